@@ -1,0 +1,91 @@
+/*
+ * TEST INFRASTRUCTURE ONLY — the CPU oracle.
+ *
+ * A from-scratch CPU restatement of the reference's correlation engine
+ * (src/FFTapps.c and src/FWTa/wavelet_def_v7.c of sergiventosa/FastPCC), used by
+ * the tests as the checker and by bench.py as the timed CPU baseline.  Nothing
+ * in the product library links, loads or calls it.
+ *
+ * Two builds of every correlation are provided:
+ *   orc_*    "reference precision": FP32 where the reference uses fftwf
+ *            (PCC2, PCC1, PCC-v, 1-bit), FP64 where it uses fftw (GNCC, WPCC),
+ *            with the same mixed-precision accumulators and roundings;
+ *   orc64_*  "all-FP64": the same algorithm with every step in double, the
+ *            yardstick against which each output type's error is stated.
+ * FFTW is replaced by the oracle's own FFT (offt.c), so FFTW's rounding is not
+ * reproduced: parity is at tolerance level, never bitwise, for FFT outputs.
+ */
+#ifndef FASTPCC_ORACLE_H
+#define FASTPCC_ORACLE_H
+#include <complex.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_PCC2 = 0, ORC_PCC1 = 1, ORC_PCCV = 2, ORC_GNCC = 3, ORC_CC1B = 4, ORC_WPCC = 5 };
+
+/* ---- building blocks (src/FFTapps.c) ---- */
+int orc_analytic_f(float complex *y, const float *x, unsigned N);            /* :104-124 */
+int orc_analytic_d(double complex *y, const double *x, unsigned N);          /* :45-80   */
+void orc_ampnormf(float complex *y, unsigned N);                             /* :178-205 */
+void orc_ampnorm(double complex *y, unsigned N);                             /* :147-175 */
+void orc_ampnormf_d(double complex *y, unsigned N);   /* AmpNormf's formula evaluated in FP64 */
+
+/* ---- drivers with the reference's signatures (src/FFTapps.h:11-16) ---- */
+int orc_pcc_set(float **y, float **x1, float **x2, int N, unsigned Tr, double v, int Lag1, int Lag2);
+int orc_pcc1_set(float **y, float **x1, float **x2, int N, unsigned Tr, int Lag1, int Lag2);
+int orc_pcc2_set(float **y, float **x1, float **x2, unsigned N, unsigned Tr, int Lag1, int Lag2);
+int orc_ccgn_set(float **y, float **x1, float **x2, unsigned N, unsigned Tr, int Lag1, int Lag2);
+int orc_cc1b_set(float **y, float **x1, float **x2, unsigned N, unsigned Tr, int Lag1, int Lag2);
+int orc_tspcc2_set(float **y, float **x1, float **x2, unsigned N, unsigned Tr, int Lag1, int Lag2,
+                   double pmin, double pmax, unsigned V, int type, double op1);
+
+/* ---- flat-array entry points for ctypes: x1/x2 are [Tr][N], y is [Tr][L] ----
+ * kind is one of ORC_*.  v is PCC-v's power; pmin/pmax/V/type/op1 are WPCC's. */
+int orc_run_f32(int kind, float *y, const float *x1, const float *x2, unsigned N, unsigned Tr,
+                int Lag1, int Lag2, double v, double pmin, double pmax, unsigned V, int type, double op1);
+int orc_run_f64(int kind, double *y, const float *x1, const float *x2, unsigned N, unsigned Tr,
+                int Lag1, int Lag2, double v, double pmin, double pmax, unsigned V, int type, double op1);
+
+/* All station pairs (i<j, p = i*S - i*(i+1)/2 + (j-i-1)), x is [S][W*N], window w
+ * = samples [w*N, (w+1)*N).  stacked != 0: y is [P][L], the mean over windows;
+ * else y is [P][W][L].  One pair per *_set call, OpenMP over windows inside it,
+ * the way the reference's one-pair-per-run CLI would be driven. */
+int orc_allpairs_f32(int kind, float *y, const float *x, unsigned S, unsigned W, unsigned N,
+                     int Lag1, int Lag2, int stacked, unsigned pair_begin, unsigned pair_end,
+                     double v, double pmin, double pmax, unsigned V, int type, double op1);
+int orc_allpairs_f64(int kind, double *y, const float *x, unsigned S, unsigned W, unsigned N,
+                     int Lag1, int Lag2, int stacked, unsigned pair_begin, unsigned pair_end,
+                     double v, double pmin, double pmax, unsigned V, int type, double op1);
+
+/* ---- wavelet filter bank (src/FWTa/wavelet_def_v7.c:204-339) ---- */
+typedef struct {
+    unsigned S;           /* number of scales = J*V             */
+    unsigned J, V;
+    unsigned Nz;          /* FFT length after the doubling rule */
+    unsigned Ls0;         /* Ls[S-1]                            */
+    double s0, K0, op1;
+    double *scale;        /* [S]                                 */
+    unsigned *Ls;         /* [S]                                 */
+    int *center;          /* [S]                                 */
+    double complex **wc;  /* [S] -> Ls[s] samples                */
+} orc_wfamily;
+
+/* Reproduces tspcc2_set's parameter setup (src/FFTapps.c:1076-1115): returns NULL
+ * for an unsupported type or lags, otherwise the family with Nz already doubled. */
+orc_wfamily *orc_wpcc_family(unsigned N, int Lag1, int Lag2, double pmin, double pmax, unsigned V,
+                             int type, double op1);
+void orc_wfamily_free(orc_wfamily *w);
+double orc_erfi(double z);
+/* Flat export for ctypes: fills scale/Ls/center (length S) and the wavelet samples
+ * concatenated (interleaved re,im) into wbuf (length 2*sum Ls); returns S or <0. */
+int orc_wpcc_family_export(unsigned N, int Lag1, int Lag2, double pmin, double pmax, unsigned V,
+                           int type, double op1, unsigned *out_meta /* S,J,V,Nz,Ls0 */,
+                           double *out_s0_K0_op1, double *scale, unsigned *Ls, int *center,
+                           double *wbuf, unsigned wbuf_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,158 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes bindings of the CPU oracle.
+
+The oracle restates the CPU branches of the reference's src/FFTapps.c and
+src/FWTa/wavelet_def_v7.c; see oracle.h for the per-function citations.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+PCC2, PCC1, PCCV, GNCC, CC1B, WPCC = range(6)
+KINDS = {"pcc2": PCC2, "pcc1": PCC1, "pccv": PCCV, "gncc": GNCC, "cc1b": CC1B, "wpcc": WPCC}
+
+_lib = None
+
+
+def build(force=False):
+    """Compile liboracle.so with the oracle's Makefile (gcc)."""
+    args = ["make", "-C", _HERE]
+    if force:
+        args.append("-B")
+    subprocess.run(args, check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        fp = ctypes.POINTER(ctypes.c_float)
+        dp = ctypes.POINTER(ctypes.c_double)
+        u, i, d = ctypes.c_uint, ctypes.c_int, ctypes.c_double
+        L.orc_run_f32.argtypes = [i, fp, fp, fp, u, u, i, i, d, d, d, u, i, d]
+        L.orc_run_f64.argtypes = [i, dp, fp, fp, u, u, i, i, d, d, d, u, i, d]
+        L.orc_allpairs_f32.argtypes = [i, fp, fp, u, u, u, i, i, i, u, u, d, d, d, u, i, d]
+        L.orc_allpairs_f64.argtypes = [i, dp, fp, u, u, u, i, i, i, u, u, d, d, d, u, i, d]
+        L.orc_analytic_f.argtypes = [ctypes.c_void_p, fp, u]
+        L.orc_analytic_d.argtypes = [ctypes.c_void_p, dp, u]
+        L.orc_ampnormf.argtypes = [ctypes.c_void_p, u]
+        L.orc_ampnorm.argtypes = [ctypes.c_void_p, u]
+        L.orc_erfi.argtypes = [d]
+        L.orc_erfi.restype = d
+        L.orc_wpcc_family_export.argtypes = [u, i, i, d, d, u, i, d, ctypes.POINTER(ctypes.c_uint),
+                                             dp, dp, ctypes.POINTER(ctypes.c_uint),
+                                             ctypes.POINTER(ctypes.c_int), dp, u]
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def out_len(kind, lag1, lag2):
+    if kind == WPCC:
+        return abs(lag2 - lag1 + 1)
+    return abs(lag2 - lag1) + 1
+
+
+def run(kind, x1, x2, lag1, lag2, f64=False, v=2.0, pmin=0.0, pmax=0.0, V=8, wtype=-1, op1=0.0):
+    """One pair over Tr windows, like the reference's *_set: x1, x2 are [Tr][N]."""
+    kind = KINDS.get(kind, kind)
+    x1 = _f32(np.atleast_2d(x1))
+    x2 = _f32(np.atleast_2d(x2))
+    Tr, N = x1.shape
+    L = out_len(kind, lag1, lag2)
+    y = np.zeros((Tr, L), dtype=np.float64 if f64 else np.float32)
+    fn = lib().orc_run_f64 if f64 else lib().orc_run_f32
+    r = fn(kind, _ptr(y, ctypes.c_double if f64 else ctypes.c_float), _ptr(x1, ctypes.c_float),
+           _ptr(x2, ctypes.c_float), N, Tr, lag1, lag2, v, pmin, pmax, V, wtype, op1)
+    if r != 0:
+        raise ValueError(f"oracle returned {r}")
+    return y
+
+
+def allpairs(kind, x, W, N, lag1, lag2, stacked=True, f64=False, pair_begin=0, pair_end=None,
+             v=2.0, pmin=0.0, pmax=0.0, V=8, wtype=-1, op1=0.0):
+    """All station pairs (i<j, row-major upper triangle) over W windows of N samples."""
+    kind = KINDS.get(kind, kind)
+    x = _f32(x)
+    S = x.shape[0]
+    P = S * (S - 1) // 2
+    if pair_end is None or pair_end > P:
+        pair_end = P
+    L = out_len(kind, lag1, lag2)
+    shape = (pair_end - pair_begin, L) if stacked else (pair_end - pair_begin, W, L)
+    y = np.zeros(shape, dtype=np.float64 if f64 else np.float32)
+    fn = lib().orc_allpairs_f64 if f64 else lib().orc_allpairs_f32
+    r = fn(kind, _ptr(y, ctypes.c_double if f64 else ctypes.c_float), _ptr(x, ctypes.c_float), S, W, N,
+           lag1, lag2, 1 if stacked else 0, pair_begin, pair_end, v, pmin, pmax, V, wtype, op1)
+    if r != 0:
+        raise ValueError(f"oracle returned {r}")
+    return y
+
+
+def analytic(x, f64=False):
+    x = np.ascontiguousarray(x, dtype=np.float64 if f64 else np.float32)
+    y = np.zeros(x.shape[0], dtype=np.complex128 if f64 else np.complex64)
+    if f64:
+        lib().orc_analytic_d(y.ctypes.data, _ptr(x, ctypes.c_double), x.shape[0])
+    else:
+        lib().orc_analytic_f(y.ctypes.data, _ptr(x, ctypes.c_float), x.shape[0])
+    return y
+
+
+def ampnormf(z):
+    z = np.array(z, dtype=np.complex64)
+    lib().orc_ampnormf(z.ctypes.data, z.shape[0])
+    return z
+
+
+def ampnorm(z):
+    z = np.array(z, dtype=np.complex128)
+    lib().orc_ampnorm(z.ctypes.data, z.shape[0])
+    return z
+
+
+def erfi(z):
+    return lib().orc_erfi(float(z))
+
+
+def wpcc_family(N, lag1, lag2, pmin, pmax, V, wtype=-1, op1=0.0):
+    """The WPCC filter bank as tspcc2_set builds it: dict of S, J, V, Nz, Ls0, s0, K0, op1,
+    scale[S], Ls[S], center[S], and the complex wavelets (list of arrays)."""
+    meta = (ctypes.c_uint * 5)()
+    fl = (ctypes.c_double * 3)()
+    r = lib().orc_wpcc_family_export(N, lag1, lag2, pmin, pmax, V, wtype, op1, meta, fl,
+                                     None, None, None, None, 0)
+    if r < 0:
+        raise ValueError(f"oracle returned {r}")
+    S = meta[0]
+    scale = np.zeros(S)
+    Ls = np.zeros(S, dtype=np.uint32)
+    center = np.zeros(S, dtype=np.int32)
+    lib().orc_wpcc_family_export(N, lag1, lag2, pmin, pmax, V, wtype, op1, meta, fl,
+                                 _ptr(scale, ctypes.c_double), _ptr(Ls, ctypes.c_uint),
+                                 _ptr(center, ctypes.c_int), None, 0)
+    tot = int(Ls.sum())
+    wbuf = np.zeros(2 * tot)
+    lib().orc_wpcc_family_export(N, lag1, lag2, pmin, pmax, V, wtype, op1, meta, fl,
+                                 _ptr(scale, ctypes.c_double), _ptr(Ls, ctypes.c_uint),
+                                 _ptr(center, ctypes.c_int), _ptr(wbuf, ctypes.c_double), 2 * tot)
+    wc = wbuf.view(np.complex128)
+    waves, off = [], 0
+    for s in range(S):
+        waves.append(wc[off:off + Ls[s]].copy())
+        off += int(Ls[s])
+    return dict(S=S, J=meta[1], V=meta[2], Nz=meta[3], Ls0=meta[4], s0=fl[0], K0=fl[1], op1=fl[2],
+                scale=scale, Ls=Ls, center=center, wc=waves)
