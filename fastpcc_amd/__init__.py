@@ -1,0 +1,8 @@
+"""fastpcc_amd: interstation phase cross-correlation on MI355X (gfx950).
+
+The compute path is libfastpcc.so (hand-written HIP kernels behind a C ABI, see
+include/fastpcc.h); this package loads it with ctypes and mirrors the reference's
+correlation interface on numpy arrays (api.py).
+"""
+from .api import (FastPCCError, DeviceRunner, allpairs, make_job, output_shape, pcc2_set)  # noqa: F401
+from . import synth  # noqa: F401

@@ -1,0 +1,63 @@
+"""Loads libfastpcc.so (the HIP library) and declares its C ABI for ctypes.
+
+There is no fallback: if the library is missing or fails to load, every entry
+point raises.  Build it with ``python -m fastpcc_amd.build`` (or
+``__graft_entry__.build()``).
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfastpcc.so")
+
+FPCC_PCC2, FPCC_PCC1, FPCC_PCCV, FPCC_GNCC, FPCC_CC1B, FPCC_WPCC = range(6)
+STACK_NONE, STACK_PLAIN, STACK_ZSUM, STACK_GRAM = range(4)
+
+
+class FpccJob(ctypes.Structure):
+    """Mirror of fpcc_job in include/fastpcc.h."""
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("stack", ctypes.c_int),
+        ("n_stations", ctypes.c_uint),
+        ("n_windows", ctypes.c_uint),
+        ("win_len", ctypes.c_uint),
+        ("station_stride", ctypes.c_ulonglong),
+        ("lag1", ctypes.c_int),
+        ("lag2", ctypes.c_int),
+        ("v", ctypes.c_double),
+        ("pmin", ctypes.c_double),
+        ("pmax", ctypes.c_double),
+        ("V", ctypes.c_uint),
+        ("wtype", ctypes.c_int),
+        ("op1", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises RuntimeError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"fastpcc: HIP library not built ({LIB_PATH}); run python -m fastpcc_amd.build")
+    L = ctypes.CDLL(LIB_PATH)
+    fpp = ctypes.POINTER(ctypes.POINTER(ctypes.c_float))
+    u, i, d = ctypes.c_uint, ctypes.c_int, ctypes.c_double
+    L.pcc2_set.argtypes = [fpp, fpp, fpp, u, u, i, i]
+    L.pcc2_set.restype = i
+    jp = ctypes.POINTER(FpccJob)
+    L.fpcc_output_len.argtypes = [jp]
+    L.fpcc_output_len.restype = ctypes.c_size_t
+    L.fpcc_workspace_size.argtypes = [jp]
+    L.fpcc_workspace_size.restype = ctypes.c_size_t
+    L.fpcc_allpairs_device.argtypes = [jp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.c_void_p]
+    L.fpcc_allpairs_device.restype = i
+    L.fpcc_allpairs.argtypes = [jp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    L.fpcc_allpairs.restype = i
+    _lib = L
+    return L

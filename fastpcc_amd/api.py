@@ -1,0 +1,125 @@
+"""Host-side mirror of the reference's correlation interface (src/FFTapps.h).
+
+Each function takes numpy arrays shaped like the reference's row-pointer arrays
+([Tr][N] for x1 and x2) and returns y as [Tr][L], calling the HIP library
+through its C ABI.  Return codes of the C layer are raised as exceptions with the
+reference's meaning (-1 invalid/GPU failure, -2 unsupported option, -3 lags too
+large).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import FpccJob
+
+_ERR = {-1: "invalid argument or GPU failure", -2: "unsupported option", -3: "|lag| >= N or unsupported type"}
+
+
+class FastPCCError(RuntimeError):
+    def __init__(self, code, where):
+        super().__init__(f"{where} returned {code}: {_ERR.get(code, 'error')}")
+        self.code = code
+
+
+def _rows(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    ptrs = (ctypes.POINTER(ctypes.c_float) * a.shape[0])()
+    for t in range(a.shape[0]):
+        ptrs[t] = a[t].ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    return a, ptrs
+
+
+def _lag_len(lag1, lag2):
+    return abs(lag2 - lag1) + 1
+
+
+def pcc2_set(x1, x2, lag1, lag2):
+    """PCC2 of one station pair over Tr windows (src/FFTapps.c:785-881)."""
+    x1 = np.atleast_2d(x1)
+    x2 = np.atleast_2d(x2)
+    if x1.shape != x2.shape:
+        raise ValueError("x1 and x2 must have the same shape")
+    Tr, N = x1.shape
+    L = _lag_len(lag1, lag2)
+    y = np.zeros((Tr, L), dtype=np.float32)
+    _, py = _rows(y)
+    a1, p1 = _rows(x1)
+    a2, p2 = _rows(x2)
+    r = _lib.lib().pcc2_set(py, p1, p2, N, Tr, lag1, lag2)
+    if r != 0:
+        raise FastPCCError(r, "pcc2_set")
+    return y
+
+
+KIND = {"pcc2": _lib.FPCC_PCC2, "pcc1": _lib.FPCC_PCC1, "pccv": _lib.FPCC_PCCV, "gncc": _lib.FPCC_GNCC,
+        "cc1b": _lib.FPCC_CC1B, "wpcc": _lib.FPCC_WPCC}
+STACK = {"none": _lib.STACK_NONE, "plain": _lib.STACK_PLAIN, "zsum": _lib.STACK_ZSUM, "gram": _lib.STACK_GRAM}
+
+
+def make_job(kind, n_stations, n_windows, win_len, lag1, lag2, stack="plain", station_stride=None, v=2.0,
+             pmin=0.0, pmax=0.0, V=8, wtype=-1, op1=0.0):
+    job = FpccJob()
+    job.kind = KIND.get(kind, kind)
+    job.stack = STACK.get(stack, stack)
+    job.n_stations = n_stations
+    job.n_windows = n_windows
+    job.win_len = win_len
+    job.station_stride = station_stride if station_stride is not None else n_windows * win_len
+    job.lag1, job.lag2 = lag1, lag2
+    job.v, job.pmin, job.pmax, job.V, job.wtype, job.op1 = v, pmin, pmax, V, wtype, op1
+    return job
+
+
+def output_shape(job):
+    P = job.n_stations * (job.n_stations - 1) // 2
+    L = _lag_len(job.lag1, job.lag2)
+    return (P, job.n_windows, L) if job.stack == _lib.STACK_NONE else (P, L)
+
+
+def allpairs(x, n_windows, win_len, lag1, lag2, kind="pcc2", stack="plain", **kw):
+    """All station pairs from host data x [S][>= W*N]; returns y on the host."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    job = make_job(kind, x.shape[0], n_windows, win_len, lag1, lag2, stack, station_stride=x.shape[1], **kw)
+    n = _lib.lib().fpcc_output_len(ctypes.byref(job))
+    if n == 0:
+        raise FastPCCError(-1, "fpcc_output_len")
+    y = np.zeros(output_shape(job), dtype=np.float32)
+    r = _lib.lib().fpcc_allpairs(ctypes.byref(job), x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                 y.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    if r != 0:
+        raise FastPCCError(r, "fpcc_allpairs")
+    return y
+
+
+class DeviceRunner:
+    """Device-resident all-pairs runs on torch-allocated HBM (torch is plumbing:
+    allocation and the stream handle).  The workspace is allocated once."""
+
+    def __init__(self, job, device="cuda:0"):
+        import torch
+        self.torch = torch
+        self.job = job
+        self.device = torch.device(device)
+        L = _lib.lib()
+        self.nout = L.fpcc_output_len(ctypes.byref(job))
+        if self.nout == 0:
+            raise FastPCCError(-1, "fpcc_output_len")
+        self.ws_bytes = L.fpcc_workspace_size(ctypes.byref(job))
+        self.ws = torch.empty(max(self.ws_bytes, 1), dtype=torch.uint8, device=self.device)
+        self.y = torch.empty(self.nout, dtype=torch.float32, device=self.device)
+
+    def run(self, x_dev, stream=None):
+        torch = self.torch
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        assert x_dev.dtype == torch.float32 and x_dev.is_contiguous() and x_dev.device == self.device
+        need = (self.job.n_stations - 1) * self.job.station_stride + self.job.n_windows * self.job.win_len
+        if x_dev.numel() < need:
+            raise ValueError("input tensor too small for the job")
+        r = _lib.lib().fpcc_allpairs_device(ctypes.byref(self.job), ctypes.c_void_p(x_dev.data_ptr()),
+                                            ctypes.c_void_p(self.y.data_ptr()), ctypes.c_void_p(self.ws.data_ptr()),
+                                            self.ws_bytes, ctypes.c_void_p(stream.cuda_stream))
+        if r != 0:
+            raise FastPCCError(r, "fpcc_allpairs_device")
+        return self.y.view(output_shape(self.job))

@@ -1,0 +1,49 @@
+// Internal host-side interfaces of the fastpcc engine (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdio.h>
+#include "fft_core.hpp"
+
+namespace fpcc {
+
+// Returns a message on failure and keeps the first error of a call sequence.
+#define FPCC_CHECK(expr)                                                                      \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess) {                                                               \
+            fprintf(stderr, "fastpcc: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(_e), \
+                    __FILE__, __LINE__);                                                      \
+            return -1;                                                                        \
+        }                                                                                     \
+    } while (0)
+
+// Device-resident plan of a length-n complex FFT in precision T (float/double):
+// the pass list and the twiddle table (FP64-generated, rounded once to T).
+struct DevPlan {
+    FftPlan plan;
+    const void *tw;  // C2<T>*, on the current device
+    bool ok;
+};
+
+// Plan for length n (cached per device and precision).  ok=false if n has a
+// prime factor above 7 or needs more than kMaxPasses passes.
+const DevPlan &get_plan(int n, bool f64);
+// Table of exp(+2*pi*i*k/nz), k in [0, nz/2), for the half-length real-output
+// inverse (cached per device and precision).
+const void *get_c2r_twiddles(int nz, bool f64);
+// Radix list for n (host); returns false if unsupported.
+bool factor_plan(int n, FftPlan &p);
+
+// Smallest power of two >= v
+static inline unsigned pow2_ceil(unsigned v)
+{
+    unsigned r = 1;
+    while (r < v) r <<= 1;
+    return r;
+}
+
+// Nz rule of the reference (src/FFTapps.c:805)
+unsigned nz_rule(unsigned N, unsigned M);
+
+}  // namespace fpcc

@@ -1,0 +1,306 @@
+// Device FFT core for gfx950: register butterflies + LDS-staged Stockham passes.
+//
+// A length-n transform runs inside one workgroup.  n is factored into radices
+// from {16, 8, 4, 2, 9, 3, 25, 5, 7}; pass p reads in[b + r*n/R], applies the
+// twiddles W_{Ns*R}^{r*k} (k = b mod Ns, Ns = product of earlier radices), does
+// a radix-R DFT in registers and writes out[(b/Ns)*Ns*R + k + r*Ns] (Stockham
+// autosort: natural order in and out, no bit reversal).  Passes ping-pong between
+// two LDS buffers.  Twiddles come from a per-plan table generated in FP64 on the
+// host and rounded once to the working precision; entry (r-1)*Ns + k of pass p
+// holds W_{Ns*R}^{r*k} of the forward transform (the inverse uses its conjugate).
+#pragma once
+#include <utility>
+#include "cplx.hpp"
+
+namespace fpcc {
+
+constexpr int kMaxPasses = 10;
+
+// Plan of one transform length, passed to kernels by value.
+struct FftPlan {
+    int n;
+    int npass;
+    int radix[kMaxPasses];
+    int tw_off[kMaxPasses];  // offset of each pass's twiddles in the table
+};
+
+// ---- compile-time loop ----
+template <int... Is, class F>
+__host__ __device__ __forceinline__ void sfor_impl(std::integer_sequence<int, Is...>, F &&f)
+{
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, class F>
+__host__ __device__ __forceinline__ void sfor(F &&f)
+{
+    sfor_impl(std::make_integer_sequence<int, N>{}, f);
+}
+
+// multiply by the constant W_R^M = exp(DIR*2*pi*i*M/R), folding the trivial cases
+template <int DIR, int R, int M, typename T>
+__host__ __device__ __forceinline__ C2<T> twc(C2<T> a)
+{
+    constexpr int m = ((M % R) + R) % R;
+    if constexpr (m == 0) {
+        return a;
+    } else if constexpr (2 * m == R) {
+        return C2<T>{-a.x, -a.y};
+    } else if constexpr (4 * m == R) {
+        return rot<DIR>(a);
+    } else if constexpr (4 * m == 3 * R) {
+        return rot<-DIR>(a);
+    } else {
+        constexpr T c = (T)ct_cos2pi(m, R);
+        constexpr T s = (T)(DIR * ct_sin2pi(m, R));
+        return C2<T>{a.x * c - a.y * s, a.x * s + a.y * c};
+    }
+}
+
+constexpr int first_factor(int r)
+{
+    return (r % 4 == 0 && r != 4) ? 4 : (r % 2 == 0 && r != 2) ? 2 : (r % 3 == 0 && r != 3) ? 3 : (r % 5 == 0 && r != 5) ? 5 : r;
+}
+
+// ---- in-register DFTs ----
+template <int R, int DIR>
+struct Dft {
+    // composite: R = A*B, Cooley-Tukey in registers with constant twiddles
+    static constexpr int A = first_factor(R);
+    static constexpr int B = R / A;
+    static_assert(A > 1 && B > 1, "unsupported radix");
+    template <typename T>
+    __host__ __device__ __forceinline__ static void run(C2<T> *v)
+    {
+        C2<T> t[R];
+        sfor<B>([&](auto b) {
+            C2<T> col[A];
+            sfor<A>([&](auto a) { col[a()] = v[B * a() + b()]; });
+            Dft<A, DIR>::run(col);
+            sfor<A>([&](auto k1) { t[b() * A + k1()] = twc<DIR, R, b() * k1()>(col[k1()]); });
+        });
+        sfor<A>([&](auto k1) {
+            C2<T> row[B];
+            sfor<B>([&](auto b) { row[b()] = t[b() * A + k1()]; });
+            Dft<B, DIR>::run(row);
+            sfor<B>([&](auto k2) { v[k1() + A * k2()] = row[k2()]; });
+        });
+    }
+};
+
+template <int DIR>
+struct Dft<2, DIR> {
+    template <typename T>
+    __host__ __device__ __forceinline__ static void run(C2<T> *v)
+    {
+        C2<T> a = v[0], b = v[1];
+        v[0] = a + b;
+        v[1] = a - b;
+    }
+};
+
+template <int DIR>
+struct Dft<4, DIR> {
+    template <typename T>
+    __host__ __device__ __forceinline__ static void run(C2<T> *v)
+    {
+        C2<T> s02 = v[0] + v[2], d02 = v[0] - v[2];
+        C2<T> s13 = v[1] + v[3], d13 = rot<DIR>(v[1] - v[3]);
+        v[0] = s02 + s13;
+        v[2] = s02 - s13;
+        v[1] = d02 + d13;
+        v[3] = d02 - d13;
+    }
+};
+
+template <int DIR>
+struct Dft<8, DIR> {
+    template <typename T>
+    __host__ __device__ __forceinline__ static void run(C2<T> *v)
+    {
+        // radix-2 over two radix-4 halves (even / odd samples)
+        C2<T> e[4] = {v[0], v[2], v[4], v[6]};
+        C2<T> o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, DIR>::run(e);
+        Dft<4, DIR>::run(o);
+        constexpr T h = (T)0.70710678118654752440;
+        C2<T> o1 = C2<T>{(o[1].x - DIR * o[1].y) * h, (o[1].y + DIR * o[1].x) * h};  // * W8
+        C2<T> o2 = rot<DIR>(o[2]);                                                  // * W8^2
+        C2<T> o3 = C2<T>{(-o[3].x - DIR * o[3].y) * h, (-o[3].y + DIR * o[3].x) * h}; // * W8^3
+        v[0] = e[0] + o[0];
+        v[4] = e[0] - o[0];
+        v[1] = e[1] + o1;
+        v[5] = e[1] - o1;
+        v[2] = e[2] + o2;
+        v[6] = e[2] - o2;
+        v[3] = e[3] + o3;
+        v[7] = e[3] - o3;
+    }
+};
+
+template <int DIR>
+struct Dft<3, DIR> {
+    template <typename T>
+    __host__ __device__ __forceinline__ static void run(C2<T> *v)
+    {
+        constexpr T s3 = (T)0.86602540378443864676;
+        C2<T> s = v[1] + v[2];
+        C2<T> d = s3 * (v[1] - v[2]);
+        C2<T> t = C2<T>{v[0].x - (T)0.5 * s.x, v[0].y - (T)0.5 * s.y};
+        v[0] = v[0] + s;
+        C2<T> rd = rot<DIR>(d);
+        v[1] = t + rd;
+        v[2] = t - rd;
+    }
+};
+
+template <int DIR>
+struct Dft<5, DIR> {
+    template <typename T>
+    __host__ __device__ __forceinline__ static void run(C2<T> *v)
+    {
+        constexpr T c1 = (T)0.30901699437494742410;   // cos(2pi/5)
+        constexpr T c2 = (T)-0.80901699437494742410;  // cos(4pi/5)
+        constexpr T q1 = (T)(DIR * 0.95105651629515357212);  // DIR*sin(2pi/5)
+        constexpr T q2 = (T)(DIR * 0.58778525229247312917);  // DIR*sin(4pi/5)
+        C2<T> a0 = v[0];
+        C2<T> s7 = v[1] + v[4], s10 = v[1] - v[4], s8 = v[2] + v[3], s9 = v[2] - v[3];
+        v[0] = a0 + s7 + s8;
+        C2<T> s5 = a0 + c1 * s7 + c2 * s8;
+        C2<T> s6 = q1 * s10 + q2 * s9;
+        C2<T> s11 = a0 + c2 * s7 + c1 * s8;
+        C2<T> s12 = q2 * s10 - q1 * s9;
+        C2<T> is6 = C2<T>{-s6.y, s6.x}, is12 = C2<T>{-s12.y, s12.x};
+        v[1] = s5 + is6;
+        v[4] = s5 - is6;
+        v[2] = s11 + is12;
+        v[3] = s11 - is12;
+    }
+};
+
+template <int DIR>
+struct Dft<7, DIR> {
+    template <typename T>
+    __host__ __device__ __forceinline__ static void run(C2<T> *v)
+    {
+        C2<T> a[7];
+        sfor<7>([&](auto j) { a[j()] = v[j()]; });
+        C2<T> sp[3], sm[3];
+        sfor<3>([&](auto j) {
+            sp[j()] = a[j() + 1] + a[6 - j()];
+            sm[j()] = a[j() + 1] - a[6 - j()];
+        });
+        v[0] = a[0] + sp[0] + sp[1] + sp[2];
+        sfor<6>([&](auto km1) {
+            constexpr int k = km1() + 1;
+            C2<T> re = a[0], im = C2<T>{0, 0};
+            sfor<3>([&](auto j) {
+                constexpr T c = (T)ct_cos2pi((j() + 1) * k, 7);
+                constexpr T s = (T)(DIR * ct_sin2pi((j() + 1) * k, 7));
+                re = re + c * sp[j()];
+                im = im + s * sm[j()];
+            });
+            v[k] = re + C2<T>{-im.y, im.x};
+        });
+    }
+};
+
+// ---- one Stockham pass over LDS (ping-pong: in != out) ----
+template <typename T, int R, int DIR>
+__device__ __forceinline__ void stockham_pass(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int n,
+                                              int Ns, const C2<T> *__restrict__ tw)
+{
+    const int nb = n / R;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        C2<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = in[b + r * nb];
+        const int k = b % Ns;
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                C2<T> w = tw[(r - 1) * Ns + k];
+                if (DIR > 0) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        Dft<R, DIR>::run(v);
+        const int base = (b - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[base + r * Ns] = v[r];
+    }
+}
+
+// Runs all passes of `plan` on data in buf0 (LDS), ping-ponging with buf1.
+// Must be called by the whole workgroup; ends with a barrier.  Returns the
+// buffer that holds the result.
+template <typename T, int DIR>
+__device__ C2<T> *fft_lds(C2<T> *buf0, C2<T> *buf1, const FftPlan &plan, const C2<T> *__restrict__ tw)
+{
+    C2<T> *in = buf0, *out = buf1;
+    int Ns = 1;
+    for (int p = 0; p < plan.npass; ++p) {
+        const int R = plan.radix[p];
+        const C2<T> *twp = tw + plan.tw_off[p];
+        switch (R) {
+        case 2: stockham_pass<T, 2, DIR>(in, out, plan.n, Ns, twp); break;
+        case 3: stockham_pass<T, 3, DIR>(in, out, plan.n, Ns, twp); break;
+        case 4: stockham_pass<T, 4, DIR>(in, out, plan.n, Ns, twp); break;
+        case 5: stockham_pass<T, 5, DIR>(in, out, plan.n, Ns, twp); break;
+        case 7: stockham_pass<T, 7, DIR>(in, out, plan.n, Ns, twp); break;
+        case 8: stockham_pass<T, 8, DIR>(in, out, plan.n, Ns, twp); break;
+        case 9: stockham_pass<T, 9, DIR>(in, out, plan.n, Ns, twp); break;
+        case 16: stockham_pass<T, 16, DIR>(in, out, plan.n, Ns, twp); break;
+        case 25: stockham_pass<T, 25, DIR>(in, out, plan.n, Ns, twp); break;
+        default: break;
+        }
+        __syncthreads();
+        C2<T> *t = in;
+        in = out;
+        out = t;
+        Ns *= R;
+    }
+    return in;
+}
+
+// ---- workgroup reductions (wave64) ----
+template <typename T>
+__device__ __forceinline__ T wave_max(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+// red must hold blockDim.x/64 elements; result broadcast to all threads
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T *red)
+{
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    T r = red[0];
+    for (int i = 1; i < nw; ++i) r = fmax(r, red[i]);
+    return r;
+}
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *red)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    T r = red[0];
+    for (int i = 1; i < nw; ++i) r += red[i];
+    return r;
+}
+
+}  // namespace fpcc

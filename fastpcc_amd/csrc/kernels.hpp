@@ -1,0 +1,17 @@
+// Launchers of the device stages (host-callable, stream-ordered, no allocation).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "cplx.hpp"
+
+namespace fpcc {
+
+// PCC2 station stage: F[s*W+w][Nz] = FFT_Nz(pad(AmpNormf(analytic_N(x_sw))))
+int launch_station_pcc(const float *d_x, long long station_stride, int S, int W, int N, int Nz, cf32 *d_F,
+                       hipStream_t st);
+// PCC2 pair stage over pairs [pair_base, pair_base+npairs) of S stations.
+// mode 0: y[p][w][L]; 1: mean over windows of per-window IFFTs; 2: IFFT of the
+// window sum (same result by linearity).
+int launch_pair_pcc2(const cf32 *d_F, int S, int W, int N, int Nz, int lag0, int L, int mode, int pair_base,
+                     int npairs, float *d_y, hipStream_t st);
+
+}  // namespace fpcc

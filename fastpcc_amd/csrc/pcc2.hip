@@ -1,0 +1,230 @@
+// PCC2 on gfx950: station stage (phasor spectra) and pair stage (cross-correlation).
+//
+// Station stage, one workgroup per (station, window), all in LDS:
+//   x[N] -> DFT_N -> analytic mask -> IDFT_N -> AmpNormf -> zero-pad -> DFT_Nz
+// which is AnalyticSignal_plan_float + AmpNormf + the zero-padded forward FFT of
+// pcc2_set (src/FFTapps.c:104-124, 178-205, 838-848), computed once per station
+// window instead of once per pair window.
+//
+// Pair stage: only Re(IFFT_Nz(conj(F1) F2)) is needed (src/FFTapps.c:851-856), so
+// the cross spectrum Y is folded to its Hermitian part and inverted as a real
+// signal with a half-length complex IFFT:
+//   Z[k] = P_k + i e^{+2 pi i k/Nz} Q_k,  k < Nz/2,
+//   P_k = (Y[k] + Y[k+M] + conj(Y[-k] + Y[M-k])) / 2,
+//   Q_k = (Y[k] - Y[k+M] + conj(Y[-k] - Y[M-k])) / 2,   M = Nz/2,
+//   z = IFFT_M(Z),  y[2m] = Re z[m],  y[2m+1] = Im z[m].
+// The lag window y[(lag0 + l) mod Nz], l < L, is scaled by 1/((float)Nz*(float)N).
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace fpcc {
+
+__global__ void __launch_bounds__(512)
+k_station_pcc(const float *__restrict__ x, long long station_stride, int W, int N, int Nz, FftPlan planN,
+              const cf32 *__restrict__ twN, FftPlan planZ, const cf32 *__restrict__ twZ, float da1,
+              cf32 *__restrict__ F)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cf32 *buf0 = reinterpret_cast<cf32 *>(smem_raw);
+    cf32 *buf1 = buf0 + Nz;
+    float *red = reinterpret_cast<float *>(buf1 + Nz);
+    const int sw = blockIdx.x;
+    const int s = sw / W, w = sw - s * W;
+    const float *in = x + (long long)s * station_stride + (long long)w * N;
+    const int T = blockDim.x;
+
+    for (int n = threadIdx.x; n < N; n += T) buf0[n] = cf32{in[n], 0.0f};
+    __syncthreads();
+    cf32 *X = fft_lds<float, -1>(buf0, buf1, planN, twN);
+    // analytic mask (AnalyticSignal_plan_float, :113-117)
+    const int half = (N + 1) >> 1;
+    const float two_da1 = 2 * da1;
+    for (int k = threadIdx.x; k < N; k += T) {
+        cf32 v = X[k];
+        float g;
+        if (k == 0) g = da1;
+        else if (k < half) g = two_da1;
+        else if (!(N & 1) && k == (N >> 1)) g = da1;
+        else g = 0.0f;
+        X[k] = cf32{v.x * g, v.y * g};
+    }
+    __syncthreads();
+    cf32 *other = (X == buf0) ? buf1 : buf0;
+    cf32 *z = fft_lds<float, +1>(X, other, planN, twN);
+    // AmpNormf (:178-205): eps = 1e-6*sqrt(max|z|^2) in double, stored as float
+    float m = 0.0f;
+    for (int n = threadIdx.x; n < N; n += T) {
+        cf32 v = z[n];
+        float a = v.x * v.x + v.y * v.y;
+        m = fmaxf(m, a);
+    }
+    m = block_max(m, red);
+    const float eps = (float)(1e-6 * sqrt((double)m));
+    cf32 *dst = (z == buf0) ? buf1 : buf0;
+    for (int n = threadIdx.x; n < Nz; n += T) {
+        cf32 o = cf32{0.0f, 0.0f};
+        if (n < N) {
+            cf32 v = z[n];
+            float r = 1.0f / (sqrtf(v.x * v.x + v.y * v.y) + eps);
+            if (__builtin_isnormal(r)) o = cf32{v.x * r, v.y * r};
+        }
+        dst[n] = o;
+    }
+    __syncthreads();
+    cf32 *Fz = fft_lds<float, -1>(dst, z, planZ, twZ);
+    cf32 *out = F + (long long)sw * Nz;
+    for (int k = threadIdx.x; k < Nz; k += T) out[k] = Fz[k];
+}
+
+__device__ __forceinline__ void pair_of(int p, int S, int &i, int &j)
+{
+    // p = i*S - i*(i+1)/2 + (j-i-1), i<j
+    double b = 2.0 * S - 1.0;
+    int ii = (int)floor((b - sqrt(b * b - 8.0 * p)) * 0.5);
+    if (ii < 0) ii = 0;
+    while (ii > 0 && ii * S - ii * (ii + 1) / 2 > p) --ii;
+    while ((ii + 1) * S - (ii + 1) * (ii + 2) / 2 <= p) ++ii;
+    i = ii;
+    j = p - (ii * S - ii * (ii + 1) / 2) + ii + 1;
+}
+
+// Fold conj(F1)F2 into the half-length spectrum Z (see file header) in LDS.
+__device__ __forceinline__ void fold_cross(const cf32 *__restrict__ F1, const cf32 *__restrict__ F2, int Nz,
+                                           const cf32 *__restrict__ twc, cf32 *Z, bool accumulate)
+{
+    const int M = Nz >> 1;
+    for (int k = threadIdx.x; k < M; k += blockDim.x) {
+        const int kn = (Nz - k) & (Nz - 1);
+        const int km = M - k;
+        cf32 y0 = cmulc(F1[k], F2[k]);
+        cf32 y1 = cmulc(F1[k + M], F2[k + M]);
+        cf32 y2 = cmulc(F1[kn], F2[kn]);
+        cf32 y3 = cmulc(F1[km], F2[km]);
+        cf32 P = cf32{0.5f * (y0.x + y1.x + y2.x + y3.x), 0.5f * (y0.y + y1.y - y2.y - y3.y)};
+        cf32 Q = cf32{0.5f * (y0.x - y1.x + y2.x - y3.x), 0.5f * (y0.y - y1.y - y2.y + y3.y)};
+        cf32 wq = cmul(twc[k], Q);
+        cf32 zk = cf32{P.x - wq.y, P.y + wq.x};
+        if (accumulate) {
+            cf32 a = Z[k];
+            Z[k] = cf32{a.x + zk.x, a.y + zk.y};
+        } else {
+            Z[k] = zk;
+        }
+    }
+}
+
+__device__ __forceinline__ float real_out(const cf32 *z, int n)
+{
+    cf32 v = z[n >> 1];
+    return (n & 1) ? v.y : v.x;
+}
+
+// mode 0: per-window outputs y[p][w][L]; 1: mean of per-window IFFTs y[p][L];
+// 2: IFFT of the window-summed folded spectrum, y[p][L] (identical by linearity).
+__global__ void __launch_bounds__(512)
+k_pair_pcc2(const cf32 *__restrict__ F, int S, int W, int Nz, FftPlan planM, const cf32 *__restrict__ twM,
+            const cf32 *__restrict__ twc, int lag0, int L, float scale, int mode, int pair_base,
+            float *__restrict__ y)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int M = Nz >> 1;
+    cf32 *buf0 = reinterpret_cast<cf32 *>(smem_raw);
+    cf32 *buf1 = buf0 + M;
+    float *acc = reinterpret_cast<float *>(buf1 + M);
+    int p, w0, w1;
+    if (mode == 0) {
+        p = blockIdx.x / W + pair_base;
+        w0 = blockIdx.x % W;
+        w1 = w0 + 1;
+    } else {
+        p = blockIdx.x + pair_base;
+        w0 = 0;
+        w1 = W;
+    }
+    int i, j;
+    pair_of(p, S, i, j);
+    const int T = blockDim.x;
+    if (mode == 2) {
+        for (int w = w0; w < w1; ++w) {
+            fold_cross(F + (long long)(i * W + w) * Nz, F + (long long)(j * W + w) * Nz, Nz, twc, buf0, w > w0);
+            __syncthreads();
+        }
+        cf32 *z = fft_lds<float, +1>(buf0, buf1, planM, twM);
+        const float sc = scale / (float)W;
+        float *dst = y + (long long)(p - pair_base) * L;
+        for (int l = threadIdx.x; l < L; l += T) {
+            int n = (lag0 + l) & (Nz - 1);
+            dst[l] = sc * real_out(z, n);
+        }
+        return;
+    }
+    for (int l = threadIdx.x; l < L; l += T) acc[l] = 0.0f;
+    for (int w = w0; w < w1; ++w) {
+        fold_cross(F + (long long)(i * W + w) * Nz, F + (long long)(j * W + w) * Nz, Nz, twc, buf0, false);
+        __syncthreads();
+        cf32 *z = fft_lds<float, +1>(buf0, buf1, planM, twM);
+        for (int l = threadIdx.x; l < L; l += T) {
+            int n = (lag0 + l) & (Nz - 1);
+            acc[l] += scale * real_out(z, n);
+        }
+        __syncthreads();
+    }
+    if (mode == 0) {
+        float *dst = y + ((long long)(p - pair_base) * W + w0) * L;
+        for (int l = threadIdx.x; l < L; l += T) dst[l] = acc[l];
+    } else {
+        const float inv = 1.0f / (float)W;
+        float *dst = y + (long long)(p - pair_base) * L;
+        for (int l = threadIdx.x; l < L; l += T) dst[l] = acc[l] * inv;
+    }
+}
+
+// gfx950 workgroups may address up to 160 KB of LDS; opt in above 64 KB.
+static void allow_lds(const void *fn, size_t bytes)
+{
+    if (bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+int launch_station_pcc(const float *d_x, long long station_stride, int S, int W, int N, int Nz, cf32 *d_F,
+                       hipStream_t st)
+{
+    const DevPlan &pN = get_plan(N, false);
+    const DevPlan &pZ = get_plan(Nz, false);
+    if (!pN.ok || !pZ.ok) {
+        fprintf(stderr, "fastpcc: unsupported FFT length (N=%d, Nz=%d)\n", N, Nz);
+        return -1;
+    }
+    if (Nz > 8192) {
+        fprintf(stderr, "fastpcc: Nz=%d exceeds the single-workgroup limit (8192)\n", Nz);
+        return -1;
+    }
+    const float da1 = (float)(1 / (double)N);
+    const int threads = 512;
+    size_t lds = sizeof(cf32) * 2 * (size_t)Nz + sizeof(float) * 16;
+    allow_lds((const void *)k_station_pcc, lds);
+    hipLaunchKernelGGL(k_station_pcc, dim3(S * W), dim3(threads), lds, st, d_x, station_stride, W, N, Nz,
+                       pN.plan, (const cf32 *)pN.tw, pZ.plan, (const cf32 *)pZ.tw, da1, d_F);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_pair_pcc2(const cf32 *d_F, int S, int W, int N, int Nz, int lag0, int L, int mode, int pair_base,
+                     int npairs, float *d_y, hipStream_t st)
+{
+    const int M = Nz / 2;
+    const DevPlan &pM = get_plan(M, false);
+    const void *twc = get_c2r_twiddles(Nz, false);
+    if (!pM.ok || !twc) return -1;
+    const float scale = 1 / ((float)Nz * (float)N);
+    const int threads = 512;
+    size_t lds = sizeof(cf32) * 2 * (size_t)M + sizeof(float) * (size_t)L;
+    int blocks = mode == 0 ? npairs * W : npairs;
+    if (blocks == 0) return 0;
+    allow_lds((const void *)k_pair_pcc2, lds);
+    hipLaunchKernelGGL(k_pair_pcc2, dim3(blocks), dim3(threads), lds, st, d_F, S, W, Nz, pM.plan,
+                       (const cf32 *)pM.tw, (const cf32 *)twc, lag0, L, scale, mode, pair_base, d_y);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace fpcc

@@ -1,0 +1,88 @@
+/*
+ * fastpcc — MI355X (gfx950) interstation correlation library, C ABI.
+ *
+ * Drop-in for the correlation entry points of sergiventosa/FastPCC: the
+ * functions below keep the reference's names, argument meanings, ownership and
+ * return codes, so that PCC_fullpair_1b.c (or any C host) links against
+ * libfastpcc.so instead of FFTapps.o/ccs_cuda.o.  All arithmetic runs in
+ * hand-written HIP kernels on the current HIP device; there is no CPU fallback:
+ * a GPU failure is returned as an error code.
+ *
+ * Return codes (as the reference):
+ *    0  success
+ *   -1  invalid argument (NULL pointer, negative length) or a GPU failure
+ *   -2  unsupported option (cuda_tspcc2_set_freq with Down_smp != 1)
+ *   -3  |Lag1| or |Lag2| >= N, or an unsupported wavelet type
+ * Diagnostics are printed to stderr.
+ *
+ * Layout: y, x1, x2 are arrays of Tr row pointers owned by the caller; x rows
+ * hold N samples, y rows hold L = |Lag2-Lag1|+1 lags (abs(Lag2-Lag1+1) for
+ * tspcc2_set, as the reference).  y[tr][l] is the correlation at lag
+ * min(Lag1,Lag2)+l, positive lags meaning x2 later than x1.
+ */
+#ifndef FASTPCC_H
+#define FASTPCC_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * Reference entry points (src/FFTapps.h:6-16).  Host buffers in and out.
+ * ------------------------------------------------------------------------- */
+
+/* PCC of power 2 (Ventosa et al. 2019), replaces pcc2_set, src/FFTapps.c:785-881. */
+int pcc2_set(float **const y, float **const x1, float **const x2, const unsigned int N,
+             const unsigned int Tr, const int Lag1, const int Lag2);
+
+/* ---------------------------------------------------------------------------
+ * Batched all-pairs entry (no counterpart in the reference, which correlates one
+ * pair per process run; src/PCC_fullpair_1b.c:467-499).
+ *
+ * Input x: n_stations rows, row s starting at x + s*station_stride, each holding
+ * n_windows consecutive windows of win_len samples (window w = samples
+ * [w*win_len, (w+1)*win_len)).  Pairs p <-> (i<j) in row-major upper-triangle
+ * order, p = i*S - i*(i+1)/2 + (j-i-1); station i is x1 (virtual source), j is x2.
+ * Output y: stack == FPCC_STACK_NONE -> [P][n_windows][L], else [P][L] holding
+ * the arithmetic mean over windows.
+ * ------------------------------------------------------------------------- */
+enum { FPCC_PCC2 = 0, FPCC_PCC1 = 1, FPCC_PCCV = 2, FPCC_GNCC = 3, FPCC_CC1B = 4, FPCC_WPCC = 5 };
+enum {
+    FPCC_STACK_NONE = 0,  /* per-window outputs                                       */
+    FPCC_STACK_PLAIN = 1, /* mean of per-window correlations (one IFFT per pair-window) */
+    FPCC_STACK_ZSUM = 2,  /* sum of folded cross spectra over windows, one IFFT per pair */
+    FPCC_STACK_GRAM = 3   /* per-frequency Hermitian rank-W Gram, one IFFT per pair     */
+};
+
+typedef struct {
+    int kind;                        /* FPCC_PCC2 ...                                  */
+    int stack;                       /* FPCC_STACK_*                                   */
+    unsigned int n_stations;         /* S                                              */
+    unsigned int n_windows;          /* W                                              */
+    unsigned int win_len;            /* N, samples per window                          */
+    unsigned long long station_stride; /* elements between station rows (>= W*N)      */
+    int lag1, lag2;                  /* lag window, lag1 <= lag2, |lag| < N            */
+    double v;                        /* PCC-v power (kind FPCC_PCCV)                   */
+    double pmin, pmax;               /* WPCC periods in samples                        */
+    unsigned int V;                  /* WPCC voices per octave                         */
+    int wtype;                       /* WPCC wavelet: -1 Morlet, -2 complete Morlet, -3 MexHat */
+    double op1;                      /* WPCC Morlet w0 (0 = pi*sqrt(2/ln 2))           */
+} fpcc_job;
+
+/* Number of output floats for a job (0 if the job is invalid). */
+size_t fpcc_output_len(const fpcc_job *job);
+/* Device workspace bytes fpcc_allpairs_device needs for a job. */
+size_t fpcc_workspace_size(const fpcc_job *job);
+/* Device buffers, stream-ordered on `stream` (a hipStream_t, NULL = default
+ * stream); no allocation, no synchronisation.  Returns 0 or an error code. */
+int fpcc_allpairs_device(const fpcc_job *job, const float *d_x, float *d_y, void *d_ws, size_t ws_bytes,
+                         void *stream);
+/* Host buffers; allocates device memory, runs, copies back, frees. */
+int fpcc_allpairs(const fpcc_job *job, const float *h_x, float *h_y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,75 @@
+"""PCC2 on the GPU against the oracle (reference-precision restatement of
+src/FFTapps.c:785-881) and the all-FP64 oracle."""
+import numpy as np
+import pytest
+
+from fastpcc_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5  # absolute, FP32 path vs the reference-precision oracle
+
+
+def _rand(Tr, N, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal((Tr, N)).astype(np.float32), rng.standard_normal((Tr, N)).astype(np.float32))
+
+
+@pytest.mark.parametrize("N,Tr,lag1,lag2", [
+    (3600, 3, -500, 500),
+    (100, 3, -40, 60),
+    (64, 2, -10, 10),
+    (1000, 2, 5, 300),
+    (3600, 2, -3599, 3599),
+    (2401, 1, -7, 0),
+])
+def test_pcc2_set_matches_oracle(fp, orc, N, Tr, lag1, lag2):
+    print(f"pcc2_set N={N} Tr={Tr} lags=({lag1},{lag2})", flush=True)
+    x1, x2 = _rand(Tr, N, N + Tr)
+    y = fp.pcc2_set(x1, x2, lag1, lag2)
+    yr = orc.run("pcc2", x1, x2, lag1, lag2)
+    y64 = orc.run("pcc2", x1, x2, lag1, lag2, f64=True)
+    err = np.abs(y - yr).max()
+    print(f"  max|gpu-oracle|={err:.3e} max|gpu-fp64|={np.abs(y - y64).max():.3e}")
+    assert err <= TOL
+
+
+def test_pcc2_set_swapped_lags(fp, orc):
+    x1, x2 = _rand(2, 500, 7)
+    y = fp.pcc2_set(x1, x2, 80, -30)  # Lag1 > Lag2: window starts at min(Lag1, Lag2)
+    yr = orc.run("pcc2", x1, x2, 80, -30)
+    assert y.shape == yr.shape == (2, 111)
+    assert np.abs(y - yr).max() <= TOL
+
+
+def test_pcc2_set_rejects_large_lags(fp):
+    x1, x2 = _rand(1, 100, 3)
+    with pytest.raises(fp.FastPCCError) as e:
+        fp.pcc2_set(x1, x2, -100, 10)
+    assert e.value.code == -3
+
+
+def test_allpairs_config1(fp, orc):
+    """8 stations x 3600 samples, 28 pairs, lags +-500, one window."""
+    x, shifts = synth.make_stations(8, 3600, 200, seed=11)
+    y = fp.allpairs(x, 1, 3600, -500, 500, stack="plain")
+    yr = orc.allpairs("pcc2", x, 1, 3600, -500, 500, stacked=True)
+    y64 = orc.allpairs("pcc2", x, 1, 3600, -500, 500, stacked=True, f64=True)
+    err = np.abs(y - yr).max()
+    print(f"C1 max|gpu-oracle|={err:.3e} max|gpu-fp64|={np.abs(y - y64).max():.3e}")
+    assert err <= TOL
+    # each pair peaks at the true relative delay
+    for p, (i, j) in enumerate(synth.pair_list(8)):
+        assert np.argmax(y[p]) - 500 == shifts[j] - shifts[i]
+
+
+@pytest.mark.parametrize("stack", ["none", "plain", "zsum"])
+def test_allpairs_windows(fp, orc, stack):
+    S, W, N = 5, 4, 3600
+    x, _ = synth.make_stations(S, W * N, 300, seed=5)
+    y = fp.allpairs(x, W, N, -1000, 1000, stack=stack)
+    yr = orc.allpairs("pcc2", x, W, N, -1000, 1000, stacked=(stack != "none"))
+    err = np.abs(y - yr).max()
+    print(f"stack={stack} max|gpu-oracle|={err:.3e}")
+    assert y.shape == yr.shape
+    assert err <= TOL
