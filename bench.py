@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the all-pairs PCC2 path on one MI355X.
+
+Workload (default): 256 stations x 86400 float32 samples (one day at 1 Hz),
+24 non-overlapping windows of 3600 samples, nfft 8192, all 32640 station pairs,
+lags +-1000, linear stack (mean) over windows.  Inputs are seeded synthetic
+records (noise + delayed band-limited common source, SNR 0.5) resident in HBM
+before the timed region.  A step is one full pass: station stage (phasor spectra
+of all 6144 station windows) + pair stage (all pair correlations, stacked).
+
+Prints one JSON line: pair-correlations/s (one (pair, window) correlation of L
+lags = one unit), per-kernel roofline of the dominant kernel from HIP events
+recorded inside the timed region, and the oracle's CPU rate on a bounded sample.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+CONFIGS = {
+    # name: stations, windows, samples per window, max lag, max shift
+    "c1": dict(S=8, W=1, N=3600, lag=500, shift=200),
+    "c2": dict(S=64, W=24, N=3600, lag=1000, shift=1000),
+    "c3": dict(S=256, W=24, N=3600, lag=1000, shift=1000),
+}
+HBM_PEAK_GBS = 8000.0
+
+
+def nz_rule(N, M):
+    n = 1
+    while n < N + M:
+        n <<= 1
+    return n
+
+
+def kernel_bytes(name, cfg, Nz):
+    """Algorithmic HBM bytes of one launch of `name`."""
+    S, W, N, lag = cfg["S"], cfg["W"], cfg["N"], cfg["lag"]
+    P, L = S * (S - 1) // 2, 2 * lag + 1
+    spectra = S * W * Nz * 8
+    if name.startswith("station"):
+        return S * W * N * 4 + spectra            # read records, write spectra
+    if name.startswith("pair_pcc2"):
+        return spectra + P * L * 4                # read spectra once, write stacked lags
+    if name.startswith("gram"):
+        return spectra + P * (Nz // 2 + 1) * 8    # read spectra, write folded cross spectra
+    if name.startswith("pair_c2r"):
+        return P * (Nz // 2 + 1) * 8 + P * L * 4  # read folded cross spectra, write lags
+    return None
+
+
+def cpu_baseline(x, cfg, seconds):
+    import oracle
+    S, W, N, lag = cfg["S"], cfg["W"], cfg["N"], cfg["lag"]
+    P = S * (S - 1) // 2
+    done, t0 = 0, time.perf_counter()
+    chunk = 4
+    while done < P:
+        n = min(chunk, P - done)
+        oracle.allpairs("pcc2", x, W, N, -lag, lag, stacked=True, pair_begin=done, pair_end=done + n)
+        done += n
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+        chunk = max(4, min(256, int(done / el * 0.25)))
+    el = time.perf_counter() - t0
+    return {"value": done * W / el, "unit": "pair-correlations/s", "cores": oracle.max_threads(), "kind": "port",
+            "sample": f"first {done} of {P} pairs x {W} windows of the same workload ({el:.1f} s), "
+                      f"oracle pcc2_set restatement, OpenMP over windows of one pair per call"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--stack", default="zsum", choices=["plain", "zsum", "gram"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the path shards over station-pair tiles; "
+              "multi-GPU runs are not part of it); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import fastpcc_amd
+    from fastpcc_amd import synth, _lib
+
+    cfg = CONFIGS[args.config]
+    S, W, N, lag = cfg["S"], cfg["W"], cfg["N"], cfg["lag"]
+    P, L = S * (S - 1) // 2, 2 * lag + 1
+    Nz = nz_rule(N, lag)
+    x, _ = synth.make_stations(S, W * N, cfg["shift"], seed=1234)
+    dev = torch.device("cuda:0")
+    x_dev = torch.from_numpy(x).to(dev)
+    job = fastpcc_amd.make_job("pcc2", S, W, N, -lag, lag, stack=args.stack)
+    runner = fastpcc_amd.DeviceRunner(job, dev)
+    stream = torch.cuda.current_stream(dev)
+    lib = _lib.lib()
+    lib.fpcc_profile_report.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+
+    for _ in range(args.warmup):
+        runner.run(x_dev, stream)
+    torch.cuda.synchronize()
+
+    check = None
+    if not args.no_check:
+        import oracle
+        y = runner.y.view(P, L).cpu().numpy()
+        npair = 2
+        yr = oracle.allpairs("pcc2", x, W, N, -lag, lag, stacked=True, pair_end=npair)
+        y64 = oracle.allpairs("pcc2", x, W, N, -lag, lag, stacked=True, pair_end=npair, f64=True)
+        check = {"pairs": npair, "max_abs_err_vs_oracle": float(np.abs(y[:npair] - yr).max()),
+                 "max_abs_err_vs_fp64": float(np.abs(y[:npair] - y64).max())}
+
+    lib.fpcc_profile_reset()
+    lib.fpcc_profile_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        runner.run(x_dev, stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lib.fpcc_profile_enable(0)
+    names = ctypes.create_string_buffer(4096)
+    tot = (ctypes.c_double * 16)()
+    cnt = (ctypes.c_longlong * 16)()
+    nk = lib.fpcc_profile_report(names, 4096, tot, cnt, 16)
+    kernels = {}
+    for k, nm in enumerate(names.value.decode().split("\n")[:nk]):
+        kernels[nm] = {"avg_ms": tot[k] / max(cnt[k], 1), "launches": int(cnt[k])}
+
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = P * W / (ms_per_step / 1e3)
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"]) if kernels else None
+    roof = None
+    if dom:
+        b = kernel_bytes(dom, cfg, Nz)
+        ach = b / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": b}
+    for k, v in kernels.items():
+        b = kernel_bytes(k, cfg, Nz)
+        if b:
+            v["algorithmic_GBps"] = round(b / (v["avg_ms"] * 1e-3) / 1e9, 1)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(x, cfg, args.cpu_seconds)
+
+    out = {
+        "metric": "PCC2 pair-correlations/s, 1 GPU",
+        "value": value,
+        "unit": "pair-correlations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded noise + delayed band-limited common source, SNR 0.5)",
+        "config": {"workload": f"PCC2 all pairs, {S} stations x {W * N} samples, {W} windows of {N}, "
+                               f"nfft {Nz}, lags +-{lag}, mean stack over windows",
+                   "stations": S, "pairs": P, "windows": W, "win_len": N, "nfft": Nz, "lags": L,
+                   "stack": args.stack, "parallelism": "1 GPU"},
+        "kernels": kernels,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "check": check,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -82,21 +82,36 @@ extern "C" size_t fpcc_workspace_size(const fpcc_job *job)
     return align_up(sizeof(cf32) * (size_t)job->n_stations * job->n_windows * Nz, 256);
 }
 
-extern "C" int fpcc_allpairs_device(const fpcc_job *job, const float *d_x, float *d_y, void *d_ws, size_t ws_bytes,
-                                    void *stream)
+extern "C" int fpcc_station_stage_device(const fpcc_job *job, const float *d_x, void *d_ws, size_t ws_bytes,
+                                         void *stream)
 {
     LagWin lw;
     unsigned Nz;
     int r = validate_job(job, lw, Nz);
     if (r) return r;
-    if (!d_x || !d_y || !d_ws || ws_bytes < fpcc_workspace_size(job)) return -1;
-    hipStream_t st = (hipStream_t)stream;
-    const int S = job->n_stations, W = job->n_windows, N = job->win_len;
-    cf32 *F = (cf32 *)d_ws;
-    r = launch_station_pcc(d_x, (long long)job->station_stride, S, W, N, Nz, F, st);
+    if (!d_x || !d_ws || ws_bytes < fpcc_workspace_size(job)) return -1;
+    return launch_station_pcc(d_x, (long long)job->station_stride, job->n_stations, job->n_windows, job->win_len,
+                              Nz, (cf32 *)d_ws, (hipStream_t)stream);
+}
+
+extern "C" int fpcc_pair_stage_device(const fpcc_job *job, void *d_ws, size_t ws_bytes, float *d_y, void *stream)
+{
+    LagWin lw;
+    unsigned Nz;
+    int r = validate_job(job, lw, Nz);
     if (r) return r;
-    const int P = S * (S - 1) / 2;
-    return launch_pair_pcc2(F, S, W, N, Nz, lw.lag0, lw.L, job->stack, 0, P, d_y, st);
+    if (!d_y || !d_ws || ws_bytes < fpcc_workspace_size(job)) return -1;
+    const int S = job->n_stations, P = S * (S - 1) / 2;
+    return launch_pair_pcc2((const cf32 *)d_ws, S, job->n_windows, job->win_len, Nz, lw.lag0, lw.L, job->stack, 0,
+                            P, d_y, (hipStream_t)stream);
+}
+
+extern "C" int fpcc_allpairs_device(const fpcc_job *job, const float *d_x, float *d_y, void *d_ws, size_t ws_bytes,
+                                    void *stream)
+{
+    int r = fpcc_station_stage_device(job, d_x, d_ws, ws_bytes, stream);
+    if (r) return r;
+    return fpcc_pair_stage_device(job, d_ws, ws_bytes, d_y, stream);
 }
 
 extern "C" int fpcc_allpairs(const fpcc_job *job, const float *h_x, float *h_y)

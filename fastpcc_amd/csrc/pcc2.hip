@@ -16,6 +16,7 @@
 // The lag window y[(lag0 + l) mod Nz], l < L, is scaled by 1/((float)Nz*(float)N).
 #include "engine.hpp"
 #include "kernels.hpp"
+#include "profile.hpp"
 
 namespace fpcc {
 
@@ -202,6 +203,7 @@ int launch_station_pcc(const float *d_x, long long station_stride, int S, int W,
     const int threads = 512;
     size_t lds = sizeof(cf32) * 2 * (size_t)Nz + sizeof(float) * 16;
     allow_lds((const void *)k_station_pcc, lds);
+    ProfScope prof("station_pcc", st);
     hipLaunchKernelGGL(k_station_pcc, dim3(S * W), dim3(threads), lds, st, d_x, station_stride, W, N, Nz,
                        pN.plan, (const cf32 *)pN.tw, pZ.plan, (const cf32 *)pZ.tw, da1, d_F);
     FPCC_CHECK(hipGetLastError());
@@ -221,6 +223,7 @@ int launch_pair_pcc2(const cf32 *d_F, int S, int W, int N, int Nz, int lag0, int
     int blocks = mode == 0 ? npairs * W : npairs;
     if (blocks == 0) return 0;
     allow_lds((const void *)k_pair_pcc2, lds);
+    ProfScope prof(mode == 2 ? "pair_pcc2_zsum" : "pair_pcc2_plain", st);
     hipLaunchKernelGGL(k_pair_pcc2, dim3(blocks), dim3(threads), lds, st, d_F, S, W, Nz, pM.plan,
                        (const cf32 *)pM.tw, (const cf32 *)twc, lag0, L, scale, mode, pair_base, d_y);
     FPCC_CHECK(hipGetLastError());

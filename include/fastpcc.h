@@ -81,6 +81,23 @@ int fpcc_allpairs_device(const fpcc_job *job, const float *d_x, float *d_y, void
                          void *stream);
 /* Host buffers; allocates device memory, runs, copies back, frees. */
 int fpcc_allpairs(const fpcc_job *job, const float *h_x, float *h_y);
+/* The two halves of fpcc_allpairs_device.  The station stage writes every
+ * (station, window) spectrum into the workspace; the pair stage reads them and
+ * writes y.  Spectra left in the workspace may be reused by later pair stages of
+ * jobs with the same kind, stations, windows, N and lags. */
+int fpcc_station_stage_device(const fpcc_job *job, const float *d_x, void *d_ws, size_t ws_bytes, void *stream);
+int fpcc_pair_stage_device(const fpcc_job *job, void *d_ws, size_t ws_bytes, float *d_y, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Kernel timing for benchmarks (no counterpart in the reference).  While
+ * enabled, each kernel launch records a HIP event pair on its stream;
+ * fpcc_profile_report waits for them and returns, per kernel name (written
+ * newline-separated into names), the summed milliseconds and launch counts.
+ * Returns the number of kernels reported.
+ * ------------------------------------------------------------------------- */
+void fpcc_profile_enable(int on);
+void fpcc_profile_reset(void);
+int fpcc_profile_report(char *names, size_t names_len, double *total_ms, long long *counts, int max_kernels);
 
 #ifdef __cplusplus
 }

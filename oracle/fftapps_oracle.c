@@ -22,6 +22,8 @@
 #define PI 3.14159265358979323846
 #endif
 
+int orc_max_threads(void) { return omp_get_max_threads(); }
+
 /* ------------------------------------------------------------------------- */
 /* analytic signal                                                           */
 /* ------------------------------------------------------------------------- */

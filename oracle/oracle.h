@@ -23,6 +23,8 @@
 extern "C" {
 #endif
 
+int orc_max_threads(void); /* OpenMP threads the drivers use */
+
 enum { ORC_PCC2 = 0, ORC_PCC1 = 1, ORC_PCCV = 2, ORC_GNCC = 3, ORC_CC1B = 4, ORC_WPCC = 5 };
 
 /* ---- building blocks (src/FFTapps.c) ---- */

@@ -43,6 +43,7 @@ def lib():
         L.orc_analytic_d.argtypes = [ctypes.c_void_p, dp, u]
         L.orc_ampnormf.argtypes = [ctypes.c_void_p, u]
         L.orc_ampnorm.argtypes = [ctypes.c_void_p, u]
+        L.orc_max_threads.restype = i
         L.orc_erfi.argtypes = [d]
         L.orc_erfi.restype = d
         L.orc_wpcc_family_export.argtypes = [u, i, i, d, d, u, i, d, ctypes.POINTER(ctypes.c_uint),
@@ -100,6 +101,10 @@ def allpairs(kind, x, W, N, lag1, lag2, stacked=True, f64=False, pair_begin=0, p
     if r != 0:
         raise ValueError(f"oracle returned {r}")
     return y
+
+
+def max_threads():
+    return lib().orc_max_threads()
 
 
 def analytic(x, f64=False):
