@@ -59,8 +59,28 @@ int validate_job(const fpcc_job *job, LagWin &lw, unsigned &Nz)
     if (r) return r;
     Nz = nz_rule(job->win_len, lw.M);
     if (job->kind != FPCC_PCC2) return -1;
-    if (job->stack < FPCC_STACK_NONE || job->stack > FPCC_STACK_ZSUM) return -1;
+    if (job->stack < FPCC_STACK_NONE || job->stack > FPCC_STACK_GRAM) return -1;
     return 0;
+}
+
+// Workspace layout: [F: S*W*Nz spectra][U: (Nz/2+1)*S*2W][G: P*ld], 256-B aligned.
+struct WsLayout {
+    size_t F, U, G, total;
+};
+WsLayout ws_layout(const fpcc_job *job, unsigned Nz)
+{
+    WsLayout l{};
+    const size_t S = job->n_stations, W = job->n_windows, P = S * (S - 1) / 2;
+    l.F = 0;
+    size_t f_bytes = align_up(sizeof(cf32) * S * W * Nz, 256);
+    l.total = f_bytes;
+    if (job->stack == FPCC_STACK_GRAM) {
+        l.U = l.total;
+        l.total += align_up(sizeof(cf32) * (Nz / 2 + 1) * S * 2 * W, 256);
+        l.G = l.total;
+        l.total += align_up(sizeof(cf32) * P * (size_t)gram_ld(Nz), 256);
+    }
+    return l;
 }
 
 }  // namespace
@@ -79,7 +99,7 @@ extern "C" size_t fpcc_workspace_size(const fpcc_job *job)
     LagWin lw;
     unsigned Nz;
     if (validate_job(job, lw, Nz)) return 0;
-    return align_up(sizeof(cf32) * (size_t)job->n_stations * job->n_windows * Nz, 256);
+    return ws_layout(job, Nz).total;
 }
 
 extern "C" int fpcc_station_stage_device(const fpcc_job *job, const float *d_x, void *d_ws, size_t ws_bytes,
@@ -101,9 +121,20 @@ extern "C" int fpcc_pair_stage_device(const fpcc_job *job, void *d_ws, size_t ws
     int r = validate_job(job, lw, Nz);
     if (r) return r;
     if (!d_y || !d_ws || ws_bytes < fpcc_workspace_size(job)) return -1;
-    const int S = job->n_stations, P = S * (S - 1) / 2;
-    return launch_pair_pcc2((const cf32 *)d_ws, S, job->n_windows, job->win_len, Nz, lw.lag0, lw.L, job->stack, 0,
-                            P, d_y, (hipStream_t)stream);
+    const int S = job->n_stations, P = S * (S - 1) / 2, W = job->n_windows, N = job->win_len;
+    hipStream_t st = (hipStream_t)stream;
+    if (job->stack == FPCC_STACK_GRAM) {
+        WsLayout l = ws_layout(job, Nz);
+        char *ws = (char *)d_ws;
+        cf32 *F = (cf32 *)(ws + l.F), *U = (cf32 *)(ws + l.U), *G = (cf32 *)(ws + l.G);
+        int ntiles = 0;
+        const int2 *tiles = gram_tiles(S, ntiles);
+        if (!tiles) return -1;
+        if ((r = launch_spectra_to_u(F, S, W, Nz, U, st))) return r;
+        if ((r = launch_gram_f32(U, S, W, Nz, tiles, ntiles, G, st))) return r;
+        return launch_pair_c2r(G, P, Nz, N, W, lw.lag0, lw.L, d_y, st);
+    }
+    return launch_pair_pcc2((const cf32 *)d_ws, S, W, N, Nz, lw.lag0, lw.L, job->stack, 0, P, d_y, st);
 }
 
 extern "C" int fpcc_allpairs_device(const fpcc_job *job, const float *d_x, float *d_y, void *d_ws, size_t ws_bytes,

@@ -14,4 +14,12 @@ int launch_station_pcc(const float *d_x, long long station_stride, int S, int W,
 int launch_pair_pcc2(const cf32 *d_F, int S, int W, int N, int Nz, int lag0, int L, int mode, int pair_base,
                      int npairs, float *d_y, hipStream_t st);
 
+// Stack over windows as a per-frequency Gram (gram.hip).
+long long gram_ld(int Nz);                       // row length of G in bins
+const int2 *gram_tiles(int S, int &ntiles);      // cached device tile table
+int launch_spectra_to_u(const cf32 *d_F, int S, int W, int Nz, cf32 *d_U, hipStream_t st);
+int launch_gram_f32(const cf32 *d_U, int S, int W, int Nz, const int2 *d_tiles, int ntiles, cf32 *d_G,
+                    hipStream_t st);
+int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int L, float *d_y, hipStream_t st);
+
 }  // namespace fpcc

@@ -63,7 +63,7 @@ def test_allpairs_config1(fp, orc):
         assert np.argmax(y[p]) - 500 == shifts[j] - shifts[i]
 
 
-@pytest.mark.parametrize("stack", ["none", "plain", "zsum"])
+@pytest.mark.parametrize("stack", ["none", "plain", "zsum", "gram"])
 def test_allpairs_windows(fp, orc, stack):
     S, W, N = 5, 4, 3600
     x, _ = synth.make_stations(S, W * N, 300, seed=5)
@@ -73,3 +73,19 @@ def test_allpairs_windows(fp, orc, stack):
     print(f"stack={stack} max|gpu-oracle|={err:.3e}")
     assert y.shape == yr.shape
     assert err <= TOL
+
+
+@pytest.mark.parametrize("S,W,N,lag", [(8, 1, 3600, 500), (37, 3, 1000, 200), (50, 2, 3600, 1000), (17, 5, 640, 100)])
+def test_gram_stack_matches_oracle(fp, orc, S, W, N, lag):
+    """MFMA Gram stack with station counts that do not fill the 16x32 tiles,
+    odd and even window counts."""
+    print(f"gram S={S} W={W} N={N} lag={lag}", flush=True)
+    x, _ = synth.make_stations(S, W * N, lag // 2, seed=S * 100 + W)
+    y = fp.allpairs(x, W, N, -lag, lag, stack="gram")
+    yp = fp.allpairs(x, W, N, -lag, lag, stack="plain")
+    npr = min(y.shape[0], 60)  # oracle on a prefix of the pairs keeps the CPU time low
+    yr = orc.allpairs("pcc2", x, W, N, -lag, lag, stacked=True, pair_end=npr)
+    err = np.abs(y[:npr] - yr).max()
+    print(f"  max|gram-oracle|={err:.3e} max|gram-plain| (all pairs)={np.abs(y - yp).max():.3e}")
+    assert err <= TOL
+    assert np.abs(y - yp).max() <= TOL
