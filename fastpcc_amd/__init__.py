@@ -4,5 +4,6 @@ The compute path is libfastpcc.so (hand-written HIP kernels behind a C ABI, see
 include/fastpcc.h); this package loads it with ctypes and mirrors the reference's
 correlation interface on numpy arrays (api.py).
 """
-from .api import (FastPCCError, DeviceRunner, allpairs, make_job, output_shape, pcc2_set)  # noqa: F401
+from .api import (FastPCCError, DeviceRunner, allpairs, make_job, output_shape, pcc2_set, ccgn_set,  # noqa: F401
+                  cc1b_set)
 from . import synth  # noqa: F401

@@ -47,8 +47,9 @@ def lib():
     L = ctypes.CDLL(LIB_PATH)
     fpp = ctypes.POINTER(ctypes.POINTER(ctypes.c_float))
     u, i, d = ctypes.c_uint, ctypes.c_int, ctypes.c_double
-    L.pcc2_set.argtypes = [fpp, fpp, fpp, u, u, i, i]
-    L.pcc2_set.restype = i
+    for name in ("pcc2_set", "ccgn_set", "cc1b_set"):
+        getattr(L, name).argtypes = [fpp, fpp, fpp, u, u, i, i]
+        getattr(L, name).restype = i
     jp = ctypes.POINTER(FpccJob)
     L.fpcc_output_len.argtypes = [jp]
     L.fpcc_output_len.restype = ctypes.c_size_t

@@ -34,8 +34,7 @@ def _lag_len(lag1, lag2):
     return abs(lag2 - lag1) + 1
 
 
-def pcc2_set(x1, x2, lag1, lag2):
-    """PCC2 of one station pair over Tr windows (src/FFTapps.c:785-881)."""
+def _pair_set(name, x1, x2, lag1, lag2):
     x1 = np.atleast_2d(x1)
     x2 = np.atleast_2d(x2)
     if x1.shape != x2.shape:
@@ -46,10 +45,25 @@ def pcc2_set(x1, x2, lag1, lag2):
     _, py = _rows(y)
     a1, p1 = _rows(x1)
     a2, p2 = _rows(x2)
-    r = _lib.lib().pcc2_set(py, p1, p2, N, Tr, lag1, lag2)
+    r = getattr(_lib.lib(), name)(py, p1, p2, N, Tr, lag1, lag2)
     if r != 0:
-        raise FastPCCError(r, "pcc2_set")
+        raise FastPCCError(r, name)
     return y
+
+
+def pcc2_set(x1, x2, lag1, lag2):
+    """PCC2 of one station pair over Tr windows (src/FFTapps.c:785-881)."""
+    return _pair_set("pcc2_set", x1, x2, lag1, lag2)
+
+
+def ccgn_set(x1, x2, lag1, lag2):
+    """GNCC of one station pair over Tr windows (src/FFTapps.c:883-978)."""
+    return _pair_set("ccgn_set", x1, x2, lag1, lag2)
+
+
+def cc1b_set(x1, x2, lag1, lag2):
+    """1-bit GNCC of one station pair over Tr windows (src/FFTapps.c:980-1073)."""
+    return _pair_set("cc1b_set", x1, x2, lag1, lag2)
 
 
 KIND = {"pcc2": _lib.FPCC_PCC2, "pcc1": _lib.FPCC_PCC1, "pccv": _lib.FPCC_PCCV, "gncc": _lib.FPCC_GNCC,

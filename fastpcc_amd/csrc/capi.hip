@@ -12,27 +12,6 @@ using namespace fpcc;
 
 namespace {
 
-struct LagWin {
-    int L, lag0;
-    unsigned M;
-};
-
-// Lag window of pcc2_set/ccgn_set/cc1b_set (src/FFTapps.c:790-804)
-int lag_window(int Lag1, int Lag2, unsigned N, LagWin &lw)
-{
-    if (Lag2 >= Lag1) {
-        lw.L = Lag2 - Lag1 + 1;
-        lw.lag0 = Lag1;
-    } else {
-        lw.L = Lag1 - Lag2 + 1;
-        lw.lag0 = Lag2;
-    }
-    unsigned ua1 = (unsigned)abs(Lag1), ua2 = (unsigned)abs(Lag2);
-    if (ua1 >= N || ua2 >= N) return -3;
-    lw.M = ua1 > ua2 ? ua1 : ua2;
-    return 0;
-}
-
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 struct DeviceBuf {
@@ -50,91 +29,222 @@ struct HostBuf {
     }
 };
 
-int validate_job(const fpcc_job *job, LagWin &lw, unsigned &Nz)
+// Resolved sizes of a job.
+struct Dims {
+    int S, W, N, L, lag0;
+    unsigned M, Nz;
+    size_t P;
+};
+
+bool stack_supported(int kind, int stack)
+{
+    switch (kind) {
+    case FPCC_PCC2: return stack >= FPCC_STACK_NONE && stack <= FPCC_STACK_GRAM;
+    case FPCC_GNCC: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN;
+    case FPCC_CC1B: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN || stack == FPCC_STACK_GRAM;
+    default: return false;
+    }
+}
+
+// Lag window and Nz rule of pcc2_set/ccgn_set/cc1b_set (src/FFTapps.c:790-805).
+int resolve(const fpcc_job *job, Dims &d)
 {
     if (!job || job->n_stations < 2 || job->n_windows < 1 || job->win_len < 2) return -1;
     if (job->station_stride < (unsigned long long)job->n_windows * job->win_len) return -1;
     if (job->lag1 > job->lag2) return -1;
-    int r = lag_window(job->lag1, job->lag2, job->win_len, lw);
-    if (r) return r;
-    Nz = nz_rule(job->win_len, lw.M);
-    if (job->kind != FPCC_PCC2) return -1;
-    if (job->stack < FPCC_STACK_NONE || job->stack > FPCC_STACK_GRAM) return -1;
+    const unsigned N = job->win_len;
+    const unsigned ua1 = (unsigned)abs(job->lag1), ua2 = (unsigned)abs(job->lag2);
+    if (ua1 >= N || ua2 >= N) return -3;
+    if (!stack_supported(job->kind, job->stack)) return -1;
+    d.S = (int)job->n_stations;
+    d.W = (int)job->n_windows;
+    d.N = (int)N;
+    d.L = job->lag2 - job->lag1 + 1;
+    d.lag0 = job->lag1;
+    d.M = ua1 > ua2 ? ua1 : ua2;
+    d.Nz = nz_rule(N, d.M);
+    d.P = (size_t)d.S * (d.S - 1) / 2;
     return 0;
 }
 
-// Workspace layout: [F: S*W*Nz spectra][U: (Nz/2+1)*S*2W][G: P*ld], 256-B aligned.
-struct WsLayout {
-    size_t F, U, G, total;
+// Workspace regions of a job (byte offsets, 256-B aligned).
+struct Layout {
+    size_t a = 0, b = 0, c = 0, d = 0, e = 0, total = 0;
 };
-WsLayout ws_layout(const fpcc_job *job, unsigned Nz)
+
+size_t take(size_t &total, size_t bytes)
 {
-    WsLayout l{};
-    const size_t S = job->n_stations, W = job->n_windows, P = S * (S - 1) / 2;
-    l.F = 0;
-    size_t f_bytes = align_up(sizeof(cf32) * S * W * Nz, 256);
-    l.total = f_bytes;
-    if (job->stack == FPCC_STACK_GRAM) {
-        l.U = l.total;
-        l.total += align_up(sizeof(cf32) * (Nz / 2 + 1) * S * 2 * W, 256);
-        l.G = l.total;
-        l.total += align_up(sizeof(cf32) * P * (size_t)gram_ld(Nz), 256);
+    size_t off = total;
+    total += align_up(bytes, 256);
+    return off;
+}
+
+Layout layout(const fpcc_job *job, const Dims &d)
+{
+    Layout l;
+    const size_t SW = (size_t)d.S * d.W, K1 = d.Nz / 2 + 1;
+    switch (job->kind) {
+    case FPCC_PCC2:
+        l.a = take(l.total, sizeof(cf32) * SW * d.Nz);  // spectra
+        if (job->stack == FPCC_STACK_GRAM) {
+            l.b = take(l.total, sizeof(cf32) * K1 * SW * 2);                  // U
+            l.c = take(l.total, sizeof(cf32) * d.P * (size_t)gram_ld(d.Nz));  // G
+        }
+        break;
+    case FPCC_GNCC:
+        l.a = take(l.total, sizeof(cf64) * SW * (size_t)real_ld(d.Nz));    // half spectra
+        l.b = take(l.total, sizeof(double) * SW * (size_t)energy_ld(d.N));  // energies
+        break;
+    case FPCC_CC1B:
+        l.a = take(l.total, sizeof(cf32) * SW * (size_t)real_ld(d.Nz));  // x >= 0 signs
+        l.b = take(l.total, sizeof(cf32) * SW * (size_t)real_ld(d.Nz));  // x > 0 signs
+        if (job->stack == FPCC_STACK_GRAM) {
+            l.c = take(l.total, sizeof(cf32) * K1 * SW);
+            l.d = take(l.total, sizeof(cf32) * K1 * SW);
+            l.e = take(l.total, sizeof(cf32) * d.P * (size_t)gram_ld(d.Nz));
+        }
+        break;
     }
     return l;
+}
+
+int station_stage(const fpcc_job *job, const Dims &d, const float *x, char *ws, hipStream_t st)
+{
+    Layout l = layout(job, d);
+    const long long stride = (long long)job->station_stride;
+    switch (job->kind) {
+    case FPCC_PCC2: return launch_station_pcc(x, stride, d.S, d.W, d.N, d.Nz, (cf32 *)(ws + l.a), st);
+    case FPCC_GNCC:
+        return launch_station_real(x, stride, d.S, d.W, d.N, d.Nz, 0, true, ws + l.a, (double *)(ws + l.b), st);
+    case FPCC_CC1B: {
+        int r = launch_station_real(x, stride, d.S, d.W, d.N, d.Nz, 1, false, ws + l.a, nullptr, st);
+        if (r) return r;
+        return launch_station_real(x, stride, d.S, d.W, d.N, d.Nz, 2, false, ws + l.b, nullptr, st);
+    }
+    }
+    return -1;
+}
+
+int pair_stage(const fpcc_job *job, const Dims &d, char *ws, float *y, hipStream_t st)
+{
+    Layout l = layout(job, d);
+    int r = 0;
+    const int mode = job->stack == FPCC_STACK_NONE ? 0 : 1;
+    switch (job->kind) {
+    case FPCC_PCC2:
+        if (job->stack == FPCC_STACK_GRAM) {
+            int ntiles = 0;
+            const int2 *tiles = gram_tiles(d.S, ntiles);
+            if (!tiles) return -1;
+            cf32 *F = (cf32 *)(ws + l.a), *U = (cf32 *)(ws + l.b), *G = (cf32 *)(ws + l.c);
+            if ((r = launch_spectra_to_u(F, d.Nz, d.S, d.W, d.Nz, 1, U, st))) return r;
+            if ((r = launch_gram_f32(U, U, d.S, 2 * d.W, d.Nz, tiles, ntiles, G, st))) return r;
+            // 1/2 of the Gram's Hermitian fold, the reference's 1/((float)Nz*(float)N), the mean over W
+            const float scale = 0.5f * (1 / ((float)d.Nz * (float)d.N)) / (float)d.W;
+            return launch_pair_c2r(G, (int)d.P, d.Nz, d.N, d.W, d.lag0, d.L, scale, 0, y, st);
+        }
+        return launch_pair_pcc2((const cf32 *)(ws + l.a), d.S, d.W, d.N, d.Nz, d.lag0, d.L, job->stack, 0,
+                                (int)d.P, y, st);
+    case FPCC_GNCC:
+        return launch_pair_gn(ws + l.a, ws + l.a, (const double *)(ws + l.b), true, d.S, d.W, d.N, d.Nz, d.lag0,
+                              d.L, 0, mode, y, st);
+    case FPCC_CC1B:
+        if (job->stack == FPCC_STACK_GRAM) {
+            int ntiles = 0;
+            const int2 *tiles = gram_tiles(d.S, ntiles);
+            if (!tiles) return -1;
+            const long long ld = real_ld(d.Nz);
+            cf32 *Ua = (cf32 *)(ws + l.c), *Ub = (cf32 *)(ws + l.d), *G = (cf32 *)(ws + l.e);
+            if ((r = launch_spectra_to_u((const cf32 *)(ws + l.a), ld, d.S, d.W, d.Nz, 0, Ua, st))) return r;
+            if ((r = launch_spectra_to_u((const cf32 *)(ws + l.b), ld, d.S, d.W, d.Nz, 0, Ub, st))) return r;
+            if ((r = launch_gram_f32(Ua, Ub, d.S, d.W, d.Nz, tiles, ntiles, G, st))) return r;
+            // ccf_lowlevel's 1/Nz, the mean over W, then 1/(N-|lag|) (gnf_lowlevel with +-1 data)
+            const float scale = (float)(1.0 / (double)d.Nz) / (float)d.W;
+            return launch_pair_c2r(G, (int)d.P, d.Nz, d.N, d.W, d.lag0, d.L, scale, 1, y, st);
+        }
+        return launch_pair_gn(ws + l.a, ws + l.b, nullptr, false, d.S, d.W, d.N, d.Nz, d.lag0, d.L, 1, mode, y,
+                              st);
+    }
+    return -1;
+}
+
+// One pair over Tr windows from host row pointers, as the reference's *_set:
+// x1 rows become station 0, x2 rows station 1, outputs per window.
+int pair_set_host(int kind, float **y, float **x1, float **x2, unsigned N, unsigned Tr, int Lag1, int Lag2,
+                  double v)
+{
+    const unsigned ua1 = (unsigned)abs(Lag1), ua2 = (unsigned)abs(Lag2);
+    if (ua1 >= N || ua2 >= N) return -3;
+    if (Tr == 0) return 0;
+    if (!y || !x1 || !x2) return -1;
+    fpcc_job job{};
+    job.kind = kind;
+    job.stack = FPCC_STACK_NONE;
+    job.n_stations = 2;
+    job.n_windows = Tr;
+    job.win_len = N;
+    job.station_stride = (unsigned long long)Tr * N;
+    job.lag1 = std::min(Lag1, Lag2);
+    job.lag2 = std::max(Lag1, Lag2);
+    job.v = v;
+    Dims d;
+    int r = resolve(&job, d);
+    if (r) return r;
+    const size_t xlen = (size_t)2 * Tr * N, ylen = (size_t)Tr * d.L;
+    HostBuf hx, hy;
+    DeviceBuf dx, dy, dws;
+    FPCC_CHECK(hipHostMalloc(&hx.p, sizeof(float) * xlen, hipHostMallocDefault));
+    FPCC_CHECK(hipHostMalloc(&hy.p, sizeof(float) * ylen, hipHostMallocDefault));
+    for (unsigned tr = 0; tr < Tr; ++tr) {
+        if (!x1[tr] || !x2[tr] || !y[tr]) return -1;
+        memcpy((float *)hx.p + (size_t)tr * N, x1[tr], sizeof(float) * N);
+        memcpy((float *)hx.p + (size_t)(Tr + tr) * N, x2[tr], sizeof(float) * N);
+    }
+    const size_t wsb = layout(&job, d).total;
+    FPCC_CHECK(hipMalloc(&dx.p, sizeof(float) * xlen));
+    FPCC_CHECK(hipMalloc(&dy.p, sizeof(float) * ylen));
+    FPCC_CHECK(hipMalloc(&dws.p, wsb));
+    FPCC_CHECK(hipMemcpy(dx.p, hx.p, sizeof(float) * xlen, hipMemcpyHostToDevice));
+    if ((r = station_stage(&job, d, (const float *)dx.p, (char *)dws.p, nullptr))) return r;
+    if ((r = pair_stage(&job, d, (char *)dws.p, (float *)dy.p, nullptr))) return r;
+    FPCC_CHECK(hipMemcpy(hy.p, dy.p, sizeof(float) * ylen, hipMemcpyDeviceToHost));
+    for (unsigned tr = 0; tr < Tr; ++tr) memcpy(y[tr], (float *)hy.p + (size_t)tr * d.L, sizeof(float) * d.L);
+    return 0;
 }
 
 }  // namespace
 
 extern "C" size_t fpcc_output_len(const fpcc_job *job)
 {
-    LagWin lw;
-    unsigned Nz;
-    if (validate_job(job, lw, Nz)) return 0;
-    size_t P = (size_t)job->n_stations * (job->n_stations - 1) / 2;
-    return P * (job->stack == FPCC_STACK_NONE ? job->n_windows : 1) * (size_t)lw.L;
+    Dims d;
+    if (resolve(job, d)) return 0;
+    return d.P * (job->stack == FPCC_STACK_NONE ? d.W : 1) * (size_t)d.L;
 }
 
 extern "C" size_t fpcc_workspace_size(const fpcc_job *job)
 {
-    LagWin lw;
-    unsigned Nz;
-    if (validate_job(job, lw, Nz)) return 0;
-    return ws_layout(job, Nz).total;
+    Dims d;
+    if (resolve(job, d)) return 0;
+    return layout(job, d).total;
 }
 
 extern "C" int fpcc_station_stage_device(const fpcc_job *job, const float *d_x, void *d_ws, size_t ws_bytes,
                                          void *stream)
 {
-    LagWin lw;
-    unsigned Nz;
-    int r = validate_job(job, lw, Nz);
+    Dims d;
+    int r = resolve(job, d);
     if (r) return r;
-    if (!d_x || !d_ws || ws_bytes < fpcc_workspace_size(job)) return -1;
-    return launch_station_pcc(d_x, (long long)job->station_stride, job->n_stations, job->n_windows, job->win_len,
-                              Nz, (cf32 *)d_ws, (hipStream_t)stream);
+    if (!d_x || !d_ws || ws_bytes < layout(job, d).total) return -1;
+    return station_stage(job, d, d_x, (char *)d_ws, (hipStream_t)stream);
 }
 
 extern "C" int fpcc_pair_stage_device(const fpcc_job *job, void *d_ws, size_t ws_bytes, float *d_y, void *stream)
 {
-    LagWin lw;
-    unsigned Nz;
-    int r = validate_job(job, lw, Nz);
+    Dims d;
+    int r = resolve(job, d);
     if (r) return r;
-    if (!d_y || !d_ws || ws_bytes < fpcc_workspace_size(job)) return -1;
-    const int S = job->n_stations, P = S * (S - 1) / 2, W = job->n_windows, N = job->win_len;
-    hipStream_t st = (hipStream_t)stream;
-    if (job->stack == FPCC_STACK_GRAM) {
-        WsLayout l = ws_layout(job, Nz);
-        char *ws = (char *)d_ws;
-        cf32 *F = (cf32 *)(ws + l.F), *U = (cf32 *)(ws + l.U), *G = (cf32 *)(ws + l.G);
-        int ntiles = 0;
-        const int2 *tiles = gram_tiles(S, ntiles);
-        if (!tiles) return -1;
-        if ((r = launch_spectra_to_u(F, S, W, Nz, U, st))) return r;
-        if ((r = launch_gram_f32(U, S, W, Nz, tiles, ntiles, G, st))) return r;
-        return launch_pair_c2r(G, P, Nz, N, W, lw.lag0, lw.L, d_y, st);
-    }
-    return launch_pair_pcc2((const cf32 *)d_ws, S, W, N, Nz, lw.lag0, lw.L, job->stack, 0, P, d_y, st);
+    if (!d_y || !d_ws || ws_bytes < layout(job, d).total) return -1;
+    return pair_stage(job, d, (char *)d_ws, d_y, (hipStream_t)stream);
 }
 
 extern "C" int fpcc_allpairs_device(const fpcc_job *job, const float *d_x, float *d_y, void *d_ws, size_t ws_bytes,
@@ -147,57 +257,39 @@ extern "C" int fpcc_allpairs_device(const fpcc_job *job, const float *d_x, float
 
 extern "C" int fpcc_allpairs(const fpcc_job *job, const float *h_x, float *h_y)
 {
-    size_t ylen = fpcc_output_len(job);
-    if (!ylen || !h_x || !h_y) {
-        LagWin lw;
-        unsigned Nz;
-        int r = job ? validate_job(job, lw, Nz) : -1;
-        return r ? r : -1;
-    }
-    size_t ws = fpcc_workspace_size(job);
-    size_t xlen = (size_t)(job->n_stations - 1) * job->station_stride + (size_t)job->n_windows * job->win_len;
+    Dims d;
+    int r = resolve(job, d);
+    if (r) return r;
+    if (!h_x || !h_y) return -1;
+    const size_t ylen = fpcc_output_len(job), ws = layout(job, d).total;
+    const size_t xlen = (size_t)(job->n_stations - 1) * job->station_stride + (size_t)job->n_windows * job->win_len;
     DeviceBuf dx, dy, dws;
     FPCC_CHECK(hipMalloc(&dx.p, sizeof(float) * xlen));
     FPCC_CHECK(hipMalloc(&dy.p, sizeof(float) * ylen));
     FPCC_CHECK(hipMalloc(&dws.p, ws));
     FPCC_CHECK(hipMemcpy(dx.p, h_x, sizeof(float) * xlen, hipMemcpyHostToDevice));
-    int r = fpcc_allpairs_device(job, (const float *)dx.p, (float *)dy.p, dws.p, ws, nullptr);
+    r = fpcc_allpairs_device(job, (const float *)dx.p, (float *)dy.p, dws.p, ws, nullptr);
     if (r) return r;
     FPCC_CHECK(hipMemcpy(h_y, dy.p, sizeof(float) * ylen, hipMemcpyDeviceToHost));
     return 0;
 }
 
-// ---- reference-compatible entries (src/FFTapps.h) ----
+// ---- reference-compatible entries (src/FFTapps.h:11-16) ----
 
 extern "C" int pcc2_set(float **const y, float **const x1, float **const x2, const unsigned int N,
                         const unsigned int Tr, const int Lag1, const int Lag2)
 {
-    LagWin lw;
-    int r = lag_window(Lag1, Lag2, N, lw);
-    if (r) return r;
-    if (Tr == 0) return 0;
-    if (!y || !x1 || !x2) return -1;
-    const unsigned Nz = nz_rule(N, lw.M);
-    // stations 0 (= x1 windows) and 1 (= x2 windows), one pair, per-window output
-    const size_t xlen = (size_t)2 * Tr * N, ylen = (size_t)Tr * lw.L;
-    HostBuf hx, hy;
-    DeviceBuf dx, dy, dF;
-    FPCC_CHECK(hipHostMalloc(&hx.p, sizeof(float) * xlen, hipHostMallocDefault));
-    FPCC_CHECK(hipHostMalloc(&hy.p, sizeof(float) * ylen, hipHostMallocDefault));
-    for (unsigned tr = 0; tr < Tr; ++tr) {
-        if (!x1[tr] || !x2[tr] || !y[tr]) return -1;
-        memcpy((float *)hx.p + (size_t)tr * N, x1[tr], sizeof(float) * N);
-        memcpy((float *)hx.p + (size_t)(Tr + tr) * N, x2[tr], sizeof(float) * N);
-    }
-    FPCC_CHECK(hipMalloc(&dx.p, sizeof(float) * xlen));
-    FPCC_CHECK(hipMalloc(&dy.p, sizeof(float) * ylen));
-    FPCC_CHECK(hipMalloc(&dF.p, sizeof(cf32) * 2 * Tr * (size_t)Nz));
-    FPCC_CHECK(hipMemcpy(dx.p, hx.p, sizeof(float) * xlen, hipMemcpyHostToDevice));
-    r = launch_station_pcc((const float *)dx.p, (long long)Tr * N, 2, Tr, N, Nz, (cf32 *)dF.p, nullptr);
-    if (r) return r;
-    r = launch_pair_pcc2((const cf32 *)dF.p, 2, Tr, N, Nz, lw.lag0, lw.L, 0, 0, 1, (float *)dy.p, nullptr);
-    if (r) return r;
-    FPCC_CHECK(hipMemcpy(hy.p, dy.p, sizeof(float) * ylen, hipMemcpyDeviceToHost));
-    for (unsigned tr = 0; tr < Tr; ++tr) memcpy(y[tr], (float *)hy.p + (size_t)tr * lw.L, sizeof(float) * lw.L);
-    return 0;
+    return pair_set_host(FPCC_PCC2, y, x1, x2, N, Tr, Lag1, Lag2, 2.0);
+}
+
+extern "C" int ccgn_set(float **const y, float **const x1, float **const x2, const unsigned int N,
+                        const unsigned int Tr, const int Lag1, const int Lag2)
+{
+    return pair_set_host(FPCC_GNCC, y, x1, x2, N, Tr, Lag1, Lag2, 2.0);
+}
+
+extern "C" int cc1b_set(float **const y, float **const x1, float **const x2, const unsigned int N,
+                        const unsigned int Tr, const int Lag1, const int Lag2)
+{
+    return pair_set_host(FPCC_CC1B, y, x1, x2, N, Tr, Lag1, Lag2, 2.0);
 }

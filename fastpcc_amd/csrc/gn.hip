@@ -1,0 +1,253 @@
+// GNCC and 1-bit GNCC (src/FFTapps.c:883-1073) on gfx950.
+//
+// Station stage, one workgroup per (station, window): samples (GNCC) or their
+// signs (1-bit; x >= 0 -> +1 for the virtual source, x > 0 -> +1 for the
+// receiver, src/FFTapps.c:1037 and :1042) are zero-padded to Nz and transformed
+// with a half-length complex FFT: z[m] = x[2m] + i x[2m+1], Z = FFT_{Nz/2}(z),
+//   X[k] = (Z[k] + conj Z[M-k])/2 + e^{-2 pi i k/Nz} (Z[k] - conj Z[M-k])/(2i),
+// k in [0, M], M = Nz/2 (Z[M] = Z[0]).  GNCC runs in FP64 like the reference
+// (fftw_plan_dft_r2c_1d) and also writes prefix sums of x^2 in FP64 for the
+// overlap energies; 1-bit runs in FP32 (fftwf_plan_dft_r2c_1d).
+//
+// Pair stage: H = conj(X1) X2 (Hermitian half spectrum of a real correlation),
+// c2r through the same half-length trick, lag window y[(lag0+l) mod Nz] / Nz,
+// then division by sqrt(E1(lag) E2(lag)) with E1 = sum of x1^2 over
+// [max(0,-lag), min(N,N-lag)), E2 = sum of x2^2 over [max(0,lag), min(N,N+lag))
+// (gn_lowlevel, :372-429); for 1-bit both energies equal the overlap length.
+#include "engine.hpp"
+#include "kernels.hpp"
+#include "profile.hpp"
+
+namespace fpcc {
+
+template <typename T>
+__device__ __forceinline__ T sample_val(float v, int sign_rule)
+{
+    if (sign_rule == 1) return v >= 0.0f ? T(1) : T(-1);
+    if (sign_rule == 2) return v > 0.0f ? T(1) : T(-1);
+    return (T)v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(512)
+k_station_real(const float *__restrict__ x, long long station_stride, int W, int N, int Nz, int sign_rule,
+               FftPlan planM, const C2<T> *__restrict__ twM, const C2<T> *__restrict__ twc,
+               C2<T> *__restrict__ X, long long ldx, double *__restrict__ E, long long lde)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int M = Nz >> 1;
+    C2<T> *buf0 = reinterpret_cast<C2<T> *>(smem_raw);
+    C2<T> *buf1 = buf0 + M;
+    double *scan = reinterpret_cast<double *>(buf1 + M);  // blockDim.x partial sums
+    const int sw = blockIdx.x;
+    const int s = sw / W, w = sw - s * W;
+    const float *in = x + (long long)s * station_stride + (long long)w * N;
+    const int T_ = blockDim.x;
+    for (int m = threadIdx.x; m < M; m += T_) {
+        const int n0 = 2 * m, n1 = 2 * m + 1;
+        T e = n0 < N ? sample_val<T>(in[n0], sign_rule) : T(0);
+        T o = n1 < N ? sample_val<T>(in[n1], sign_rule) : T(0);
+        buf0[m] = C2<T>{e, o};
+    }
+    __syncthreads();
+    C2<T> *Z = fft_lds<T, -1>(buf0, buf1, planM, twM);
+    C2<T> *out = X + (long long)sw * ldx;
+    for (int k = threadIdx.x; k <= M; k += T_) {
+        C2<T> zk = Z[k == M ? 0 : k];
+        C2<T> zm = Z[k == 0 ? 0 : M - k];
+        C2<T> zc = C2<T>{zm.x, -zm.y};
+        C2<T> ev = C2<T>{(zk.x + zc.x) * T(0.5), (zk.y + zc.y) * T(0.5)};
+        C2<T> d = C2<T>{zk.x - zc.x, zk.y - zc.y};
+        C2<T> od = C2<T>{d.y * T(0.5), -d.x * T(0.5)};  // d / (2i)
+        C2<T> wk = (k == M) ? C2<T>{T(-1), T(0)} : C2<T>{twc[k].x, -twc[k].y};
+        out[k] = ev + cmul(wk, od);
+    }
+    if (E) {
+        // exclusive prefix sums of x^2 (double), P[0..N]
+        const int per = (N + T_ - 1) / T_;
+        const int a = threadIdx.x * per, b = min(N, a + per);
+        double part = 0.0;
+        for (int n = a; n < b; ++n) {
+            double v = (double)in[n];
+            part += v * v;
+        }
+        scan[threadIdx.x] = part;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double run = 0.0;
+            for (int t = 0; t < T_; ++t) {
+                double v = scan[t];
+                scan[t] = run;
+                run += v;
+            }
+        }
+        __syncthreads();
+        double run = scan[threadIdx.x];
+        double *P = E + (long long)sw * lde;
+        for (int n = a; n < b; ++n) {
+            P[n] = run;
+            double v = (double)in[n];
+            run += v * v;
+        }
+        if (b == N && a < N) P[N] = run;
+        if (N == 0 && threadIdx.x == 0) P[0] = 0.0;
+    }
+}
+
+__device__ __forceinline__ void pair_of_gn(int p, int S, int &i, int &j)
+{
+    double b = 2.0 * S - 1.0;
+    int ii = (int)floor((b - sqrt(b * b - 8.0 * p)) * 0.5);
+    if (ii < 0) ii = 0;
+    while (ii > 0 && ii * S - ii * (ii + 1) / 2 > p) --ii;
+    while ((ii + 1) * S - (ii + 1) * (ii + 2) / 2 <= p) ++ii;
+    i = ii;
+    j = p - (ii * S - ii * (ii + 1) / 2) + ii + 1;
+}
+
+// mode 0: y[p][w][L]; 1: mean over windows.  norm 0: energies (GNCC); 1: overlap count.
+template <typename T>
+__global__ void __launch_bounds__(512)
+k_pair_gn(const C2<T> *__restrict__ Xa, const C2<T> *__restrict__ Xb, long long ldx,
+          const double *__restrict__ E, long long lde, int S, int W, int N, int Nz, FftPlan planM,
+          const C2<T> *__restrict__ twM, const C2<T> *__restrict__ twc, int lag0, int L, int norm, int mode,
+          float *__restrict__ y)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int M = Nz >> 1;
+    C2<T> *buf0 = reinterpret_cast<C2<T> *>(smem_raw);
+    C2<T> *buf1 = buf0 + M;
+    float *acc = reinterpret_cast<float *>(buf1 + M);
+    int p, w0, w1;
+    if (mode == 0) {
+        p = blockIdx.x / W;
+        w0 = blockIdx.x % W;
+        w1 = w0 + 1;
+    } else {
+        p = blockIdx.x;
+        w0 = 0;
+        w1 = W;
+    }
+    int i, j;
+    pair_of_gn(p, S, i, j);
+    const int T_ = blockDim.x;
+    const T inv_nz = T(1) / (T)Nz;
+    for (int l = threadIdx.x; l < L; l += T_) acc[l] = 0.0f;
+    for (int w = w0; w < w1; ++w) {
+        const C2<T> *X1 = Xa + (long long)(i * W + w) * ldx;
+        const C2<T> *X2 = Xb + (long long)(j * W + w) * ldx;
+        for (int k = threadIdx.x; k < M; k += T_) {
+            C2<T> a = cmulc(X1[k], X2[k]);
+            C2<T> c = cmulc(X1[M - k], X2[M - k]);
+            if (k == 0) {  // c2r ignores the imaginary parts of DC and Nyquist
+                a.y = T(0);
+                c.y = T(0);
+            }
+            C2<T> P = C2<T>{a.x + c.x, a.y - c.y};
+            C2<T> Q = C2<T>{a.x - c.x, a.y + c.y};
+            C2<T> wq = cmul(twc[k], Q);
+            buf0[k] = C2<T>{P.x - wq.y, P.y + wq.x};
+        }
+        __syncthreads();
+        C2<T> *z = fft_lds<T, +1>(buf0, buf1, planM, twM);
+        const double *P1 = E ? E + (long long)(i * W + w) * lde : nullptr;
+        const double *P2 = E ? E + (long long)(j * W + w) * lde : nullptr;
+        for (int l = threadIdx.x; l < L; l += T_) {
+            const int lag = lag0 + l;
+            const int n = lag & (Nz - 1);
+            C2<T> v = z[n >> 1];
+            double c = (double)(((n & 1) ? v.y : v.x) * inv_nz);
+            double den;
+            if (norm == 0) {
+                const int a1 = max(0, -lag), b1 = min(N, N - lag);
+                const int a2 = max(0, lag), b2 = min(N, N + lag);
+                double e1 = P1[b1] - P1[a1], e2 = P2[b2] - P2[a2];
+                den = sqrt(e1 * e2);
+            } else {
+                den = (double)(N - abs(lag));
+            }
+            acc[l] += (float)(c / den);
+        }
+        __syncthreads();
+    }
+    if (mode == 0) {
+        float *dst = y + ((long long)p * W + w0) * L;
+        for (int l = threadIdx.x; l < L; l += T_) dst[l] = acc[l];
+    } else {
+        const float inv = 1.0f / (float)W;
+        float *dst = y + (long long)p * L;
+        for (int l = threadIdx.x; l < L; l += T_) dst[l] = acc[l] * inv;
+    }
+}
+
+static void allow_lds(const void *fn, size_t bytes)
+{
+    if (bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+long long real_ld(int Nz) { return (long long)(Nz / 2 + 1 + 1) / 2 * 2; }
+long long energy_ld(int N) { return (long long)(N + 1 + 1) / 2 * 2; }
+
+int launch_station_real(const float *d_x, long long station_stride, int S, int W, int N, int Nz, int sign_rule,
+                        bool f64, void *d_X, double *d_E, hipStream_t st)
+{
+    const int M = Nz / 2;
+    const DevPlan &pM = get_plan(M, f64);
+    const void *twc = get_c2r_twiddles(Nz, f64);
+    if (!pM.ok || !twc) return -1;
+    const int threads = 512;
+    const size_t elem = f64 ? sizeof(cf64) : sizeof(cf32);
+    size_t lds = elem * 2 * (size_t)M + sizeof(double) * threads;
+    if (lds > 160 * 1024) {
+        fprintf(stderr, "fastpcc: Nz=%d exceeds the single-workgroup limit for real spectra\n", Nz);
+        return -1;
+    }
+    ProfScope prof(f64 ? "station_real_f64" : "station_real_f32", st);
+    if (f64) {
+        allow_lds((const void *)k_station_real<double>, lds);
+        hipLaunchKernelGGL(k_station_real<double>, dim3(S * W), dim3(threads), lds, st, d_x, station_stride, W, N,
+                           Nz, sign_rule, pM.plan, (const cf64 *)pM.tw, (const cf64 *)twc, (cf64 *)d_X, real_ld(Nz),
+                           d_E, energy_ld(N));
+    } else {
+        allow_lds((const void *)k_station_real<float>, lds);
+        hipLaunchKernelGGL(k_station_real<float>, dim3(S * W), dim3(threads), lds, st, d_x, station_stride, W, N,
+                           Nz, sign_rule, pM.plan, (const cf32 *)pM.tw, (const cf32 *)twc, (cf32 *)d_X, real_ld(Nz),
+                           d_E, energy_ld(N));
+    }
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_pair_gn(const void *d_Xa, const void *d_Xb, const double *d_E, bool f64, int S, int W, int N, int Nz,
+                   int lag0, int L, int norm, int mode, float *d_y, hipStream_t st)
+{
+    const int M = Nz / 2;
+    const DevPlan &pM = get_plan(M, f64);
+    const void *twc = get_c2r_twiddles(Nz, f64);
+    if (!pM.ok || !twc) return -1;
+    const int P = S * (S - 1) / 2;
+    const int blocks = mode == 0 ? P * W : P;
+    if (!blocks) return 0;
+    const size_t elem = f64 ? sizeof(cf64) : sizeof(cf32);
+    size_t lds = elem * 2 * (size_t)M + sizeof(float) * (size_t)L;
+    if (lds > 160 * 1024) {
+        fprintf(stderr, "fastpcc: lag window too long for the single-workgroup pair stage\n");
+        return -1;
+    }
+    ProfScope prof(f64 ? "pair_gn_f64" : "pair_gn_f32", st);
+    if (f64) {
+        allow_lds((const void *)k_pair_gn<double>, lds);
+        hipLaunchKernelGGL(k_pair_gn<double>, dim3(blocks), dim3(512), lds, st, (const cf64 *)d_Xa,
+                           (const cf64 *)d_Xb, real_ld(Nz), d_E, energy_ld(N), S, W, N, Nz, pM.plan,
+                           (const cf64 *)pM.tw, (const cf64 *)twc, lag0, L, norm, mode, d_y);
+    } else {
+        allow_lds((const void *)k_pair_gn<float>, lds);
+        hipLaunchKernelGGL(k_pair_gn<float>, dim3(blocks), dim3(512), lds, st, (const cf32 *)d_Xa,
+                           (const cf32 *)d_Xb, real_ld(Nz), d_E, energy_ld(N), S, W, N, Nz, pM.plan,
+                           (const cf32 *)pM.tw, (const cf32 *)twc, lag0, L, norm, mode, d_y);
+    }
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace fpcc

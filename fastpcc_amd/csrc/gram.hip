@@ -28,8 +28,12 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // ---- F[sw][Nz] -> U[k][s][2W], k in [0, M] ----
 // Tile: 32 frequencies x 64 station-windows, staged in LDS for both the direct
 // and the mirrored frequency.
+// mirror = 1 (analytic-signal spectra, Nz bins per row): m in [0, 2W) with the
+// conjugated mirror frequency in the second half; mirror = 0 (half spectra of real
+// signals, ldF >= Nz/2+1 bins per row): m in [0, W).
 __global__ void __launch_bounds__(256)
-k_spectra_to_u(const cf32 *__restrict__ F, int SW, int W, int Nz, cf32 *__restrict__ U)
+k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz, int mirror,
+               cf32 *__restrict__ U)
 {
     __shared__ cf32 td[64][33];
     __shared__ cf32 tm[64][33];
@@ -42,16 +46,18 @@ k_spectra_to_u(const cf32 *__restrict__ F, int SW, int W, int Nz, cf32 *__restri
         const int k = k0 + tx;
         cf32 d = cf32{0.f, 0.f}, m = cf32{0.f, 0.f};
         if (sw < SW && k <= M) {
-            d = F[(long long)sw * Nz + k];
-            cf32 mm = F[(long long)sw * Nz + ((Nz - k) & (Nz - 1))];
-            m = cf32{mm.x, -mm.y};
+            d = F[(long long)sw * ldF + k];
+            if (mirror) {
+                cf32 mm = F[(long long)sw * ldF + ((Nz - k) & (Nz - 1))];
+                m = cf32{mm.x, -mm.y};
+            }
         }
         td[r][tx] = d;
         tm[r][tx] = m;
     }
     __syncthreads();
     // write: for each k of the tile, 64 station-windows (contiguous w runs per station)
-    const int W2 = 2 * W;
+    const int W2 = mirror ? 2 * W : W;
     const int S = SW / W;
     for (int idx = threadIdx.x; idx < 32 * 64; idx += 256) {
         const int kk = idx >> 6, r = idx & 63;
@@ -60,7 +66,7 @@ k_spectra_to_u(const cf32 *__restrict__ F, int SW, int W, int Nz, cf32 *__restri
         const int s = sw / W, w = sw - s * W;
         cf32 *row = U + ((long long)k * S + s) * W2;
         row[w] = td[r][kk];
-        row[W + w] = tm[r][kk];
+        if (mirror) row[W + w] = tm[r][kk];
     }
 }
 
@@ -72,8 +78,8 @@ k_spectra_to_u(const cf32 *__restrict__ F, int SW, int W, int Nz, cf32 *__restri
 constexpr int kGramKc = 16;
 
 __global__ void __launch_bounds__(256)
-k_gram_f32(const cf32 *__restrict__ U, int S, int W, int M, const int2 *__restrict__ tiles, int ntiles,
-           int nchunks, cf32 *__restrict__ G, long long ldg)
+k_gram_f32(const cf32 *__restrict__ Ua, const cf32 *__restrict__ Ub, int S, int W2, int M,
+           const int2 *__restrict__ tiles, int ntiles, int nchunks, cf32 *__restrict__ G, long long ldg)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cf32 *stage = reinterpret_cast<cf32 *>(smem_raw);  // [16*32 pairs][kGramKc]
@@ -88,7 +94,6 @@ k_gram_f32(const cf32 *__restrict__ U, int S, int W, int M, const int2 *__restri
     const int k0 = chunk * kGramKc;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, q = lane >> 4;
-    const int W2 = 2 * W;
     const int si = bi * 16 + li, sj0 = bj * 32 + li, sj1 = sj0 + 16;
     const int nmm = (W2 + 3) / 4;  // K padded with zeros to a multiple of 4 complex values
 
@@ -97,10 +102,9 @@ k_gram_f32(const cf32 *__restrict__ U, int S, int W, int M, const int2 *__restri
         const int k = k0 + kl;
         f4v r0 = {0, 0, 0, 0}, i0 = {0, 0, 0, 0}, r1 = {0, 0, 0, 0}, i1 = {0, 0, 0, 0};
         if (k <= M) {
-            const cf32 *Uk = U + (long long)k * S * W2;
-            const cf32 *pa = Uk + (long long)si * W2;
-            const cf32 *pb0 = Uk + (long long)sj0 * W2;
-            const cf32 *pb1 = Uk + (long long)sj1 * W2;
+            const cf32 *pa = Ua + ((long long)k * S + si) * W2;
+            const cf32 *pb0 = Ub + ((long long)k * S + sj0) * W2;
+            const cf32 *pb1 = Ub + ((long long)k * S + sj1) * W2;
             const bool va = si < S, vb0 = sj0 < S, vb1 = sj1 < S;
             for (int mm = 0; mm < nmm; ++mm) {
                 const int m = 4 * mm + q;
@@ -140,7 +144,8 @@ k_gram_f32(const cf32 *__restrict__ U, int S, int W, int M, const int2 *__restri
 // Z[k] = P + i e^{+2 pi i k/Nz} Q, P = G[k] + conj(G[M-k]), Q = G[k] - conj(G[M-k])
 __global__ void __launch_bounds__(512)
 k_pair_c2r(const cf32 *__restrict__ G, long long ldg, int Nz, FftPlan planM, const cf32 *__restrict__ twM,
-           const cf32 *__restrict__ twc, int lag0, int L, float scale, float *__restrict__ y)
+           const cf32 *__restrict__ twc, int lag0, int L, float scale, int count_norm, int N,
+           float *__restrict__ y)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int M = Nz >> 1;
@@ -150,6 +155,10 @@ k_pair_c2r(const cf32 *__restrict__ G, long long ldg, int Nz, FftPlan planM, con
     const cf32 *g = G + p * ldg;
     for (int k = threadIdx.x; k < M; k += blockDim.x) {
         cf32 a = g[k], c = g[M - k];
+        if (k == 0) {  // DC and Nyquist of a real signal's spectrum are real
+            a.y = 0.f;
+            c.y = 0.f;
+        }
         cf32 P = cf32{a.x + c.x, a.y - c.y};
         cf32 Q = cf32{a.x - c.x, a.y + c.y};
         cf32 wq = cmul(twc[k], Q);
@@ -161,7 +170,9 @@ k_pair_c2r(const cf32 *__restrict__ G, long long ldg, int Nz, FftPlan planM, con
     for (int l = threadIdx.x; l < L; l += blockDim.x) {
         const int n = (lag0 + l) & (Nz - 1);
         cf32 v = z[n >> 1];
-        dst[l] = scale * ((n & 1) ? v.y : v.x);
+        float r = scale * ((n & 1) ? v.y : v.x);
+        if (count_norm) r = (float)((double)r / (double)(N - abs(lag0 + l)));
+        dst[l] = r;
     }
 }
 
@@ -199,18 +210,19 @@ const int2 *gram_tiles(int S, int &ntiles)
     return (const int2 *)d;
 }
 
-int launch_spectra_to_u(const cf32 *d_F, int S, int W, int Nz, cf32 *d_U, hipStream_t st)
+int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, int mirror, cf32 *d_U,
+                        hipStream_t st)
 {
     const int M = Nz / 2, SW = S * W;
     dim3 grid((M + 1 + 31) / 32, (SW + 63) / 64);
     ProfScope prof("spectra_to_u", st);
-    hipLaunchKernelGGL(k_spectra_to_u, grid, dim3(256), 0, st, d_F, SW, W, Nz, d_U);
+    hipLaunchKernelGGL(k_spectra_to_u, grid, dim3(256), 0, st, d_F, ldF, SW, W, Nz, mirror, d_U);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }
 
-int launch_gram_f32(const cf32 *d_U, int S, int W, int Nz, const int2 *d_tiles, int ntiles, cf32 *d_G,
-                    hipStream_t st)
+int launch_gram_f32(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int Nz, const int2 *d_tiles, int ntiles,
+                    cf32 *d_G, hipStream_t st)
 {
     const int M = Nz / 2;
     const int nchunks = (M + 1 + kGramKc - 1) / kGramKc;
@@ -218,25 +230,24 @@ int launch_gram_f32(const cf32 *d_U, int S, int W, int Nz, const int2 *d_tiles, 
     size_t lds = sizeof(cf32) * 16 * 32 * kGramKc;
     allow_lds((const void *)k_gram_f32, lds);
     ProfScope prof("gram_f32", st);
-    hipLaunchKernelGGL(k_gram_f32, dim3(slots * 8), dim3(256), lds, st, d_U, S, W, M, d_tiles, ntiles, nchunks,
-                       d_G, gram_ld(Nz));
+    hipLaunchKernelGGL(k_gram_f32, dim3(slots * 8), dim3(256), lds, st, d_Ua, d_Ub, S, W2, M, d_tiles, ntiles,
+                       nchunks, d_G, gram_ld(Nz));
     FPCC_CHECK(hipGetLastError());
     return 0;
 }
 
-int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int L, float *d_y, hipStream_t st)
+int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int L, float scale, int count_norm,
+                    float *d_y, hipStream_t st)
 {
     const int M = Nz / 2;
     const DevPlan &pM = get_plan(M, false);
     const void *twc = get_c2r_twiddles(Nz, false);
     if (!pM.ok || !twc) return -1;
-    // 1/2 of the Gram's fold, the reference's 1/((float)Nz*(float)N), the mean over W
-    const float scale = 0.5f * (1 / ((float)Nz * (float)N)) / (float)W;
     size_t lds = sizeof(cf32) * 2 * (size_t)M;
     allow_lds((const void *)k_pair_c2r, lds);
     ProfScope prof("pair_c2r", st);
     hipLaunchKernelGGL(k_pair_c2r, dim3(P), dim3(512), lds, st, d_G, gram_ld(Nz), Nz, pM.plan,
-                       (const cf32 *)pM.tw, (const cf32 *)twc, lag0, L, scale, d_y);
+                       (const cf32 *)pM.tw, (const cf32 *)twc, lag0, L, scale, count_norm, N, d_y);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }

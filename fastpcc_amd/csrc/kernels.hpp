@@ -17,9 +17,24 @@ int launch_pair_pcc2(const cf32 *d_F, int S, int W, int N, int Nz, int lag0, int
 // Stack over windows as a per-frequency Gram (gram.hip).
 long long gram_ld(int Nz);                       // row length of G in bins
 const int2 *gram_tiles(int S, int &ntiles);      // cached device tile table
-int launch_spectra_to_u(const cf32 *d_F, int S, int W, int Nz, cf32 *d_U, hipStream_t st);
-int launch_gram_f32(const cf32 *d_U, int S, int W, int Nz, const int2 *d_tiles, int ntiles, cf32 *d_G,
-                    hipStream_t st);
-int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int L, float *d_y, hipStream_t st);
+int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, int mirror, cf32 *d_U,
+                        hipStream_t st);
+// G[p][k] = sum_{m < W2} conj(Ua[k][i][m]) Ub[k][j][m] for pairs i < j, k <= Nz/2
+int launch_gram_f32(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int Nz, const int2 *d_tiles, int ntiles,
+                    cf32 *d_G, hipStream_t st);
+// y[p][l] = scale * c2r(G[p])[(lag0+l) mod Nz], divided by N-|lag| if count_norm
+int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int L, float scale, int count_norm,
+                    float *d_y, hipStream_t st);
+
+// GNCC / 1-bit (gn.hip).  Half spectra rows hold real_ld(Nz) bins, energy rows
+// energy_ld(N) prefix sums.  sign_rule 0: samples, 1: x >= 0, 2: x > 0.
+long long real_ld(int Nz);
+long long energy_ld(int N);
+int launch_station_real(const float *d_x, long long station_stride, int S, int W, int N, int Nz, int sign_rule,
+                        bool f64, void *d_X, double *d_E, hipStream_t st);
+// norm 0: sqrt(E1 E2) from prefix sums (GNCC); 1: overlap count (1-bit).
+// mode 0: y[p][w][L]; 1: mean over windows.
+int launch_pair_gn(const void *d_Xa, const void *d_Xb, const double *d_E, bool f64, int S, int W, int N, int Nz,
+                   int lag0, int L, int norm, int mode, float *d_y, hipStream_t st);
 
 }  // namespace fpcc

@@ -33,8 +33,17 @@ extern "C" {
  * Reference entry points (src/FFTapps.h:6-16).  Host buffers in and out.
  * ------------------------------------------------------------------------- */
 
-/* PCC of power 2 (Ventosa et al. 2019), replaces pcc2_set, src/FFTapps.c:785-881. */
+/* PCC of power 2 (Ventosa et al. 2019), replaces pcc2_set, src/FFTapps.c:785-881.
+ * FP32 like the reference's fftwf path. */
 int pcc2_set(float **const y, float **const x1, float **const x2, const unsigned int N,
+             const unsigned int Tr, const int Lag1, const int Lag2);
+/* Geometrically normalised correlation, replaces ccgn_set, src/FFTapps.c:883-978.
+ * FP64 FFTs and overlap energies like the reference's fftw path. */
+int ccgn_set(float **const y, float **const x1, float **const x2, const unsigned int N,
+             const unsigned int Tr, const int Lag1, const int Lag2);
+/* 1-bit GNCC, replaces cc1b_set, src/FFTapps.c:980-1073.  Signs x1 >= 0 -> +1 and
+ * x2 > 0 -> +1 (the reference's asymmetric rule, :1037 and :1042); FP32 FFTs. */
+int cc1b_set(float **const y, float **const x1, float **const x2, const unsigned int N,
              const unsigned int Tr, const int Lag1, const int Lag2);
 
 /* ---------------------------------------------------------------------------
@@ -46,7 +55,8 @@ int pcc2_set(float **const y, float **const x1, float **const x2, const unsigned
  * [w*win_len, (w+1)*win_len)).  Pairs p <-> (i<j) in row-major upper-triangle
  * order, p = i*S - i*(i+1)/2 + (j-i-1); station i is x1 (virtual source), j is x2.
  * Output y: stack == FPCC_STACK_NONE -> [P][n_windows][L], else [P][L] holding
- * the arithmetic mean over windows.
+ * the arithmetic mean over windows.  Stack modes per kind: PCC2 all; GNCC NONE,
+ * PLAIN (its per-window energies forbid stacking spectra); 1-bit NONE, PLAIN, GRAM.
  * ------------------------------------------------------------------------- */
 enum { FPCC_PCC2 = 0, FPCC_PCC1 = 1, FPCC_PCCV = 2, FPCC_GNCC = 3, FPCC_CC1B = 4, FPCC_WPCC = 5 };
 enum {

@@ -1,0 +1,83 @@
+"""GNCC and 1-bit GNCC on the GPU against the oracle (src/FFTapps.c:883-1073)
+and the committed config-1 goldens."""
+import os
+
+import numpy as np
+import pytest
+
+from fastpcc_amd import synth
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _rand(Tr, N, seed, zeros=0.0):
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((Tr, N)).astype(np.float32)
+    b = rng.standard_normal((Tr, N)).astype(np.float32)
+    if zeros:
+        a[rng.random((Tr, N)) < zeros] = 0.0
+        b[rng.random((Tr, N)) < zeros] = 0.0
+    return a, b
+
+
+@pytest.mark.parametrize("N,Tr,lag1,lag2", [(3600, 3, -1000, 1000), (100, 2, -99, 99), (1000, 2, 5, 300),
+                                            (2401, 1, -7, 0), (640, 4, 60, -20)])
+def test_ccgn_set_matches_oracle(fp, orc, N, Tr, lag1, lag2):
+    print(f"ccgn_set N={N} Tr={Tr} lags=({lag1},{lag2})", flush=True)
+    x1, x2 = _rand(Tr, N, N + 1)
+    y = fp.ccgn_set(x1, x2, lag1, lag2)
+    yr = orc.run("gncc", x1, x2, lag1, lag2)
+    err = np.abs(y - yr).max()
+    print(f"  max|gpu-oracle|={err:.3e}")
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("N,Tr,lag1,lag2,zeros", [(3600, 3, -1000, 1000, 0.0), (100, 2, -99, 99, 0.0),
+                                                  (1000, 2, -300, 300, 0.3), (512, 3, -50, 70, 0.6)])
+def test_cc1b_set_matches_oracle(fp, orc, N, Tr, lag1, lag2, zeros):
+    """With exact zeros the asymmetric sign rule (x1 >= 0, x2 > 0) changes the
+    result by about 2/N per zero, far above the tolerance."""
+    print(f"cc1b_set N={N} Tr={Tr} lags=({lag1},{lag2}) zeros={zeros}", flush=True)
+    x1, x2 = _rand(Tr, N, N + 2, zeros)
+    y = fp.cc1b_set(x1, x2, lag1, lag2)
+    yr = orc.run("cc1b", x1, x2, lag1, lag2)
+    err = np.abs(y - yr).max()
+    print(f"  max|gpu-oracle|={err:.3e}")
+    assert err <= TOL
+    if zeros:
+        # the same data with the symmetric rule would differ: check that it does
+        alt = orc.run("cc1b", x2, x1, -lag2, -lag1)[:, ::-1]
+        assert np.abs(alt - yr).max() > 10 * TOL
+
+
+def test_cc1b_zero_records_give_minus_one(fp):
+    z = np.zeros((2, 256), np.float32)
+    y = fp.cc1b_set(z, z, -5, 5)
+    assert np.abs(y + 1).max() < 1e-6
+
+
+@pytest.mark.parametrize("kind,stack", [("gncc", "none"), ("gncc", "plain"), ("cc1b", "none"),
+                                        ("cc1b", "plain"), ("cc1b", "gram")])
+def test_allpairs_gn(fp, orc, kind, stack):
+    S, W, N = 9, 3, 2000
+    x, _ = synth.make_stations(S, W * N, 300, seed=17)
+    if kind == "cc1b":
+        x[:, ::97] = 0.0  # exact zeros exercise the sign rule
+    y = fp.allpairs(x, W, N, -600, 600, kind=kind, stack=stack)
+    yr = orc.allpairs(kind, x, W, N, -600, 600, stacked=(stack != "none"))
+    err = np.abs(y - yr).max()
+    print(f"{kind}/{stack} max|gpu-oracle|={err:.3e}")
+    assert y.shape == yr.shape and err <= TOL
+
+
+@pytest.mark.parametrize("kind,stack", [("pcc2", "plain"), ("pcc2", "gram"), ("gncc", "plain"), ("cc1b", "plain"),
+                                        ("cc1b", "gram")])
+def test_golden_config1(fp, kind, stack):
+    x = np.load(os.path.join(G, "c1_x.npy"), allow_pickle=False)
+    gold = np.load(os.path.join(G, f"c1_{kind}_fp64.npy"), allow_pickle=False)
+    y = fp.allpairs(x, 1, 3600, -500, 500, kind=kind, stack=stack)
+    err = np.abs(y - gold).max()
+    print(f"{kind}/{stack} vs golden (FP64): {err:.3e}")
+    assert err <= TOL

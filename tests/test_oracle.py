@@ -1,0 +1,364 @@
+"""Pins of the CPU oracle (restatement of src/FFTapps.c and src/FWTa/wavelet_def_v7.c).
+
+The reference ships no tests or golden vectors for this path (its only example
+script checks nothing), so every pin below comes from an independent route:
+library routines (scipy.signal.hilbert, numpy.correlate, scipy.special.erfi),
+closed forms, direct time-domain sums instead of FFTs, identities between two
+different algorithms of the oracle, and known-answer cases.
+"""
+import numpy as np
+import pytest
+import scipy.signal as ss
+import scipy.special as sp
+
+from fastpcc_amd import synth
+
+
+def _phasor64(x):
+    z = ss.hilbert(np.asarray(x, dtype=np.float64))
+    eps = 1e-6 * np.abs(z).max()
+    return z / (np.abs(z) + eps)
+
+
+def _lags(l1, l2):
+    return np.arange(min(l1, l2), max(l1, l2) + 1)
+
+
+# ---------------------------------------------------------------- analytic signal (a1)
+
+@pytest.mark.parametrize("N", [37, 64, 100, 127, 360, 3600, 4099])
+def test_analytic_matches_scipy_hilbert(orc, N):
+    """Mask 1/N at DC, 2/N below Nyquist, 1/N at Nyquist (src/FFTapps.c:113-117)
+    equals scipy's Hilbert construction; 127 and 4099 exercise Bluestein."""
+    x = np.random.default_rng(N).standard_normal(N)
+    assert np.abs(orc.analytic(x, f64=True) - ss.hilbert(x)).max() < 1e-12
+    assert np.abs(orc.analytic(x.astype(np.float32)) - ss.hilbert(x)).max() < 5e-6
+
+
+@pytest.mark.parametrize("N,k", [(64, 5), (100, 1), (3600, 37), (97, 12)])
+def test_analytic_of_cosine_is_exponential(orc, N, k):
+    n = np.arange(N)
+    z = orc.analytic(np.cos(2 * np.pi * k * n / N), f64=True)
+    assert np.abs(z - np.exp(2j * np.pi * k * n / N)).max() < 1e-12
+
+
+def test_analytic_dc_and_nyquist(orc):
+    N = 64
+    n = np.arange(N)
+    assert np.abs(orc.analytic(np.full(N, 2.5), f64=True) - 2.5).max() < 1e-13
+    alt = (-1.0) ** n
+    assert np.abs(orc.analytic(alt, f64=True) - alt).max() < 1e-13
+
+
+# ---------------------------------------------------------------- phasor regularisers (a2, a3)
+
+def test_ampnormf_known_answer(orc):
+    """max|z| = 5 -> eps = 5e-6 (src/FFTapps.c:188): (3+4i)/(5+5e-6)."""
+    z = orc.ampnormf(np.array([3 + 4j, 0, 1 + 0j], dtype=np.complex64))
+    assert abs(z[0] - (3 + 4j) / 5.000005) < 1e-7
+    assert z[1] == 0
+    assert abs(z[2] - 1 / 1.000005) < 1e-7
+
+
+def test_ampnorm_known_answer(orc):
+    """FP64 variant: eps = 1e-6 * max|z|^2 = 25e-6, z / sqrt(|z|^2 + eps) (:147-175)."""
+    z = orc.ampnorm(np.array([3 + 4j, 1j], dtype=np.complex128))
+    assert abs(z[0] - (3 + 4j) / np.sqrt(25 + 25e-6)) < 1e-15
+    assert abs(z[1] - 1j / np.sqrt(1 + 25e-6)) < 1e-15
+
+
+def test_ampnorm_zero_vector_is_zero(orc):
+    """All-zero input: the reciprocal is infinite, the isnormal guard gives 0."""
+    assert not np.any(orc.ampnormf(np.zeros(8, np.complex64)))
+    assert not np.any(orc.ampnorm(np.zeros(8, np.complex128)))
+
+
+def test_ampnormf_properties(orc):
+    rng = np.random.default_rng(3)
+    z = (rng.standard_normal(500) + 1j * rng.standard_normal(500)).astype(np.complex64)
+    p = orc.ampnormf(z)
+    assert np.all(np.abs(p) < 1)
+    nz = np.abs(z) > 1e-3
+    assert np.abs(np.angle(p[nz] / z[nz])).max() < 1e-6              # phase kept
+    assert np.abs(orc.ampnormf(7.0 * z) - p).max() < 1e-6             # scale invariant
+
+
+# ---------------------------------------------------------------- PCC2 (a4) vs direct sums
+
+def _direct_pcc2(x1, x2, lags):
+    p1, p2 = _phasor64(x1), _phasor64(x2)
+    N = len(x1)
+    out = []
+    for lag in lags:
+        n = np.arange(max(0, -lag), min(N, N - lag))
+        out.append(np.real(np.sum(np.conj(p1[n]) * p2[n + lag])) / N)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("N,l1,l2", [(300, -40, 60), (97, -96, 96), (256, 10, 100), (500, 80, -30)])
+def test_pcc2_vs_direct_formula(orc, N, l1, l2):
+    rng = np.random.default_rng(N)
+    x1 = rng.standard_normal((2, N)).astype(np.float32)
+    x2 = rng.standard_normal((2, N)).astype(np.float32)
+    ref = np.array([_direct_pcc2(x1[t], x2[t], _lags(l1, l2)) for t in range(2)])
+    assert np.abs(orc.run("pcc2", x1, x2, l1, l2, f64=True) - ref).max() < 1e-12
+    assert np.abs(orc.run("pcc2", x1, x2, l1, l2) - ref).max() < 1e-6
+
+
+def test_pcc2_autocorrelation_peak(orc):
+    """x2 == x1: lag 0 equals (1/N) sum |phi|^2, just below 1, and is the maximum."""
+    x = np.random.default_rng(0).standard_normal((1, 1000)).astype(np.float32)
+    y = orc.run("pcc2", x, x, -50, 50, f64=True)[0]
+    assert np.argmax(y) == 50 and 0.999 < y[50] < 1.0
+
+
+def test_pcc2_delayed_copy_peaks_at_delay(orc):
+    x, sh = synth.make_stations(2, 3600, 200, snr=2.0, seed=4)
+    y = orc.run("pcc2", x[:1], x[1:], -300, 300)[0]
+    assert np.argmax(y) - 300 == sh[1] - sh[0]
+
+
+def test_pcc2_pair_swap(orc):
+    """y_ji[lag] = y_ij[-lag]."""
+    rng = np.random.default_rng(9)
+    x1 = rng.standard_normal((1, 400)).astype(np.float32)
+    x2 = rng.standard_normal((1, 400)).astype(np.float32)
+    a = orc.run("pcc2", x1, x2, -30, 30, f64=True)[0]
+    b = orc.run("pcc2", x2, x1, -30, 30, f64=True)[0]
+    assert np.abs(a - b[::-1]).max() < 1e-13
+
+
+# ---------------------------------------------------------------- PCC1 / PCC-v (a5, a6)
+
+def _direct_pcc_power(x1, x2, lags, v):
+    p1, p2 = _phasor64(x1), _phasor64(x2)
+    N = len(x1)
+    M = np.abs(p2[None, :] + p1[:, None]) ** v - np.abs(p2[None, :] - p1[:, None]) ** v  # [n1][n2]
+    out = []
+    for lag in lags:
+        d = np.diagonal(M, offset=lag)  # pairs phi1[n], phi2[n+lag]
+        out.append(d.sum() / (2 ** (v / 2) * (N - abs(lag))))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("v", [1.0, 1.5, 2.0, 3.0])
+def test_pccv_vs_diagonal_sums(orc, v):
+    """Sums along the diagonals of the |phi2[m] +- phi1[n]|^v matrix."""
+    rng = np.random.default_rng(int(v * 10))
+    N = 150
+    x1 = rng.standard_normal((1, N)).astype(np.float32)
+    x2 = rng.standard_normal((1, N)).astype(np.float32)
+    ref = _direct_pcc_power(x1[0], x2[0], _lags(-30, 40), v)
+    assert np.abs(orc.run("pccv", x1, x2, -30, 40, f64=True, v=v)[0] - ref).max() < 1e-12
+    assert np.abs(orc.run("pccv", x1, x2, -30, 40, v=v)[0] - ref).max() < 1e-5
+
+
+def test_pccv2_equals_scaled_pcc2(orc):
+    """|a+b|^2 - |a-b|^2 = 4 Re(a conj b): PCC-v(2) = 2N/(N-|lag|) * PCC2 (two
+    different algorithms: time-domain lowlevel vs zero-padded FFTs)."""
+    rng = np.random.default_rng(21)
+    N = 600
+    x1 = rng.standard_normal((2, N)).astype(np.float32)
+    x2 = rng.standard_normal((2, N)).astype(np.float32)
+    lags = _lags(-100, 80)
+    yv = orc.run("pccv", x1, x2, -100, 80, f64=True, v=2.0)
+    y2 = orc.run("pcc2", x1, x2, -100, 80, f64=True)
+    assert np.abs(yv - y2 * 2 * N / (N - np.abs(lags))).max() < 1e-12
+
+
+def test_pcc1_equals_scaled_pccv1(orc):
+    """PCC1 divides by 2*count (:276), PCC-v by 2^(v/2)*count (:334)."""
+    rng = np.random.default_rng(22)
+    x1 = rng.standard_normal((2, 300)).astype(np.float32)
+    x2 = rng.standard_normal((2, 300)).astype(np.float32)
+    y1 = orc.run("pcc1", x1, x2, -60, 60, f64=True)
+    yv = orc.run("pccv", x1, x2, -60, 60, f64=True, v=1.0)
+    assert np.abs(y1 - yv * np.sqrt(2) / 2).max() < 1e-13
+    y1f = orc.run("pcc1", x1, x2, -60, 60)
+    assert np.abs(y1f - y1).max() < 1e-5
+
+
+def test_pcc1_identical_signals(orc):
+    """phi1 == phi2: |2 phi| - 0 summed over N, /2N -> mean |phi| at lag 0, just below 1."""
+    x = np.random.default_rng(5).standard_normal((1, 700)).astype(np.float32)
+    y = orc.run("pcc1", x, x, -5, 5, f64=True)[0]
+    assert np.argmax(y) == 5 and 0.99 < y[5] < 1.0
+
+
+# ---------------------------------------------------------------- GNCC and 1-bit (a7, a8)
+
+def _direct_gncc(a, b, lags, onebit=False):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if onebit:
+        a = np.where(a >= 0, 1.0, -1.0)
+        b = np.where(b > 0, 1.0, -1.0)
+    N = len(a)
+    full = np.correlate(b, a, "full")  # full[k + N - 1] = sum_n b[n + k] a[n]
+    out = []
+    for lag in lags:
+        n = np.arange(max(0, -lag), min(N, N - lag))
+        out.append(full[lag + N - 1] / np.sqrt(np.sum(a[n] ** 2) * np.sum(b[n + lag] ** 2)))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("kind", ["gncc", "cc1b"])
+@pytest.mark.parametrize("N,l1,l2", [(300, -40, 60), (1000, -999, 999), (257, 3, 50)])
+def test_gn_vs_numpy_correlate(orc, kind, N, l1, l2):
+    rng = np.random.default_rng(N + l1)
+    x1 = rng.standard_normal((2, N)).astype(np.float32)
+    x2 = rng.standard_normal((2, N)).astype(np.float32)
+    ref = np.array([_direct_gncc(x1[t], x2[t], _lags(l1, l2), kind == "cc1b") for t in range(2)])
+    assert np.abs(orc.run(kind, x1, x2, l1, l2, f64=True) - ref).max() < 1e-11
+    # FP32 FFT rounding is divided by the overlap length: largest at |lag| -> N-1
+    assert np.abs(orc.run(kind, x1, x2, l1, l2) - ref).max() < 1e-5
+
+
+def test_gncc_delayed_copy_is_exactly_one(orc):
+    """x2[n+d] = x1[n] over the overlap: normalised correlation 1 at lag d."""
+    rng = np.random.default_rng(8)
+    s = rng.standard_normal(1100)
+    d = 37
+    x1 = s[100:1100].astype(np.float32)[None]
+    x2 = s[100 - d:1100 - d].astype(np.float32)[None]
+    y = orc.run("gncc", x1, x2, -60, 60)[0]
+    assert np.argmax(y) == 60 + d and abs(y[60 + d] - 1) < 1e-12
+
+
+def test_onebit_sign_rule_is_asymmetric(orc):
+    """Zeros map to +1 for station 1 (x >= 0) and to -1 for station 2 (x > 0)
+    (src/FFTapps.c:1037, :1042): zero records correlate to exactly -1."""
+    z = np.zeros((1, 64), np.float32)
+    y = orc.run("cc1b", z, z, -3, 3)[0]
+    assert np.all(y == -1.0)
+    y64 = orc.run("cc1b", z, z, -3, 3, f64=True)[0]
+    assert np.all(y64 == -1.0)
+
+
+def test_onebit_scale_invariance(orc):
+    rng = np.random.default_rng(2)
+    x1 = rng.standard_normal((1, 400)).astype(np.float32)
+    x2 = rng.standard_normal((1, 400)).astype(np.float32)
+    a = orc.run("cc1b", x1, x2, -20, 20)
+    b = orc.run("cc1b", 3 * x1, 0.5 * x2, -20, 20)
+    assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------- lag and Nz rules (a11)
+
+@pytest.mark.parametrize("kind", ["pcc2", "gncc", "cc1b"])
+def test_too_large_lag_returns_minus3(orc, kind):
+    x = np.zeros((1, 100), np.float32)
+    with pytest.raises(ValueError, match="-3"):
+        orc.run(kind, x, x, -100, 5)
+
+
+def test_swapped_lags_same_window(orc):
+    rng = np.random.default_rng(4)
+    x1 = rng.standard_normal((1, 300)).astype(np.float32)
+    x2 = rng.standard_normal((1, 300)).astype(np.float32)
+    a = orc.run("pcc2", x1, x2, 50, -20)
+    b = orc.run("pcc2", x1, x2, -20, 50)
+    assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------- all pairs + stack (a13, a14)
+
+def test_pair_index_map():
+    S = 9
+    pl = synth.pair_list(S)
+    assert len(pl) == S * (S - 1) // 2
+    for p, (i, j) in enumerate(pl):
+        assert synth.pair_index(i, j, S) == p
+
+
+def test_allpairs_stack_is_mean_of_windows(orc):
+    x, _ = synth.make_stations(4, 3 * 500, 50, seed=6)
+    per = orc.allpairs("pcc2", x, 3, 500, -40, 40, stacked=False, f64=True)
+    st = orc.allpairs("pcc2", x, 3, 500, -40, 40, stacked=True, f64=True)
+    assert np.abs(per.mean(axis=1) - st).max() < 1e-15
+    # pair (1, 3) over its windows = one *_set call on those rows
+    w = x.reshape(4, 3, 500)
+    one = orc.run("pcc2", w[1], w[3], -40, 40, f64=True)
+    assert np.abs(per[synth.pair_index(1, 3, 4)] - one).max() == 0
+
+
+# ---------------------------------------------------------------- WPCC filter bank (a10)
+
+def test_erfi_matches_scipy(orc):
+    for z in [-2.0, -0.3, 0.0, 0.7, 1.5, 3.0]:
+        assert abs(orc.erfi(z) - sp.erfi(z)) <= 1e-12 * max(1.0, abs(sp.erfi(z)))
+
+
+def test_wavelet_family_config4_table(orc):
+    """Morlet, pmin 2, pmax 200, V 8 (src/FFTapps.c:1101-1110): J = round(1/8 +
+    log2 100) = 7, S = 56, s0 = 2 w0/(2 pi) with w0 = pi sqrt(2/ln 2), largest
+    scale s0 2^(55/8), Ls = 2 ceil(5 s) + 1, Nz = 131072 without doubling."""
+    f = orc.wpcc_family(86400, -2000, 2000, 2.0, 200.0, 8, -1, 0.0)
+    w0 = np.pi * np.sqrt(2 / np.log(2))
+    assert (f["S"], f["J"], f["V"], f["Nz"]) == (56, 7, 8, 131072)
+    assert abs(f["op1"] - w0) < 1e-15 and abs(f["s0"] - 2 * w0 / (2 * np.pi)) < 1e-14
+    assert abs(f["scale"][-1] - f["s0"] * 2 ** (55 / 8)) < 1e-9
+    assert f["Ls0"] == 2 * int(np.ceil(5 * f["scale"][-1])) + 1 == 1995
+    assert abs(f["K0"] - np.sum(1 / f["scale"])) < 1e-12
+    assert round(f["s0"], 4) == 1.6986 and round(f["K0"], 4) == 7.0378
+
+
+def test_morlet_samples_closed_form(orc):
+    f = orc.wpcc_family(1000, -50, 50, 4.0, 16.0, 2, -1, 0.0)
+    w0 = f["op1"]
+    for s in [0, f["S"] - 1]:
+        sc, Ls, c = f["scale"][s], int(f["Ls"][s]), int(f["center"][s])
+        assert c == Ls // 2
+        t = (np.arange(Ls) - c) / sc
+        ref = (np.pi ** -0.25) / np.sqrt(sc) * np.exp(1j * w0 * t) * np.exp(-t * t / 2)
+        assert np.abs(f["wc"][s] - ref).max() < 1e-14
+        # w(u) = conj(w(-u)) about the centre
+        assert np.abs(f["wc"][s] - np.conj(f["wc"][s][::-1])).max() < 1e-14
+
+
+def test_doubling_rule(orc):
+    """Nz doubles when Nz - (N+M) < Ls0 (src/FFTapps.c:1112)."""
+    f = orc.wpcc_family(1000, -20, 20, 4.0, 100.0, 2, -3, 0.0)
+    base = 1 << int(np.ceil(np.log2(1020)))
+    assert f["Nz"] == (2 * base if base - 1020 < f["Ls0"] else base)
+
+
+# ---------------------------------------------------------------- WPCC (a9) without FFTs
+
+def _direct_wpcc(x1, x2, l1, l2, fam):
+    """Circular CWT by direct convolution in the Nz buffer, AmpNorm (FP64), and
+    circular correlation sums, weighted by 1/scale and C = 1/(K0 N)."""
+    Nz, Ls0, N = fam["Nz"], fam["Ls0"], len(x1)
+
+    def cwt(x, s):
+        buf = np.zeros(Nz)
+        buf[Ls0:Ls0 + N] = x
+        w, c = fam["wc"][s], int(fam["center"][s])
+        out = np.zeros(Nz, complex)
+        for t in range(len(w)):  # out[n] = sum_t w[t] buf[n - t + c]
+            out += w[t] * np.roll(buf, t - c)
+        a2 = np.abs(out) ** 2
+        return out / np.sqrt(a2 + 1e-6 * a2.max())
+
+    lags = _lags(l1, l2)
+    y = np.zeros(len(lags))
+    for s in range(fam["S"]):
+        p1, p2 = cwt(x1, s), cwt(x2, s)
+        for li, lag in enumerate(lags):
+            y[li] += np.real(np.sum(np.conj(p1) * np.roll(p2, -lag))) / fam["scale"][s]
+    return y / (fam["K0"] * N)
+
+
+@pytest.mark.parametrize("wtype", [-1, -3])
+def test_wpcc_vs_direct_time_domain(orc, wtype):
+    rng = np.random.default_rng(-wtype)
+    N, l1, l2 = 120, -15, 15
+    x1 = rng.standard_normal((1, N)).astype(np.float32)
+    x2 = rng.standard_normal((1, N)).astype(np.float32)
+    fam = orc.wpcc_family(N, l1, l2, 4.0, 16.0, 2, wtype, 0.0)
+    ref = _direct_wpcc(x1[0].astype(np.float64), x2[0].astype(np.float64), l1, l2, fam)
+    y64 = orc.run("wpcc", x1, x2, l1, l2, f64=True, pmin=4.0, pmax=16.0, V=2, wtype=wtype)[0]
+    assert np.abs(y64 - ref).max() < 1e-12
+    y = orc.run("wpcc", x1, x2, l1, l2, pmin=4.0, pmax=16.0, V=2, wtype=wtype)[0]
+    assert np.abs(y - ref).max() < 1e-6
