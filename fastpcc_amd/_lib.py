@@ -47,9 +47,17 @@ def lib():
     L = ctypes.CDLL(LIB_PATH)
     fpp = ctypes.POINTER(ctypes.POINTER(ctypes.c_float))
     u, i, d = ctypes.c_uint, ctypes.c_int, ctypes.c_double
-    for name in ("pcc2_set", "ccgn_set", "cc1b_set"):
+    for name in ("pcc2_set", "ccgn_set", "cc1b_set", "pcc1_set"):
         getattr(L, name).argtypes = [fpp, fpp, fpp, u, u, i, i]
         getattr(L, name).restype = i
+    L.pcc_set.argtypes = [fpp, fpp, fpp, i, u, d, i, i]
+    L.pcc_set.restype = i
+    L.pcc1_highlevel.argtypes = [fpp, fpp, fpp, i, u, i, i, ctypes.c_void_p]
+    L.pcc1_highlevel.restype = i
+    L.pcc_highlevel.argtypes = [fpp, fpp, fpp, i, u, ctypes.c_float, i, i, ctypes.c_void_p]
+    L.pcc_highlevel.restype = i
+    L.pcc1_highlevel2.argtypes = [fpp, fpp, fpp, i, u, i, i]
+    L.pcc1_highlevel2.restype = None
     jp = ctypes.POINTER(FpccJob)
     L.fpcc_output_len.argtypes = [jp]
     L.fpcc_output_len.restype = ctypes.c_size_t

@@ -61,6 +61,59 @@ def ccgn_set(x1, x2, lag1, lag2):
     return _pair_set("ccgn_set", x1, x2, lag1, lag2)
 
 
+def pcc1_set(x1, x2, lag1, lag2):
+    """PCC1 of one station pair over Tr windows (src/FFTapps.c:611-783).  Rows
+    hold Lag2-Lag1+1 values; the reference's lag rules apply (see include/fastpcc.h)."""
+    return _time_set("pcc1_set", x1, x2, lag1, lag2)
+
+
+def pcc_set(x1, x2, v, lag1, lag2):
+    """PCC of power v of one station pair over Tr windows (src/FFTapps.c:475-608)."""
+    return _time_set("pcc_set", x1, x2, lag1, lag2, v=v)
+
+
+def pcc1_highlevel(z1, z2, lag1, lag2, y=None):
+    """Analytic-signal entry of the reference's GPU engine (src/ccs_cuda.h:10):
+    z1, z2 are [Tr][N] complex64 analytic signals; returns y [Tr][L] (only the
+    computable lags are written into y, which starts as zeros or the given array)."""
+    return _time_set("pcc1_highlevel", z1, z2, lag1, lag2, complex_in=True, y=y)
+
+
+def pcc_highlevel(z1, z2, v, lag1, lag2, y=None):
+    """PCC-v analytic-signal entry (src/ccs_cuda.h:9)."""
+    return _time_set("pcc_highlevel", z1, z2, lag1, lag2, v=v, complex_in=True, y=y)
+
+
+def _time_set(name, x1, x2, lag1, lag2, v=None, complex_in=False, y=None):
+    dt = np.complex64 if complex_in else np.float32
+    x1 = np.ascontiguousarray(np.atleast_2d(x1), dtype=dt)
+    x2 = np.ascontiguousarray(np.atleast_2d(x2), dtype=dt)
+    if x1.shape != x2.shape:
+        raise ValueError("x1 and x2 must have the same shape")
+    Tr, N = x1.shape
+    L = lag2 - lag1 + 1
+    if y is None:
+        y = np.zeros((Tr, max(L, 0)), dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    _, py = _rows(y) if L > 0 else (None, (ctypes.POINTER(ctypes.c_float) * max(Tr, 1))())
+    p1 = (ctypes.POINTER(ctypes.c_float) * Tr)(*[x1[t].ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+                                                  for t in range(Tr)])
+    p2 = (ctypes.POINTER(ctypes.c_float) * Tr)(*[x2[t].ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+                                                  for t in range(Tr)])
+    fn = getattr(_lib.lib(), name)
+    if name == "pcc_set":
+        r = fn(py, p1, p2, N, Tr, v, lag1, lag2)
+    elif name == "pcc_highlevel":
+        r = fn(py, p1, p2, N, Tr, v, lag1, lag2, None)
+    elif name == "pcc1_highlevel":
+        r = fn(py, p1, p2, N, Tr, lag1, lag2, None)
+    else:
+        r = fn(py, p1, p2, N, Tr, lag1, lag2)
+    if r != 0:
+        raise FastPCCError(r, name)
+    return y
+
+
 def cc1b_set(x1, x2, lag1, lag2):
     """1-bit GNCC of one station pair over Tr windows (src/FFTapps.c:980-1073)."""
     return _pair_set("cc1b_set", x1, x2, lag1, lag2)

@@ -1,5 +1,6 @@
 // C ABI of libfastpcc (declared in include/fastpcc.h).
 #include <math.h>
+#include <semaphore.h>
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
@@ -42,6 +43,8 @@ bool stack_supported(int kind, int stack)
     case FPCC_PCC2: return stack >= FPCC_STACK_NONE && stack <= FPCC_STACK_GRAM;
     case FPCC_GNCC: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN;
     case FPCC_CC1B: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN || stack == FPCC_STACK_GRAM;
+    case FPCC_PCC1:
+    case FPCC_PCCV: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN;
     default: return false;
     }
 }
@@ -104,6 +107,10 @@ Layout layout(const fpcc_job *job, const Dims &d)
             l.e = take(l.total, sizeof(cf32) * d.P * (size_t)gram_ld(d.Nz));
         }
         break;
+    case FPCC_PCC1:
+    case FPCC_PCCV:
+        l.a = take(l.total, sizeof(cf32) * SW * (size_t)d.N);  // phasors
+        break;
     }
     return l;
 }
@@ -121,6 +128,8 @@ int station_stage(const fpcc_job *job, const Dims &d, const float *x, char *ws, 
         if (r) return r;
         return launch_station_real(x, stride, d.S, d.W, d.N, d.Nz, 2, false, ws + l.b, nullptr, st);
     }
+    case FPCC_PCC1:
+    case FPCC_PCCV: return launch_station_phasor(x, stride, d.S, d.W, d.N, (cf32 *)(ws + l.a), st);
     }
     return -1;
 }
@@ -164,8 +173,67 @@ int pair_stage(const fpcc_job *job, const Dims &d, char *ws, float *y, hipStream
         }
         return launch_pair_gn(ws + l.a, ws + l.b, nullptr, false, d.S, d.W, d.N, d.Nz, d.lag0, d.L, 1, mode, y,
                               st);
+    case FPCC_PCC1:
+    case FPCC_PCCV:
+        return launch_pcc_time((const cf32 *)(ws + l.a), d.S, d.W, d.N, d.lag0, d.L,
+                               job->kind == FPCC_PCC1 ? 1.0 : job->v, job->kind == FPCC_PCC1, mode, y, d.L, 0, st);
     }
     return -1;
+}
+
+// PCC1 / PCC-v for one pair over Tr windows (pcc1_set, src/FFTapps.c:611-783;
+// pcc_set, :475-608).  Their lag handling differs from the FFT types: no -3, a
+// lag range disjoint from [-N, N] returns 0 untouched, L < 0 returns -1, and only
+// lags l in [l1, L') are computed (pcc1f_lowlevel, :251-278), the rest of each
+// row being zero.  If complex_in != 0 the inputs are analytic signals (the
+// *_highlevel entries): x1/x2 hold Tr complex rows and the phasor regulariser is
+// applied on the GPU; `anok`, if given, is waited on per trace before reading it.
+int pcc_time_host(int pcc1, float **y, const float *const *x1, const float *const *x2, int N, unsigned Tr,
+                  double v, int Lag1, int Lag2, int complex_in, sem_t *anok, bool zero_rows)
+{
+    if (Lag1 > N || Lag2 < -N) return 0;
+    const int L = Lag2 - Lag1 + 1;
+    if (!x1 || !x2 || L < 0 || !y) return -1;
+    if (Tr == 0 || L == 0 || N < 1) return 0;
+    int Lp = L;
+    if (Lag2 > N) Lp -= (Lag2 - N);
+    const int l1 = (Lag1 >= -N) ? 0 : -(Lag1 + N);
+    const size_t cw = complex_in ? 2 : 1;
+    const size_t xlen = (size_t)2 * Tr * N * cw, ylen = (size_t)Tr * L;
+    HostBuf hx, hy;
+    DeviceBuf dx, dy, dphi;
+    FPCC_CHECK(hipHostMalloc(&hx.p, sizeof(float) * xlen, hipHostMallocDefault));
+    FPCC_CHECK(hipHostMalloc(&hy.p, sizeof(float) * ylen, hipHostMallocDefault));
+    for (unsigned tr = 0; tr < Tr; ++tr) {
+        if (anok) sem_wait(&anok[tr]);
+        if (!x1[tr] || !x2[tr] || !y[tr]) return -1;
+        memcpy((float *)hx.p + (size_t)tr * N * cw, x1[tr], sizeof(float) * N * cw);
+        memcpy((float *)hx.p + (size_t)(Tr + tr) * N * cw, x2[tr], sizeof(float) * N * cw);
+    }
+    FPCC_CHECK(hipMalloc(&dx.p, sizeof(float) * xlen));
+    FPCC_CHECK(hipMalloc(&dy.p, sizeof(float) * ylen));
+    FPCC_CHECK(hipMemcpy(dx.p, hx.p, sizeof(float) * xlen, hipMemcpyHostToDevice));
+    FPCC_CHECK(hipMemset(dy.p, 0, sizeof(float) * ylen));
+    int r;
+    const cf32 *phi;
+    if (complex_in) {
+        if ((r = launch_ampnormf_rows((cf32 *)dx.p, 2 * (int)Tr, N, nullptr))) return r;
+        phi = (const cf32 *)dx.p;
+    } else {
+        FPCC_CHECK(hipMalloc(&dphi.p, sizeof(cf32) * 2 * Tr * (size_t)N));
+        if ((r = launch_station_phasor((const float *)dx.p, (long long)Tr * N, 2, Tr, N, (cf32 *)dphi.p, nullptr)))
+            return r;
+        phi = (const cf32 *)dphi.p;
+    }
+    if ((r = launch_pcc_time(phi, 2, Tr, N, Lag1 + l1, Lp - l1, v, pcc1, 0, (float *)dy.p, L, l1, nullptr)))
+        return r;
+    FPCC_CHECK(hipMemcpy(hy.p, dy.p, sizeof(float) * ylen, hipMemcpyDeviceToHost));
+    for (unsigned tr = 0; tr < Tr; ++tr) {
+        const float *src = (float *)hy.p + (size_t)tr * L;
+        if (zero_rows) memcpy(y[tr], src, sizeof(float) * L);
+        else if (Lp > l1) memcpy(y[tr] + l1, src + l1, sizeof(float) * (Lp - l1));
+    }
+    return 0;
 }
 
 // One pair over Tr windows from host row pointers, as the reference's *_set:
@@ -292,4 +360,38 @@ extern "C" int cc1b_set(float **const y, float **const x1, float **const x2, con
                         const unsigned int Tr, const int Lag1, const int Lag2)
 {
     return pair_set_host(FPCC_CC1B, y, x1, x2, N, Tr, Lag1, Lag2, 2.0);
+}
+
+extern "C" int pcc1_set(float **const y, float **const x1, float **const x2, const int N, const unsigned int Tr,
+                        const int Lag1, const int Lag2)
+{
+    return pcc_time_host(1, y, x1, x2, N, Tr, 1.0, Lag1, Lag2, 0, nullptr, true);
+}
+
+extern "C" int pcc_set(float **const y, float **const x1, float **const x2, const int N, const unsigned int Tr,
+                       const double v, const int Lag1, const int Lag2)
+{
+    return pcc_time_host(0, y, x1, x2, N, Tr, v, Lag1, Lag2, 0, nullptr, true);
+}
+
+// ---- reference GPU-engine entries (src/ccs_cuda.h:8-19) ----
+
+extern "C" int pcc1_highlevel(float **y, _Complex float **x1, _Complex float **x2, int N, unsigned int Tr, int Lag1,
+                              int Lag2, sem_t *anok)
+{
+    return pcc_time_host(1, y, (const float *const *)x1, (const float *const *)x2, N, Tr, 1.0, Lag1, Lag2, 1, anok,
+                         false);
+}
+
+extern "C" int pcc_highlevel(float **y, _Complex float **x1, _Complex float **x2, int N, unsigned int Tr, float v,
+                             int Lag1, int Lag2, sem_t *anok)
+{
+    return pcc_time_host(0, y, (const float *const *)x1, (const float *const *)x2, N, Tr, (double)v, Lag1, Lag2, 1,
+                         anok, false);
+}
+
+extern "C" void pcc1_highlevel2(float **y, float **x1, float **x2, int N, unsigned int Tr, int Lag1, int Lag2)
+{
+    int r = pcc_time_host(1, y, x1, x2, N, Tr, 1.0, Lag1, Lag2, 0, nullptr, false);
+    if (r) fprintf(stderr, "fastpcc: pcc1_highlevel2 failed (%d)\n", r);
 }

@@ -8,6 +8,15 @@ namespace fpcc {
 // PCC2 station stage: F[s*W+w][Nz] = FFT_Nz(pad(AmpNormf(analytic_N(x_sw))))
 int launch_station_pcc(const float *d_x, long long station_stride, int S, int W, int N, int Nz, cf32 *d_F,
                        hipStream_t st);
+// AmpNormf in place on `rows` rows of N complex values.
+int launch_ampnormf_rows(cf32 *d_z, int rows, int N, hipStream_t st);
+// Phasors only (analytic signal + AmpNormf), phi[s*W+w][N].
+int launch_station_phasor(const float *d_x, long long station_stride, int S, int W, int N, cf32 *d_phi,
+                          hipStream_t st);
+// Time-domain PCC1 (pcc1 = 1) or PCC-v over lags lag_first .. lag_first+L-1;
+// output element l of row r at y[r*out_ld + out_off + l] (rows: pair*W+w or pair).
+int launch_pcc_time(const cf32 *d_phi, int S, int W, int N, int lag_first, int L, double v, int pcc1, int mode,
+                    float *d_y, int out_ld, int out_off, hipStream_t st);
 // PCC2 pair stage over pairs [pair_base, pair_base+npairs) of S stations.
 // mode 0: y[p][w][L]; 1: mean over windows of per-window IFFTs; 2: IFFT of the
 // window sum (same result by linearity).

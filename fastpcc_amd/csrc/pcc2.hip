@@ -23,7 +23,7 @@ namespace fpcc {
 __global__ void __launch_bounds__(512)
 k_station_pcc(const float *__restrict__ x, long long station_stride, int W, int N, int Nz, FftPlan planN,
               const cf32 *__restrict__ twN, FftPlan planZ, const cf32 *__restrict__ twZ, float da1,
-              cf32 *__restrict__ F)
+              cf32 *__restrict__ F, cf32 *__restrict__ phi)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cf32 *buf0 = reinterpret_cast<cf32 *>(smem_raw);
@@ -61,6 +61,15 @@ k_station_pcc(const float *__restrict__ x, long long station_stride, int W, int 
     }
     m = block_max(m, red);
     const float eps = (float)(1e-6 * sqrt((double)m));
+    if (phi) {  // phasor-only output (time-domain PCC1 / PCC-v)
+        cf32 *o = phi + (long long)sw * N;
+        for (int n = threadIdx.x; n < N; n += T) {
+            cf32 v = z[n];
+            float r = 1.0f / (sqrtf(v.x * v.x + v.y * v.y) + eps);
+            o[n] = __builtin_isnormal(r) ? cf32{v.x * r, v.y * r} : cf32{0.0f, 0.0f};
+        }
+        return;
+    }
     cf32 *dst = (z == buf0) ? buf1 : buf0;
     for (int n = threadIdx.x; n < Nz; n += T) {
         cf32 o = cf32{0.0f, 0.0f};
@@ -75,6 +84,25 @@ k_station_pcc(const float *__restrict__ x, long long station_stride, int W, int 
     cf32 *Fz = fft_lds<float, -1>(dst, z, planZ, twZ);
     cf32 *out = F + (long long)sw * Nz;
     for (int k = threadIdx.x; k < Nz; k += T) out[k] = Fz[k];
+}
+
+// AmpNormf (src/FFTapps.c:178-205) in place on rows of N complex values.
+__global__ void __launch_bounds__(256) k_ampnormf_rows(cf32 *__restrict__ z, int N)
+{
+    __shared__ float red[4];
+    cf32 *row = z + (long long)blockIdx.x * N;
+    float m = 0.0f;
+    for (int n = threadIdx.x; n < N; n += blockDim.x) {
+        cf32 v = row[n];
+        m = fmaxf(m, v.x * v.x + v.y * v.y);
+    }
+    m = block_max(m, red);
+    const float eps = (float)(1e-6 * sqrt((double)m));
+    for (int n = threadIdx.x; n < N; n += blockDim.x) {
+        cf32 v = row[n];
+        float r = 1.0f / (sqrtf(v.x * v.x + v.y * v.y) + eps);
+        row[n] = __builtin_isnormal(r) ? cf32{v.x * r, v.y * r} : cf32{0.0f, 0.0f};
+    }
 }
 
 __device__ __forceinline__ void pair_of(int p, int S, int &i, int &j)
@@ -205,7 +233,26 @@ int launch_station_pcc(const float *d_x, long long station_stride, int S, int W,
     allow_lds((const void *)k_station_pcc, lds);
     ProfScope prof("station_pcc", st);
     hipLaunchKernelGGL(k_station_pcc, dim3(S * W), dim3(threads), lds, st, d_x, station_stride, W, N, Nz,
-                       pN.plan, (const cf32 *)pN.tw, pZ.plan, (const cf32 *)pZ.tw, da1, d_F);
+                       pN.plan, (const cf32 *)pN.tw, pZ.plan, (const cf32 *)pZ.tw, da1, d_F, (cf32 *)nullptr);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_station_phasor(const float *d_x, long long station_stride, int S, int W, int N, cf32 *d_phi,
+                          hipStream_t st)
+{
+    const DevPlan &pN = get_plan(N, false);
+    if (!pN.ok || N > 8192) {
+        fprintf(stderr, "fastpcc: unsupported analytic-signal length N=%d\n", N);
+        return -1;
+    }
+    const float da1 = (float)(1 / (double)N);
+    const int threads = 512;
+    size_t lds = sizeof(cf32) * 2 * (size_t)N + sizeof(float) * 16;
+    allow_lds((const void *)k_station_pcc, lds);
+    ProfScope prof("station_phasor", st);
+    hipLaunchKernelGGL(k_station_pcc, dim3(S * W), dim3(threads), lds, st, d_x, station_stride, W, N, N, pN.plan,
+                       (const cf32 *)pN.tw, pN.plan, (const cf32 *)pN.tw, da1, (cf32 *)nullptr, d_phi);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }
@@ -230,4 +277,15 @@ int launch_pair_pcc2(const cf32 *d_F, int S, int W, int N, int Nz, int lag0, int
     return 0;
 }
 
+}  // namespace fpcc
+
+namespace fpcc {
+int launch_ampnormf_rows(cf32 *d_z, int rows, int N, hipStream_t st)
+{
+    if (rows <= 0) return 0;
+    ProfScope prof("ampnormf_rows", st);
+    hipLaunchKernelGGL(k_ampnormf_rows, dim3(rows), dim3(256), 0, st, d_z, N);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
 }  // namespace fpcc

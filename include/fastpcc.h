@@ -24,6 +24,7 @@
 #define FASTPCC_H
 
 #include <stddef.h>
+#include <semaphore.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -46,6 +47,36 @@ int ccgn_set(float **const y, float **const x1, float **const x2, const unsigned
 int cc1b_set(float **const y, float **const x1, float **const x2, const unsigned int N,
              const unsigned int Tr, const int Lag1, const int Lag2);
 
+/* PCC of power 1, replaces pcc1_set, src/FFTapps.c:611-783 (CPU branch):
+ * analytic signal + AmpNormf in FP32, then the time-domain sum of
+ * |phi2[n+lag]+phi1[n]| - |phi2[n+lag]-phi1[n]| over the overlap / (2 (N-|lag|)).
+ * Lag handling as the reference: returns 0 without output if Lag1 > N or
+ * Lag2 < -N, -1 if Lag2 < Lag1, rows zeroed outside |lag| <= N. */
+int pcc1_set(float **const y, float **const x1, float **const x2, const int N, const unsigned int Tr,
+             const int Lag1, const int Lag2);
+/* PCC of power v, replaces pcc_set, src/FFTapps.c:475-608 (CPU branch): terms
+ * |.|^v, divided by 2^(v/2) (N-|lag|).  Lag handling as pcc1_set. */
+int pcc_set(float **const y, float **const x1, float **const x2, const int N, const unsigned int Tr,
+            const double v, const int Lag1, const int Lag2);
+
+/* ---------------------------------------------------------------------------
+ * Reference GPU-engine entry points (src/ccs_cuda.h:8-19).  x1/x2 of the
+ * *_highlevel pair are the un-normalised analytic signals (Tr rows of N complex
+ * values) that the reference's CPU producer computes; when anok is not NULL,
+ * anok[tr] is waited on (sem_wait) before row tr is read, in trace order, as the
+ * reference's consumer does.  The phasor regulariser and the PCC-v divisor follow
+ * the reference's CPU path (AmpNormf with 1e-6 max|z|; 2^(v/2) (N-|lag|)), not
+ * its CUDA kernels, so every entry gives the same numbers as pcc1_set/pcc_set.
+ * Only lags l in [l1, L') are written (rows are not zeroed), as the reference.
+ * ------------------------------------------------------------------------- */
+int pcc1_highlevel(float **y, _Complex float **x1, _Complex float **x2, int N, unsigned int Tr, int Lag1, int Lag2,
+                   sem_t *anok);
+int pcc_highlevel(float **y, _Complex float **x1, _Complex float **x2, int N, unsigned int Tr, float v, int Lag1,
+                  int Lag2, sem_t *anok);
+/* Real inputs, PCC1 end to end on the GPU; never called by the reference
+ * (src/FFTapps.c:745 sits in a dead branch).  Errors are printed. */
+void pcc1_highlevel2(float **y, float **x1, float **x2, int N, unsigned int Tr, int Lag1, int Lag2);
+
 /* ---------------------------------------------------------------------------
  * Batched all-pairs entry (no counterpart in the reference, which correlates one
  * pair per process run; src/PCC_fullpair_1b.c:467-499).
@@ -56,7 +87,8 @@ int cc1b_set(float **const y, float **const x1, float **const x2, const unsigned
  * order, p = i*S - i*(i+1)/2 + (j-i-1); station i is x1 (virtual source), j is x2.
  * Output y: stack == FPCC_STACK_NONE -> [P][n_windows][L], else [P][L] holding
  * the arithmetic mean over windows.  Stack modes per kind: PCC2 all; GNCC NONE,
- * PLAIN (its per-window energies forbid stacking spectra); 1-bit NONE, PLAIN, GRAM.
+ * PLAIN (its per-window energies forbid stacking spectra); 1-bit NONE, PLAIN, GRAM;
+ * PCC1 and PCC-v NONE, PLAIN (time-domain sums, N <= 8192).
  * ------------------------------------------------------------------------- */
 enum { FPCC_PCC2 = 0, FPCC_PCC1 = 1, FPCC_PCCV = 2, FPCC_GNCC = 3, FPCC_CC1B = 4, FPCC_WPCC = 5 };
 enum {
