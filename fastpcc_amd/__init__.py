@@ -5,5 +5,6 @@ include/fastpcc.h); this package loads it with ctypes and mirrors the reference'
 correlation interface on numpy arrays (api.py).
 """
 from .api import (FastPCCError, DeviceRunner, allpairs, make_job, output_shape, pcc2_set, ccgn_set,  # noqa: F401
-                  cc1b_set, pcc1_set, pcc_set, pcc1_highlevel, pcc_highlevel)
+                  cc1b_set, pcc1_set, pcc_set, pcc1_highlevel, pcc_highlevel, tspcc2_set,
+                  cuda_tspcc2_set_freq)
 from . import synth  # noqa: F401

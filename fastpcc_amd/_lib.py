@@ -58,6 +58,12 @@ def lib():
     L.pcc_highlevel.restype = i
     L.pcc1_highlevel2.argtypes = [fpp, fpp, fpp, i, u, i, i]
     L.pcc1_highlevel2.restype = None
+    L.tspcc2_set.argtypes = [fpp, fpp, fpp, u, u, i, i, d, d, u, i, d]
+    L.tspcc2_set.restype = i
+    L.cuda_tspcc2_set_freq.argtypes = [fpp, fpp, fpp, u, u, u, i, i, u, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_uint),
+                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint), ctypes.c_float]
+    L.cuda_tspcc2_set_freq.restype = i
     jp = ctypes.POINTER(FpccJob)
     L.fpcc_output_len.argtypes = [jp]
     L.fpcc_output_len.restype = ctypes.c_size_t

@@ -114,6 +114,51 @@ def _time_set(name, x1, x2, lag1, lag2, v=None, complex_in=False, y=None):
     return y
 
 
+def tspcc2_set(x1, x2, lag1, lag2, pmin, pmax, V, wtype=-1, op1=0.0):
+    """Wavelet PCC of one station pair over Tr windows (src/FFTapps.c:1076-1255);
+    pmin/pmax in samples.  Rows hold abs(lag2-lag1+1) lags from min(lag1, lag2)."""
+    x1 = np.atleast_2d(x1)
+    x2 = np.atleast_2d(x2)
+    Tr, N = x1.shape
+    L = abs(lag2 - lag1 + 1)
+    y = np.zeros((Tr, L), dtype=np.float32)
+    _, py = _rows(y)
+    a1, p1 = _rows(x1)
+    a2, p2 = _rows(x2)
+    r = _lib.lib().tspcc2_set(py, p1, p2, N, Tr, lag1, lag2, pmin, pmax, V, wtype, op1)
+    if r != 0:
+        raise FastPCCError(r, "tspcc2_set")
+    return y
+
+
+def cuda_tspcc2_set_freq(x1, x2, Nz, lag1, lag2, scale, wc, Ls, center, C, down_smp=None):
+    """The reference GPU engine's WPCC entry (src/ccs_cuda.h:12) with a caller-built
+    filter bank: scale[S], wavelets wc (list of complex128 arrays), Ls[S], center[S]."""
+    x1 = np.atleast_2d(x1)
+    x2 = np.atleast_2d(x2)
+    Tr, N = x1.shape
+    L = abs(lag2 - lag1) + 1
+    y = np.zeros((Tr, L), dtype=np.float32)
+    _, py = _rows(y)
+    a1, p1 = _rows(x1)
+    a2, p2 = _rows(x2)
+    S = len(scale)
+    sc = np.ascontiguousarray(scale, dtype=np.float64)
+    waves = [np.ascontiguousarray(w, dtype=np.complex128) for w in wc]
+    wp = (ctypes.c_void_p * S)(*[w.ctypes.data for w in waves])
+    ls = np.ascontiguousarray(Ls, dtype=np.uint32)
+    ce = np.ascontiguousarray(center, dtype=np.int32)
+    ds = np.ascontiguousarray(np.ones(S) if down_smp is None else down_smp, dtype=np.uint32)
+    r = _lib.lib().cuda_tspcc2_set_freq(py, p1, p2, N, Tr, Nz, lag1, lag2, S,
+                                        sc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), wp,
+                                        ls.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
+                                        ce.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                        ds.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), C)
+    if r != 0:
+        raise FastPCCError(r, "cuda_tspcc2_set_freq")
+    return y
+
+
 def cc1b_set(x1, x2, lag1, lag2):
     """1-bit GNCC of one station pair over Tr windows (src/FFTapps.c:980-1073)."""
     return _pair_set("cc1b_set", x1, x2, lag1, lag2)

@@ -8,6 +8,7 @@
 #include "../../include/fastpcc.h"
 #include "engine.hpp"
 #include "kernels.hpp"
+#include "wavelet.hpp"
 
 using namespace fpcc;
 
@@ -35,6 +36,7 @@ struct Dims {
     int S, W, N, L, lag0;
     unsigned M, Nz;
     size_t P;
+    const WFamily *fam;  // WPCC only
 };
 
 bool stack_supported(int kind, int stack)
@@ -45,6 +47,7 @@ bool stack_supported(int kind, int stack)
     case FPCC_CC1B: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN || stack == FPCC_STACK_GRAM;
     case FPCC_PCC1:
     case FPCC_PCCV: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN;
+    case FPCC_WPCC: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN || stack == FPCC_STACK_GRAM;
     default: return false;
     }
 }
@@ -67,6 +70,13 @@ int resolve(const fpcc_job *job, Dims &d)
     d.M = ua1 > ua2 ? ua1 : ua2;
     d.Nz = nz_rule(N, d.M);
     d.P = (size_t)d.S * (d.S - 1) / 2;
+    d.fam = nullptr;
+    if (job->kind == FPCC_WPCC) {
+        int err = 0;
+        d.fam = get_wfamily(N, job->lag1, job->lag2, job->pmin, job->pmax, job->V, job->wtype, job->op1, &err);
+        if (!d.fam) return err ? err : -1;
+        d.Nz = d.fam->Nz;
+    }
     return 0;
 }
 
@@ -91,7 +101,7 @@ Layout layout(const fpcc_job *job, const Dims &d)
         l.a = take(l.total, sizeof(cf32) * SW * d.Nz);  // spectra
         if (job->stack == FPCC_STACK_GRAM) {
             l.b = take(l.total, sizeof(cf32) * K1 * SW * 2);                  // U
-            l.c = take(l.total, sizeof(cf32) * d.P * (size_t)gram_ld(d.Nz));  // G
+            l.c = take(l.total, sizeof(cf32) * d.P * (size_t)gram_ld(d.Nz / 2 + 1));  // G
         }
         break;
     case FPCC_GNCC:
@@ -104,12 +114,15 @@ Layout layout(const fpcc_job *job, const Dims &d)
         if (job->stack == FPCC_STACK_GRAM) {
             l.c = take(l.total, sizeof(cf32) * K1 * SW);
             l.d = take(l.total, sizeof(cf32) * K1 * SW);
-            l.e = take(l.total, sizeof(cf32) * d.P * (size_t)gram_ld(d.Nz));
+            l.e = take(l.total, sizeof(cf32) * d.P * (size_t)gram_ld(d.Nz / 2 + 1));
         }
         break;
     case FPCC_PCC1:
     case FPCC_PCCV:
         l.a = take(l.total, sizeof(cf32) * SW * (size_t)d.N);  // phasors
+        break;
+    case FPCC_WPCC:
+        l.a = take(l.total, wpcc_workspace(d.S, d.W, d.N, (int)d.Nz, (int)d.fam->S));
         break;
     }
     return l;
@@ -130,6 +143,7 @@ int station_stage(const fpcc_job *job, const Dims &d, const float *x, char *ws, 
     }
     case FPCC_PCC1:
     case FPCC_PCCV: return launch_station_phasor(x, stride, d.S, d.W, d.N, (cf32 *)(ws + l.a), st);
+    case FPCC_WPCC: return wpcc_station_stage(x, stride, d.S, d.W, d.N, *d.fam, ws + l.a, st);
     }
     return -1;
 }
@@ -146,8 +160,10 @@ int pair_stage(const fpcc_job *job, const Dims &d, char *ws, float *y, hipStream
             const int2 *tiles = gram_tiles(d.S, ntiles);
             if (!tiles) return -1;
             cf32 *F = (cf32 *)(ws + l.a), *U = (cf32 *)(ws + l.b), *G = (cf32 *)(ws + l.c);
-            if ((r = launch_spectra_to_u(F, d.Nz, d.S, d.W, d.Nz, 1, U, st))) return r;
-            if ((r = launch_gram_f32(U, U, d.S, 2 * d.W, d.Nz, tiles, ntiles, G, st))) return r;
+            const int nk = d.Nz / 2 + 1;
+            if ((r = launch_spectra_to_u(F, d.Nz, d.S, d.W, d.Nz, nk, 1, U, st))) return r;
+            if ((r = launch_gram_f32(U, U, d.S, 2 * d.W, 0, 2 * d.W, nk, tiles, ntiles, G, gram_ld(nk), st)))
+                return r;
             // 1/2 of the Gram's Hermitian fold, the reference's 1/((float)Nz*(float)N), the mean over W
             const float scale = 0.5f * (1 / ((float)d.Nz * (float)d.N)) / (float)d.W;
             return launch_pair_c2r(G, (int)d.P, d.Nz, d.N, d.W, d.lag0, d.L, scale, 0, y, st);
@@ -164,9 +180,10 @@ int pair_stage(const fpcc_job *job, const Dims &d, char *ws, float *y, hipStream
             if (!tiles) return -1;
             const long long ld = real_ld(d.Nz);
             cf32 *Ua = (cf32 *)(ws + l.c), *Ub = (cf32 *)(ws + l.d), *G = (cf32 *)(ws + l.e);
-            if ((r = launch_spectra_to_u((const cf32 *)(ws + l.a), ld, d.S, d.W, d.Nz, 0, Ua, st))) return r;
-            if ((r = launch_spectra_to_u((const cf32 *)(ws + l.b), ld, d.S, d.W, d.Nz, 0, Ub, st))) return r;
-            if ((r = launch_gram_f32(Ua, Ub, d.S, d.W, d.Nz, tiles, ntiles, G, st))) return r;
+            const int nk = d.Nz / 2 + 1;
+            if ((r = launch_spectra_to_u((const cf32 *)(ws + l.a), ld, d.S, d.W, d.Nz, nk, 0, Ua, st))) return r;
+            if ((r = launch_spectra_to_u((const cf32 *)(ws + l.b), ld, d.S, d.W, d.Nz, nk, 0, Ub, st))) return r;
+            if ((r = launch_gram_f32(Ua, Ub, d.S, d.W, 0, d.W, nk, tiles, ntiles, G, gram_ld(nk), st))) return r;
             // ccf_lowlevel's 1/Nz, the mean over W, then 1/(N-|lag|) (gnf_lowlevel with +-1 data)
             const float scale = (float)(1.0 / (double)d.Nz) / (float)d.W;
             return launch_pair_c2r(G, (int)d.P, d.Nz, d.N, d.W, d.lag0, d.L, scale, 1, y, st);
@@ -177,8 +194,39 @@ int pair_stage(const fpcc_job *job, const Dims &d, char *ws, float *y, hipStream
     case FPCC_PCCV:
         return launch_pcc_time((const cf32 *)(ws + l.a), d.S, d.W, d.N, d.lag0, d.L,
                                job->kind == FPCC_PCC1 ? 1.0 : job->v, job->kind == FPCC_PCC1, mode, y, d.L, 0, st);
+    case FPCC_WPCC: return wpcc_pair_stage(d.S, d.W, d.N, *d.fam, d.lag0, d.L, mode == 0, ws + l.a, y, st);
     }
     return -1;
+}
+
+// WPCC of one pair over Tr windows with a given family (tspcc2_set and
+// cuda_tspcc2_set_freq): x1 rows station 0, x2 rows station 1, L lags from lag0.
+int wpcc_pair_host(const WFamily &f, float **y, float **x1, float **x2, unsigned N, unsigned Tr, int lag0, int L)
+{
+    if (Tr == 0 || L <= 0) return 0;
+    if (!y || !x1 || !x2) return -1;
+    const size_t xlen = (size_t)2 * Tr * N, ylen = (size_t)Tr * L;
+    const size_t wsb = wpcc_workspace(2, (int)Tr, (int)N, (int)f.Nz, (int)f.S);
+    HostBuf hx, hy;
+    DeviceBuf dx, dy, dws;
+    FPCC_CHECK(hipHostMalloc(&hx.p, sizeof(float) * xlen, hipHostMallocDefault));
+    FPCC_CHECK(hipHostMalloc(&hy.p, sizeof(float) * ylen, hipHostMallocDefault));
+    for (unsigned tr = 0; tr < Tr; ++tr) {
+        if (!x1[tr] || !x2[tr] || !y[tr]) return -1;
+        memcpy((float *)hx.p + (size_t)tr * N, x1[tr], sizeof(float) * N);
+        memcpy((float *)hx.p + (size_t)(Tr + tr) * N, x2[tr], sizeof(float) * N);
+    }
+    FPCC_CHECK(hipMalloc(&dx.p, sizeof(float) * xlen));
+    FPCC_CHECK(hipMalloc(&dy.p, sizeof(float) * ylen));
+    FPCC_CHECK(hipMalloc(&dws.p, wsb));
+    FPCC_CHECK(hipMemcpy(dx.p, hx.p, sizeof(float) * xlen, hipMemcpyHostToDevice));
+    int r = wpcc_station_stage((const float *)dx.p, (long long)Tr * N, 2, (int)Tr, (int)N, f, (char *)dws.p, nullptr);
+    if (r) return r;
+    if ((r = wpcc_pair_stage(2, (int)Tr, (int)N, f, lag0, L, true, (char *)dws.p, (float *)dy.p, nullptr)))
+        return r;
+    FPCC_CHECK(hipMemcpy(hy.p, dy.p, sizeof(float) * ylen, hipMemcpyDeviceToHost));
+    for (unsigned tr = 0; tr < Tr; ++tr) memcpy(y[tr], (float *)hy.p + (size_t)tr * L, sizeof(float) * L);
+    return 0;
 }
 
 // PCC1 / PCC-v for one pair over Tr windows (pcc1_set, src/FFTapps.c:611-783;
@@ -394,4 +442,57 @@ extern "C" void pcc1_highlevel2(float **y, float **x1, float **x2, int N, unsign
 {
     int r = pcc_time_host(1, y, x1, x2, N, Tr, 1.0, Lag1, Lag2, 0, nullptr, false);
     if (r) fprintf(stderr, "fastpcc: pcc1_highlevel2 failed (%d)\n", r);
+}
+
+// WPCC, replaces tspcc2_set (src/FFTapps.c:1076-1255).  L = abs(Lag2-Lag1+1) and
+// rows are zeroed before the family is built, as the reference.
+extern "C" int tspcc2_set(float **const y, float **const x1, float **const x2, const unsigned int N,
+                          const unsigned int Tr, const int Lag1, const int Lag2, double pmin, double pmax,
+                          unsigned int V, int type, double op1)
+{
+    const int L = abs(Lag2 - Lag1 + 1);
+    const unsigned ua1 = (unsigned)abs(Lag1), ua2 = (unsigned)abs(Lag2);
+    if (ua1 >= N || ua2 >= N) return -3;
+    if (y)
+        for (unsigned tr = 0; tr < Tr; ++tr)
+            if (y[tr]) memset(y[tr], 0, sizeof(float) * L);
+    int err = 0;
+    const WFamily *f = get_wfamily(N, Lag1, Lag2, pmin, pmax, V, type, op1, &err);
+    if (!f) return err ? err : -1;
+    const int lag0 = Lag2 >= Lag1 ? Lag1 : Lag2;
+    return wpcc_pair_host(*f, y, x1, x2, N, Tr, lag0, L);
+}
+
+// Reference GPU WPCC entry (src/ccs_cuda.h:12, src/ccs_cuda.cu:898-1036): the
+// caller supplies Nz, the S wavelets (FP64, length Ls[s], origin center[s]) and
+// C = 1/(K0 N).  Down_smp must be 1 (-2 otherwise).  Phasors follow the CPU
+// regulariser z / sqrt(|z|^2 + 1e-6 max|z|^2), not the CUDA threshold.
+extern "C" int cuda_tspcc2_set_freq(float **y0, float **x01, float **x02, unsigned int N, unsigned int Tr,
+                                    unsigned int Nz, const int Lag1, const int Lag2, unsigned int S, double *scale,
+                                    _Complex double **wc, unsigned int *Ls, int *center, unsigned int *Down_smp,
+                                    float C)
+{
+    if (!scale || !wc || !Ls || !center || !Down_smp || S == 0) return -1;
+    for (unsigned s = 0; s < S; ++s)
+        if (Down_smp[s] != 1) return -2;
+    if (Nz == 0 || (Nz & (Nz - 1)) || !(C > 0)) return -1;
+    WFamily f;
+    f.type = 0;
+    f.S = S;
+    f.Nz = Nz;
+    f.Ls0 = Ls[S - 1];
+    f.K0 = 1.0 / ((double)C * (double)N);
+    f.scale.assign(scale, scale + S);
+    f.Ls.assign(Ls, Ls + S);
+    f.center.assign(center, center + S);
+    f.wc.resize(S);
+    for (unsigned s = 0; s < S; ++s) {
+        if (!wc[s] || Ls[s] > Nz || center[s] < 0 || (unsigned)center[s] > Ls[s]) return -1;
+        const double *p = (const double *)wc[s];
+        f.wc[s].resize(Ls[s]);
+        for (unsigned u = 0; u < Ls[s]; ++u) f.wc[s][u] = std::complex<double>(p[2 * u], p[2 * u + 1]);
+    }
+    const int Lags = Lag2 >= Lag1 ? Lag2 - Lag1 + 1 : Lag1 - Lag2 + 1;
+    const int lag0 = Lag2 >= Lag1 ? Lag1 : Lag2;
+    return wpcc_pair_host(f, y0, x01, x02, N, Tr, lag0, Lags);
 }

@@ -32,6 +32,8 @@ const DevPlan &get_plan(int n, bool f64);
 // Table of exp(+2*pi*i*k/nz), k in [0, nz/2), for the half-length real-output
 // inverse (cached per device and precision).
 const void *get_c2r_twiddles(int nz, bool f64);
+// Table of exp(-2*pi*i*k/n), k in [0, n) (cached per device and precision).
+const void *get_full_twiddles(int n, bool f64);
 // Radix list for n (host); returns false if unsupported.
 bool factor_plan(int n, FftPlan &p);
 

@@ -262,6 +262,61 @@ __device__ C2<T> *fft_lds(C2<T> *buf0, C2<T> *buf1, const FftPlan &plan, const C
     return in;
 }
 
+// ---- batched variant: nb transforms of length plan.n, transform t at t*ld ----
+template <typename T, int R, int DIR>
+__device__ __forceinline__ void stockham_pass_batch(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int n,
+                                                    int Ns, const C2<T> *__restrict__ tw, int nbatch, int ld)
+{
+    const int nb = n / R;
+    const int total = nb * nbatch;
+    for (int bb = threadIdx.x; bb < total; bb += blockDim.x) {
+        const int t = bb / nb, b = bb - t * nb;
+        const C2<T> *src = in + t * ld;
+        C2<T> *dst = out + t * ld;
+        C2<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = src[b + r * nb];
+        const int k = b % Ns;
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                C2<T> w = tw[(r - 1) * Ns + k];
+                if (DIR > 0) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        Dft<R, DIR>::run(v);
+        const int base = (b - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[r];
+    }
+}
+
+template <typename T, int DIR>
+__device__ C2<T> *fft_lds_batch(C2<T> *buf0, C2<T> *buf1, const FftPlan &plan, const C2<T> *__restrict__ tw,
+                                int nbatch, int ld)
+{
+    C2<T> *in = buf0, *out = buf1;
+    int Ns = 1;
+    for (int p = 0; p < plan.npass; ++p) {
+        const int R = plan.radix[p];
+        const C2<T> *twp = tw + plan.tw_off[p];
+        switch (R) {
+        case 2: stockham_pass_batch<T, 2, DIR>(in, out, plan.n, Ns, twp, nbatch, ld); break;
+        case 4: stockham_pass_batch<T, 4, DIR>(in, out, plan.n, Ns, twp, nbatch, ld); break;
+        case 8: stockham_pass_batch<T, 8, DIR>(in, out, plan.n, Ns, twp, nbatch, ld); break;
+        case 16: stockham_pass_batch<T, 16, DIR>(in, out, plan.n, Ns, twp, nbatch, ld); break;
+        default: break;  // large transforms are powers of two
+        }
+        __syncthreads();
+        C2<T> *t = in;
+        in = out;
+        out = t;
+        Ns *= R;
+    }
+    return in;
+}
+
 // ---- workgroup reductions (wave64) ----
 template <typename T>
 __device__ __forceinline__ T wave_max(T v)

@@ -121,4 +121,26 @@ const void *get_c2r_twiddles(int nz, bool f64)
     return d;
 }
 
+const void *get_full_twiddles(int n, bool f64)
+{
+    static std::map<std::tuple<int, int, bool>, const void *> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto key = std::make_tuple(dev, n, f64);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    std::vector<double> t;
+    t.reserve(2 * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+        double c, s;
+        tw_fwd(k, n, c, s);
+        t.push_back(c);
+        t.push_back(s);
+    }
+    const void *d = f64 ? upload_table<double>(t) : upload_table<float>(t);
+    cache.emplace(key, d);
+    return d;
+}
+
 }  // namespace fpcc

@@ -32,12 +32,11 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // conjugated mirror frequency in the second half; mirror = 0 (half spectra of real
 // signals, ldF >= Nz/2+1 bins per row): m in [0, W).
 __global__ void __launch_bounds__(256)
-k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz, int mirror,
+k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz, int nk, int mirror,
                cf32 *__restrict__ U)
 {
     __shared__ cf32 td[64][33];
     __shared__ cf32 tm[64][33];
-    const int M = Nz >> 1;
     const int k0 = blockIdx.x * 32;
     const int sw0 = blockIdx.y * 64;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
@@ -45,7 +44,7 @@ k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz,
         const int sw = sw0 + r;
         const int k = k0 + tx;
         cf32 d = cf32{0.f, 0.f}, m = cf32{0.f, 0.f};
-        if (sw < SW && k <= M) {
+        if (sw < SW && k < nk) {
             d = F[(long long)sw * ldF + k];
             if (mirror) {
                 cf32 mm = F[(long long)sw * ldF + ((Nz - k) & (Nz - 1))];
@@ -62,7 +61,7 @@ k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz,
     for (int idx = threadIdx.x; idx < 32 * 64; idx += 256) {
         const int kk = idx >> 6, r = idx & 63;
         const int k = k0 + kk, sw = sw0 + r;
-        if (k > M || sw >= SW) continue;
+        if (k >= nk || sw >= SW) continue;
         const int s = sw / W, w = sw - s * W;
         cf32 *row = U + ((long long)k * S + s) * W2;
         row[w] = td[r][kk];
@@ -78,7 +77,7 @@ k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz,
 constexpr int kGramKc = 16;
 
 __global__ void __launch_bounds__(256)
-k_gram_f32(const cf32 *__restrict__ Ua, const cf32 *__restrict__ Ub, int S, int W2, int M,
+k_gram_f32(const cf32 *__restrict__ Ua, const cf32 *__restrict__ Ub, int S, int W2, int m0, int K, int nk,
            const int2 *__restrict__ tiles, int ntiles, int nchunks, cf32 *__restrict__ G, long long ldg)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -95,20 +94,20 @@ k_gram_f32(const cf32 *__restrict__ Ua, const cf32 *__restrict__ Ub, int S, int 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, q = lane >> 4;
     const int si = bi * 16 + li, sj0 = bj * 32 + li, sj1 = sj0 + 16;
-    const int nmm = (W2 + 3) / 4;  // K padded with zeros to a multiple of 4 complex values
+    const int nmm = (K + 3) / 4;  // K padded with zeros to a multiple of 4 complex values
 
     for (int t = 0; t < kGramKc / 4; ++t) {
         const int kl = wave + 4 * t;
         const int k = k0 + kl;
         f4v r0 = {0, 0, 0, 0}, i0 = {0, 0, 0, 0}, r1 = {0, 0, 0, 0}, i1 = {0, 0, 0, 0};
-        if (k <= M) {
-            const cf32 *pa = Ua + ((long long)k * S + si) * W2;
-            const cf32 *pb0 = Ub + ((long long)k * S + sj0) * W2;
-            const cf32 *pb1 = Ub + ((long long)k * S + sj1) * W2;
+        if (k < nk) {
+            const cf32 *pa = Ua + ((long long)k * S + si) * W2 + m0;
+            const cf32 *pb0 = Ub + ((long long)k * S + sj0) * W2 + m0;
+            const cf32 *pb1 = Ub + ((long long)k * S + sj1) * W2 + m0;
             const bool va = si < S, vb0 = sj0 < S, vb1 = sj1 < S;
             for (int mm = 0; mm < nmm; ++mm) {
                 const int m = 4 * mm + q;
-                const bool vm = m < W2;
+                const bool vm = m < K;
                 cf32 a = (va && vm) ? pa[m] : cf32{0.f, 0.f};
                 cf32 b0 = (vb0 && vm) ? pb0[m] : cf32{0.f, 0.f};
                 cf32 b1 = (vb1 && vm) ? pb1[m] : cf32{0.f, 0.f};
@@ -182,7 +181,7 @@ static void allow_lds(const void *fn, size_t bytes)
     if (bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-long long gram_ld(int Nz) { return ((long long)(Nz / 2 + 1) + kGramKc - 1) / kGramKc * kGramKc; }
+long long gram_ld(int nk) { return ((long long)nk + kGramKc - 1) / kGramKc * kGramKc; }
 
 // Station tiles (16 rows x 32 columns) holding at least one pair i < j, as a
 // device table cached per device and station count.
@@ -210,28 +209,28 @@ const int2 *gram_tiles(int S, int &ntiles)
     return (const int2 *)d;
 }
 
-int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, int mirror, cf32 *d_U,
+int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, int nk, int mirror, cf32 *d_U,
                         hipStream_t st)
 {
-    const int M = Nz / 2, SW = S * W;
-    dim3 grid((M + 1 + 31) / 32, (SW + 63) / 64);
+    const int SW = S * W;
+    dim3 grid((nk + 31) / 32, (SW + 63) / 64);
     ProfScope prof("spectra_to_u", st);
-    hipLaunchKernelGGL(k_spectra_to_u, grid, dim3(256), 0, st, d_F, ldF, SW, W, Nz, mirror, d_U);
+    hipLaunchKernelGGL(k_spectra_to_u, grid, dim3(256), 0, st, d_F, ldF, SW, W, Nz, nk, mirror, d_U);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }
 
-int launch_gram_f32(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int Nz, const int2 *d_tiles, int ntiles,
-                    cf32 *d_G, hipStream_t st)
+int launch_gram_f32(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int m0, int K, int nk, const int2 *d_tiles,
+                    int ntiles, cf32 *d_G, long long ldg, hipStream_t st)
 {
-    const int M = Nz / 2;
-    const int nchunks = (M + 1 + kGramKc - 1) / kGramKc;
+    const int nchunks = (nk + kGramKc - 1) / kGramKc;
+    if (ldg < (long long)nchunks * kGramKc) return -1;
     const int slots = (nchunks + 7) / 8 * ntiles;
     size_t lds = sizeof(cf32) * 16 * 32 * kGramKc;
     allow_lds((const void *)k_gram_f32, lds);
     ProfScope prof("gram_f32", st);
-    hipLaunchKernelGGL(k_gram_f32, dim3(slots * 8), dim3(256), lds, st, d_Ua, d_Ub, S, W2, M, d_tiles, ntiles,
-                       nchunks, d_G, gram_ld(Nz));
+    hipLaunchKernelGGL(k_gram_f32, dim3(slots * 8), dim3(256), lds, st, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles,
+                       ntiles, nchunks, d_G, ldg);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }
@@ -246,7 +245,7 @@ int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int 
     size_t lds = sizeof(cf32) * 2 * (size_t)M;
     allow_lds((const void *)k_pair_c2r, lds);
     ProfScope prof("pair_c2r", st);
-    hipLaunchKernelGGL(k_pair_c2r, dim3(P), dim3(512), lds, st, d_G, gram_ld(Nz), Nz, pM.plan,
+    hipLaunchKernelGGL(k_pair_c2r, dim3(P), dim3(512), lds, st, d_G, gram_ld(Nz / 2 + 1), Nz, pM.plan,
                        (const cf32 *)pM.tw, (const cf32 *)twc, lag0, L, scale, count_norm, N, d_y);
     FPCC_CHECK(hipGetLastError());
     return 0;

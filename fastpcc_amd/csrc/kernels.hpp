@@ -24,13 +24,16 @@ int launch_pair_pcc2(const cf32 *d_F, int S, int W, int N, int Nz, int lag0, int
                      int npairs, float *d_y, hipStream_t st);
 
 // Stack over windows as a per-frequency Gram (gram.hip).
-long long gram_ld(int Nz);                       // row length of G in bins
+long long gram_ld(int nk);                       // row length of G for nk bins
 const int2 *gram_tiles(int S, int &ntiles);      // cached device tile table
-int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, int mirror, cf32 *d_U,
+// U[k][s][m] for k < nk from rows F[s*W+w] of ldF bins; mirror 1 appends the
+// conjugated bin Nz-k as m = W+w.
+int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, int nk, int mirror, cf32 *d_U,
                         hipStream_t st);
-// G[p][k] = sum_{m < W2} conj(Ua[k][i][m]) Ub[k][j][m] for pairs i < j, k <= Nz/2
-int launch_gram_f32(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int Nz, const int2 *d_tiles, int ntiles,
-                    cf32 *d_G, hipStream_t st);
+// G[p][k] = sum_{m0 <= m < m0+K} conj(Ua[k][i][m]) Ub[k][j][m] for pairs i < j, k < nk
+// (U rows hold W2 values per station).
+int launch_gram_f32(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int m0, int K, int nk, const int2 *d_tiles,
+                    int ntiles, cf32 *d_G, long long ldg, hipStream_t st);
 // y[p][l] = scale * c2r(G[p])[(lag0+l) mod Nz], divided by N-|lag| if count_norm
 int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int L, float scale, int count_norm,
                     float *d_y, hipStream_t st);
@@ -45,5 +48,14 @@ int launch_station_real(const float *d_x, long long station_stride, int S, int W
 // mode 0: y[p][w][L]; 1: mean over windows.
 int launch_pair_gn(const void *d_Xa, const void *d_Xb, const double *d_E, bool f64, int S, int W, int N, int Nz,
                    int lag0, int L, int norm, int mode, float *d_y, hipStream_t st);
+
+// WPCC (wpcc.hip): workspace bytes and the two stages for S stations x W windows.
+struct WFamily;
+size_t wpcc_workspace(int S, int W, int N, int Nz, int nscales);
+int wpcc_station_stage(const float *d_x, long long station_stride, int S, int W, int N, const WFamily &f,
+                       char *ws, hipStream_t st);
+// per_window: y[p][w][L] (one Gram per window), else mean over windows y[p][L].
+int wpcc_pair_stage(int S, int W, int N, const WFamily &f, int lag0, int L, bool per_window, char *ws, float *d_y,
+                    hipStream_t st);
 
 }  // namespace fpcc

@@ -59,6 +59,15 @@ int pcc1_set(float **const y, float **const x1, float **const x2, const int N, c
 int pcc_set(float **const y, float **const x1, float **const x2, const int N, const unsigned int Tr,
             const double v, const int Lag1, const int Lag2);
 
+/* Wavelet phase cross-correlation, replaces tspcc2_set, src/FFTapps.c:1076-1255.
+ * pmin/pmax are periods in samples; type -1 Morlet, -2 complete Morlet, -3
+ * complex Mexican hat; op1 is the Morlet w0 (0 = pi sqrt(2/ln 2)).  The filter
+ * bank is built in FP64 as CreateWaveletFamily (src/FWTa/wavelet_def_v7.c:204-295);
+ * the FFTs of Nz points run in FP32 on the GPU.  Rows hold abs(Lag2-Lag1+1) lags
+ * starting at min(Lag1, Lag2) and are zeroed first; -3 for an unknown type. */
+int tspcc2_set(float **const y, float **const x1, float **const x2, const unsigned int N, const unsigned int Tr,
+               const int Lag1, const int Lag2, double pmin, double pmax, unsigned int V, int type, double op1);
+
 /* ---------------------------------------------------------------------------
  * Reference GPU-engine entry points (src/ccs_cuda.h:8-19).  x1/x2 of the
  * *_highlevel pair are the un-normalised analytic signals (Tr rows of N complex
@@ -76,6 +85,14 @@ int pcc_highlevel(float **y, _Complex float **x1, _Complex float **x2, int N, un
 /* Real inputs, PCC1 end to end on the GPU; never called by the reference
  * (src/FFTapps.c:745 sits in a dead branch).  Errors are printed. */
 void pcc1_highlevel2(float **y, float **x1, float **x2, int N, unsigned int Tr, int Lag1, int Lag2);
+/* WPCC with a caller-built filter bank (src/ccs_cuda.cu:898-1036): Nz (power of
+ * two), S wavelets wc[s] of Ls[s] FP64 complex samples with origin center[s],
+ * scales, and C = 1/(K0 N).  -2 if any Down_smp[s] != 1.  Writes |Lag2-Lag1|+1
+ * lags per window from min(Lag1, Lag2).  The phasor regulariser is the CPU one
+ * (z / sqrt(|z|^2 + 1e-6 max|z|^2)), not the reference CUDA threshold. */
+int cuda_tspcc2_set_freq(float **y0, float **x01, float **x02, unsigned int N, unsigned int Tr, unsigned int Nz,
+                         const int Lag1, const int Lag2, unsigned int S, double *scale, _Complex double **wc,
+                         unsigned int *Ls, int *center, unsigned int *Down_smp, float C);
 
 /* ---------------------------------------------------------------------------
  * Batched all-pairs entry (no counterpart in the reference, which correlates one
@@ -88,7 +105,8 @@ void pcc1_highlevel2(float **y, float **x1, float **x2, int N, unsigned int Tr, 
  * Output y: stack == FPCC_STACK_NONE -> [P][n_windows][L], else [P][L] holding
  * the arithmetic mean over windows.  Stack modes per kind: PCC2 all; GNCC NONE,
  * PLAIN (its per-window energies forbid stacking spectra); 1-bit NONE, PLAIN, GRAM;
- * PCC1 and PCC-v NONE, PLAIN (time-domain sums, N <= 8192).
+ * PCC1 and PCC-v NONE, PLAIN (time-domain sums, N <= 8192); WPCC NONE, PLAIN
+ * (= GRAM: the sum over scales and windows is one per-frequency Gram).
  * ------------------------------------------------------------------------- */
 enum { FPCC_PCC2 = 0, FPCC_PCC1 = 1, FPCC_PCCV = 2, FPCC_GNCC = 3, FPCC_CC1B = 4, FPCC_WPCC = 5 };
 enum {
