@@ -1,0 +1,362 @@
+// Two-pass (four-step) FFT kernels with fused pro- and epilogues; see bigfft.hpp.
+#include <cmath>
+#include <map>
+#include <mutex>
+#include "bigfft.hpp"
+#include "profile.hpp"
+
+namespace fpcc {
+
+// Split n = n1*n2 with n1 <= 512 (column pass in LDS), n2 <= 2048 (row pass),
+// both 7-smooth, n1 as close to sqrt(n) as allowed and n2 >= 16 when possible.
+static BigPlan make_big_plan(int n)
+{
+    BigPlan b;
+    if (n < 64) return b;
+    int best = 0;
+    double bestscore = 1e300;
+    for (int d = 4; d <= 512 && d <= n; ++d) {
+        if (n % d) continue;
+        const int e = n / d;
+        if (e > 2048) continue;
+        FftPlan t1, t2;
+        if (!factor_plan(d, t1) || !factor_plan(e, t2)) continue;
+        double score = std::fabs(std::log((double)d) - 0.5 * std::log((double)n));
+        if (e < kBigCols) score += 100;  // partial column tiles waste lanes
+        if (score < bestscore) {
+            bestscore = score;
+            best = d;
+        }
+    }
+    if (!best) return b;
+    b.n = n;
+    b.n1 = best;
+    b.n2 = n / best;
+    const DevPlan &a = get_plan(b.n1, false), &c = get_plan(b.n2, false);
+    if (!a.ok || !c.ok) return b;
+    b.p1 = a.plan;
+    b.p2 = c.plan;
+    b.tw1 = (const cf32 *)a.tw;
+    b.tw2 = (const cf32 *)c.tw;
+    b.twN = (const cf32 *)get_full_twiddles(n, false);
+    b.ok = b.twN != nullptr;
+    return b;
+}
+
+const BigPlan &big_plan(int n)
+{
+    static std::map<std::pair<int, int>, BigPlan> cache;
+    static std::mutex mu;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find({dev, n});
+    if (it != cache.end()) return it->second;
+    return cache.emplace(std::make_pair(dev, n), make_big_plan(n)).first->second;
+}
+
+__device__ __forceinline__ void pair_of_big(int p, int S, int &i, int &j)
+{
+    double b = 2.0 * S - 1.0;
+    int ii = (int)floor((b - sqrt(b * b - 8.0 * p)) * 0.5);
+    if (ii < 0) ii = 0;
+    while (ii > 0 && ii * S - ii * (ii + 1) / 2 > p) --ii;
+    while ((ii + 1) * S - (ii + 1) * (ii + 2) / 2 <= p) ++ii;
+    i = ii;
+    j = p - (ii * S - ii * (ii + 1) / 2) + ii + 1;
+}
+
+// index of W_n^{c r} in the full table: c < N2 and r < N1, so c*r < n
+__device__ __forceinline__ int twidx(int c, int r) { return c * r; }
+
+__global__ void __launch_bounds__(256)
+k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, const cf32 *__restrict__ twN)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int ld = N1 + 1;  // padded column stride (the loads write with stride ld)
+    cf32 *buf0 = reinterpret_cast<cf32 *>(smem_raw);
+    cf32 *buf1 = buf0 + kBigCols * ld;
+    float *red = reinterpret_cast<float *>(buf1 + kBigCols * ld);
+    const int b = blockIdx.y;
+    const int c0 = blockIdx.x * kBigCols;
+    const long long base = (long long)b * a.bstride;
+    const long long sbase = (long long)b * (a.src_bstride ? a.src_bstride : a.bstride);
+    const int n = N1 * N2;
+    const int op = a.op;
+    float eps = 0.f;
+    if (op == COL_FWD_AMPNORM) eps = 1e-6f * a.maxbuf[b];
+    if (op == COL_FWD_AMPNORMF) eps = (float)(1e-6 * sqrt((double)a.maxbuf[b]));
+    const float *xr = nullptr;
+    if (op == COL_FWD_INPUT) {
+        const int s = b / a.W, w = b - s * a.W;
+        xr = a.x + (long long)s * a.x_stride + (long long)w * a.N;
+    }
+    for (int idx = threadIdx.x; idx < kBigCols * N1; idx += blockDim.x) {
+        const int t = idx % kBigCols, r = idx / kBigCols;
+        const int c = c0 + t;
+        cf32 v = cf32{0.f, 0.f};
+        if (c < N2) {
+            const long long g = (long long)r * N2 + c;
+            if (op == COL_FWD_INPUT) {
+                const long long m = g - a.off;
+                if (m >= 0 && m < a.N) {
+                    const float xv = xr[m];
+                    float s;
+                    if (a.sign_rule == 1) s = xv >= 0.f ? 1.f : -1.f;
+                    else if (a.sign_rule == 2) s = xv > 0.f ? 1.f : -1.f;
+                    else s = xv * a.in_scale;
+                    v = cf32{s, 0.f};
+                }
+            } else if (op == COL_FWD_AMPNORM) {
+                const cf32 u = a.src[sbase + g];
+                const float rr = 1.0f / sqrtf(u.x * u.x + u.y * u.y + eps);
+                if (__builtin_isnormal(rr)) v = cf32{u.x * rr, u.y * rr};
+            } else if (op == COL_FWD_AMPNORMF) {
+                if (g < a.nvalid) {
+                    const cf32 u = a.src[sbase + g];
+                    const float rr = 1.0f / (sqrtf(u.x * u.x + u.y * u.y) + eps);
+                    if (__builtin_isnormal(rr)) v = cf32{u.x * rr, u.y * rr};
+                }
+            } else {
+                v = a.src[sbase + g];
+            }
+        }
+        buf0[t * ld + r] = v;
+    }
+    __syncthreads();
+    const bool inv = op == COL_INV_MAX || op == COL_INV_LAGWIN;
+    cf32 *res = inv ? fft_lds_batch<float, +1>(buf0, buf1, p1, tw1, kBigCols, ld)
+                    : fft_lds_batch<float, -1>(buf0, buf1, p1, tw1, kBigCols, ld);
+    float m = 0.f;
+    // LAGWIN batch decode
+    int p = 0, w = 0, i = 0, j = 0;
+    if (op == COL_INV_LAGWIN) {
+        const int q = b / a.Wb;
+        w = a.wsel + (b - q * a.Wb);
+        p = a.pair0 + q;
+        if (a.norm == 1) pair_of_big(p, a.S, i, j);
+    }
+    for (int idx = threadIdx.x; idx < kBigCols * N1; idx += blockDim.x) {
+        const int t = idx % kBigCols, r = idx / kBigCols;
+        const int c = c0 + t;
+        if (c >= N2) continue;
+        const long long g = (long long)r * N2 + c;
+        cf32 v = res[t * ld + r];
+        if (!inv) {
+            a.dst[base + g] = cmul(v, twN[twidx(c, r)]);  // W_n^{n2 k1}
+        } else if (op == COL_INV_MAX) {
+            a.dst[base + g] = v;
+            m = fmaxf(m, v.x * v.x + v.y * v.y);
+        } else {
+            long long d = g - a.lag0;  // in (-n, 2n): lag0 is in (-n, n)
+            if (d < 0) d += n;
+            if (d >= n) d -= n;
+            const int l = (int)d;
+            if (l < a.L) {
+                const int lag = a.lag0 + l;
+                double o = (double)(a.scale * v.x);
+                if (a.norm == 1) {
+                    const double *P1 = a.E + (long long)(i * a.Wfull + w) * a.lde;
+                    const double *P2 = a.E + (long long)(j * a.Wfull + w) * a.lde;
+                    const int N = a.Nsig;
+                    const int a1 = max(0, -lag), b1 = min(N, N - lag);
+                    const int a2 = max(0, lag), b2 = min(N, N + lag);
+                    o = o / sqrt((P1[b1] - P1[a1]) * (P2[b2] - P2[a2]));
+                } else if (a.norm == 2) {
+                    o = o / (double)(a.Nsig - abs(lag));
+                }
+                if (a.accumulate) a.y[(long long)p * a.L + l] += (float)o;
+                else a.y[((long long)p * a.Wfull + w) * a.L + l] = (float)o;
+            }
+        }
+    }
+    if (op == COL_INV_MAX) {
+        m = block_max(m, red);
+        if (threadIdx.x == 0) atomicMax(reinterpret_cast<int *>(a.maxbuf + b), __float_as_int(m));
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, const cf32 *__restrict__ twN)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cf32 *buf0 = reinterpret_cast<cf32 *>(smem_raw);
+    cf32 *buf1 = buf0 + kBigRows * N2;
+    const int b = blockIdx.y;
+    const int r0 = blockIdx.x * kBigRows;
+    const long long base = (long long)b * a.bstride;
+    const long long n = (long long)N1 * N2;
+    const int op = a.op;
+    const cf32 *Xs = nullptr, *Fs = nullptr;
+    if (op == ROW_INV_MUL) {
+        const int sw = b / a.S, s = b - sw * a.S;
+        Xs = a.X + sw * n;
+        Fs = a.fw + s * n;
+    }
+    int ci = 0, cj = 0, cw0 = 0, cw1 = 0;
+    const cf32 *F2 = a.F2 ? a.F2 : a.F;
+    if (op == ROW_INV_CROSS) {
+        const int q = b / a.Wb;
+        pair_of_big(a.pair0 + q, a.Sst, ci, cj);
+        cw0 = a.wsum ? 0 : a.wsel + (b - q * a.Wb);
+        cw1 = a.wsum ? a.Wfull : cw0 + 1;
+    }
+    for (int idx = threadIdx.x; idx < kBigRows * N2; idx += blockDim.x) {
+        const int rr = idx / N2, c = idx - rr * N2;
+        const int r = r0 + rr;
+        cf32 v = cf32{0.f, 0.f};
+        if (r < N1) {
+            const long long g = (long long)r * N2 + c;
+            if (op == ROW_INV_MUL) {
+                v = cmul(Xs[g], Fs[g]);
+            } else if (op == ROW_INV_CROSS) {
+                for (int w = cw0; w < cw1; ++w) {
+                    const cf32 t = cmulc(a.F[(long long)(ci * a.Wfull + w) * n + g], F2[(long long)(cj * a.Wfull + w) * n + g]);
+                    v = cf32{v.x + t.x, v.y + t.y};
+                }
+            } else {
+                v = a.src[base + g];
+            }
+        }
+        buf0[idx] = v;
+    }
+    __syncthreads();
+    const bool inv = op == ROW_INV_MUL || op == ROW_INV_PLAIN || op == ROW_INV_CROSS;
+    cf32 *res = (inv ? fft_lds_batch<float, +1>(buf0, buf1, p2, tw2, kBigRows, N2)
+                     : fft_lds_batch<float, -1>(buf0, buf1, p2, tw2, kBigRows, N2));
+    const float sc = (op == ROW_FWD_SCALE) ? a.wscale[b % a.S] : 1.0f;
+    const int half = (a.N + 1) >> 1;
+    if (op == ROW_FWD_MASK || op == ROW_MASK_INV) {
+        for (int idx = threadIdx.x; idx < kBigRows * N2; idx += blockDim.x) {
+            const int rr = idx / N2, c = idx - rr * N2;
+            const long long k = (r0 + rr) + (long long)N1 * c;  // natural bin
+            float gm;
+            if (k == 0) gm = a.da1;
+            else if (k < half) gm = 2 * a.da1;
+            else if (!(a.N & 1) && k == (a.N >> 1)) gm = a.da1;
+            else gm = 0.f;
+            res[idx] = cf32{res[idx].x * gm, res[idx].y * gm};
+        }
+        if (op == ROW_MASK_INV) {
+            __syncthreads();
+            cf32 *other = (res == buf0) ? buf1 : buf0;
+            res = fft_lds_batch<float, +1>(res, other, p2, tw2, kBigRows, N2);
+        }
+    }
+    const bool inv_out = inv || op == ROW_MASK_INV;
+    for (int idx = threadIdx.x; idx < kBigRows * N2; idx += blockDim.x) {
+        const int rr = idx / N2, c = idx - rr * N2;
+        const int r = r0 + rr;
+        if (r >= N1) continue;
+        const long long g = (long long)r * N2 + c;
+        cf32 v = res[idx];
+        if (inv_out) {
+            v = cmulc(twN[twidx(c, r)], v);  // W_n^{-n2 k1}
+        } else if (op == ROW_FWD_SCALE) {
+            v = cf32{v.x * sc, v.y * sc};
+        }
+        a.dst[base + g] = v;
+    }
+}
+
+// One workgroup per station-window: P[n] = sum_{m<n} x[m]^2 in FP64, n in [0, N].
+__global__ void __launch_bounds__(512)
+k_energy_prefix(const float *__restrict__ x, long long station_stride, int W, int N, double *__restrict__ E,
+                long long lde)
+{
+    __shared__ double part[512];
+    const int sw = blockIdx.x;
+    const int s = sw / W, w = sw - s * W;
+    const float *in = x + (long long)s * station_stride + (long long)w * N;
+    const int T = blockDim.x;
+    const int per = (N + T - 1) / T;
+    const int a0 = threadIdx.x * per, b0 = min(N, a0 + per);
+    double acc = 0.0;
+    for (int n = a0; n < b0; ++n) {
+        const double v = (double)in[n];
+        acc += v * v;
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double run = 0.0;
+        for (int t = 0; t < T; ++t) {
+            const double v = part[t];
+            part[t] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    double run = part[threadIdx.x];
+    double *P = E + (long long)sw * lde;
+    for (int n = a0; n < b0; ++n) {
+        P[n] = run;
+        const double v = (double)in[n];
+        run += v * v;
+    }
+    if (a0 < N && b0 == N) P[N] = run;
+}
+
+int launch_energy_prefix(const float *d_x, long long station_stride, int S, int W, int N, double *d_E, long long lde,
+                         hipStream_t st)
+{
+    ProfScope prof("energy_prefix", st);
+    hipLaunchKernelGGL(k_energy_prefix, dim3(S * W), dim3(512), 0, st, d_x, station_stride, W, N, d_E, lde);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+__global__ void k_fill_zero(float *p, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+static void allow_lds(const void *fn, size_t bytes)
+{
+    if (bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+int run_cols(const BigPlan &bp, const ColArgs &a, int nbatch, hipStream_t st, const char *name)
+{
+    if (nbatch <= 0) return 0;
+    if (!bp.ok) return -1;
+    if (nbatch > 65535) {
+        fprintf(stderr, "fastpcc: %d transforms exceed one launch\n", nbatch);
+        return -1;
+    }
+    const size_t lds = sizeof(cf32) * 2 * kBigCols * (size_t)(bp.n1 + 1) + sizeof(float) * 8;
+    allow_lds((const void *)k_big_cols, lds);
+    ProfScope prof(name, st);
+    hipLaunchKernelGGL(k_big_cols, dim3((bp.n2 + kBigCols - 1) / kBigCols, nbatch), dim3(256), lds, st, a, bp.n1,
+                       bp.n2, bp.p1, bp.tw1, bp.twN);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+int run_rows(const BigPlan &bp, const RowArgs &a, int nbatch, hipStream_t st, const char *name)
+{
+    if (nbatch <= 0) return 0;
+    if (!bp.ok) return -1;
+    if (nbatch > 65535) {
+        fprintf(stderr, "fastpcc: %d transforms exceed one launch\n", nbatch);
+        return -1;
+    }
+    const size_t lds = sizeof(cf32) * 2 * kBigRows * (size_t)bp.n2;
+    allow_lds((const void *)k_big_rows, lds);
+    ProfScope prof(name, st);
+    hipLaunchKernelGGL(k_big_rows, dim3((bp.n1 + kBigRows - 1) / kBigRows, nbatch), dim3(256), lds, st, a, bp.n1,
+                       bp.n2, bp.p2, bp.tw2, bp.twN);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+int fill_zero(float *p, long long n, hipStream_t st)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace fpcc

@@ -37,7 +37,20 @@ struct Dims {
     unsigned M, Nz;
     size_t P;
     const WFamily *fam;  // WPCC only
+    bool large;          // windows beyond the single-workgroup kernels
 };
+
+// The single-workgroup kernels need Nz <= 8192 and, for the analytic signal, a
+// 7-smooth N <= 8192; anything else goes through the two-pass FFTs.
+bool needs_large(int kind, int N, unsigned Nz)
+{
+    if (kind == FPCC_WPCC) return false;
+    FftPlan p;
+    const bool analytic = kind == FPCC_PCC2 || kind == FPCC_PCC1 || kind == FPCC_PCCV;
+    if (analytic && (N > 8192 || !factor_plan(N, p))) return true;
+    if (kind == FPCC_PCC1 || kind == FPCC_PCCV) return false;
+    return Nz > 8192;
+}
 
 bool stack_supported(int kind, int stack)
 {
@@ -71,6 +84,11 @@ int resolve(const fpcc_job *job, Dims &d)
     d.Nz = nz_rule(N, d.M);
     d.P = (size_t)d.S * (d.S - 1) / 2;
     d.fam = nullptr;
+    d.large = needs_large(job->kind, d.N, d.Nz);
+    if (d.large && !large_supported(job->kind, d.N, (int)d.Nz)) {
+        fprintf(stderr, "fastpcc: unsupported window length N=%d (Nz=%u)\n", d.N, d.Nz);
+        return -1;
+    }
     if (job->kind == FPCC_WPCC) {
         int err = 0;
         d.fam = get_wfamily(N, job->lag1, job->lag2, job->pmin, job->pmax, job->V, job->wtype, job->op1, &err);
@@ -96,6 +114,10 @@ Layout layout(const fpcc_job *job, const Dims &d)
 {
     Layout l;
     const size_t SW = (size_t)d.S * d.W, K1 = d.Nz / 2 + 1;
+    if (d.large) {
+        l.a = take(l.total, large_workspace(job->kind, job->stack, d.S, d.W, d.N, (int)d.Nz));
+        return l;
+    }
     switch (job->kind) {
     case FPCC_PCC2:
         l.a = take(l.total, sizeof(cf32) * SW * d.Nz);  // spectra
@@ -132,6 +154,7 @@ int station_stage(const fpcc_job *job, const Dims &d, const float *x, char *ws, 
 {
     Layout l = layout(job, d);
     const long long stride = (long long)job->station_stride;
+    if (d.large) return large_station_stage(job->kind, x, stride, d.S, d.W, d.N, (int)d.Nz, job->stack, ws + l.a, st);
     switch (job->kind) {
     case FPCC_PCC2: return launch_station_pcc(x, stride, d.S, d.W, d.N, d.Nz, (cf32 *)(ws + l.a), st);
     case FPCC_GNCC:
@@ -152,6 +175,10 @@ int pair_stage(const fpcc_job *job, const Dims &d, char *ws, float *y, hipStream
 {
     Layout l = layout(job, d);
     int r = 0;
+    if (d.large) {
+        const int stack = job->stack == FPCC_STACK_ZSUM ? FPCC_STACK_PLAIN : job->stack;
+        return large_pair_stage(job->kind, stack, d.S, d.W, d.N, (int)d.Nz, d.lag0, d.L, job->v, ws + l.a, y, st);
+    }
     const int mode = job->stack == FPCC_STACK_NONE ? 0 : 1;
     switch (job->kind) {
     case FPCC_PCC2:
@@ -264,12 +291,24 @@ int pcc_time_host(int pcc1, float **y, const float *const *x1, const float *cons
     FPCC_CHECK(hipMemset(dy.p, 0, sizeof(float) * ylen));
     int r;
     const cf32 *phi;
+    FftPlan fpl;
     if (complex_in) {
         if ((r = launch_ampnormf_rows((cf32 *)dx.p, 2 * (int)Tr, N, nullptr))) return r;
         phi = (const cf32 *)dx.p;
-    } else {
+    } else if (N <= 8192 && factor_plan(N, fpl)) {
         FPCC_CHECK(hipMalloc(&dphi.p, sizeof(cf32) * 2 * Tr * (size_t)N));
         if ((r = launch_station_phasor((const float *)dx.p, (long long)Tr * N, 2, Tr, N, (cf32 *)dphi.p, nullptr)))
+            return r;
+        phi = (const cf32 *)dphi.p;
+    } else {
+        // the phasors lead the large-window workspace (rows of N complex values)
+        if (!large_supported(FPCC_PCC1, N, N)) {
+            fprintf(stderr, "fastpcc: analytic signal of N=%d needs a 7-smooth length\n", N);
+            return -1;
+        }
+        FPCC_CHECK(hipMalloc(&dphi.p, large_workspace(FPCC_PCC1, FPCC_STACK_NONE, 2, (int)Tr, N, N)));
+        if ((r = large_station_stage(FPCC_PCC1, (const float *)dx.p, (long long)Tr * N, 2, (int)Tr, N, N,
+                                     FPCC_STACK_NONE, (char *)dphi.p, nullptr)))
             return r;
         phi = (const cf32 *)dphi.p;
     }

@@ -303,10 +303,15 @@ __device__ C2<T> *fft_lds_batch(C2<T> *buf0, C2<T> *buf1, const FftPlan &plan, c
         const C2<T> *twp = tw + plan.tw_off[p];
         switch (R) {
         case 2: stockham_pass_batch<T, 2, DIR>(in, out, plan.n, Ns, twp, nbatch, ld); break;
+        case 3: stockham_pass_batch<T, 3, DIR>(in, out, plan.n, Ns, twp, nbatch, ld); break;
         case 4: stockham_pass_batch<T, 4, DIR>(in, out, plan.n, Ns, twp, nbatch, ld); break;
+        case 5: stockham_pass_batch<T, 5, DIR>(in, out, plan.n, Ns, twp, nbatch, ld); break;
+        case 7: stockham_pass_batch<T, 7, DIR>(in, out, plan.n, Ns, twp, nbatch, ld); break;
         case 8: stockham_pass_batch<T, 8, DIR>(in, out, plan.n, Ns, twp, nbatch, ld); break;
+        case 9: stockham_pass_batch<T, 9, DIR>(in, out, plan.n, Ns, twp, nbatch, ld); break;
         case 16: stockham_pass_batch<T, 16, DIR>(in, out, plan.n, Ns, twp, nbatch, ld); break;
-        default: break;  // large transforms are powers of two
+        case 25: stockham_pass_batch<T, 25, DIR>(in, out, plan.n, Ns, twp, nbatch, ld); break;
+        default: break;
         }
         __syncthreads();
         C2<T> *t = in;

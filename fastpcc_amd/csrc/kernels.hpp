@@ -49,6 +49,14 @@ int launch_station_real(const float *d_x, long long station_stride, int S, int W
 int launch_pair_gn(const void *d_Xa, const void *d_Xb, const double *d_E, bool f64, int S, int W, int N, int Nz,
                    int lag0, int L, int norm, int mode, float *d_y, hipStream_t st);
 
+// Windows longer than one workgroup's LDS (large.hip).
+bool large_supported(int kind, int N, int Nz);
+size_t large_workspace(int kind, int stack, int S, int W, int N, int Nz);
+int large_station_stage(int kind, const float *x, long long stride, int S, int W, int N, int Nz, int stack,
+                        char *ws, hipStream_t st);
+int large_pair_stage(int kind, int stack, int S, int W, int N, int Nz, int lag0, int L, double v, char *ws, float *y,
+                     hipStream_t st);
+
 // WPCC (wpcc.hip): workspace bytes and the two stages for S stations x W windows.
 struct WFamily;
 size_t wpcc_workspace(int S, int W, int N, int Nz, int nscales);

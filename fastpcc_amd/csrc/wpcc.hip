@@ -24,237 +24,9 @@
 #include "kernels.hpp"
 #include "profile.hpp"
 #include "wavelet.hpp"
+#include "bigfft.hpp"
 
 namespace fpcc {
-
-constexpr int kBigCols = 16;  // columns per workgroup (128-byte row segments)
-constexpr int kBigRows = 4;   // rows per workgroup
-
-struct BigPlan {
-    int n = 0, n1 = 0, n2 = 0;
-    FftPlan p1{}, p2{};
-    const cf32 *tw1 = nullptr, *tw2 = nullptr, *twN = nullptr;
-    bool ok = false;
-};
-
-static BigPlan make_big_plan(int n)
-{
-    BigPlan b;
-    if (n < 64 || (n & (n - 1))) return b;
-    int lg = 0;
-    while ((1 << lg) < n) ++lg;
-    // N1 <= 512 rows keeps the column pass within LDS (16 columns, two buffers);
-    // N2 >= 16 columns fills one column workgroup; N2 <= 2048 bounds the row pass.
-    b.n = n;
-    b.n1 = 1 << (lg / 2);
-    if (b.n1 > 512) b.n1 = 512;
-    b.n2 = n / b.n1;
-    if (b.n2 < kBigCols) {
-        b.n2 = kBigCols;
-        b.n1 = n / kBigCols;
-    }
-    if (b.n1 < kBigRows || b.n2 > 2048) return b;
-    const DevPlan &a = get_plan(b.n1, false), &c = get_plan(b.n2, false);
-    if (!a.ok || !c.ok) return b;
-    b.p1 = a.plan;
-    b.p2 = c.plan;
-    b.tw1 = (const cf32 *)a.tw;
-    b.tw2 = (const cf32 *)c.tw;
-    b.twN = (const cf32 *)get_full_twiddles(n, false);
-    b.ok = b.twN != nullptr;
-    return b;
-}
-
-enum { COL_FWD_INPUT = 0, COL_FWD_PLAIN = 1, COL_FWD_AMPNORM = 2, COL_INV_MAX = 3, COL_INV_LAGWIN = 4 };
-enum { ROW_FWD_PLAIN = 0, ROW_FWD_SCALE = 1, ROW_INV_MUL = 2, ROW_INV_PLAIN = 3 };
-
-struct ColArgs {
-    int op;
-    const cf32 *src;
-    cf32 *dst;
-    long long bstride;  // elements between batches of src/dst
-    // COL_FWD_INPUT: batch = station-window sw
-    const float *x;
-    long long x_stride;
-    int W, N, Ls0;
-    float inv_nz;
-    // COL_INV_MAX writes, COL_FWD_AMPNORM reads max|z|^2 per batch
-    float *maxbuf;
-    // COL_INV_LAGWIN: batch b -> row y_row0 + b*y_rowstep of L floats
-    float *y;
-    int lag0, L, y_row0, y_rowstep;
-    float scale;
-};
-
-struct RowArgs {
-    int op;
-    const cf32 *src;
-    cf32 *dst;
-    long long bstride;
-    // ROW_INV_MUL: batch b = sw*S + s multiplies X[sw] by fw[s]
-    const cf32 *X, *fw;
-    int S;
-    // ROW_FWD_SCALE: batch b multiplied by wscale[b % S]
-    const float *wscale;
-};
-
-__global__ void __launch_bounds__(256)
-k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, const cf32 *__restrict__ twN)
-{
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int ld = N1 + 1;  // padded column stride (the loads write with stride ld)
-    cf32 *buf0 = reinterpret_cast<cf32 *>(smem_raw);
-    cf32 *buf1 = buf0 + kBigCols * ld;
-    float *red = reinterpret_cast<float *>(buf1 + kBigCols * ld);
-    const int b = blockIdx.y;
-    const int c0 = blockIdx.x * kBigCols;
-    const long long base = (long long)b * a.bstride;
-    const int Nz = N1 * N2;
-    const int op = a.op;
-    float eps = 0.f;
-    if (op == COL_FWD_AMPNORM) eps = 1e-6f * a.maxbuf[b];
-    const float *xr = nullptr;
-    if (op == COL_FWD_INPUT) {
-        const int s = b / a.W, w = b - s * a.W;
-        xr = a.x + (long long)s * a.x_stride + (long long)w * a.N;
-    }
-    for (int idx = threadIdx.x; idx < kBigCols * N1; idx += blockDim.x) {
-        const int t = idx % kBigCols, r = idx / kBigCols;
-        const long long g = (long long)r * N2 + c0 + t;
-        cf32 v;
-        if (op == COL_FWD_INPUT) {
-            const int n = (int)g - a.Ls0;
-            v = (n >= 0 && n < a.N) ? cf32{xr[n] * a.inv_nz, 0.f} : cf32{0.f, 0.f};
-        } else if (op == COL_FWD_AMPNORM) {
-            const cf32 u = a.src[base + g];
-            const float rr = 1.0f / sqrtf(u.x * u.x + u.y * u.y + eps);
-            v = __builtin_isnormal(rr) ? cf32{u.x * rr, u.y * rr} : cf32{0.f, 0.f};
-        } else {
-            v = a.src[base + g];
-        }
-        buf0[t * ld + r] = v;
-    }
-    __syncthreads();
-    const bool inv = op == COL_INV_MAX || op == COL_INV_LAGWIN;
-    cf32 *res = inv ? fft_lds_batch<float, +1>(buf0, buf1, p1, tw1, kBigCols, ld)
-                    : fft_lds_batch<float, -1>(buf0, buf1, p1, tw1, kBigCols, ld);
-    float m = 0.f;
-    for (int idx = threadIdx.x; idx < kBigCols * N1; idx += blockDim.x) {
-        const int t = idx % kBigCols, r = idx / kBigCols;
-        const int c = c0 + t;
-        const long long g = (long long)r * N2 + c;
-        cf32 v = res[t * ld + r];
-        if (!inv) {
-            a.dst[base + g] = cmul(v, twN[c * r]);  // W_Nz^{n2 k1}
-        } else if (op == COL_INV_MAX) {
-            a.dst[base + g] = v;
-            m = fmaxf(m, v.x * v.x + v.y * v.y);
-        } else {
-            const int l = (int)((g - a.lag0) & (long long)(Nz - 1));
-            if (l < a.L) a.y[((long long)a.y_row0 + (long long)b * a.y_rowstep) * a.L + l] = a.scale * v.x;
-        }
-    }
-    if (op == COL_INV_MAX) {
-        m = block_max(m, red);
-        if (threadIdx.x == 0) atomicMax(reinterpret_cast<int *>(a.maxbuf + b), __float_as_int(m));
-    }
-}
-
-__global__ void __launch_bounds__(256)
-k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, const cf32 *__restrict__ twN)
-{
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    cf32 *buf0 = reinterpret_cast<cf32 *>(smem_raw);
-    cf32 *buf1 = buf0 + kBigRows * N2;
-    const int b = blockIdx.y;
-    const int r0 = blockIdx.x * kBigRows;
-    const long long base = (long long)b * a.bstride;
-    const long long Nz = (long long)N1 * N2;
-    const int op = a.op;
-    const cf32 *Xs = nullptr, *Fs = nullptr;
-    if (op == ROW_INV_MUL) {
-        const int sw = b / a.S, s = b - sw * a.S;
-        Xs = a.X + sw * Nz;
-        Fs = a.fw + s * Nz;
-    }
-    for (int idx = threadIdx.x; idx < kBigRows * N2; idx += blockDim.x) {
-        const int rr = idx / N2, c = idx - rr * N2;
-        const long long g = (long long)(r0 + rr) * N2 + c;
-        buf0[idx] = (op == ROW_INV_MUL) ? cmul(Xs[g], Fs[g]) : a.src[base + g];
-    }
-    __syncthreads();
-    const bool inv = op == ROW_INV_MUL || op == ROW_INV_PLAIN;
-    cf32 *res = inv ? fft_lds_batch<float, +1>(buf0, buf1, p2, tw2, kBigRows, N2)
-                    : fft_lds_batch<float, -1>(buf0, buf1, p2, tw2, kBigRows, N2);
-    const float sc = (op == ROW_FWD_SCALE) ? a.wscale[b % a.S] : 1.0f;
-    for (int idx = threadIdx.x; idx < kBigRows * N2; idx += blockDim.x) {
-        const int rr = idx / N2, c = idx - rr * N2;
-        const int r = r0 + rr;
-        const long long g = (long long)r * N2 + c;
-        cf32 v = res[idx];
-        if (inv) {
-            v = cmulc(twN[c * r], v);  // W_Nz^{-n2 k1}
-        } else if (op == ROW_FWD_SCALE) {
-            v = cf32{v.x * sc, v.y * sc};
-        }
-        a.dst[base + g] = v;
-    }
-}
-
-__global__ void k_fill_zero(float *p, long long n)
-{
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.f;
-}
-
-static void allow_lds(const void *fn, size_t bytes)
-{
-    if (bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-}
-
-static int run_cols(const BigPlan &bp, const ColArgs &a, int nbatch, hipStream_t st, const char *name)
-{
-    if (nbatch <= 0) return 0;
-    if (nbatch > 65535) {
-        fprintf(stderr, "fastpcc: %d transforms exceed one launch\n", nbatch);
-        return -1;
-    }
-    const size_t lds = sizeof(cf32) * 2 * kBigCols * (size_t)(bp.n1 + 1) + sizeof(float) * 8;
-    allow_lds((const void *)k_big_cols, lds);
-    ProfScope prof(name, st);
-    hipLaunchKernelGGL(k_big_cols, dim3(bp.n2 / kBigCols, nbatch), dim3(256), lds, st, a, bp.n1, bp.n2, bp.p1,
-                       bp.tw1, bp.twN);
-    FPCC_CHECK(hipGetLastError());
-    return 0;
-}
-
-static int run_rows(const BigPlan &bp, const RowArgs &a, int nbatch, hipStream_t st, const char *name)
-{
-    if (nbatch <= 0) return 0;
-    if (nbatch > 65535) {
-        fprintf(stderr, "fastpcc: %d transforms exceed one launch\n", nbatch);
-        return -1;
-    }
-    const size_t lds = sizeof(cf32) * 2 * kBigRows * (size_t)bp.n2;
-    allow_lds((const void *)k_big_rows, lds);
-    ProfScope prof(name, st);
-    hipLaunchKernelGGL(k_big_rows, dim3(bp.n1 / kBigRows, nbatch), dim3(256), lds, st, a, bp.n1, bp.n2, bp.p2,
-                       bp.tw2, bp.twN);
-    FPCC_CHECK(hipGetLastError());
-    return 0;
-}
-
-static const BigPlan &big_plan(int n)
-{
-    static std::map<std::pair<int, int>, BigPlan> cache;
-    static std::mutex mu;
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    std::lock_guard<std::mutex> lk(mu);
-    auto it = cache.find({dev, n});
-    if (it != cache.end()) return it->second;
-    return cache.emplace(std::make_pair(dev, n), make_big_plan(n)).first->second;
-}
 
 // Filter-bank spectra W_s (permuted order) and sqrt(1/(scale_s Nz)), cached per
 // device and family (the wavelets are rounded to FP32 before their FFT).
@@ -392,8 +164,8 @@ int wpcc_station_stage(const float *d_x, long long station_stride, int S, int W,
     ca.x_stride = station_stride;
     ca.W = W;
     ca.N = N;
-    ca.Ls0 = (int)f.Ls0;
-    ca.inv_nz = (float)(1.0 / (double)Nz);
+    ca.off = (int)f.Ls0;
+    ca.in_scale = (float)(1.0 / (double)Nz);
     if ((r = run_cols(bp, ca, SW, st, "wpcc_input_cols"))) return r;
     RowArgs ra{};
     ra.op = ROW_FWD_PLAIN;
@@ -404,8 +176,7 @@ int wpcc_station_stage(const float *d_x, long long station_stride, int S, int W,
     if ((r = run_rows(bp, ra, SW, st, "wpcc_input_rows"))) return r;
     // per (sw, s): IFFT(X W_s) with max|z|^2, AmpNorm, FFT, * sqrt(1/(scale Nz))
     const long long nb = (long long)SW * nsc;
-    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, mx, nb);
-    FPCC_CHECK(hipGetLastError());
+    if ((r = fill_zero(mx, nb, st))) return r;
     ra = RowArgs{};
     ra.op = ROW_INV_MUL;
     ra.dst = B;
@@ -467,8 +238,10 @@ int wpcc_pair_stage(int S, int W, int N, const WFamily &f, int lag0, int L, bool
         ca.y = d_y;
         ca.lag0 = lag0;
         ca.L = L;
-        ca.y_row0 = per_window ? w : 0;
-        ca.y_rowstep = per_window ? W : 1;
+        ca.S = S;
+        ca.Wb = 1;
+        ca.wsel = per_window ? w : 0;
+        ca.Wfull = per_window ? W : 1;
         ca.scale = (float)(per_window ? C : C / W);
         if ((r = run_cols(bp, ca, P, st, "wpcc_pair_inv_cols"))) return r;
     }

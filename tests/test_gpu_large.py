@@ -1,0 +1,72 @@
+"""Windows beyond one workgroup's LDS (N > 8192 or Nz > 8192): the two-pass FFT
+path against the oracle.  N = 21600 is the reference's own example size (day
+files at 4 s, examples/G/*), giving Nz = 32768 for lags of +-3000."""
+import numpy as np
+import pytest
+
+from fastpcc_amd import synth
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _rand(Tr, N, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((Tr, N)).astype(np.float32), rng.standard_normal((Tr, N)).astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", ["pcc2", "gncc", "cc1b"])
+def test_reference_example_size(fp, orc, kind):
+    N, Tr, l1, l2 = 21600, 2, -3000, 3000
+    print(f"{kind} N={N} Tr={Tr} lags=({l1},{l2})", flush=True)
+    x1, x2 = _rand(Tr, N, 21600)
+    fn = {"pcc2": fp.pcc2_set, "gncc": fp.ccgn_set, "cc1b": fp.cc1b_set}[kind]
+    y = fn(x1, x2, l1, l2)
+    yr = orc.run(kind, x1, x2, l1, l2)
+    y64 = orc.run(kind, x1, x2, l1, l2, f64=True)
+    err = np.abs(y - yr).max()
+    print(f"  max|gpu-oracle|={err:.3e} max|gpu-fp64|={np.abs(y - y64).max():.3e}")
+    assert err <= TOL
+
+
+def test_pcc1_large_window(fp, orc):
+    x1, x2 = _rand(1, 21600, 5)
+    y = fp.pcc1_set(x1, x2, -400, 400)
+    yr = orc.run("pcc1", x1, x2, -400, 400)
+    err = np.abs(y - yr).max()
+    print(f"pcc1 N=21600 max|gpu-oracle|={err:.3e}")
+    assert err <= TOL
+
+
+def test_pcc2_one_day_at_1hz(fp, orc):
+    """86400 samples, lags +-2000: Nz = 131072 (the longest serving window)."""
+    x, sh = synth.make_stations(2, 86400, 1000, seed=9)
+    y = fp.pcc2_set(x[:1], x[1:], -2000, 2000)
+    yr = orc.run("pcc2", x[:1], x[1:], -2000, 2000)
+    err = np.abs(y - yr).max()
+    print(f"pcc2 N=86400 max|gpu-oracle|={err:.3e}")
+    assert err <= TOL
+    assert np.argmax(y[0]) - 2000 == sh[1] - sh[0]
+
+
+def test_gncc_prime_length(fp, orc):
+    """GNCC and 1-bit need no DFT of length N: any N works."""
+    x1, x2 = _rand(2, 10007, 11)
+    for kind, fn in (("gncc", fp.ccgn_set), ("cc1b", fp.cc1b_set)):
+        y = fn(x1, x2, -700, 900)
+        assert np.abs(y - orc.run(kind, x1, x2, -700, 900)).max() <= TOL
+
+
+@pytest.mark.parametrize("kind,stack", [("pcc2", "none"), ("pcc2", "plain"), ("pcc2", "gram"), ("gncc", "none"),
+                                        ("gncc", "plain"), ("cc1b", "plain"), ("cc1b", "gram"), ("pcc1", "plain")])
+def test_allpairs_large(fp, orc, kind, stack):
+    S, W, N = 5, 2, 12000
+    x, _ = synth.make_stations(S, W * N, 500, seed=41)
+    if kind == "cc1b":
+        x[:, ::101] = 0.0
+    lag = 300 if kind == "pcc1" else 2000
+    y = fp.allpairs(x, W, N, -lag, lag, kind=kind, stack=stack)
+    yr = orc.allpairs(kind, x, W, N, -lag, lag, stacked=(stack != "none"))
+    err = np.abs(y - yr).max()
+    print(f"{kind}/{stack} N={N} max|gpu-oracle|={err:.3e}")
+    assert y.shape == yr.shape and err <= TOL
