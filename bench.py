@@ -29,6 +29,7 @@ CONFIGS = {
     "c3": dict(S=256, W=24, N=3600, lag=1000, shift=1000),
 }
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"gram_f32": 157.3, "gram_split16": 2500.0}  # dense FP32 / FP16 MFMA
 
 
 def nz_rule(N, M):
@@ -36,6 +37,16 @@ def nz_rule(N, M):
     while n < N + M:
         n <<= 1
     return n
+
+
+def kernel_flops(name, cfg, Nz):
+    """Algorithmic flops of one Gram launch: P pairs x (Nz/2+1) bins x 2W complex
+    multiply-adds (8 flops each) of the Hermitian-folded stack."""
+    if not name.startswith("gram"):
+        return None
+    S, W = cfg["S"], cfg["W"]
+    P = S * (S - 1) // 2
+    return P * (Nz // 2 + 1) * 2 * W * 8
 
 
 def kernel_bytes(name, cfg, Nz):
@@ -80,7 +91,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
-    ap.add_argument("--stack", default="gram", choices=["plain", "zsum", "gram"])
+    ap.add_argument("--stack", default="gram", choices=["plain", "zsum", "gram", "gram16"])
+    ap.add_argument("--variants", default="plain,gram16",
+                    help="other stack modes timed briefly for comparison (comma list, '' for none)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -144,15 +157,52 @@ def main():
     value = P * W / (ms_per_step / 1e3)
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"]) if kernels else None
     roof = None
-    if dom:
-        b = kernel_bytes(dom, cfg, Nz)
-        ach = b / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": b}
     for k, v in kernels.items():
         b = kernel_bytes(k, cfg, Nz)
         if b:
             v["algorithmic_GBps"] = round(b / (v["avg_ms"] * 1e-3) / 1e9, 1)
+        f = kernel_flops(k, cfg, Nz)
+        if f:
+            v["algorithmic_TFLOPs"] = round(f / (v["avg_ms"] * 1e-3) / 1e12, 2)
+    if dom:
+        f = kernel_flops(dom, cfg, Nz)
+        if f and dom in MFMA_PEAK_TFLOPS:
+            ach = f / (kernels[dom]["avg_ms"] * 1e-3) / 1e12
+            peak = MFMA_PEAK_TFLOPS[dom]
+            roof = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+                    "frac": round(ach / peak, 4), "traffic": None, "algorithmic_flops": f}
+        else:
+            b = kernel_bytes(dom, cfg, Nz)
+            ach = b / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
+            roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": b}
+
+    variants = {}
+    for vs in [v for v in args.variants.split(",") if v and v != args.stack]:
+        vjob = fastpcc_amd.make_job("pcc2", S, W, N, -lag, lag, stack=vs)
+        vr = fastpcc_amd.DeviceRunner(vjob, dev)
+        vr.run(x_dev, stream)
+        torch.cuda.synchronize()
+        nst = 3 if vs == "plain" else 10
+        ta = time.perf_counter()
+        for _ in range(nst):
+            vr.run(x_dev, stream)
+        torch.cuda.synchronize()
+        vms = (time.perf_counter() - ta) * 1e3 / nst
+        ent = {"ms_per_step": vms, "value": P * W / (vms / 1e3), "steps": nst}
+        if not args.no_check:
+            import oracle
+            yv = vr.y.view(P, L).cpu().numpy()[:2]
+            y64 = oracle.allpairs("pcc2", x, W, N, -lag, lag, stacked=True, pair_end=2, f64=True)
+            ent["max_abs_err_vs_fp64"] = float(np.abs(yv - y64).max())
+        variants[vs] = ent
+        del vr
+    speedup = None
+    if "plain" in variants:
+        speedup = round(value / variants["plain"]["value"], 2)
+        for name, ent in variants.items():
+            if name != "plain":
+                ent["speedup_vs_plain"] = round(ent["value"] / variants["plain"]["value"], 2)
 
     cpu = None
     if not args.no_cpu:
@@ -169,13 +219,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.stack != "gram16" else "f16x2(split hi+lo)/f32 acc",
         "data": "synthetic (seeded noise + delayed band-limited common source, SNR 0.5)",
         "config": {"workload": f"PCC2 all pairs, {S} stations x {W * N} samples, {W} windows of {N}, "
                                f"nfft {Nz}, lags +-{lag}, mean stack over windows",
                    "stations": S, "pairs": P, "windows": W, "win_len": N, "nfft": Nz, "lags": L,
                    "stack": args.stack, "parallelism": "1 GPU"},
         "kernels": kernels,
+        "variants": variants,
+        "speedup_vs_plain": speedup,
         "roofline": roof,
         "cpu_baseline": cpu,
         "check": check,

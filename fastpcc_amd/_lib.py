@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfastpcc.so")
 
 FPCC_PCC2, FPCC_PCC1, FPCC_PCCV, FPCC_GNCC, FPCC_CC1B, FPCC_WPCC = range(6)
-STACK_NONE, STACK_PLAIN, STACK_ZSUM, STACK_GRAM = range(4)
+STACK_NONE, STACK_PLAIN, STACK_ZSUM, STACK_GRAM, STACK_GRAM16 = range(5)
 
 
 class FpccJob(ctypes.Structure):

@@ -166,7 +166,8 @@ def cc1b_set(x1, x2, lag1, lag2):
 
 KIND = {"pcc2": _lib.FPCC_PCC2, "pcc1": _lib.FPCC_PCC1, "pccv": _lib.FPCC_PCCV, "gncc": _lib.FPCC_GNCC,
         "cc1b": _lib.FPCC_CC1B, "wpcc": _lib.FPCC_WPCC}
-STACK = {"none": _lib.STACK_NONE, "plain": _lib.STACK_PLAIN, "zsum": _lib.STACK_ZSUM, "gram": _lib.STACK_GRAM}
+STACK = {"none": _lib.STACK_NONE, "plain": _lib.STACK_PLAIN, "zsum": _lib.STACK_ZSUM, "gram": _lib.STACK_GRAM,
+         "gram16": _lib.STACK_GRAM16}
 
 
 def make_job(kind, n_stations, n_windows, win_len, lag1, lag2, stack="plain", station_stride=None, v=2.0,

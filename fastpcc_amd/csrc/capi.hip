@@ -55,9 +55,11 @@ bool needs_large(int kind, int N, unsigned Nz)
 bool stack_supported(int kind, int stack)
 {
     switch (kind) {
-    case FPCC_PCC2: return stack >= FPCC_STACK_NONE && stack <= FPCC_STACK_GRAM;
+    case FPCC_PCC2: return stack >= FPCC_STACK_NONE && stack <= FPCC_STACK_GRAM16;
     case FPCC_GNCC: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN;
-    case FPCC_CC1B: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN || stack == FPCC_STACK_GRAM;
+    case FPCC_CC1B:
+        return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN || stack == FPCC_STACK_GRAM ||
+               stack == FPCC_STACK_GRAM16;
     case FPCC_PCC1:
     case FPCC_PCCV: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN;
     case FPCC_WPCC: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN || stack == FPCC_STACK_GRAM;
@@ -100,8 +102,10 @@ int resolve(const fpcc_job *job, Dims &d)
 
 // Workspace regions of a job (byte offsets, 256-B aligned).
 struct Layout {
-    size_t a = 0, b = 0, c = 0, d = 0, e = 0, total = 0;
+    size_t a = 0, b = 0, c = 0, d = 0, e = 0, mx = 0, total = 0;
 };
+
+bool is_gram(int stack) { return stack == FPCC_STACK_GRAM || stack == FPCC_STACK_GRAM16; }
 
 size_t take(size_t &total, size_t bytes)
 {
@@ -121,9 +125,10 @@ Layout layout(const fpcc_job *job, const Dims &d)
     switch (job->kind) {
     case FPCC_PCC2:
         l.a = take(l.total, sizeof(cf32) * SW * d.Nz);  // spectra
-        if (job->stack == FPCC_STACK_GRAM) {
-            l.b = take(l.total, sizeof(cf32) * K1 * SW * 2);                  // U
+        if (is_gram(job->stack)) {
+            l.b = take(l.total, sizeof(cf32) * K1 * SW * 2);                          // U
             l.c = take(l.total, sizeof(cf32) * d.P * (size_t)gram_ld(d.Nz / 2 + 1));  // G
+            l.mx = take(l.total, sizeof(float));                                      // max |U|
         }
         break;
     case FPCC_GNCC:
@@ -133,10 +138,11 @@ Layout layout(const fpcc_job *job, const Dims &d)
     case FPCC_CC1B:
         l.a = take(l.total, sizeof(cf32) * SW * (size_t)real_ld(d.Nz));  // x >= 0 signs
         l.b = take(l.total, sizeof(cf32) * SW * (size_t)real_ld(d.Nz));  // x > 0 signs
-        if (job->stack == FPCC_STACK_GRAM) {
+        if (is_gram(job->stack)) {
             l.c = take(l.total, sizeof(cf32) * K1 * SW);
             l.d = take(l.total, sizeof(cf32) * K1 * SW);
             l.e = take(l.total, sizeof(cf32) * d.P * (size_t)gram_ld(d.Nz / 2 + 1));
+            l.mx = take(l.total, sizeof(float));
         }
         break;
     case FPCC_PCC1:
@@ -182,15 +188,19 @@ int pair_stage(const fpcc_job *job, const Dims &d, char *ws, float *y, hipStream
     const int mode = job->stack == FPCC_STACK_NONE ? 0 : 1;
     switch (job->kind) {
     case FPCC_PCC2:
-        if (job->stack == FPCC_STACK_GRAM) {
+        if (is_gram(job->stack)) {
             int ntiles = 0;
             const int2 *tiles = gram_tiles(d.S, ntiles);
             if (!tiles) return -1;
             cf32 *F = (cf32 *)(ws + l.a), *U = (cf32 *)(ws + l.b), *G = (cf32 *)(ws + l.c);
+            float *mx = (float *)(ws + l.mx);
+            const bool s16 = job->stack == FPCC_STACK_GRAM16;
             const int nk = d.Nz / 2 + 1;
-            if ((r = launch_spectra_to_u(F, d.Nz, d.S, d.W, d.Nz, nk, 1, U, st))) return r;
-            if ((r = launch_gram_f32(U, U, d.S, 2 * d.W, 0, 2 * d.W, nk, tiles, ntiles, G, gram_ld(nk), st)))
-                return r;
+            if (s16 && (r = fill_zero(mx, 1, st))) return r;
+            if ((r = launch_spectra_to_u(F, d.Nz, d.S, d.W, d.Nz, nk, 1, U, st, s16 ? mx : nullptr))) return r;
+            r = s16 ? launch_gram_split16(U, U, d.S, 2 * d.W, 0, 2 * d.W, nk, tiles, ntiles, mx, G, gram_ld(nk), st)
+                    : launch_gram_f32(U, U, d.S, 2 * d.W, 0, 2 * d.W, nk, tiles, ntiles, G, gram_ld(nk), st);
+            if (r) return r;
             // 1/2 of the Gram's Hermitian fold, the reference's 1/((float)Nz*(float)N), the mean over W
             const float scale = 0.5f * (1 / ((float)d.Nz * (float)d.N)) / (float)d.W;
             return launch_pair_c2r(G, (int)d.P, d.Nz, d.N, d.W, d.lag0, d.L, scale, 0, y, st);
@@ -201,16 +211,25 @@ int pair_stage(const fpcc_job *job, const Dims &d, char *ws, float *y, hipStream
         return launch_pair_gn(ws + l.a, ws + l.a, (const double *)(ws + l.b), true, d.S, d.W, d.N, d.Nz, d.lag0,
                               d.L, 0, mode, y, st);
     case FPCC_CC1B:
-        if (job->stack == FPCC_STACK_GRAM) {
+        if (is_gram(job->stack)) {
             int ntiles = 0;
             const int2 *tiles = gram_tiles(d.S, ntiles);
             if (!tiles) return -1;
             const long long ld = real_ld(d.Nz);
             cf32 *Ua = (cf32 *)(ws + l.c), *Ub = (cf32 *)(ws + l.d), *G = (cf32 *)(ws + l.e);
+            float *mx = (float *)(ws + l.mx);
+            const bool s16 = job->stack == FPCC_STACK_GRAM16;
             const int nk = d.Nz / 2 + 1;
-            if ((r = launch_spectra_to_u((const cf32 *)(ws + l.a), ld, d.S, d.W, d.Nz, nk, 0, Ua, st))) return r;
-            if ((r = launch_spectra_to_u((const cf32 *)(ws + l.b), ld, d.S, d.W, d.Nz, nk, 0, Ub, st))) return r;
-            if ((r = launch_gram_f32(Ua, Ub, d.S, d.W, 0, d.W, nk, tiles, ntiles, G, gram_ld(nk), st))) return r;
+            if (s16 && (r = fill_zero(mx, 1, st))) return r;
+            if ((r = launch_spectra_to_u((const cf32 *)(ws + l.a), ld, d.S, d.W, d.Nz, nk, 0, Ua, st,
+                                         s16 ? mx : nullptr)))
+                return r;
+            if ((r = launch_spectra_to_u((const cf32 *)(ws + l.b), ld, d.S, d.W, d.Nz, nk, 0, Ub, st,
+                                         s16 ? mx : nullptr)))
+                return r;
+            r = s16 ? launch_gram_split16(Ua, Ub, d.S, d.W, 0, d.W, nk, tiles, ntiles, mx, G, gram_ld(nk), st)
+                    : launch_gram_f32(Ua, Ub, d.S, d.W, 0, d.W, nk, tiles, ntiles, G, gram_ld(nk), st);
+            if (r) return r;
             // ccf_lowlevel's 1/Nz, the mean over W, then 1/(N-|lag|) (gnf_lowlevel with +-1 data)
             const float scale = (float)(1.0 / (double)d.Nz) / (float)d.W;
             return launch_pair_c2r(G, (int)d.P, d.Nz, d.N, d.W, d.lag0, d.L, scale, 1, y, st);

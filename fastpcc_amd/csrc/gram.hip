@@ -33,10 +33,12 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // signals, ldF >= Nz/2+1 bins per row): m in [0, W).
 __global__ void __launch_bounds__(256)
 k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz, int nk, int mirror,
-               cf32 *__restrict__ U)
+               cf32 *__restrict__ U, float *__restrict__ umax)
 {
     __shared__ cf32 td[64][33];
     __shared__ cf32 tm[64][33];
+    __shared__ float red[4];
+    float amax = 0.f;
     const int k0 = blockIdx.x * 32;
     const int sw0 = blockIdx.y * 64;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
@@ -53,8 +55,13 @@ k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz,
         }
         td[r][tx] = d;
         tm[r][tx] = m;
+        amax = fmaxf(amax, fmaxf(fmaxf(fabsf(d.x), fabsf(d.y)), fmaxf(fabsf(m.x), fabsf(m.y))));
     }
     __syncthreads();
+    if (umax) {  // largest |component| of U, for the split-FP16 Gram's scaling
+        amax = block_max(amax, red);
+        if (threadIdx.x == 0) atomicMax(reinterpret_cast<int *>(umax), __float_as_int(amax));
+    }
     // write: for each k of the tile, 64 station-windows (contiguous w runs per station)
     const int W2 = mirror ? 2 * W : W;
     const int S = SW / W;
@@ -139,6 +146,117 @@ k_gram_f32(const cf32 *__restrict__ Ua, const cf32 *__restrict__ Ub, int S, int 
     }
 }
 
+// ---- Gram on MFMA f16 16x16x32 with split hi+lo operands ----
+// Every U component v (scaled by sigma = 2^e, exact) is split into
+// hi = fp16(v), lo = fp16(v - hi); products use hi*hi + hi*lo + lo*hi with FP32
+// accumulation (the dropped lo*lo is below 2^-22 relative).  sigma puts the
+// largest |component| near 2^14, inside FP16's range with normal lo parts down
+// to 2^-8 of it; results are multiplied by 1/sigma^2.  Operand layouts (verified
+// on gfx950): A[l%16][8*(l/16)+e], B[8*(l/16)+e][l%16]; the K index (m, c)
+// interleaves components so a lane's 8 values are 4 consecutive complex U[m].
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void split8(const float *v, float sigma, h8v &hi, h8v &lo)
+{
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const float s = v[e] * sigma;
+        const _Float16 h = (_Float16)s;
+        hi[e] = h;
+        lo[e] = (_Float16)(s - (float)h);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_gram_split16(const cf32 *__restrict__ Ua, const cf32 *__restrict__ Ub, int S, int W2, int m0, int K, int nk,
+               const int2 *__restrict__ tiles, int ntiles, int nchunks, const float *__restrict__ umax,
+               cf32 *__restrict__ G, long long ldg)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cf32 *stage = reinterpret_cast<cf32 *>(smem_raw);  // [16*32 pairs][kGramKc]
+    const int b = blockIdx.x;
+    const int xcd = b & 7, slot = b >> 3;
+    const int chunk = (slot / ntiles) * 8 + xcd;
+    const int tile = slot % ntiles;
+    if (chunk >= nchunks) return;
+    const int bi = tiles[tile].x, bj = tiles[tile].y;
+    const int k0 = chunk * kGramKc;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, q = lane >> 4;
+    const int si = bi * 16 + li, sj0 = bj * 32 + li, sj1 = sj0 + 16;
+    const bool va = si < S, vb0 = sj0 < S, vb1 = sj1 < S;
+    // sigma = 2^(14 - ceil(log2 max)), exact; inv2 = 1/sigma^2
+    const float mx = umax[0];
+    int ex = 0;
+    if (mx > 0.f) (void)frexpf(mx, &ex);
+    const float sigma = ldexpf(1.0f, 14 - ex), inv2 = ldexpf(1.0f, 2 * (ex - 14));
+    const int nsteps = (K + 15) / 16;  // 16 complex values (K = 32 reals) per MFMA
+
+    for (int t = 0; t < kGramKc / 4; ++t) {
+        const int kl = wave + 4 * t;
+        const int k = k0 + kl;
+        f4v r0 = {0, 0, 0, 0}, i0 = {0, 0, 0, 0}, r1 = {0, 0, 0, 0}, i1 = {0, 0, 0, 0};
+        if (k < nk) {
+            const cf32 *pa = Ua + ((long long)k * S + si) * W2 + m0;
+            const cf32 *pb0 = Ub + ((long long)k * S + sj0) * W2 + m0;
+            const cf32 *pb1 = Ub + ((long long)k * S + sj1) * W2 + m0;
+            for (int stp = 0; stp < nsteps; ++stp) {
+                float av[8], b0v[8], b1v[8], b0r[8], b1r[8];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int m = 16 * stp + 4 * q + u;
+                    const bool vm = m < K;
+                    const cf32 a = (va && vm) ? pa[m] : cf32{0.f, 0.f};
+                    const cf32 c0 = (vb0 && vm) ? pb0[m] : cf32{0.f, 0.f};
+                    const cf32 c1 = (vb1 && vm) ? pb1[m] : cf32{0.f, 0.f};
+                    av[2 * u] = a.x;
+                    av[2 * u + 1] = a.y;
+                    b0v[2 * u] = c0.x;
+                    b0v[2 * u + 1] = c0.y;
+                    b1v[2 * u] = c1.x;
+                    b1v[2 * u + 1] = c1.y;
+                    b0r[2 * u] = c0.y;  // Im G: (ur_i, ui_i) . (ui_j, -ur_j)
+                    b0r[2 * u + 1] = -c0.x;
+                    b1r[2 * u] = c1.y;
+                    b1r[2 * u + 1] = -c1.x;
+                }
+                h8v ah, al, bh, bl;
+                split8(av, sigma, ah, al);
+                split8(b0v, sigma, bh, bl);
+                r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, r0, 0, 0, 0);
+                r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, r0, 0, 0, 0);
+                r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, r0, 0, 0, 0);
+                split8(b0r, sigma, bh, bl);
+                i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, i0, 0, 0, 0);
+                i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, i0, 0, 0, 0);
+                i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, i0, 0, 0, 0);
+                split8(b1v, sigma, bh, bl);
+                r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, r1, 0, 0, 0);
+                r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, r1, 0, 0, 0);
+                r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, r1, 0, 0, 0);
+                split8(b1r, sigma, bh, bl);
+                i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, i1, 0, 0, 0);
+                i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, i1, 0, 0, 0);
+                i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, i1, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int il = 4 * q + r;
+            stage[(il * 32 + li) * kGramKc + kl] = cf32{r0[r] * inv2, i0[r] * inv2};
+            stage[(il * 32 + li + 16) * kGramKc + kl] = cf32{r1[r] * inv2, i1[r] * inv2};
+        }
+    }
+    __syncthreads();
+    const int kk = threadIdx.x & 15;
+    for (int pl = threadIdx.x >> 4; pl < 16 * 32; pl += 16) {
+        const int i = bi * 16 + (pl >> 5), j = bj * 32 + (pl & 31);
+        if (i >= j || j >= S) continue;
+        const long long p = (long long)i * S - (long long)i * (i + 1) / 2 + (j - i - 1);
+        G[p * ldg + k0 + kk] = stage[pl * kGramKc + kk];
+    }
+}
+
 // ---- per pair: folded spectrum -> real lags ----
 // Z[k] = P + i e^{+2 pi i k/Nz} Q, P = G[k] + conj(G[M-k]), Q = G[k] - conj(G[M-k])
 __global__ void __launch_bounds__(512)
@@ -210,12 +328,12 @@ const int2 *gram_tiles(int S, int &ntiles)
 }
 
 int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, int nk, int mirror, cf32 *d_U,
-                        hipStream_t st)
+                        hipStream_t st, float *d_umax)
 {
     const int SW = S * W;
     dim3 grid((nk + 31) / 32, (SW + 63) / 64);
     ProfScope prof("spectra_to_u", st);
-    hipLaunchKernelGGL(k_spectra_to_u, grid, dim3(256), 0, st, d_F, ldF, SW, W, Nz, nk, mirror, d_U);
+    hipLaunchKernelGGL(k_spectra_to_u, grid, dim3(256), 0, st, d_F, ldF, SW, W, Nz, nk, mirror, d_U, d_umax);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }
@@ -231,6 +349,22 @@ int launch_gram_f32(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int m0, i
     ProfScope prof("gram_f32", st);
     hipLaunchKernelGGL(k_gram_f32, dim3(slots * 8), dim3(256), lds, st, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles,
                        ntiles, nchunks, d_G, ldg);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_gram_split16(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int m0, int K, int nk,
+                        const int2 *d_tiles, int ntiles, const float *d_umax, cf32 *d_G, long long ldg,
+                        hipStream_t st)
+{
+    const int nchunks = (nk + kGramKc - 1) / kGramKc;
+    if (ldg < (long long)nchunks * kGramKc) return -1;
+    const int slots = (nchunks + 7) / 8 * ntiles;
+    size_t lds = sizeof(cf32) * 16 * 32 * kGramKc;
+    allow_lds((const void *)k_gram_split16, lds);
+    ProfScope prof("gram_split16", st);
+    hipLaunchKernelGGL(k_gram_split16, dim3(slots * 8), dim3(256), lds, st, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles,
+                       ntiles, nchunks, d_umax, d_G, ldg);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }

@@ -23,7 +23,7 @@ namespace {
 size_t al(size_t v) { return (v + 255) / 256 * 256; }
 
 struct LargeLayout {
-    size_t Z = 0, F = 0, F2 = 0, mx = 0, E = 0, C = 0, U = 0, U2 = 0, G = 0, total = 0;
+    size_t Z = 0, F = 0, F2 = 0, mx = 0, E = 0, C = 0, U = 0, U2 = 0, G = 0, umax = 0, total = 0;
 };
 
 size_t take(size_t &t, size_t bytes)
@@ -54,7 +54,7 @@ LargeLayout large_layout(int kind, int stack, int S, int W, int N, int Nz)
     LargeLayout l;
     size_t t = 0;
     const size_t SW = (size_t)S * W, P = (size_t)S * (S - 1) / 2;
-    const bool gram = stack == FPCC_STACK_GRAM;
+    const bool gram = stack == FPCC_STACK_GRAM || stack == FPCC_STACK_GRAM16;
     switch (kind) {
     case FPCC_PCC2:
         l.Z = take(t, sizeof(cf32) * SW * N);
@@ -80,6 +80,7 @@ LargeLayout large_layout(int kind, int stack, int S, int W, int N, int Nz)
             l.U = take(t, sizeof(cf32) * (size_t)Nz * SW);
             if (kind == FPCC_CC1B) l.U2 = take(t, sizeof(cf32) * (size_t)Nz * SW);
             l.G = take(t, sizeof(cf32) * P * (size_t)gram_ld(Nz));
+            l.umax = take(t, sizeof(float));
         } else {
             const bool per_window = stack == FPCC_STACK_NONE;
             const long long nb = per_window ? (long long)P * W : (long long)P;
@@ -220,16 +221,22 @@ int large_pair_stage(int kind, int stack, int S, int W, int N, int Nz, int lag0,
     const int norm = kind == FPCC_GNCC ? 1 : kind == FPCC_CC1B ? 2 : 0;
     const cf32 *F = (const cf32 *)(ws + l.F);
     const cf32 *F2 = kind == FPCC_CC1B ? (const cf32 *)(ws + l.F2) : F;
-    if (stack == FPCC_STACK_GRAM) {  // PCC2, 1-bit: stack through the Gram over all Nz bins
+    if (stack == FPCC_STACK_GRAM || stack == FPCC_STACK_GRAM16) {  // PCC2, 1-bit: Gram over all Nz bins
         int ntiles = 0;
         const int2 *tiles = gram_tiles(S, ntiles);
         if (!tiles) return -1;
         cf32 *U = (cf32 *)(ws + l.U), *U2 = kind == FPCC_CC1B ? (cf32 *)(ws + l.U2) : U;
         cf32 *G = (cf32 *)(ws + l.G);
+        float *mx = (float *)(ws + l.umax);
+        const bool s16 = stack == FPCC_STACK_GRAM16;
         const long long ldg = gram_ld(Nz);
-        if ((r = launch_spectra_to_u(F, Nz, S, W, Nz, Nz, 0, U, st))) return r;
-        if (kind == FPCC_CC1B && (r = launch_spectra_to_u(F2, Nz, S, W, Nz, Nz, 0, U2, st))) return r;
-        if ((r = launch_gram_f32(U, U2, S, W, 0, W, Nz, tiles, ntiles, G, ldg, st))) return r;
+        if (s16 && (r = fill_zero(mx, 1, st))) return r;
+        if ((r = launch_spectra_to_u(F, Nz, S, W, Nz, Nz, 0, U, st, s16 ? mx : nullptr))) return r;
+        if (kind == FPCC_CC1B && (r = launch_spectra_to_u(F2, Nz, S, W, Nz, Nz, 0, U2, st, s16 ? mx : nullptr)))
+            return r;
+        r = s16 ? launch_gram_split16(U, U2, S, W, 0, W, Nz, tiles, ntiles, mx, G, ldg, st)
+                : launch_gram_f32(U, U2, S, W, 0, W, Nz, tiles, ntiles, G, ldg, st);
+        if (r) return r;
         RowArgs ra;
         ra.op = ROW_INV_PLAIN;
         ra.src = G;

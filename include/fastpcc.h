@@ -104,7 +104,8 @@ int cuda_tspcc2_set_freq(float **y0, float **x01, float **x02, unsigned int N, u
  * order, p = i*S - i*(i+1)/2 + (j-i-1); station i is x1 (virtual source), j is x2.
  * Output y: stack == FPCC_STACK_NONE -> [P][n_windows][L], else [P][L] holding
  * the arithmetic mean over windows.  Stack modes per kind: PCC2 all; GNCC NONE,
- * PLAIN (its per-window energies forbid stacking spectra); 1-bit NONE, PLAIN, GRAM;
+ * PLAIN (its per-window energies forbid stacking spectra); 1-bit NONE, PLAIN, GRAM,
+ * GRAM16;
  * PCC1 and PCC-v NONE, PLAIN (time-domain sums, N <= 8192); WPCC NONE, PLAIN
  * (= GRAM: the sum over scales and windows is one per-frequency Gram).
  * ------------------------------------------------------------------------- */
@@ -113,7 +114,8 @@ enum {
     FPCC_STACK_NONE = 0,  /* per-window outputs                                       */
     FPCC_STACK_PLAIN = 1, /* mean of per-window correlations (one IFFT per pair-window) */
     FPCC_STACK_ZSUM = 2,  /* sum of folded cross spectra over windows, one IFFT per pair */
-    FPCC_STACK_GRAM = 3   /* per-frequency Hermitian rank-W Gram, one IFFT per pair     */
+    FPCC_STACK_GRAM = 3,  /* per-frequency Hermitian rank-W Gram (FP32 MFMA), one IFFT per pair */
+    FPCC_STACK_GRAM16 = 4 /* the same Gram on FP16 MFMA with split hi+lo operands            */
 };
 
 typedef struct {
