@@ -322,6 +322,70 @@ __device__ C2<T> *fft_lds_batch(C2<T> *buf0, C2<T> *buf1, const FftPlan &plan, c
     return in;
 }
 
+// ---- compile-time FFT: radices, strides and thread count fixed at compile time ----
+// In place in one LDS buffer: each pass reads all of a thread's butterfly inputs
+// into registers, waits, then writes; LDS index i lives at i + i/16 (one pad slot
+// per 16 complex values breaks the power-of-two strides of the early passes).
+// The twiddle table is the runtime plan's (same radix order, checked by the host).
+__host__ __device__ constexpr int ct_pad(int i) { return i + (i >> 4); }
+template <int N>
+constexpr int ct_lds_elems()
+{
+    return N + N / 16 + 1;
+}
+
+template <typename T, int N, int NT, int DIR, int R, int NS>
+__device__ __forceinline__ void ct_pass(C2<T> *lds, const C2<T> *__restrict__ tw)
+{
+    constexpr int NB = N / R;
+    constexpr int BPT = (NB + NT - 1) / NT;
+    C2<T> v[BPT][R];
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+        const int b = threadIdx.x + i * NT;
+        if (NB % NT == 0 || b < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[i][r] = lds[ct_pad(b + r * NB)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+        const int b = threadIdx.x + i * NT;
+        if (NB % NT == 0 || b < NB) {
+            const int k = b % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    C2<T> w = tw[(r - 1) * NS + k];
+                    if (DIR > 0) w.y = -w.y;
+                    v[i][r] = cmul(v[i][r], w);
+                }
+            }
+            Dft<R, DIR>::run(v[i]);
+            const int base = (b - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[ct_pad(base + r * NS)] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int N, int NT, int DIR, int NS, int OFF, int R, int... Rest>
+__device__ __forceinline__ void ct_passes(C2<T> *lds, const C2<T> *__restrict__ tw)
+{
+    ct_pass<T, N, NT, DIR, R, NS>(lds, tw + OFF);
+    if constexpr (sizeof...(Rest) > 0) ct_passes<T, N, NT, DIR, NS * R, OFF + (R - 1) * NS, Rest...>(lds, tw);
+}
+
+// Length N = product of Rs, NT threads, data at lds[ct_pad(i)]; ends with a barrier.
+template <typename T, int N, int NT, int DIR, int... Rs>
+__device__ __forceinline__ void ct_fft(C2<T> *lds, const C2<T> *__restrict__ tw)
+{
+    static_assert((Rs * ... * 1) == N, "radices must multiply to N");
+    ct_passes<T, N, NT, DIR, 1, 0, Rs...>(lds, tw);
+}
+
 // ---- workgroup reductions (wave64) ----
 template <typename T>
 __device__ __forceinline__ T wave_max(T v)

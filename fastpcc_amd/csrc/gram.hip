@@ -293,6 +293,41 @@ k_pair_c2r(const cf32 *__restrict__ G, long long ldg, int Nz, FftPlan planM, con
     }
 }
 
+// Specialisation for Nz = 8192 (M = 4096 = 16^3): compile-time in-place FFT, 35 KB
+// of LDS, four workgroups per CU.
+__global__ void __launch_bounds__(256)
+k_pair_c2r_4096(const cf32 *__restrict__ G, long long ldg, const cf32 *__restrict__ twM,
+                const cf32 *__restrict__ twc, int lag0, int L, float scale, int count_norm, int N,
+                float *__restrict__ y)
+{
+    constexpr int M = 4096, Nz = 8192;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cf32 *lds = reinterpret_cast<cf32 *>(smem_raw);
+    const long long p = blockIdx.x;
+    const cf32 *g = G + p * ldg;
+    for (int k = threadIdx.x; k < M; k += 256) {
+        cf32 a = g[k], c = g[M - k];
+        if (k == 0) {
+            a.y = 0.f;
+            c.y = 0.f;
+        }
+        cf32 P = cf32{a.x + c.x, a.y - c.y};
+        cf32 Q = cf32{a.x - c.x, a.y + c.y};
+        cf32 wq = cmul(twc[k], Q);
+        lds[ct_pad(k)] = cf32{P.x - wq.y, P.y + wq.x};
+    }
+    __syncthreads();
+    ct_fft<float, M, 256, +1, 16, 16, 16>(lds, twM);
+    float *dst = y + p * L;
+    for (int l = threadIdx.x; l < L; l += 256) {
+        const int n = (lag0 + l) & (Nz - 1);
+        cf32 v = lds[ct_pad(n >> 1)];
+        float r = scale * ((n & 1) ? v.y : v.x);
+        if (count_norm) r = (float)((double)r / (double)(N - abs(lag0 + l)));
+        dst[l] = r;
+    }
+}
+
 // ---- host side ----
 static void allow_lds(const void *fn, size_t bytes)
 {
@@ -376,6 +411,15 @@ int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int 
     const DevPlan &pM = get_plan(M, false);
     const void *twc = get_c2r_twiddles(Nz, false);
     if (!pM.ok || !twc) return -1;
+    if (M == 4096 && pM.plan.npass == 3 && pM.plan.radix[0] == 16 && pM.plan.radix[1] == 16 &&
+        pM.plan.radix[2] == 16) {
+        size_t lds = sizeof(cf32) * ct_lds_elems<4096>();
+        ProfScope prof("pair_c2r", st);
+        hipLaunchKernelGGL(k_pair_c2r_4096, dim3(P), dim3(256), lds, st, d_G, gram_ld(Nz / 2 + 1),
+                           (const cf32 *)pM.tw, (const cf32 *)twc, lag0, L, scale, count_norm, N, d_y);
+        FPCC_CHECK(hipGetLastError());
+        return 0;
+    }
     size_t lds = sizeof(cf32) * 2 * (size_t)M;
     allow_lds((const void *)k_pair_c2r, lds);
     ProfScope prof("pair_c2r", st);

@@ -14,6 +14,7 @@
 //   Q_k = (Y[k] - Y[k+M] + conj(Y[-k] - Y[M-k])) / 2,   M = Nz/2,
 //   z = IFFT_M(Z),  y[2m] = Re z[m],  y[2m+1] = Im z[m].
 // The lag window y[(lag0 + l) mod Nz], l < L, is scaled by 1/((float)Nz*(float)N).
+#include <initializer_list>
 #include "engine.hpp"
 #include "kernels.hpp"
 #include "profile.hpp"
@@ -84,6 +85,64 @@ k_station_pcc(const float *__restrict__ x, long long station_stride, int W, int 
     cf32 *Fz = fft_lds<float, -1>(dst, z, planZ, twZ);
     cf32 *out = F + (long long)sw * Nz;
     for (int k = threadIdx.x; k < Nz; k += T) out[k] = Fz[k];
+}
+
+// Station stage specialised for N = 3600, Nz = 8192 (one day at 1 Hz cut in
+// hourly windows): compile-time FFTs (radices 16*9*25 and 16*16*16*2), in place
+// in one padded LDS buffer, so two workgroups share a CU.
+constexpr int kSpN = 3600, kSpNz = 8192, kSpT = 256;
+
+__global__ void __launch_bounds__(kSpT)
+k_station_pcc_3600(const float *__restrict__ x, long long station_stride, int W, const cf32 *__restrict__ twN,
+                   const cf32 *__restrict__ twZ, float da1, cf32 *__restrict__ F)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cf32 *lds = reinterpret_cast<cf32 *>(smem_raw);
+    float *red = reinterpret_cast<float *>(lds + ct_lds_elems<kSpNz>());
+    const int sw = blockIdx.x;
+    const int s = sw / W, w = sw - s * W;
+    const float *in = x + (long long)s * station_stride + (long long)w * kSpN;
+    for (int n = threadIdx.x; n < kSpN; n += kSpT) lds[ct_pad(n)] = cf32{in[n], 0.0f};
+    __syncthreads();
+    ct_fft<float, kSpN, kSpT, -1, 16, 9, 25>(lds, twN);
+    constexpr int half = (kSpN + 1) >> 1;
+    const float two_da1 = 2 * da1;
+    for (int k = threadIdx.x; k < kSpN; k += kSpT) {
+        const float g = k == 0 ? da1 : k < half ? two_da1 : k == (kSpN >> 1) ? da1 : 0.0f;
+        cf32 v = lds[ct_pad(k)];
+        lds[ct_pad(k)] = cf32{v.x * g, v.y * g};
+    }
+    __syncthreads();
+    ct_fft<float, kSpN, kSpT, +1, 16, 9, 25>(lds, twN);
+    float m = 0.0f;
+    for (int n = threadIdx.x; n < kSpN; n += kSpT) {
+        cf32 v = lds[ct_pad(n)];
+        m = fmaxf(m, v.x * v.x + v.y * v.y);
+    }
+    m = block_max(m, red);
+    const float eps = (float)(1e-6 * sqrt((double)m));
+    for (int n = threadIdx.x; n < kSpNz; n += kSpT) {
+        cf32 o = cf32{0.0f, 0.0f};
+        if (n < kSpN) {
+            cf32 v = lds[ct_pad(n)];
+            float r = 1.0f / (sqrtf(v.x * v.x + v.y * v.y) + eps);
+            if (__builtin_isnormal(r)) o = cf32{v.x * r, v.y * r};
+        }
+        lds[ct_pad(n)] = o;
+    }
+    __syncthreads();
+    ct_fft<float, kSpNz, kSpT, -1, 16, 16, 16, 2>(lds, twZ);
+    cf32 *out = F + (long long)sw * kSpNz;
+    for (int k = threadIdx.x; k < kSpNz; k += kSpT) out[k] = lds[ct_pad(k)];
+}
+
+static bool radices_are(const FftPlan &p, std::initializer_list<int> rs)
+{
+    if (p.npass != (int)rs.size()) return false;
+    int i = 0;
+    for (int r : rs)
+        if (p.radix[i++] != r) return false;
+    return true;
 }
 
 // AmpNormf (src/FFTapps.c:178-205) in place on rows of N complex values.
@@ -228,6 +287,15 @@ int launch_station_pcc(const float *d_x, long long station_stride, int S, int W,
         return -1;
     }
     const float da1 = (float)(1 / (double)N);
+    if (N == kSpN && Nz == kSpNz && radices_are(pN.plan, {16, 9, 25}) && radices_are(pZ.plan, {16, 16, 16, 2})) {
+        size_t lds = sizeof(cf32) * ct_lds_elems<kSpNz>() + sizeof(float) * 8;
+        allow_lds((const void *)k_station_pcc_3600, lds);
+        ProfScope prof("station_pcc", st);
+        hipLaunchKernelGGL(k_station_pcc_3600, dim3(S * W), dim3(kSpT), lds, st, d_x, station_stride, W,
+                           (const cf32 *)pN.tw, (const cf32 *)pZ.tw, da1, d_F);
+        FPCC_CHECK(hipGetLastError());
+        return 0;
+    }
     const int threads = 512;
     size_t lds = sizeof(cf32) * 2 * (size_t)Nz + sizeof(float) * 16;
     allow_lds((const void *)k_station_pcc, lds);
