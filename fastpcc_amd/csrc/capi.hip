@@ -102,7 +102,7 @@ int resolve(const fpcc_job *job, Dims &d)
 
 // Workspace regions of a job (byte offsets, 256-B aligned).
 struct Layout {
-    size_t a = 0, b = 0, c = 0, d = 0, e = 0, mx = 0, total = 0;
+    size_t a = 0, b = 0, c = 0, d = 0, e = 0, total = 0;
 };
 
 bool is_gram(int stack) { return stack == FPCC_STACK_GRAM || stack == FPCC_STACK_GRAM16; }
@@ -128,7 +128,6 @@ Layout layout(const fpcc_job *job, const Dims &d)
         if (is_gram(job->stack)) {
             l.b = take(l.total, sizeof(cf32) * K1 * SW * 2);                          // U
             l.c = take(l.total, sizeof(cf32) * d.P * (size_t)gram_ld(d.Nz / 2 + 1));  // G
-            l.mx = take(l.total, sizeof(float));                                      // max |U|
         }
         break;
     case FPCC_GNCC:
@@ -142,7 +141,6 @@ Layout layout(const fpcc_job *job, const Dims &d)
             l.c = take(l.total, sizeof(cf32) * K1 * SW);
             l.d = take(l.total, sizeof(cf32) * K1 * SW);
             l.e = take(l.total, sizeof(cf32) * d.P * (size_t)gram_ld(d.Nz / 2 + 1));
-            l.mx = take(l.total, sizeof(float));
         }
         break;
     case FPCC_PCC1:
@@ -193,12 +191,12 @@ int pair_stage(const fpcc_job *job, const Dims &d, char *ws, float *y, hipStream
             const int2 *tiles = gram_tiles(d.S, ntiles);
             if (!tiles) return -1;
             cf32 *F = (cf32 *)(ws + l.a), *U = (cf32 *)(ws + l.b), *G = (cf32 *)(ws + l.c);
-            float *mx = (float *)(ws + l.mx);
             const bool s16 = job->stack == FPCC_STACK_GRAM16;
+            // |F| <= N: the spectra are sums of N unit-bounded phasors
+            const float sigma = s16 ? gram16_sigma(d.N) : 0.f;
             const int nk = d.Nz / 2 + 1;
-            if (s16 && (r = fill_zero(mx, 1, st))) return r;
-            if ((r = launch_spectra_to_u(F, d.Nz, d.S, d.W, d.Nz, nk, 1, U, st, s16 ? mx : nullptr))) return r;
-            r = s16 ? launch_gram_split16(U, U, d.S, 2 * d.W, 0, 2 * d.W, nk, tiles, ntiles, mx, G, gram_ld(nk), st)
+            if ((r = launch_spectra_to_u(F, d.Nz, d.S, d.W, d.Nz, nk, 1, U, st, sigma))) return r;
+            r = s16 ? launch_gram_split16(U, U, d.S, 2 * d.W, 0, 2 * d.W, nk, tiles, ntiles, sigma, G, gram_ld(nk), st)
                     : launch_gram_f32(U, U, d.S, 2 * d.W, 0, 2 * d.W, nk, tiles, ntiles, G, gram_ld(nk), st);
             if (r) return r;
             // 1/2 of the Gram's Hermitian fold, the reference's 1/((float)Nz*(float)N), the mean over W
@@ -217,17 +215,15 @@ int pair_stage(const fpcc_job *job, const Dims &d, char *ws, float *y, hipStream
             if (!tiles) return -1;
             const long long ld = real_ld(d.Nz);
             cf32 *Ua = (cf32 *)(ws + l.c), *Ub = (cf32 *)(ws + l.d), *G = (cf32 *)(ws + l.e);
-            float *mx = (float *)(ws + l.mx);
             const bool s16 = job->stack == FPCC_STACK_GRAM16;
+            // |F| <= N: spectra of N signs
+            const float sigma = s16 ? gram16_sigma(d.N) : 0.f;
             const int nk = d.Nz / 2 + 1;
-            if (s16 && (r = fill_zero(mx, 1, st))) return r;
-            if ((r = launch_spectra_to_u((const cf32 *)(ws + l.a), ld, d.S, d.W, d.Nz, nk, 0, Ua, st,
-                                         s16 ? mx : nullptr)))
+            if ((r = launch_spectra_to_u((const cf32 *)(ws + l.a), ld, d.S, d.W, d.Nz, nk, 0, Ua, st, sigma)))
                 return r;
-            if ((r = launch_spectra_to_u((const cf32 *)(ws + l.b), ld, d.S, d.W, d.Nz, nk, 0, Ub, st,
-                                         s16 ? mx : nullptr)))
+            if ((r = launch_spectra_to_u((const cf32 *)(ws + l.b), ld, d.S, d.W, d.Nz, nk, 0, Ub, st, sigma)))
                 return r;
-            r = s16 ? launch_gram_split16(Ua, Ub, d.S, d.W, 0, d.W, nk, tiles, ntiles, mx, G, gram_ld(nk), st)
+            r = s16 ? launch_gram_split16(Ua, Ub, d.S, d.W, 0, d.W, nk, tiles, ntiles, sigma, G, gram_ld(nk), st)
                     : launch_gram_f32(Ua, Ub, d.S, d.W, 0, d.W, nk, tiles, ntiles, G, gram_ld(nk), st);
             if (r) return r;
             // ccf_lowlevel's 1/Nz, the mean over W, then 1/(N-|lag|) (gnf_lowlevel with +-1 data)

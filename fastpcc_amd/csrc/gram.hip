@@ -24,6 +24,28 @@
 namespace fpcc {
 
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+
+// hi = fp16(v), lo = fp16(v - hi) of one scaled component.
+__device__ __forceinline__ unsigned short split_hi_lo(float v, unsigned short &lo)
+{
+    const _Float16 h = (_Float16)v;
+    const _Float16 l = (_Float16)(v - (float)h);
+    lo = __builtin_bit_cast(unsigned short, l);
+    return __builtin_bit_cast(unsigned short, h);
+}
+
+// One U element: FP32 complex, or (sigma > 0) the split pair
+// {hi.re | hi.im << 16, lo.re | lo.im << 16} of sigma * value.
+__device__ __forceinline__ uint2 u_element(cf32 v, float sigma)
+{
+    if (sigma == 0.f) return uint2{__float_as_uint(v.x), __float_as_uint(v.y)};
+    unsigned short lr, li;
+    const unsigned short hr = split_hi_lo(v.x * sigma, lr);
+    const unsigned short hi = split_hi_lo(v.y * sigma, li);
+    return uint2{(unsigned)hr | ((unsigned)hi << 16), (unsigned)lr | ((unsigned)li << 16)};
+}
 
 // ---- F[sw][Nz] -> U[k][s][2W], k in [0, M] ----
 // Tile: 32 frequencies x 64 station-windows, staged in LDS for both the direct
@@ -32,13 +54,11 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // conjugated mirror frequency in the second half; mirror = 0 (half spectra of real
 // signals, ldF >= Nz/2+1 bins per row): m in [0, W).
 __global__ void __launch_bounds__(256)
-k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz, int nk, int mirror,
-               cf32 *__restrict__ U, float *__restrict__ umax)
+k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz, int nk, int mirror, float sigma,
+               uint2 *__restrict__ U)
 {
     __shared__ cf32 td[64][33];
     __shared__ cf32 tm[64][33];
-    __shared__ float red[4];
-    float amax = 0.f;
     const int k0 = blockIdx.x * 32;
     const int sw0 = blockIdx.y * 64;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
@@ -55,13 +75,8 @@ k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz,
         }
         td[r][tx] = d;
         tm[r][tx] = m;
-        amax = fmaxf(amax, fmaxf(fmaxf(fabsf(d.x), fabsf(d.y)), fmaxf(fabsf(m.x), fabsf(m.y))));
     }
     __syncthreads();
-    if (umax) {  // largest |component| of U, for the split-FP16 Gram's scaling
-        amax = block_max(amax, red);
-        if (threadIdx.x == 0) atomicMax(reinterpret_cast<int *>(umax), __float_as_int(amax));
-    }
     // write: for each k of the tile, 64 station-windows (contiguous w runs per station)
     const int W2 = mirror ? 2 * W : W;
     const int S = SW / W;
@@ -70,190 +85,223 @@ k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz,
         const int k = k0 + kk, sw = sw0 + r;
         if (k >= nk || sw >= SW) continue;
         const int s = sw / W, w = sw - s * W;
-        cf32 *row = U + ((long long)k * S + s) * W2;
-        row[w] = td[r][kk];
-        if (mirror) row[W + w] = tm[r][kk];
+        uint2 *row = U + ((long long)k * S + s) * W2;
+        row[w] = u_element(td[r][kk], sigma);
+        if (mirror) row[W + w] = u_element(tm[r][kk], sigma);
     }
 }
 
-// ---- Gram on MFMA f32 16x16x4 ----
-// Workgroup: station tile I (16) x J (32) for a chunk of 16 frequencies; wave q
-// computes frequencies k0+q, k0+q+4, ...  Operand layouts (verified on gfx950):
-// A[l%16][l/16], B[l/16][l%16], D[4*(l/16)+r][l%16].
-// K index: (m, component) with m = 4*mm + l/16 over 2W = 4*nmm complex values.
-constexpr int kGramKc = 16;
+// ---- Gram on MFMA ----
+// Workgroup (512 threads, 8 waves): a 64 x 64 block of stations (rows of Ua x
+// rows of Ub) for a chunk of kGF consecutive frequencies.  Per step (one
+// frequency, up to kGK complex K values) both station blocks are staged through
+// LDS (double-buffered, register prefetch of the next step); wave (ti, cj)
+// owns the 16 x 32 sub-block rows 16ti.., columns 32cj.. and keeps its
+// accumulators for all kGF frequencies in registers, so each pair's kGF bins
+// leave as one 64-byte segment (staged through LDS in two halves).
+// SPLIT = 0: FP32 MFMA 16x16x4, A[l%16][l/16], B[l/16][l%16], D[4(l/16)+r][l%16];
+//   K index (m, component), lane quarter q holds m = 8mm + 2q + e (e = 0, 1).
+// SPLIT = 1: U holds sigma-scaled FP16 hi and lo parts; FP16 MFMA 16x16x32 with
+//   A[l%16][8(l/16)+e], B[8(l/16)+e][l%16]; lane quarter q holds the 4 complex
+//   values m = 16st + 4q + u as 8 halves (re, im interleaved).
+//   Re G = Ah.Bh + Ah.Bl + Al.Bh, Im G from the rotated A' = (-ui, ur); the
+//   dropped lo.lo term is below 2^-22 relative.  Results are scaled by oscale.
+// LDS row (dwords): FP32: 2*kGK components; split: kGK hi words then kGK lo words;
+// + 4 dwords of padding, which puts the 16 rows of a 16-byte operand read on
+// distinct banks.
+constexpr int kGB = 64;
+constexpr int kGF = 8;
+constexpr int kGK = 64;
+constexpr int kGRow = 2 * kGK + 4;
+constexpr int kGBuf = 2 * kGB * kGRow;  // dwords per staging buffer (A rows, then B rows)
+constexpr int kGOut = 2 * kGF + 2;      // dwords per staged pair in the epilogue
+constexpr int kGThreads = 512;
+constexpr size_t kGramLds = (size_t)4 *
+    (2 * kGBuf > kGB * (kGB / 2) * kGOut ? 2 * kGBuf : kGB * (kGB / 2) * kGOut);
 
-__global__ void __launch_bounds__(256)
-k_gram_f32(const cf32 *__restrict__ Ua, const cf32 *__restrict__ Ub, int S, int W2, int m0, int K, int nk,
-           const int2 *__restrict__ tiles, int ntiles, int nchunks, cf32 *__restrict__ G, long long ldg)
+__device__ __forceinline__ u4v rot_cplx16(u4v v)
+{
+    // (re, im) -> (-im, re) on each packed FP16 complex value
+    u4v r;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) r[e] = __builtin_amdgcn_alignbit(v[e], v[e], 16) ^ 0x8000u;
+    return r;
+}
+
+template <int SPLIT>
+__global__ void __launch_bounds__(kGThreads)
+k_gram(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, int W2, int m0, int K, int nk,
+       const int2 *__restrict__ blocks, int nblk, int nchunks, float oscale, cf32 *__restrict__ G, long long ldg)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    cf32 *stage = reinterpret_cast<cf32 *>(smem_raw);  // [16*32 pairs][kGramKc]
-    // XCD-aware order: the 8 XCDs each sweep their own frequency chunks so that a
-    // chunk's U rows are fetched into one L2 and shared by all tiles of the chunk.
-    const int b = blockIdx.x;
-    const int xcd = b & 7, slot = b >> 3;
-    const int chunk = (slot / ntiles) * 8 + xcd;
-    const int tile = slot % ntiles;
+    unsigned *lds = reinterpret_cast<unsigned *>(smem_raw);
+    // XCD-aware order: blocks b, b+8, ... run on one XCD; each XCD takes pairs of
+    // adjacent frequency chunks (one 128-byte line of every G row) and all
+    // station blocks of a chunk back to back, sharing the chunk's U rows in L2.
+    const int b = blockIdx.x, xcd = b & 7, slot = b >> 3;
+    const int cp = slot / nblk, blk = slot - cp * nblk;
+    const int chunk = ((cp >> 1) * 8 + xcd) * 2 + (cp & 1);
     if (chunk >= nchunks) return;
-    const int bi = tiles[tile].x, bj = tiles[tile].y;
-    const int k0 = chunk * kGramKc;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int bi = blocks[blk].x, bj = blocks[blk].y;
+    const bool same = bi == bj && Ua == Ub;
+    const int k0 = chunk * kGF;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int li = lane & 15, q = lane >> 4;
-    const int si = bi * 16 + li, sj0 = bj * 32 + li, sj1 = sj0 + 16;
-    const int nmm = (K + 3) / 4;  // K padded with zeros to a multiple of 4 complex values
+    const int ti = wave & 3, cj = wave >> 2;
+    const bool active = !(bi == bj && 2 * cj + 1 < ti);  // diagonal block: skip sub-blocks below it
+    const int nkc = (K + kGK - 1) / kGK;
+    const int nsteps = kGF * nkc;
+    const int nper = same ? kGB * kGK / kGThreads : 2 * kGB * kGK / kGThreads;  // elements per thread
+    constexpr int gran = SPLIT ? 16 : 8;
 
-    for (int t = 0; t < kGramKc / 4; ++t) {
-        const int kl = wave + 4 * t;
-        const int k = k0 + kl;
-        f4v r0 = {0, 0, 0, 0}, i0 = {0, 0, 0, 0}, r1 = {0, 0, 0, 0}, i1 = {0, 0, 0, 0};
-        if (k < nk) {
-            const cf32 *pa = Ua + ((long long)k * S + si) * W2 + m0;
-            const cf32 *pb0 = Ub + ((long long)k * S + sj0) * W2 + m0;
-            const cf32 *pb1 = Ub + ((long long)k * S + sj1) * W2 + m0;
-            const bool va = si < S, vb0 = sj0 < S, vb1 = sj1 < S;
-            for (int mm = 0; mm < nmm; ++mm) {
-                const int m = 4 * mm + q;
-                const bool vm = m < K;
-                cf32 a = (va && vm) ? pa[m] : cf32{0.f, 0.f};
-                cf32 b0 = (vb0 && vm) ? pb0[m] : cf32{0.f, 0.f};
-                cf32 b1 = (vb1 && vm) ? pb1[m] : cf32{0.f, 0.f};
-                r0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b0.x, r0, 0, 0, 0);
-                r0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b0.y, r0, 0, 0, 0);
-                i0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b0.y, i0, 0, 0, 0);
-                i0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, -b0.x, i0, 0, 0, 0);
-                r1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b1.x, r1, 0, 0, 0);
-                r1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b1.y, r1, 0, 0, 0);
-                i1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b1.y, i1, 0, 0, 0);
-                i1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, -b1.x, i1, 0, 0, 0);
+    uint2 pre[2 * kGB * kGK / kGThreads];
+    auto fetch = [&](int s) {
+        const int f = s / nkc, kc = s - f * nkc;
+        const int k = k0 + f;
+        const int kcc = min(kGK, K - kc * kGK);
+#pragma unroll
+        for (int r = 0; r < 2 * kGB * kGK / kGThreads; ++r) {
+            if (r < nper) {
+                const int e = t + kGThreads * r, row = e >> 6, mm = e & 63;
+                const int side = row >> 6;
+                const int st = (side ? bj : bi) * kGB + (row & 63);
+                const uint2 *U = side ? Ub : Ua;
+                uint2 v = uint2{0u, 0u};
+                if (k < nk && st < S && mm < kcc) v = U[((long long)k * S + st) * W2 + m0 + kc * kGK + mm];
+                pre[r] = v;
             }
         }
+    };
+    auto stash = [&](int s, int buf) {
+        const int f = s / nkc, kc = s - f * nkc;
+        (void)f;
+        const int kcc = min(kGK, K - kc * kGK);
+        const int kcg = (kcc + gran - 1) / gran * gran;
+        unsigned *base = lds + buf * kGBuf;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int il = 4 * q + r;
-            stage[(il * 32 + li) * kGramKc + kl] = cf32{r0[r], i0[r]};
-            stage[(il * 32 + li + 16) * kGramKc + kl] = cf32{r1[r], i1[r]};
-        }
-    }
-    __syncthreads();
-    // write pair rows: 16 consecutive threads store the 16 bins of one pair
-    const int kk = threadIdx.x & 15;
-    for (int pl = threadIdx.x >> 4; pl < 16 * 32; pl += 16) {
-        const int i = bi * 16 + (pl >> 5), j = bj * 32 + (pl & 31);
-        if (i >= j || j >= S) continue;
-        const long long p = (long long)i * S - (long long)i * (i + 1) / 2 + (j - i - 1);
-        G[p * ldg + k0 + kk] = stage[pl * kGramKc + kk];
-    }
-}
-
-// ---- Gram on MFMA f16 16x16x32 with split hi+lo operands ----
-// Every U component v (scaled by sigma = 2^e, exact) is split into
-// hi = fp16(v), lo = fp16(v - hi); products use hi*hi + hi*lo + lo*hi with FP32
-// accumulation (the dropped lo*lo is below 2^-22 relative).  sigma puts the
-// largest |component| near 2^14, inside FP16's range with normal lo parts down
-// to 2^-8 of it; results are multiplied by 1/sigma^2.  Operand layouts (verified
-// on gfx950): A[l%16][8*(l/16)+e], B[8*(l/16)+e][l%16]; the K index (m, c)
-// interleaves components so a lane's 8 values are 4 consecutive complex U[m].
-typedef _Float16 h8v __attribute__((ext_vector_type(8)));
-
-__device__ __forceinline__ void split8(const float *v, float sigma, h8v &hi, h8v &lo)
-{
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const float s = v[e] * sigma;
-        const _Float16 h = (_Float16)s;
-        hi[e] = h;
-        lo[e] = (_Float16)(s - (float)h);
-    }
-}
-
-__global__ void __launch_bounds__(256)
-k_gram_split16(const cf32 *__restrict__ Ua, const cf32 *__restrict__ Ub, int S, int W2, int m0, int K, int nk,
-               const int2 *__restrict__ tiles, int ntiles, int nchunks, const float *__restrict__ umax,
-               cf32 *__restrict__ G, long long ldg)
-{
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    cf32 *stage = reinterpret_cast<cf32 *>(smem_raw);  // [16*32 pairs][kGramKc]
-    const int b = blockIdx.x;
-    const int xcd = b & 7, slot = b >> 3;
-    const int chunk = (slot / ntiles) * 8 + xcd;
-    const int tile = slot % ntiles;
-    if (chunk >= nchunks) return;
-    const int bi = tiles[tile].x, bj = tiles[tile].y;
-    const int k0 = chunk * kGramKc;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int li = lane & 15, q = lane >> 4;
-    const int si = bi * 16 + li, sj0 = bj * 32 + li, sj1 = sj0 + 16;
-    const bool va = si < S, vb0 = sj0 < S, vb1 = sj1 < S;
-    // sigma = 2^(14 - ceil(log2 max)), exact; inv2 = 1/sigma^2
-    const float mx = umax[0];
-    int ex = 0;
-    if (mx > 0.f) (void)frexpf(mx, &ex);
-    const float sigma = ldexpf(1.0f, 14 - ex), inv2 = ldexpf(1.0f, 2 * (ex - 14));
-    const int nsteps = (K + 15) / 16;  // 16 complex values (K = 32 reals) per MFMA
-
-    for (int t = 0; t < kGramKc / 4; ++t) {
-        const int kl = wave + 4 * t;
-        const int k = k0 + kl;
-        f4v r0 = {0, 0, 0, 0}, i0 = {0, 0, 0, 0}, r1 = {0, 0, 0, 0}, i1 = {0, 0, 0, 0};
-        if (k < nk) {
-            const cf32 *pa = Ua + ((long long)k * S + si) * W2 + m0;
-            const cf32 *pb0 = Ub + ((long long)k * S + sj0) * W2 + m0;
-            const cf32 *pb1 = Ub + ((long long)k * S + sj1) * W2 + m0;
-            for (int stp = 0; stp < nsteps; ++stp) {
-                float av[8], b0v[8], b1v[8], b0r[8], b1r[8];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int m = 16 * stp + 4 * q + u;
-                    const bool vm = m < K;
-                    const cf32 a = (va && vm) ? pa[m] : cf32{0.f, 0.f};
-                    const cf32 c0 = (vb0 && vm) ? pb0[m] : cf32{0.f, 0.f};
-                    const cf32 c1 = (vb1 && vm) ? pb1[m] : cf32{0.f, 0.f};
-                    av[2 * u] = a.x;
-                    av[2 * u + 1] = a.y;
-                    b0v[2 * u] = c0.x;
-                    b0v[2 * u + 1] = c0.y;
-                    b1v[2 * u] = c1.x;
-                    b1v[2 * u + 1] = c1.y;
-                    b0r[2 * u] = c0.y;  // Im G: (ur_i, ui_i) . (ui_j, -ur_j)
-                    b0r[2 * u + 1] = -c0.x;
-                    b1r[2 * u] = c1.y;
-                    b1r[2 * u + 1] = -c1.x;
+        for (int r = 0; r < 2 * kGB * kGK / kGThreads; ++r) {
+            if (r < nper) {
+                const int e = t + kGThreads * r, row = e >> 6, mm = e & 63;
+                if (mm < kcg) {
+                    unsigned *rp = base + row * kGRow;
+                    if (SPLIT) {
+                        rp[mm] = pre[r].x;
+                        rp[kGK + mm] = pre[r].y;
+                    } else {
+                        *reinterpret_cast<uint2 *>(rp + 2 * mm) = pre[r];
+                    }
                 }
-                h8v ah, al, bh, bl;
-                split8(av, sigma, ah, al);
-                split8(b0v, sigma, bh, bl);
-                r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, r0, 0, 0, 0);
-                r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, r0, 0, 0, 0);
-                r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, r0, 0, 0, 0);
-                split8(b0r, sigma, bh, bl);
-                i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, i0, 0, 0, 0);
-                i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, i0, 0, 0, 0);
-                i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, i0, 0, 0, 0);
-                split8(b1v, sigma, bh, bl);
-                r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, r1, 0, 0, 0);
-                r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, r1, 0, 0, 0);
-                r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, r1, 0, 0, 0);
-                split8(b1r, sigma, bh, bl);
-                i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, i1, 0, 0, 0);
-                i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, i1, 0, 0, 0);
-                i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, i1, 0, 0, 0);
             }
         }
+    };
+
+    f4v acc[kGF][4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int il = 4 * q + r;
-            stage[(il * 32 + li) * kGramKc + kl] = cf32{r0[r] * inv2, i0[r] * inv2};
-            stage[(il * 32 + li + 16) * kGramKc + kl] = cf32{r1[r] * inv2, i1[r] * inv2};
+    for (int f = 0; f < kGF; ++f)
+#pragma unroll
+        for (int a = 0; a < 4; ++a) acc[f][a] = f4v{0.f, 0.f, 0.f, 0.f};
+
+    fetch(0);
+    stash(0, 0);
+    __syncthreads();
+    const int arow = 16 * ti + li;
+    const int brow = (same ? 0 : kGB) + 32 * cj + li;
+#pragma unroll
+    for (int f = 0; f < kGF; ++f) {
+        for (int kc = 0; kc < nkc; ++kc) {
+            const int s = f * nkc + kc;
+            const int buf = s & 1;
+            if (s + 1 < nsteps) fetch(s + 1);
+            const int kcc = min(kGK, K - kc * kGK);
+            const int kcg = (kcc + gran - 1) / gran * gran;
+            const unsigned *base = lds + buf * kGBuf;
+            if (active) {
+                f4v r0 = acc[f][0], i0 = acc[f][1], r1 = acc[f][2], i1 = acc[f][3];
+                if (SPLIT) {
+                    const unsigned *pa = base + arow * kGRow + 4 * q;
+                    const unsigned *pb0 = base + brow * kGRow + 4 * q;
+                    const unsigned *pb1 = pb0 + 16 * kGRow;
+                    for (int stp = 0; stp < kcg / 16; ++stp) {
+                        const u4v ah = *reinterpret_cast<const u4v *>(pa + 16 * stp);
+                        const u4v al = *reinterpret_cast<const u4v *>(pa + kGK + 16 * stp);
+                        const u4v bh0 = *reinterpret_cast<const u4v *>(pb0 + 16 * stp);
+                        const u4v bl0 = *reinterpret_cast<const u4v *>(pb0 + kGK + 16 * stp);
+                        const u4v bh1 = *reinterpret_cast<const u4v *>(pb1 + 16 * stp);
+                        const u4v bl1 = *reinterpret_cast<const u4v *>(pb1 + kGK + 16 * stp);
+                        const h8v Ah = __builtin_bit_cast(h8v, ah), Al = __builtin_bit_cast(h8v, al);
+                        const h8v Rh = __builtin_bit_cast(h8v, rot_cplx16(ah));
+                        const h8v Rl = __builtin_bit_cast(h8v, rot_cplx16(al));
+                        const h8v Bh0 = __builtin_bit_cast(h8v, bh0), Bl0 = __builtin_bit_cast(h8v, bl0);
+                        const h8v Bh1 = __builtin_bit_cast(h8v, bh1), Bl1 = __builtin_bit_cast(h8v, bl1);
+                        r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah, Bh0, r0, 0, 0, 0);
+                        i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Rh, Bh0, i0, 0, 0, 0);
+                        r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah, Bh1, r1, 0, 0, 0);
+                        i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Rh, Bh1, i1, 0, 0, 0);
+                        r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah, Bl0, r0, 0, 0, 0);
+                        i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Rh, Bl0, i0, 0, 0, 0);
+                        r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah, Bl1, r1, 0, 0, 0);
+                        i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Rh, Bl1, i1, 0, 0, 0);
+                        r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al, Bh0, r0, 0, 0, 0);
+                        i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Rl, Bh0, i0, 0, 0, 0);
+                        r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al, Bh1, r1, 0, 0, 0);
+                        i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Rl, Bh1, i1, 0, 0, 0);
+                    }
+                } else {
+                    const float *pa = reinterpret_cast<const float *>(base) + arow * kGRow + 4 * q;
+                    const float *pb0 = reinterpret_cast<const float *>(base) + brow * kGRow + 4 * q;
+                    const float *pb1 = pb0 + 16 * kGRow;
+                    for (int mm = 0; mm < kcg / 8; ++mm) {
+                        const f4v a = *reinterpret_cast<const f4v *>(pa + 16 * mm);  // (re, im) of 2 values
+                        const f4v b0 = *reinterpret_cast<const f4v *>(pb0 + 16 * mm);
+                        const f4v b1 = *reinterpret_cast<const f4v *>(pb1 + 16 * mm);
+#pragma unroll
+                        for (int e = 0; e < 2; ++e) {
+                            const float ar = a[2 * e], ai = a[2 * e + 1];
+                            r0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b0[2 * e], r0, 0, 0, 0);
+                            i0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b0[2 * e + 1], i0, 0, 0, 0);
+                            r1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b1[2 * e], r1, 0, 0, 0);
+                            i1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b1[2 * e + 1], i1, 0, 0, 0);
+                            r0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, b0[2 * e + 1], r0, 0, 0, 0);
+                            i0 = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, b0[2 * e], i0, 0, 0, 0);
+                            r1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, b1[2 * e + 1], r1, 0, 0, 0);
+                            i1 = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, b1[2 * e], i1, 0, 0, 0);
+                        }
+                    }
+                }
+                acc[f][0] = r0;
+                acc[f][1] = i0;
+                acc[f][2] = r1;
+                acc[f][3] = i1;
+            }
+            if (s + 1 < nsteps) stash(s + 1, buf ^ 1);
+            __syncthreads();
         }
     }
-    __syncthreads();
-    const int kk = threadIdx.x & 15;
-    for (int pl = threadIdx.x >> 4; pl < 16 * 32; pl += 16) {
-        const int i = bi * 16 + (pl >> 5), j = bj * 32 + (pl & 31);
-        if (i >= j || j >= S) continue;
-        const long long p = (long long)i * S - (long long)i * (i + 1) / 2 + (j - i - 1);
-        G[p * ldg + k0 + kk] = stage[pl * kGramKc + kk];
+
+    // epilogue: pair (il, jl) of a 64 x 32 half block at dwords [(il*32 + jl)*kGOut, +2*kGF)
+    float *out = reinterpret_cast<float *>(lds);
+    for (int h = 0; h < 2; ++h) {
+        if (cj == h && active) {
+#pragma unroll
+            for (int f = 0; f < kGF; ++f)
+#pragma unroll
+                for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int il = 16 * ti + 4 * q + r, jl = 16 * tt + li;
+                        *reinterpret_cast<float2 *>(out + (il * 32 + jl) * kGOut + 2 * f) =
+                            float2{acc[f][2 * tt][r] * oscale, acc[f][2 * tt + 1][r] * oscale};
+                    }
+        }
+        __syncthreads();
+        for (int e = t; e < kGB * 32 * kGF; e += kGThreads) {
+            const int pl = e >> 3, f = e & 7;
+            const int i = bi * kGB + (pl >> 5), j = bj * kGB + 32 * h + (pl & 31);
+            if (i >= j || j >= S) continue;
+            const long long p = (long long)i * S - (long long)i * (i + 1) / 2 + (j - i - 1);
+            G[p * ldg + k0 + f] = *reinterpret_cast<const cf32 *>(out + pl * kGOut + 2 * f);
+        }
+        __syncthreads();
     }
 }
 
@@ -334,10 +382,10 @@ static void allow_lds(const void *fn, size_t bytes)
     if (bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-long long gram_ld(int nk) { return ((long long)nk + kGramKc - 1) / kGramKc * kGramKc; }
+long long gram_ld(int nk) { return ((long long)nk + 15) / 16 * 16; }
 
-// Station tiles (16 rows x 32 columns) holding at least one pair i < j, as a
-// device table cached per device and station count.
+// Station blocks (64 x 64, bi <= bj) holding at least one pair i < j, as a device
+// table cached per device and station count.
 const int2 *gram_tiles(int S, int &ntiles)
 {
     static std::map<std::pair<int, int>, std::pair<const int2 *, int>> cache;
@@ -351,9 +399,8 @@ const int2 *gram_tiles(int S, int &ntiles)
         return it->second.first;
     }
     std::vector<int2> t;
-    for (int bi = 0; bi * 16 < S; ++bi)
-        for (int bj = 0; bj * 32 < S; ++bj)
-            if (bj * 32 + 31 > bi * 16) t.push_back(int2{bi, bj});
+    for (int bi = 0; bi * kGB < S; ++bi)
+        for (int bj = bi; bj * kGB < S; ++bj) t.push_back(int2{bi, bj});
     void *d = nullptr;
     if (hipMalloc(&d, sizeof(int2) * t.size()) != hipSuccess) return nullptr;
     if (hipMemcpy(d, t.data(), sizeof(int2) * t.size(), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
@@ -362,13 +409,44 @@ const int2 *gram_tiles(int S, int &ntiles)
     return (const int2 *)d;
 }
 
-int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, int nk, int mirror, cf32 *d_U,
-                        hipStream_t st, float *d_umax)
+float gram16_sigma(int bound)
+{
+    int e = 0;
+    while ((1 << e) < bound && e < 30) ++e;
+    return ldexpf(1.0f, 14 - e);
+}
+
+int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, int nk, int mirror, void *d_U,
+                        hipStream_t st, float sigma)
 {
     const int SW = S * W;
     dim3 grid((nk + 31) / 32, (SW + 63) / 64);
     ProfScope prof("spectra_to_u", st);
-    hipLaunchKernelGGL(k_spectra_to_u, grid, dim3(256), 0, st, d_F, ldF, SW, W, Nz, nk, mirror, d_U, d_umax);
+    hipLaunchKernelGGL(k_spectra_to_u, grid, dim3(256), 0, st, d_F, ldF, SW, W, Nz, nk, mirror, sigma,
+                       (uint2 *)d_U);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+static int launch_gram(bool split, const void *d_Ua, const void *d_Ub, int S, int W2, int m0, int K, int nk,
+                       const int2 *d_tiles, int ntiles, float oscale, cf32 *d_G, long long ldg, hipStream_t st)
+{
+    if (S < 2 || K < 1 || nk < 1 || m0 < 0 || m0 + K > W2) return -1;
+    const int nchunks = (nk + kGF - 1) / kGF;
+    if (ldg < (long long)nchunks * kGF) return -1;
+    const long long slots = (long long)(nchunks + 15) / 16 * 2 * ntiles;
+    if (slots * 8 > 0x7fffffffLL) return -1;
+    const void *fn = split ? (const void *)k_gram<1> : (const void *)k_gram<0>;
+    allow_lds(fn, kGramLds);
+    ProfScope prof(split ? "gram_split16" : "gram_f32", st);
+    if (split)
+        hipLaunchKernelGGL(k_gram<1>, dim3((unsigned)(slots * 8)), dim3(kGThreads), kGramLds, st,
+                           (const uint2 *)d_Ua, (const uint2 *)d_Ub, S, W2, m0, K, nk, d_tiles, ntiles, nchunks,
+                           oscale, d_G, ldg);
+    else
+        hipLaunchKernelGGL(k_gram<0>, dim3((unsigned)(slots * 8)), dim3(kGThreads), kGramLds, st,
+                           (const uint2 *)d_Ua, (const uint2 *)d_Ub, S, W2, m0, K, nk, d_tiles, ntiles, nchunks,
+                           oscale, d_G, ldg);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }
@@ -376,32 +454,14 @@ int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, in
 int launch_gram_f32(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int m0, int K, int nk, const int2 *d_tiles,
                     int ntiles, cf32 *d_G, long long ldg, hipStream_t st)
 {
-    const int nchunks = (nk + kGramKc - 1) / kGramKc;
-    if (ldg < (long long)nchunks * kGramKc) return -1;
-    const int slots = (nchunks + 7) / 8 * ntiles;
-    size_t lds = sizeof(cf32) * 16 * 32 * kGramKc;
-    allow_lds((const void *)k_gram_f32, lds);
-    ProfScope prof("gram_f32", st);
-    hipLaunchKernelGGL(k_gram_f32, dim3(slots * 8), dim3(256), lds, st, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles,
-                       ntiles, nchunks, d_G, ldg);
-    FPCC_CHECK(hipGetLastError());
-    return 0;
+    return launch_gram(false, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles, ntiles, 1.0f, d_G, ldg, st);
 }
 
-int launch_gram_split16(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int m0, int K, int nk,
-                        const int2 *d_tiles, int ntiles, const float *d_umax, cf32 *d_G, long long ldg,
-                        hipStream_t st)
+int launch_gram_split16(const void *d_Ua, const void *d_Ub, int S, int W2, int m0, int K, int nk,
+                        const int2 *d_tiles, int ntiles, float sigma, cf32 *d_G, long long ldg, hipStream_t st)
 {
-    const int nchunks = (nk + kGramKc - 1) / kGramKc;
-    if (ldg < (long long)nchunks * kGramKc) return -1;
-    const int slots = (nchunks + 7) / 8 * ntiles;
-    size_t lds = sizeof(cf32) * 16 * 32 * kGramKc;
-    allow_lds((const void *)k_gram_split16, lds);
-    ProfScope prof("gram_split16", st);
-    hipLaunchKernelGGL(k_gram_split16, dim3(slots * 8), dim3(256), lds, st, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles,
-                       ntiles, nchunks, d_umax, d_G, ldg);
-    FPCC_CHECK(hipGetLastError());
-    return 0;
+    if (!(sigma > 0.f)) return -1;
+    return launch_gram(true, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles, ntiles, 1.0f / (sigma * sigma), d_G, ldg, st);
 }
 
 int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int L, float scale, int count_norm,

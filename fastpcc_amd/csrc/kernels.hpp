@@ -27,21 +27,24 @@ int launch_pair_pcc2(const cf32 *d_F, int S, int W, int N, int Nz, int lag0, int
 long long gram_ld(int nk);                       // row length of G for nk bins
 const int2 *gram_tiles(int S, int &ntiles);      // cached device tile table
 // U[k][s][m] for k < nk from rows F[s*W+w] of ldF bins; mirror 1 appends the
-// conjugated bin Nz-k as m = W+w.
-// If d_umax is given (zeroed beforehand) it receives max |component| of U.
-int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, int nk, int mirror, cf32 *d_U,
-                        hipStream_t st, float *d_umax = nullptr);
+// conjugated bin Nz-k as m = W+w.  sigma = 0: FP32 complex elements; sigma > 0:
+// 8-byte split elements {fp16 hi re|im, fp16 lo re|im} of sigma * value for
+// launch_gram_split16.
+int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, int nk, int mirror, void *d_U,
+                        hipStream_t st, float sigma = 0.f);
 // G[p][k] = sum_{m0 <= m < m0+K} conj(Ua[k][i][m]) Ub[k][j][m] for pairs i < j, k < nk
-// (U rows hold W2 values per station).
+// (U rows hold W2 values per station); bins nk .. ldg-1 of the last chunk are zeroed.
 int launch_gram_f32(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int m0, int K, int nk, const int2 *d_tiles,
                     int ntiles, cf32 *d_G, long long ldg, hipStream_t st);
-// The same Gram on FP16 MFMA with split hi+lo operands scaled by a power of two
-// from *d_umax (max |component| over Ua and Ub).
-int launch_gram_split16(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int m0, int K, int nk,
-                        const int2 *d_tiles, int ntiles, const float *d_umax, cf32 *d_G, long long ldg,
-                        hipStream_t st);
-// y[p][l] = scale * c2r(G[p])[(lag0+l) mod Nz], divided by N-|lag| if count_norm
+// The same Gram on FP16 MFMA from split elements scaled by sigma (results are
+// divided by sigma^2).
+int launch_gram_split16(const void *d_Ua, const void *d_Ub, int S, int W2, int m0, int K, int nk,
+                        const int2 *d_tiles, int ntiles, float sigma, cf32 *d_G, long long ldg, hipStream_t st);
+// Power-of-two split scale for |component| <= bound: largest scaled value <= 2^14,
+// inside FP16's range with lo parts normal down to 2^-8 of it.
+float gram16_sigma(int bound);
 int fill_zero(float *p, long long n, hipStream_t st);
+// y[p][l] = scale * c2r(G[p])[(lag0+l) mod Nz], divided by N-|lag| if count_norm
 int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int L, float scale, int count_norm,
                     float *d_y, hipStream_t st);
 

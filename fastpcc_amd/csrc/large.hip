@@ -23,7 +23,7 @@ namespace {
 size_t al(size_t v) { return (v + 255) / 256 * 256; }
 
 struct LargeLayout {
-    size_t Z = 0, F = 0, F2 = 0, mx = 0, E = 0, C = 0, U = 0, U2 = 0, G = 0, umax = 0, total = 0;
+    size_t Z = 0, F = 0, F2 = 0, mx = 0, E = 0, C = 0, U = 0, U2 = 0, G = 0, total = 0;
 };
 
 size_t take(size_t &t, size_t bytes)
@@ -80,7 +80,6 @@ LargeLayout large_layout(int kind, int stack, int S, int W, int N, int Nz)
             l.U = take(t, sizeof(cf32) * (size_t)Nz * SW);
             if (kind == FPCC_CC1B) l.U2 = take(t, sizeof(cf32) * (size_t)Nz * SW);
             l.G = take(t, sizeof(cf32) * P * (size_t)gram_ld(Nz));
-            l.umax = take(t, sizeof(float));
         } else {
             const bool per_window = stack == FPCC_STACK_NONE;
             const long long nb = per_window ? (long long)P * W : (long long)P;
@@ -227,14 +226,13 @@ int large_pair_stage(int kind, int stack, int S, int W, int N, int Nz, int lag0,
         if (!tiles) return -1;
         cf32 *U = (cf32 *)(ws + l.U), *U2 = kind == FPCC_CC1B ? (cf32 *)(ws + l.U2) : U;
         cf32 *G = (cf32 *)(ws + l.G);
-        float *mx = (float *)(ws + l.umax);
         const bool s16 = stack == FPCC_STACK_GRAM16;
+        // |F| <= N: spectra of N unit-bounded phasors or signs
+        const float sigma = s16 ? gram16_sigma(N) : 0.f;
         const long long ldg = gram_ld(Nz);
-        if (s16 && (r = fill_zero(mx, 1, st))) return r;
-        if ((r = launch_spectra_to_u(F, Nz, S, W, Nz, Nz, 0, U, st, s16 ? mx : nullptr))) return r;
-        if (kind == FPCC_CC1B && (r = launch_spectra_to_u(F2, Nz, S, W, Nz, Nz, 0, U2, st, s16 ? mx : nullptr)))
-            return r;
-        r = s16 ? launch_gram_split16(U, U2, S, W, 0, W, Nz, tiles, ntiles, mx, G, ldg, st)
+        if ((r = launch_spectra_to_u(F, Nz, S, W, Nz, Nz, 0, U, st, sigma))) return r;
+        if (kind == FPCC_CC1B && (r = launch_spectra_to_u(F2, Nz, S, W, Nz, Nz, 0, U2, st, sigma))) return r;
+        r = s16 ? launch_gram_split16(U, U2, S, W, 0, W, Nz, tiles, ntiles, sigma, G, ldg, st)
                 : launch_gram_f32(U, U2, S, W, 0, W, Nz, tiles, ntiles, G, ldg, st);
         if (r) return r;
         RowArgs ra;
