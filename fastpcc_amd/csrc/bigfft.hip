@@ -2,6 +2,8 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <complex>
+#include <vector>
 #include "bigfft.hpp"
 #include "profile.hpp"
 
@@ -55,6 +57,89 @@ const BigPlan &big_plan(int n)
     return cache.emplace(std::make_pair(dev, n), make_big_plan(n)).first->second;
 }
 
+// FP64 radix-2 forward FFT (e^{-2 pi i jk/n}) in place, n a power of two; host only,
+// for the Bluestein kernel spectrum.
+static void host_fft_pow2(std::vector<std::complex<double>> &a)
+{
+    const size_t n = a.size();
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    std::vector<std::complex<double>> w(n / 2);
+    for (size_t j = 0; j < n / 2; ++j) {
+        const double ang = -2.0 * M_PI * (double)j / (double)n;
+        w[j] = std::complex<double>(std::cos(ang), std::sin(ang));
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const size_t step = n / len;
+        for (size_t i = 0; i < n; i += len)
+            for (size_t k = 0; k < len / 2; ++k) {
+                const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w[k * step];
+                a[i + k] = u + v;
+                a[i + k + len / 2] = u - v;
+            }
+    }
+}
+
+static Bluestein make_bluestein(int N)
+{
+    Bluestein bs;
+    if (N < 2) return bs;
+    int n = 64;
+    while (n < 2 * N - 1) n <<= 1;
+    if (!big_plan(n).ok) return bs;
+    // w_m = exp(-i pi m^2 / N), with m^2 reduced mod 2N exactly
+    std::vector<std::complex<double>> w(N), b(n, 0.0);
+    for (int m = 0; m < N; ++m) {
+        const long long q = ((long long)m * m) % (2LL * N);
+        const double ang = -M_PI * (double)q / (double)N;
+        w[m] = std::complex<double>(std::cos(ang), std::sin(ang));
+    }
+    b[0] = std::conj(w[0]);
+    for (int m = 1; m < N; ++m) b[m] = b[n - m] = std::conj(w[m]);
+    host_fft_pow2(b);
+    std::vector<cf32> wf(N), tf(n);
+    for (int m = 0; m < N; ++m) wf[m] = cf32{(float)w[m].real(), (float)w[m].imag()};
+    for (int k = 0; k < n; ++k) tf[k] = cf32{(float)(b[k].real() / n), (float)(b[k].imag() / n)};
+    void *dw = nullptr, *dt = nullptr;
+    if (hipMalloc(&dw, sizeof(cf32) * N) != hipSuccess) return bs;
+    if (hipMalloc(&dt, sizeof(cf32) * n) != hipSuccess) {
+        (void)hipFree(dw);
+        return bs;
+    }
+    if (hipMemcpy(dw, wf.data(), sizeof(cf32) * N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dt, tf.data(), sizeof(cf32) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(dw);
+        (void)hipFree(dt);
+        return bs;
+    }
+    bs.N = N;
+    bs.n = n;
+    bs.chirp = (const cf32 *)dw;
+    bs.T = (const cf32 *)dt;
+    bs.ok = true;
+    return bs;
+}
+
+const Bluestein &bluestein(int N)
+{
+    static std::map<std::pair<int, int>, Bluestein> cache;
+    static std::mutex mu;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find({dev, N});
+        if (it != cache.end()) return it->second;
+    }
+    Bluestein bs = make_bluestein(N);  // big_plan takes its own lock
+    std::lock_guard<std::mutex> lk(mu);
+    return cache.emplace(std::make_pair(dev, N), bs).first->second;
+}
+
 __device__ __forceinline__ void pair_of_big(int p, int S, int &i, int &j)
 {
     double b = 2.0 * S - 1.0;
@@ -87,7 +172,7 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
     if (op == COL_FWD_AMPNORM) eps = 1e-6f * a.maxbuf[b];
     if (op == COL_FWD_AMPNORMF) eps = (float)(1e-6 * sqrt((double)a.maxbuf[b]));
     const float *xr = nullptr;
-    if (op == COL_FWD_INPUT) {
+    if (op == COL_FWD_INPUT || op == COL_FWD_CHIRP) {
         const int s = b / a.W, w = b - s * a.W;
         xr = a.x + (long long)s * a.x_stride + (long long)w * a.N;
     }
@@ -97,7 +182,7 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
         cf32 v = cf32{0.f, 0.f};
         if (c < N2) {
             const long long g = (long long)r * N2 + c;
-            if (op == COL_FWD_INPUT) {
+            if (op == COL_FWD_INPUT || op == COL_FWD_CHIRP) {
                 const long long m = g - a.off;
                 if (m >= 0 && m < a.N) {
                     const float xv = xr[m];
@@ -106,6 +191,7 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
                     else if (a.sign_rule == 2) s = xv > 0.f ? 1.f : -1.f;
                     else s = xv * a.in_scale;
                     v = cf32{s, 0.f};
+                    if (op == COL_FWD_CHIRP) v = cf32{s * a.chirp[m].x, s * a.chirp[m].y};
                 }
             } else if (op == COL_FWD_AMPNORM) {
                 const cf32 u = a.src[sbase + g];
@@ -124,9 +210,28 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
         buf0[t * ld + r] = v;
     }
     __syncthreads();
-    const bool inv = op == COL_INV_MAX || op == COL_INV_LAGWIN;
+    const bool inv = op == COL_INV_MAX || op == COL_INV_LAGWIN || op == COL_CHIRP_MID || op == COL_INV_CHIRP_MAX;
     cf32 *res = inv ? fft_lds_batch<float, +1>(buf0, buf1, p1, tw1, kBigCols, ld)
                     : fft_lds_batch<float, -1>(buf0, buf1, p1, tw1, kBigCols, ld);
+    if (op == COL_CHIRP_MID) {
+        // second transform's chirped input: conj(mask X) w = mask conj(conv) (|w| = 1)
+        const int half = (a.N + 1) >> 1;
+        for (int idx = threadIdx.x; idx < kBigCols * N1; idx += blockDim.x) {
+            const int t = idx % kBigCols, r = idx / kBigCols;
+            const long long g = (long long)r * N2 + c0 + t;
+            float gm = 0.f;
+            if (c0 + t < N2 && g < a.N) {
+                if (g == 0) gm = a.da1;
+                else if (g < half) gm = 2 * a.da1;
+                else if (!(a.N & 1) && g == (a.N >> 1)) gm = a.da1;
+            }
+            const cf32 v = res[t * ld + r];
+            res[t * ld + r] = cf32{gm * v.x, -gm * v.y};
+        }
+        __syncthreads();
+        res = fft_lds_batch<float, -1>(res, res == buf0 ? buf1 : buf0, p1, tw1, kBigCols, ld);
+    }
+    const bool fwd_out = !inv || op == COL_CHIRP_MID;
     float m = 0.f;
     // LAGWIN batch decode
     int p = 0, w = 0, i = 0, j = 0;
@@ -142,8 +247,15 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
         if (c >= N2) continue;
         const long long g = (long long)r * N2 + c;
         cf32 v = res[t * ld + r];
-        if (!inv) {
+        if (fwd_out) {
             a.dst[base + g] = cmul(v, twN[twidx(c, r)]);  // W_n^{n2 k1}
+        } else if (op == COL_INV_CHIRP_MAX) {
+            if (g < a.N) {
+                const cf32 d = cmul(a.chirp[g], v);
+                const cf32 z = cf32{d.x, -d.y};
+                a.dst[(long long)b * a.dst_bstride + g] = z;
+                m = fmaxf(m, z.x * z.x + z.y * z.y);
+            }
         } else if (op == COL_INV_MAX) {
             a.dst[base + g] = v;
             m = fmaxf(m, v.x * v.x + v.y * v.y);
@@ -170,7 +282,7 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
             }
         }
     }
-    if (op == COL_INV_MAX) {
+    if (op == COL_INV_MAX || op == COL_INV_CHIRP_MAX) {
         m = block_max(m, red);
         if (threadIdx.x == 0) atomicMax(reinterpret_cast<int *>(a.maxbuf + b), __float_as_int(m));
     }
@@ -226,6 +338,14 @@ k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, 
                      : fft_lds_batch<float, -1>(buf0, buf1, p2, tw2, kBigRows, N2));
     const float sc = (op == ROW_FWD_SCALE) ? a.wscale[b % a.S] : 1.0f;
     const int half = (a.N + 1) >> 1;
+    if (op == ROW_MUL_INV) {
+        for (int idx = threadIdx.x; idx < kBigRows * N2; idx += blockDim.x) {
+            const int rr = idx / N2, c = idx - rr * N2;
+            if (r0 + rr < N1) res[idx] = cmul(res[idx], a.T[(r0 + rr) + (long long)N1 * c]);
+        }
+        __syncthreads();
+        res = fft_lds_batch<float, +1>(res, res == buf0 ? buf1 : buf0, p2, tw2, kBigRows, N2);
+    }
     if (op == ROW_FWD_MASK || op == ROW_MASK_INV) {
         for (int idx = threadIdx.x; idx < kBigRows * N2; idx += blockDim.x) {
             const int rr = idx / N2, c = idx - rr * N2;
@@ -243,7 +363,7 @@ k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, 
             res = fft_lds_batch<float, +1>(res, other, p2, tw2, kBigRows, N2);
         }
     }
-    const bool inv_out = inv || op == ROW_MASK_INV;
+    const bool inv_out = inv || op == ROW_MASK_INV || op == ROW_MUL_INV;
     for (int idx = threadIdx.x; idx < kBigRows * N2; idx += blockDim.x) {
         const int rr = idx / N2, c = idx - rr * N2;
         const int r = r0 + rr;

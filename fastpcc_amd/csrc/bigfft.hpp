@@ -31,6 +31,11 @@ enum ColOp {
     COL_FWD_AMPNORMF = 3,    // src / (|src| + 1e-6 sqrt(max)), 0 past nvalid (AmpNormf + zero pad)
     COL_INV_MAX = 4,         // inverse, natural out, per-batch max|z|^2 into maxbuf
     COL_INV_LAGWIN = 5,      // inverse, Re of the lag window into y (normalised)
+    // Bluestein analytic signal of N samples on an n-point grid (n >= 2N-1), chirp
+    // w_m = exp(-i pi m^2 / N):
+    COL_FWD_CHIRP = 6,       // x[g] * w_g (g < N), forward
+    COL_CHIRP_MID = 7,       // inverse -> mask_g * conj(conv_g) (g < N) -> forward
+    COL_INV_CHIRP_MAX = 8,   // inverse -> z_g = conj(w_g conv_g), g < N, into dst rows of dst_bstride; max|z|^2
 };
 
 enum RowOp {
@@ -41,6 +46,7 @@ enum RowOp {
     ROW_INV_PLAIN = 4,
     ROW_INV_CROSS = 5,  // inverse of conj(F[i,w]) F2[j,w] for the batch's pair and window (or summed over w)
     ROW_MASK_INV = 6,   // forward row FFT, analytic mask, inverse row FFT (both row passes of the analytic signal)
+    ROW_MUL_INV = 7,    // forward row FFT, * T[natural bin], inverse row FFT (Bluestein convolution)
 };
 
 struct ColArgs {
@@ -57,6 +63,10 @@ struct ColArgs {
     // COL_INV_MAX writes, COL_FWD_AMPNORM(F) read max|z|^2 per batch
     float *maxbuf = nullptr;
     int nvalid = 0;  // COL_FWD_AMPNORMF: valid samples per src row
+    // Bluestein ops: chirp table (N values), mask scale, output row stride
+    const cf32 *chirp = nullptr;
+    float da1 = 0.f;
+    long long dst_bstride = 0;
     // COL_INV_LAGWIN: batch b -> pair p = b / Wb, window w = wsel + b % Wb
     float *y = nullptr;
     int lag0 = 0, L = 0;
@@ -79,11 +89,21 @@ struct RowArgs {
     const float *wscale = nullptr;  // ROW_FWD_SCALE
     int N = 0;                      // ROW_FWD_MASK
     float da1 = 0.f;
+    const cf32 *T = nullptr;        // ROW_MUL_INV: n values in natural bin order
     // ROW_INV_CROSS: spectra F[s*Wfull + w][n] (station i) and F2 (station j, F if null);
     // batch b -> pair p = b / Wb, window wsel + b % Wb; wsum != 0 sums over all Wfull windows
     const cf32 *F = nullptr, *F2 = nullptr;
     int Sst = 2, Wb = 1, wsel = 0, Wfull = 1, wsum = 0, pair0 = 0;  // pair p = pair0 + b / Wb
 };
+
+// Bluestein tables for an N-point DFT on the n-point grid: chirp w_m (m < N) and
+// T = FFT_n(b)/n, b = conj(w) wrapped circularly; FP64 on the host, cached per device.
+struct Bluestein {
+    int N = 0, n = 0;
+    const cf32 *chirp = nullptr, *T = nullptr;
+    bool ok = false;
+};
+const Bluestein &bluestein(int N);
 
 int run_cols(const BigPlan &bp, const ColArgs &a, int nbatch, hipStream_t st, const char *name);
 int run_rows(const BigPlan &bp, const RowArgs &a, int nbatch, hipStream_t st, const char *name);

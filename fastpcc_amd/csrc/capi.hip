@@ -41,7 +41,8 @@ struct Dims {
 };
 
 // The single-workgroup kernels need Nz <= 8192 and, for the analytic signal, a
-// 7-smooth N <= 8192; anything else goes through the two-pass FFTs.
+// 7-smooth N <= 8192; anything else goes through the two-pass FFTs (Bluestein for
+// analytic signals of other lengths).
 bool needs_large(int kind, int N, unsigned Nz)
 {
     if (kind == FPCC_WPCC) return false;
@@ -87,6 +88,10 @@ int resolve(const fpcc_job *job, Dims &d)
     d.P = (size_t)d.S * (d.S - 1) / 2;
     d.fam = nullptr;
     d.large = needs_large(job->kind, d.N, d.Nz);
+    // The two-pass transforms start at 64 points.  A longer zero pad leaves the
+    // correlations unchanged: the 1/(Nz N) and 1/Nz scales cancel the unnormalised
+    // inverse FFT and Nz >= N + M keeps the lag window free of wrap-around.
+    if (d.large && d.Nz < 64) d.Nz = 64;
     if (d.large && !large_supported(job->kind, d.N, (int)d.Nz)) {
         fprintf(stderr, "fastpcc: unsupported window length N=%d (Nz=%u)\n", d.N, d.Nz);
         return -1;
@@ -318,7 +323,7 @@ int pcc_time_host(int pcc1, float **y, const float *const *x1, const float *cons
     } else {
         // the phasors lead the large-window workspace (rows of N complex values)
         if (!large_supported(FPCC_PCC1, N, N)) {
-            fprintf(stderr, "fastpcc: analytic signal of N=%d needs a 7-smooth length\n", N);
+            fprintf(stderr, "fastpcc: no analytic-signal plan for N=%d\n", N);
             return -1;
         }
         FPCC_CHECK(hipMalloc(&dphi.p, large_workspace(FPCC_PCC1, FPCC_STACK_NONE, 2, (int)Tr, N, N)));

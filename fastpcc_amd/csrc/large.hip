@@ -1,8 +1,10 @@
 // Station and pair stages for windows longer than one workgroup's LDS
-// (N > 8192 or Nz > 8192), on the two-pass FFTs of bigfft.hip.
+// (N > 8192 or Nz > 8192) and for analytic signals of lengths that are not 7-smooth,
+// on the two-pass FFTs of bigfft.hip.
 //
-// PCC2: analytic signal as DFT_N -> mask -> IDFT_N (N 7-smooth, four-step with the
-//   mask and both row passes fused), max|z|^2 per window, AmpNormf and zero pad
+// PCC2: analytic signal as DFT_N -> mask -> IDFT_N (7-smooth N: four-step with the
+//   mask and both row passes fused; any other N: Bluestein chirp convolutions on a
+//   power-of-two grid), max|z|^2 per window, AmpNormf and zero pad
 //   fused into the first pass of FFT_Nz; spectra stay in permuted bin order.
 // PCC1 / PCC-v: the same analytic signal, AmpNormf per row, then the time-domain kernel.
 // GNCC / 1-bit: complex FFT_Nz of the zero-padded samples (or signs); energies as
@@ -23,8 +25,18 @@ namespace {
 size_t al(size_t v) { return (v + 255) / 256 * 256; }
 
 struct LargeLayout {
-    size_t Z = 0, F = 0, F2 = 0, mx = 0, E = 0, C = 0, U = 0, U2 = 0, G = 0, total = 0;
+    size_t Z = 0, F = 0, F2 = 0, mx = 0, E = 0, C = 0, U = 0, U2 = 0, G = 0, B = 0, total = 0;
 };
+
+// Analytic signals of lengths without a two-pass split run as Bluestein
+// convolutions on a power-of-two grid of bluestein_n(N) points.
+int bluestein_n(int N)
+{
+    if (big_plan(N).ok) return 0;
+    int n = 64;
+    while (n < 2 * N - 1) n <<= 1;
+    return n;
+}
 
 size_t take(size_t &t, size_t bytes)
 {
@@ -60,11 +72,13 @@ LargeLayout large_layout(int kind, int stack, int S, int W, int N, int Nz)
         l.Z = take(t, sizeof(cf32) * SW * N);
         l.F = take(t, sizeof(cf32) * SW * Nz);
         l.mx = take(t, sizeof(float) * SW);
+        l.B = take(t, sizeof(cf32) * SW * (size_t)bluestein_n(N));
         break;
     case FPCC_PCC1:
     case FPCC_PCCV:
         l.Z = take(t, sizeof(cf32) * SW * N);
         l.mx = take(t, sizeof(float) * SW);
+        l.B = take(t, sizeof(cf32) * SW * (size_t)bluestein_n(N));
         break;
     case FPCC_GNCC:
         l.F = take(t, sizeof(cf32) * SW * Nz);
@@ -93,9 +107,10 @@ LargeLayout large_layout(int kind, int stack, int S, int W, int N, int Nz)
 
 bool large_supported(int kind, int N, int Nz)
 {
-    if (!big_plan(Nz).ok) return false;
-    if (kind == FPCC_PCC2 || kind == FPCC_PCC1 || kind == FPCC_PCCV) return big_plan(N).ok;
-    return true;
+    const bool analytic = kind == FPCC_PCC2 || kind == FPCC_PCC1 || kind == FPCC_PCCV;
+    if (analytic && !big_plan(N).ok && !bluestein(N).ok) return false;
+    if (kind == FPCC_PCC1 || kind == FPCC_PCCV) return true;  // time-domain lags: no Nz transforms
+    return big_plan(Nz).ok;
 }
 
 size_t large_workspace(int kind, int stack, int S, int W, int N, int Nz)
@@ -103,14 +118,64 @@ size_t large_workspace(int kind, int stack, int S, int W, int N, int Nz)
     return large_layout(kind, stack, S, W, N, Nz).total;
 }
 
-// analytic signal of every station-window into Z (natural order), max|z|^2 into mx
-static int analytic_big(const float *x, long long stride, int S, int W, int N, cf32 *Z, float *mx, hipStream_t st)
+// Bluestein: z = IDFT_N(mask DFT_N(x)) with both N-point DFTs as chirp
+// convolutions on the n-point grid (five two-pass stages through the scratch B).
+static int analytic_bluestein(const float *x, long long stride, int S, int W, int N, cf32 *Z, float *mx, cf32 *B,
+                              hipStream_t st)
 {
-    const BigPlan &bp = big_plan(N);
-    if (!bp.ok) {
-        fprintf(stderr, "fastpcc: analytic signal of N=%d needs a 7-smooth length\n", N);
+    const Bluestein &bs = bluestein(N);
+    if (!bs.ok) {
+        fprintf(stderr, "fastpcc: no analytic-signal plan for N=%d\n", N);
         return -1;
     }
+    const BigPlan &bp = big_plan(bs.n);
+    const int SW = S * W;
+    int r = fill_zero(mx, SW, st);
+    if (r) return r;
+    ColArgs ca;
+    ca.op = COL_FWD_CHIRP;
+    ca.dst = B;
+    ca.bstride = bs.n;
+    ca.x = x;
+    ca.x_stride = stride;
+    ca.W = W;
+    ca.N = N;
+    ca.chirp = bs.chirp;
+    if ((r = run_cols(bp, ca, SW, st, "bluestein_fwd_cols"))) return r;
+    RowArgs ra;
+    ra.op = ROW_MUL_INV;
+    ra.src = B;
+    ra.dst = B;
+    ra.bstride = bs.n;
+    ra.T = bs.T;
+    if ((r = run_rows(bp, ra, SW, st, "bluestein_conv_rows"))) return r;
+    ca = ColArgs{};
+    ca.op = COL_CHIRP_MID;
+    ca.src = B;
+    ca.dst = B;
+    ca.bstride = bs.n;
+    ca.N = N;
+    ca.da1 = (float)(1 / (double)N);
+    if ((r = run_cols(bp, ca, SW, st, "bluestein_mid_cols"))) return r;
+    if ((r = run_rows(bp, ra, SW, st, "bluestein_conv_rows"))) return r;
+    ca = ColArgs{};
+    ca.op = COL_INV_CHIRP_MAX;
+    ca.src = B;
+    ca.bstride = bs.n;
+    ca.dst = Z;
+    ca.dst_bstride = N;
+    ca.N = N;
+    ca.chirp = bs.chirp;
+    ca.maxbuf = mx;
+    return run_cols(bp, ca, SW, st, "bluestein_inv_cols");
+}
+
+// analytic signal of every station-window into Z (natural order), max|z|^2 into mx
+static int analytic_big(const float *x, long long stride, int S, int W, int N, cf32 *Z, float *mx, cf32 *B,
+                        hipStream_t st)
+{
+    const BigPlan &bp = big_plan(N);
+    if (!bp.ok) return analytic_bluestein(x, stride, S, W, N, Z, mx, B, st);
     const int SW = S * W;
     int r = fill_zero(mx, SW, st);
     if (r) return r;
@@ -151,7 +216,7 @@ int large_station_stage(int kind, const float *x, long long stride, int S, int W
     case FPCC_PCC2: {
         cf32 *Z = (cf32 *)(ws + l.Z), *F = (cf32 *)(ws + l.F);
         float *mx = (float *)(ws + l.mx);
-        if ((r = analytic_big(x, stride, S, W, N, Z, mx, st))) return r;
+        if ((r = analytic_big(x, stride, S, W, N, Z, mx, (cf32 *)(ws + l.B), st))) return r;
         ColArgs ca;
         ca.op = COL_FWD_AMPNORMF;
         ca.src = Z;
@@ -171,7 +236,7 @@ int large_station_stage(int kind, const float *x, long long stride, int S, int W
     case FPCC_PCC1:
     case FPCC_PCCV: {
         cf32 *Z = (cf32 *)(ws + l.Z);
-        if ((r = analytic_big(x, stride, S, W, N, Z, (float *)(ws + l.mx), st))) return r;
+        if ((r = analytic_big(x, stride, S, W, N, Z, (float *)(ws + l.mx), (cf32 *)(ws + l.B), st))) return r;
         return launch_ampnormf_rows(Z, SW, N, st);
     }
     case FPCC_GNCC:
