@@ -34,6 +34,50 @@ class FpccJob(ctypes.Structure):
     ]
 
 
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
+
+
+class ServerConfig(ctypes.Structure):
+    """Mirror of fpcc_server_config."""
+    _fields_ = [
+        ("device_bytes", ctypes.c_size_t),
+        ("pinned_bytes", ctypes.c_size_t),
+        ("max_inflight", ctypes.c_int),
+        ("n_streams", ctypes.c_int),
+        ("max_bypass", ctypes.c_int),
+        ("fail_every", ctypes.c_int),
+    ]
+
+
+class Request(ctypes.Structure):
+    """Mirror of fpcc_request."""
+    _fields_ = [
+        ("job", FpccJob),
+        ("x", ctypes.POINTER(ctypes.c_float)),
+        ("y", ctypes.POINTER(ctypes.c_float)),
+    ]
+
+
+class RequestStatus(ctypes.Structure):
+    """Mirror of fpcc_request_status."""
+    _fields_ = [
+        ("state", ctypes.c_int),
+        ("rc", ctypes.c_int),
+        ("t_submit", ctypes.c_double),
+        ("t_start", ctypes.c_double),
+        ("t_done", ctypes.c_double),
+    ]
+
+
+class ServingStats(ctypes.Structure):
+    """Mirror of fpcc_serving_stats."""
+    _fields_ = [(n, ctypes.c_ulonglong) for n in ("submitted", "completed", "failed", "cancelled", "rejected",
+                                                  "queued", "inflight", "inflight_peak")] + \
+               [(n, ctypes.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "pair_correlations",
+                                               "busy_s")] + \
+               [(n, ctypes.c_size_t) for n in ("device_bytes", "device_used", "device_peak")]
+
+
 _lib = None
 
 
@@ -74,5 +118,20 @@ def lib():
     L.fpcc_allpairs_device.restype = i
     L.fpcc_allpairs.argtypes = [jp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
     L.fpcc_allpairs.restype = i
+    vp, ull = ctypes.c_void_p, ctypes.c_ulonglong
+    L.fpcc_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+    L.fpcc_server_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ull)]
+    L.fpcc_server_poll.argtypes = [vp, ull, ctypes.POINTER(RequestStatus)]
+    L.fpcc_server_wait.argtypes = [vp, ull, i, ctypes.POINTER(RequestStatus)]
+    L.fpcc_server_cancel.argtypes = [vp, ull]
+    L.fpcc_server_forget.argtypes = [vp, ull]
+    L.fpcc_server_stats.argtypes = [vp, ctypes.POINTER(ServingStats)]
+    L.fpcc_server_destroy.argtypes = [vp]
+    L.fpcc_host_alloc.argtypes = [ctypes.c_size_t]
+    L.fpcc_host_alloc.restype = vp
+    L.fpcc_host_free.argtypes = [vp]
+    L.fpcc_host_free.restype = None
+    for name in ("create", "submit", "poll", "wait", "cancel", "forget", "stats", "destroy"):
+        getattr(L, "fpcc_server_" + name).restype = i
     _lib = L
     return L

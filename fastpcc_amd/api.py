@@ -236,3 +236,115 @@ class DeviceRunner:
         if r != 0:
             raise FastPCCError(r, "fpcc_allpairs_device")
         return self.y.view(output_shape(self.job))
+
+
+_ERR[-4] = "request exceeds the server's memory budget"
+
+
+class _PinnedBlock:
+    def __init__(self, nbytes):
+        self.p = _lib.lib().fpcc_host_alloc(nbytes)
+        if not self.p:
+            raise FastPCCError(-1, "fpcc_host_alloc")
+
+    def __del__(self):
+        if self.p:
+            _lib.lib().fpcc_host_free(self.p)
+            self.p = None
+
+
+def pinned_empty(shape, dtype=np.float32):
+    """numpy array in pinned host memory of the library's HIP runtime (freed with
+    the last view), for Server buffers copied by DMA."""
+    dtype = np.dtype(dtype)
+    n = int(np.prod(shape))
+    nbytes = max(n * dtype.itemsize, 1)
+    blk = _PinnedBlock(nbytes)
+    buf = (ctypes.c_byte * nbytes).from_address(blk.p)
+    buf._owner = blk
+    return np.frombuffer(buf, dtype=dtype, count=n).reshape(shape)
+
+
+class Server:
+    """Request server over fpcc_server_* (include/fastpcc.h): continuous batching
+    of all-pairs jobs with admission by HBM footprint.  submit() keeps references
+    to x and y until the request is finished; y is filled in place."""
+
+    def __init__(self, device_bytes=0, pinned_bytes=0, max_inflight=0, n_streams=0, max_bypass=0, fail_every=0):
+        cfg = _lib.ServerConfig(device_bytes, pinned_bytes, max_inflight, n_streams, max_bypass, fail_every)
+        h = ctypes.c_void_p()
+        r = _lib.lib().fpcc_server_create(ctypes.byref(cfg), ctypes.byref(h))
+        if r != 0:
+            raise FastPCCError(r, "fpcc_server_create")
+        self._h = h
+        self._keep = {}
+
+    def submit(self, job, x, y):
+        """x: float32 array whose rows (station_stride apart) hold the job's input;
+        y: float32 array of fpcc_output_len(job) elements.  Returns the request id."""
+        assert x.dtype == np.float32 and x.flags.c_contiguous and y.dtype == np.float32 and y.flags.c_contiguous
+        need = (job.n_stations - 1) * job.station_stride + job.n_windows * job.win_len
+        if x.size < need or y.size < _lib.lib().fpcc_output_len(ctypes.byref(job)):
+            raise ValueError("x or y too small for the job")
+        req = _lib.Request(job, x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                           y.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        rid = ctypes.c_ulonglong()
+        r = _lib.lib().fpcc_server_submit(self._h, ctypes.byref(req), ctypes.byref(rid))
+        if r != 0:
+            raise FastPCCError(r, "fpcc_server_submit")
+        self._keep[rid.value] = (x, y)
+        return rid.value
+
+    def _status(self, st):
+        return {"state": st.state, "rc": st.rc, "t_submit": st.t_submit, "t_start": st.t_start,
+                "t_done": st.t_done}
+
+    def poll(self, rid):
+        st = _lib.RequestStatus()
+        r = _lib.lib().fpcc_server_poll(self._h, rid, ctypes.byref(st))
+        if r != 0:
+            raise FastPCCError(r, "fpcc_server_poll")
+        return self._status(st)
+
+    def wait(self, rid, timeout_ms=-1):
+        """Status dict once the request is final; None on timeout."""
+        st = _lib.RequestStatus()
+        r = _lib.lib().fpcc_server_wait(self._h, rid, timeout_ms, ctypes.byref(st))
+        if r == 1:
+            return None
+        if r != 0:
+            raise FastPCCError(r, "fpcc_server_wait")
+        if st.state >= _lib.REQ_DONE:
+            self._keep.pop(rid, None)
+        return self._status(st)
+
+    def cancel(self, rid):
+        return _lib.lib().fpcc_server_cancel(self._h, rid)
+
+    def forget(self, rid):
+        return _lib.lib().fpcc_server_forget(self._h, rid)
+
+    def stats(self):
+        s = _lib.ServingStats()
+        r = _lib.lib().fpcc_server_stats(self._h, ctypes.byref(s))
+        if r != 0:
+            raise FastPCCError(r, "fpcc_server_stats")
+        return {n: getattr(s, n) for n, _ in s._fields_}
+
+    def close(self):
+        if self._h:
+            _lib.lib().fpcc_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

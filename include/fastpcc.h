@@ -151,6 +151,82 @@ int fpcc_station_stage_device(const fpcc_job *job, const float *d_x, void *d_ws,
 int fpcc_pair_stage_device(const fpcc_job *job, void *d_ws, size_t ws_bytes, float *d_y, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Request server (no counterpart in the reference, whose driver correlates one
+ * pair per process run, src/PCC_fullpair_1b.c:467-499): continuous batching of
+ * all-pairs jobs on the current HIP device.
+ *
+ * fpcc_server_create reserves one device arena (device_bytes; 0 = 40% of free
+ * HBM) and one pinned host staging arena (pinned_bytes; 0 = 512 MiB).  A request
+ * is admitted when its input, output and workspace (fpcc_workspace_size) fit in
+ * the device arena and its pageable buffers fit in the staging arena; requests
+ * are admitted in submission order, except that later ones may pass a request
+ * that does not fit yet, at most max_bypass times (0 = 8).  Up to max_inflight
+ * (0 = 16) requests run at once on n_streams (0 = 4) HIP streams, overlapping
+ * their copies and kernels.  Pinned caller buffers are copied by DMA; pageable
+ * ones through the staging arena.  x must stay valid and unchanged, and y
+ * untouched, until the request reaches a final state.  fail_every > 0 makes
+ * every fail_every-th request fail before its kernels launch (fault injection
+ * for tests).
+ *
+ * Return codes: 0 ok; -1 invalid argument or GPU failure; -3 |lag| >= N;
+ * -4 the request's footprint exceeds the server's arenas (never admissible);
+ * fpcc_server_wait returns 1 on timeout, fpcc_server_cancel 1 if the request had
+ * already finished.  Cancelling a running request drops its result (y is not
+ * written through the staging arena; a pinned y may already hold it).
+ * Latencies run from submission to the result in y.  All functions are
+ * thread-safe; destroy cancels queued requests and waits for running ones.
+ * ------------------------------------------------------------------------- */
+enum { FPCC_REQ_QUEUED = 0, FPCC_REQ_RUNNING = 1, FPCC_REQ_DONE = 2, FPCC_REQ_FAILED = 3, FPCC_REQ_CANCELLED = 4 };
+
+typedef struct {
+    size_t device_bytes;  /* device arena for request buffers (0: 40% of free HBM) */
+    size_t pinned_bytes;  /* pinned staging arena for pageable buffers (0: 512 MiB) */
+    int max_inflight;     /* requests on the device at once (0: 16)               */
+    int n_streams;        /* HIP streams (0: 4)                                   */
+    int max_bypass;       /* times a waiting request may be passed (0: 8)          */
+    int fail_every;       /* fault injection: fail every k-th request (0: off)    */
+} fpcc_server_config;
+
+typedef struct {
+    fpcc_job job;   /* one all-pairs job (layout as fpcc_allpairs)        */
+    const float *x; /* host input: n_stations rows of station_stride floats */
+    float *y;       /* host output: fpcc_output_len(&job) floats            */
+} fpcc_request;
+
+typedef struct {
+    int state;                     /* FPCC_REQ_*                              */
+    int rc;                        /* error code of a failed request          */
+    double t_submit, t_start, t_done; /* seconds since the server was created */
+} fpcc_request_status;
+
+typedef struct {
+    unsigned long long submitted, completed, failed, cancelled, rejected;
+    unsigned long long queued, inflight, inflight_peak;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* latency of completed requests          */
+    double pair_correlations;              /* sum of pairs x windows delivered        */
+    double busy_s;                         /* first admission to last completion      */
+    size_t device_bytes, device_used, device_peak;
+} fpcc_serving_stats;
+
+typedef struct fpcc_server fpcc_server;
+
+/* Pinned (page-locked) host memory from the library's own HIP runtime, for
+ * request buffers the server copies by DMA (NULL on failure). */
+void *fpcc_host_alloc(size_t bytes);
+void fpcc_host_free(void *p);
+
+int fpcc_server_create(const fpcc_server_config *cfg, fpcc_server **server);
+int fpcc_server_submit(fpcc_server *server, const fpcc_request *req, unsigned long long *id);
+int fpcc_server_poll(fpcc_server *server, unsigned long long id, fpcc_request_status *status);
+/* timeout_ms < 0 waits without limit */
+int fpcc_server_wait(fpcc_server *server, unsigned long long id, int timeout_ms, fpcc_request_status *status);
+int fpcc_server_cancel(fpcc_server *server, unsigned long long id);
+/* Drops the record of a finished request (its id becomes unknown). */
+int fpcc_server_forget(fpcc_server *server, unsigned long long id);
+int fpcc_server_stats(fpcc_server *server, fpcc_serving_stats *stats);
+int fpcc_server_destroy(fpcc_server *server);
+
+/* ---------------------------------------------------------------------------
  * Kernel timing for benchmarks (no counterpart in the reference).  While
  * enabled, each kernel launch records a HIP event pair on its stream;
  * fpcc_profile_report waits for them and returns, per kernel name (written
