@@ -2,8 +2,6 @@
 #include <cmath>
 #include <map>
 #include <mutex>
-#include <complex>
-#include <vector>
 #include "bigfft.hpp"
 #include "profile.hpp"
 
@@ -55,89 +53,6 @@ const BigPlan &big_plan(int n)
     auto it = cache.find({dev, n});
     if (it != cache.end()) return it->second;
     return cache.emplace(std::make_pair(dev, n), make_big_plan(n)).first->second;
-}
-
-// FP64 radix-2 forward FFT (e^{-2 pi i jk/n}) in place, n a power of two; host only,
-// for the Bluestein kernel spectrum.
-static void host_fft_pow2(std::vector<std::complex<double>> &a)
-{
-    const size_t n = a.size();
-    for (size_t i = 1, j = 0; i < n; ++i) {
-        size_t bit = n >> 1;
-        for (; j & bit; bit >>= 1) j ^= bit;
-        j ^= bit;
-        if (i < j) std::swap(a[i], a[j]);
-    }
-    std::vector<std::complex<double>> w(n / 2);
-    for (size_t j = 0; j < n / 2; ++j) {
-        const double ang = -2.0 * M_PI * (double)j / (double)n;
-        w[j] = std::complex<double>(std::cos(ang), std::sin(ang));
-    }
-    for (size_t len = 2; len <= n; len <<= 1) {
-        const size_t step = n / len;
-        for (size_t i = 0; i < n; i += len)
-            for (size_t k = 0; k < len / 2; ++k) {
-                const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w[k * step];
-                a[i + k] = u + v;
-                a[i + k + len / 2] = u - v;
-            }
-    }
-}
-
-static Bluestein make_bluestein(int N)
-{
-    Bluestein bs;
-    if (N < 2) return bs;
-    int n = 64;
-    while (n < 2 * N - 1) n <<= 1;
-    if (!big_plan(n).ok) return bs;
-    // w_m = exp(-i pi m^2 / N), with m^2 reduced mod 2N exactly
-    std::vector<std::complex<double>> w(N), b(n, 0.0);
-    for (int m = 0; m < N; ++m) {
-        const long long q = ((long long)m * m) % (2LL * N);
-        const double ang = -M_PI * (double)q / (double)N;
-        w[m] = std::complex<double>(std::cos(ang), std::sin(ang));
-    }
-    b[0] = std::conj(w[0]);
-    for (int m = 1; m < N; ++m) b[m] = b[n - m] = std::conj(w[m]);
-    host_fft_pow2(b);
-    std::vector<cf32> wf(N), tf(n);
-    for (int m = 0; m < N; ++m) wf[m] = cf32{(float)w[m].real(), (float)w[m].imag()};
-    for (int k = 0; k < n; ++k) tf[k] = cf32{(float)(b[k].real() / n), (float)(b[k].imag() / n)};
-    void *dw = nullptr, *dt = nullptr;
-    if (hipMalloc(&dw, sizeof(cf32) * N) != hipSuccess) return bs;
-    if (hipMalloc(&dt, sizeof(cf32) * n) != hipSuccess) {
-        (void)hipFree(dw);
-        return bs;
-    }
-    if (hipMemcpy(dw, wf.data(), sizeof(cf32) * N, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dt, tf.data(), sizeof(cf32) * n, hipMemcpyHostToDevice) != hipSuccess) {
-        (void)hipFree(dw);
-        (void)hipFree(dt);
-        return bs;
-    }
-    bs.N = N;
-    bs.n = n;
-    bs.chirp = (const cf32 *)dw;
-    bs.T = (const cf32 *)dt;
-    bs.ok = true;
-    return bs;
-}
-
-const Bluestein &bluestein(int N)
-{
-    static std::map<std::pair<int, int>, Bluestein> cache;
-    static std::mutex mu;
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        auto it = cache.find({dev, N});
-        if (it != cache.end()) return it->second;
-    }
-    Bluestein bs = make_bluestein(N);  // big_plan takes its own lock
-    std::lock_guard<std::mutex> lk(mu);
-    return cache.emplace(std::make_pair(dev, N), bs).first->second;
 }
 
 __device__ __forceinline__ void pair_of_big(int p, int S, int &i, int &j)
@@ -341,7 +256,7 @@ k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, 
     if (op == ROW_MUL_INV) {
         for (int idx = threadIdx.x; idx < kBigRows * N2; idx += blockDim.x) {
             const int rr = idx / N2, c = idx - rr * N2;
-            if (r0 + rr < N1) res[idx] = cmul(res[idx], a.T[(r0 + rr) + (long long)N1 * c]);
+            if (r0 + rr < N1) res[idx] = cmul(res[idx], a.T[(long long)(r0 + rr) * N2 + c]);
         }
         __syncthreads();
         res = fft_lds_batch<float, +1>(res, res == buf0 ? buf1 : buf0, p2, tw2, kBigRows, N2);
@@ -435,6 +350,53 @@ __global__ void k_fill_zero(float *p, long long n)
 static void allow_lds(const void *fn, size_t bytes)
 {
     if (bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+int bluestein_n(int N)
+{
+    int n = 64;
+    while (n < 2 * N - 1) n <<= 1;
+    return n;
+}
+
+// b[m] = conj(w_m)/n for m < N and b[n-m] = conj(w_m)/n for 0 < m < N (n >= 2N-1
+// keeps the two ranges apart), zero between; chirp[m] = w_m.
+__global__ void k_bluestein_chirp(int N, int n, cf32 *__restrict__ chirp, cf32 *__restrict__ b)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const int j = m < N ? m : (m > n - N ? n - m : -1);
+    cf32 v = cf32{0.f, 0.f};
+    if (j >= 0) {
+        const long long q = ((long long)j * j) % (2LL * N);  // exact reduction of the phase
+        double sn, cs;
+        sincospi(-(double)q / (double)N, &sn, &cs);
+        if (m < N) chirp[m] = cf32{(float)cs, (float)sn};
+        const double inv = 1.0 / (double)n;
+        v = cf32{(float)(cs * inv), (float)(-sn * inv)};
+    }
+    b[m] = v;
+}
+
+int bluestein_tables(int N, int n, cf32 *chirp, cf32 *T, hipStream_t st)
+{
+    const BigPlan &bp = big_plan(n);
+    if (!bp.ok || n < 2 * N - 1) return -1;
+    hipLaunchKernelGGL(k_bluestein_chirp, dim3((n + 255) / 256), dim3(256), 0, st, N, n, chirp, T);
+    FPCC_CHECK(hipGetLastError());
+    ColArgs ca;
+    ca.op = COL_FWD_PLAIN;
+    ca.src = T;
+    ca.dst = T;
+    ca.bstride = n;
+    int r = run_cols(bp, ca, 1, st, "bluestein_table_cols");
+    if (r) return r;
+    RowArgs ra;
+    ra.op = ROW_FWD_PLAIN;
+    ra.src = T;
+    ra.dst = T;
+    ra.bstride = n;
+    return run_rows(bp, ra, 1, st, "bluestein_table_rows");
 }
 
 int run_cols(const BigPlan &bp, const ColArgs &a, int nbatch, hipStream_t st, const char *name)

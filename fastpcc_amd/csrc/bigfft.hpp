@@ -46,7 +46,7 @@ enum RowOp {
     ROW_INV_PLAIN = 4,
     ROW_INV_CROSS = 5,  // inverse of conj(F[i,w]) F2[j,w] for the batch's pair and window (or summed over w)
     ROW_MASK_INV = 6,   // forward row FFT, analytic mask, inverse row FFT (both row passes of the analytic signal)
-    ROW_MUL_INV = 7,    // forward row FFT, * T[natural bin], inverse row FFT (Bluestein convolution)
+    ROW_MUL_INV = 7,    // forward row FFT, * T (same permuted position), inverse row FFT (Bluestein convolution)
 };
 
 struct ColArgs {
@@ -89,21 +89,19 @@ struct RowArgs {
     const float *wscale = nullptr;  // ROW_FWD_SCALE
     int N = 0;                      // ROW_FWD_MASK
     float da1 = 0.f;
-    const cf32 *T = nullptr;        // ROW_MUL_INV: n values in natural bin order
+    const cf32 *T = nullptr;        // ROW_MUL_INV: n values in the two-pass (permuted) order
     // ROW_INV_CROSS: spectra F[s*Wfull + w][n] (station i) and F2 (station j, F if null);
     // batch b -> pair p = b / Wb, window wsel + b % Wb; wsum != 0 sums over all Wfull windows
     const cf32 *F = nullptr, *F2 = nullptr;
     int Sst = 2, Wb = 1, wsel = 0, Wfull = 1, wsum = 0, pair0 = 0;  // pair p = pair0 + b / Wb
 };
 
-// Bluestein tables for an N-point DFT on the n-point grid: chirp w_m (m < N) and
-// T = FFT_n(b)/n, b = conj(w) wrapped circularly; FP64 on the host, cached per device.
-struct Bluestein {
-    int N = 0, n = 0;
-    const cf32 *chirp = nullptr, *T = nullptr;
-    bool ok = false;
-};
-const Bluestein &bluestein(int N);
+// Bluestein grid for an N-point DFT: the smallest power of two >= max(64, 2N-1).
+int bluestein_n(int N);
+// Tables of the N-point DFT on the n-point grid, computed on the stream into
+// caller memory: chirp w_m = exp(-i pi m^2/N) (m < N, FP64 phases) and
+// T = FFT_n(b)/n in the two-pass (permuted) order, b = conj(w) wrapped circularly.
+int bluestein_tables(int N, int n, cf32 *chirp, cf32 *T, hipStream_t st);
 
 int run_cols(const BigPlan &bp, const ColArgs &a, int nbatch, hipStream_t st, const char *name);
 int run_rows(const BigPlan &bp, const RowArgs &a, int nbatch, hipStream_t st, const char *name);

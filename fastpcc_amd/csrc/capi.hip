@@ -36,7 +36,7 @@ struct Dims {
     int S, W, N, L, lag0;
     unsigned M, Nz;
     size_t P;
-    const WFamily *fam;  // WPCC only
+    std::shared_ptr<const WFamily> fam;  // WPCC only
     bool large;          // windows beyond the single-workgroup kernels
 };
 
@@ -516,7 +516,7 @@ extern "C" int tspcc2_set(float **const y, float **const x1, float **const x2, c
         for (unsigned tr = 0; tr < Tr; ++tr)
             if (y[tr]) memset(y[tr], 0, sizeof(float) * L);
     int err = 0;
-    const WFamily *f = get_wfamily(N, Lag1, Lag2, pmin, pmax, V, type, op1, &err);
+    const std::shared_ptr<const WFamily> f = get_wfamily(N, Lag1, Lag2, pmin, pmax, V, type, op1, &err);
     if (!f) return err ? err : -1;
     const int lag0 = Lag2 >= Lag1 ? Lag1 : Lag2;
     return wpcc_pair_host(*f, y, x1, x2, N, Tr, lag0, L);

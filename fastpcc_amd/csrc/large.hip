@@ -29,13 +29,15 @@ struct LargeLayout {
 };
 
 // Analytic signals of lengths without a two-pass split run as Bluestein
-// convolutions on a power-of-two grid of bluestein_n(N) points.
-int bluestein_n(int N)
+// convolutions on a power-of-two grid; 0 when N has a split.
+int bluestein_grid(int N) { return big_plan(N).ok ? 0 : bluestein_n(N); }
+
+// Bluestein scratch: the SW rows of the grid, then the chirp (N) and kernel
+// spectrum (n) tables.
+size_t bluestein_bytes(int SW, int N)
 {
-    if (big_plan(N).ok) return 0;
-    int n = 64;
-    while (n < 2 * N - 1) n <<= 1;
-    return n;
+    const int n = bluestein_grid(N);
+    return n ? sizeof(cf32) * ((size_t)SW * n + N + n) : 0;
 }
 
 size_t take(size_t &t, size_t bytes)
@@ -72,13 +74,13 @@ LargeLayout large_layout(int kind, int stack, int S, int W, int N, int Nz)
         l.Z = take(t, sizeof(cf32) * SW * N);
         l.F = take(t, sizeof(cf32) * SW * Nz);
         l.mx = take(t, sizeof(float) * SW);
-        l.B = take(t, sizeof(cf32) * SW * (size_t)bluestein_n(N));
+        l.B = take(t, bluestein_bytes((int)SW, N));
         break;
     case FPCC_PCC1:
     case FPCC_PCCV:
         l.Z = take(t, sizeof(cf32) * SW * N);
         l.mx = take(t, sizeof(float) * SW);
-        l.B = take(t, sizeof(cf32) * SW * (size_t)bluestein_n(N));
+        l.B = take(t, bluestein_bytes((int)SW, N));
         break;
     case FPCC_GNCC:
         l.F = take(t, sizeof(cf32) * SW * Nz);
@@ -108,7 +110,7 @@ LargeLayout large_layout(int kind, int stack, int S, int W, int N, int Nz)
 bool large_supported(int kind, int N, int Nz)
 {
     const bool analytic = kind == FPCC_PCC2 || kind == FPCC_PCC1 || kind == FPCC_PCCV;
-    if (analytic && !big_plan(N).ok && !bluestein(N).ok) return false;
+    if (analytic && !big_plan(N).ok && !big_plan(bluestein_n(N)).ok) return false;
     if (kind == FPCC_PCC1 || kind == FPCC_PCCV) return true;  // time-domain lags: no Nz transforms
     return big_plan(Nz).ok;
 }
@@ -123,37 +125,39 @@ size_t large_workspace(int kind, int stack, int S, int W, int N, int Nz)
 static int analytic_bluestein(const float *x, long long stride, int S, int W, int N, cf32 *Z, float *mx, cf32 *B,
                               hipStream_t st)
 {
-    const Bluestein &bs = bluestein(N);
-    if (!bs.ok) {
+    const int n = bluestein_n(N);
+    const BigPlan &bp = big_plan(n);
+    if (!bp.ok) {
         fprintf(stderr, "fastpcc: no analytic-signal plan for N=%d\n", N);
         return -1;
     }
-    const BigPlan &bp = big_plan(bs.n);
     const int SW = S * W;
-    int r = fill_zero(mx, SW, st);
+    cf32 *chirp = B + (size_t)SW * n, *T = chirp + N;
+    int r = bluestein_tables(N, n, chirp, T, st);
     if (r) return r;
+    if ((r = fill_zero(mx, SW, st))) return r;
     ColArgs ca;
     ca.op = COL_FWD_CHIRP;
     ca.dst = B;
-    ca.bstride = bs.n;
+    ca.bstride = n;
     ca.x = x;
     ca.x_stride = stride;
     ca.W = W;
     ca.N = N;
-    ca.chirp = bs.chirp;
+    ca.chirp = chirp;
     if ((r = run_cols(bp, ca, SW, st, "bluestein_fwd_cols"))) return r;
     RowArgs ra;
     ra.op = ROW_MUL_INV;
     ra.src = B;
     ra.dst = B;
-    ra.bstride = bs.n;
-    ra.T = bs.T;
+    ra.bstride = n;
+    ra.T = T;
     if ((r = run_rows(bp, ra, SW, st, "bluestein_conv_rows"))) return r;
     ca = ColArgs{};
     ca.op = COL_CHIRP_MID;
     ca.src = B;
     ca.dst = B;
-    ca.bstride = bs.n;
+    ca.bstride = n;
     ca.N = N;
     ca.da1 = (float)(1 / (double)N);
     if ((r = run_cols(bp, ca, SW, st, "bluestein_mid_cols"))) return r;
@@ -161,11 +165,11 @@ static int analytic_bluestein(const float *x, long long stride, int S, int W, in
     ca = ColArgs{};
     ca.op = COL_INV_CHIRP_MAX;
     ca.src = B;
-    ca.bstride = bs.n;
+    ca.bstride = n;
     ca.dst = Z;
     ca.dst_bstride = N;
     ca.N = N;
-    ca.chirp = bs.chirp;
+    ca.chirp = chirp;
     ca.maxbuf = mx;
     return run_cols(bp, ca, SW, st, "bluestein_inv_cols");
 }

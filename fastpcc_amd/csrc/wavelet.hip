@@ -110,23 +110,32 @@ int build_wfamily(unsigned N, int Lag1, int Lag2, double pmin, double pmax, unsi
     return 0;
 }
 
-const WFamily *get_wfamily(unsigned N, int Lag1, int Lag2, double pmin, double pmax, unsigned V, int type,
-                           double op1, int *err)
+std::shared_ptr<const WFamily> get_wfamily(unsigned N, int Lag1, int Lag2, double pmin, double pmax, unsigned V,
+                                           int type, double op1, int *err)
 {
     typedef std::tuple<unsigned, int, int, double, double, unsigned, int, double> Key;
-    static std::map<Key, std::unique_ptr<WFamily>> cache;
+    static std::map<Key, std::pair<unsigned long long, std::shared_ptr<const WFamily>>> cache;
+    static unsigned long long tick = 0;
     static std::mutex mu;
     std::lock_guard<std::mutex> lk(mu);
     const Key key(N, Lag1, Lag2, pmin, pmax, V, type, op1);
     auto it = cache.find(key);
     if (it != cache.end()) {
         *err = 0;
-        return it->second.get();
+        it->second.first = ++tick;
+        return it->second.second;
     }
-    std::unique_ptr<WFamily> f(new WFamily);
+    auto f = std::make_shared<WFamily>();
     *err = build_wfamily(N, Lag1, Lag2, pmin, pmax, V, type, op1, *f);
     if (*err) return nullptr;
-    return (cache[key] = std::move(f)).get();
+    if (cache.size() >= kWFamilyCache) {  // drop the least recently used
+        auto old = cache.begin();
+        for (auto c = cache.begin(); c != cache.end(); ++c)
+            if (c->second.first < old->second.first) old = c;
+        cache.erase(old);
+    }
+    cache[key] = {++tick, f};
+    return f;
 }
 
 }  // namespace fpcc
