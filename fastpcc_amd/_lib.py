@@ -46,6 +46,7 @@ class ServerConfig(ctypes.Structure):
         ("n_streams", ctypes.c_int),
         ("max_bypass", ctypes.c_int),
         ("fail_every", ctypes.c_int),
+        ("cache_bytes", ctypes.c_size_t),
     ]
 
 
@@ -55,6 +56,7 @@ class Request(ctypes.Structure):
         ("job", FpccJob),
         ("x", ctypes.POINTER(ctypes.c_float)),
         ("y", ctypes.POINTER(ctypes.c_float)),
+        ("station_keys", ctypes.POINTER(ctypes.c_ulonglong)),
     ]
 
 
@@ -72,7 +74,9 @@ class RequestStatus(ctypes.Structure):
 class ServingStats(ctypes.Structure):
     """Mirror of fpcc_serving_stats."""
     _fields_ = [(n, ctypes.c_ulonglong) for n in ("submitted", "completed", "failed", "cancelled", "rejected",
-                                                  "queued", "inflight", "inflight_peak")] + \
+                                                  "cache_hits", "cache_misses")] + \
+               [("cache_used", ctypes.c_size_t)] + \
+               [(n, ctypes.c_ulonglong) for n in ("queued", "inflight", "inflight_peak")] + \
                [(n, ctypes.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "pair_correlations",
                                                "busy_s")] + \
                [(n, ctypes.c_size_t) for n in ("device_bytes", "device_used", "device_peak")]

@@ -270,8 +270,10 @@ class Server:
     of all-pairs jobs with admission by HBM footprint.  submit() keeps references
     to x and y until the request is finished; y is filled in place."""
 
-    def __init__(self, device_bytes=0, pinned_bytes=0, max_inflight=0, n_streams=0, max_bypass=0, fail_every=0):
-        cfg = _lib.ServerConfig(device_bytes, pinned_bytes, max_inflight, n_streams, max_bypass, fail_every)
+    def __init__(self, device_bytes=0, pinned_bytes=0, max_inflight=0, n_streams=0, max_bypass=0, fail_every=0,
+                 cache_bytes=0):
+        cfg = _lib.ServerConfig(device_bytes, pinned_bytes, max_inflight, n_streams, max_bypass, fail_every,
+                                cache_bytes)
         h = ctypes.c_void_p()
         r = _lib.lib().fpcc_server_create(ctypes.byref(cfg), ctypes.byref(h))
         if r != 0:
@@ -279,20 +281,27 @@ class Server:
         self._h = h
         self._keep = {}
 
-    def submit(self, job, x, y):
+    def submit(self, job, x, y, keys=None):
         """x: float32 array whose rows (station_stride apart) hold the job's input;
-        y: float32 array of fpcc_output_len(job) elements.  Returns the request id."""
+        y: float32 array of fpcc_output_len(job) elements; keys: optional per-station
+        64-bit identities for the resident-spectra cache.  Returns the request id."""
         assert x.dtype == np.float32 and x.flags.c_contiguous and y.dtype == np.float32 and y.flags.c_contiguous
         need = (job.n_stations - 1) * job.station_stride + job.n_windows * job.win_len
         if x.size < need or y.size < _lib.lib().fpcc_output_len(ctypes.byref(job)):
             raise ValueError("x or y too small for the job")
+        kp = None
+        if keys is not None:
+            keys = np.ascontiguousarray(keys, dtype=np.uint64)
+            if keys.size != job.n_stations:
+                raise ValueError("one key per station")
+            kp = keys.ctypes.data_as(ctypes.POINTER(ctypes.c_ulonglong))
         req = _lib.Request(job, x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
-                           y.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+                           y.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), kp)
         rid = ctypes.c_ulonglong()
         r = _lib.lib().fpcc_server_submit(self._h, ctypes.byref(req), ctypes.byref(rid))
         if r != 0:
             raise FastPCCError(r, "fpcc_server_submit")
-        self._keep[rid.value] = (x, y)
+        self._keep[rid.value] = (x, y, keys)
         return rid.value
 
     def _status(self, st):

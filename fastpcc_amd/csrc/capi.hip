@@ -159,6 +159,47 @@ Layout layout(const fpcc_job *job, const Dims &d)
     return l;
 }
 
+}  // namespace
+
+namespace fpcc {
+int job_station_regions(const fpcc_job *job, size_t *off, size_t *per)
+{
+    Dims d;
+    if (resolve(job, d)) return 0;
+    const Layout l = layout(job, d);
+    const size_t W = (size_t)d.W;
+    if (d.large) {
+        const int n = large_station_regions(job->kind, job->stack, d.S, d.W, d.N, (int)d.Nz, off, per);
+        for (int k = 0; k < n; ++k) off[k] += l.a;
+        return n;
+    }
+    switch (job->kind) {
+    case FPCC_PCC2: off[0] = l.a; per[0] = sizeof(cf32) * W * d.Nz; return 1;
+    case FPCC_GNCC:
+        off[0] = l.a;
+        per[0] = sizeof(cf64) * W * (size_t)real_ld(d.Nz);
+        off[1] = l.b;
+        per[1] = sizeof(double) * W * (size_t)energy_ld(d.N);
+        return 2;
+    case FPCC_CC1B:
+        off[0] = l.a;
+        off[1] = l.b;
+        per[0] = per[1] = sizeof(cf32) * W * (size_t)real_ld(d.Nz);
+        return 2;
+    case FPCC_PCC1:
+    case FPCC_PCCV: off[0] = l.a; per[0] = sizeof(cf32) * W * d.N; return 1;
+    case FPCC_WPCC: {
+        const int n = wpcc_station_regions(d.S, d.W, (int)d.Nz, (int)d.fam->S, off, per);
+        for (int k = 0; k < n; ++k) off[k] += l.a;
+        return n;
+    }
+    }
+    return 0;
+}
+}  // namespace fpcc
+
+namespace {
+
 int station_stage(const fpcc_job *job, const Dims &d, const float *x, char *ws, hipStream_t st)
 {
     Layout l = layout(job, d);

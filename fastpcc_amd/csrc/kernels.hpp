@@ -1,6 +1,7 @@
 // Launchers of the device stages (host-callable, stream-ordered, no allocation).
 #pragma once
 #include <hip/hip_runtime.h>
+#include "../../include/fastpcc.h"
 #include "cplx.hpp"
 
 namespace fpcc {
@@ -66,14 +67,24 @@ int large_station_stage(int kind, const float *x, long long stride, int S, int W
                         char *ws, hipStream_t st);
 int large_pair_stage(int kind, int stack, int S, int W, int N, int Nz, int lag0, int L, double v, char *ws, float *y,
                      hipStream_t st);
+// Workspace regions holding the station stage's per-station output (what the pair
+// stage reads): station s at off[k] + s*per[k]; returns the region count.
+int large_station_regions(int kind, int stack, int S, int W, int N, int Nz, size_t *off, size_t *per);
 
 // WPCC (wpcc.hip): workspace bytes and the two stages for S stations x W windows.
 struct WFamily;
 size_t wpcc_workspace(int S, int W, int N, int Nz, int nscales);
 int wpcc_station_stage(const float *d_x, long long station_stride, int S, int W, int N, const WFamily &f,
                        char *ws, hipStream_t st);
+int wpcc_station_regions(int S, int W, int Nz, int nscales, size_t *off, size_t *per);
 // per_window: y[p][w][L] (one Gram per window), else mean over windows y[p][L].
 int wpcc_pair_stage(int S, int W, int N, const WFamily &f, int lag0, int L, bool per_window, char *ws, float *d_y,
                     hipStream_t st);
+
+// Station-output regions of a job's workspace (any kind and path), as
+// large_station_regions; 0 if the job is invalid.  Stations are independent: a
+// station's rows are the same in any job with the same kind, windows, N and Nz
+// (and, for WPCC, filter parameters).
+int job_station_regions(const fpcc_job *job, size_t *off, size_t *per);
 
 }  // namespace fpcc

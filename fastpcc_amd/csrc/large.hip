@@ -120,6 +120,30 @@ size_t large_workspace(int kind, int stack, int S, int W, int N, int Nz)
     return large_layout(kind, stack, S, W, N, Nz).total;
 }
 
+int large_station_regions(int kind, int stack, int S, int W, int N, int Nz, size_t *off, size_t *per)
+{
+    const LargeLayout l = large_layout(kind, stack, S, W, N, Nz);
+    const size_t spec = sizeof(cf32) * (size_t)W * Nz;
+    switch (kind) {
+    case FPCC_PCC2: off[0] = l.F; per[0] = spec; return 1;
+    case FPCC_PCC1:
+    case FPCC_PCCV: off[0] = l.Z; per[0] = sizeof(cf32) * (size_t)W * N; return 1;
+    case FPCC_GNCC:
+        off[0] = l.F;
+        per[0] = spec;
+        off[1] = l.E;
+        per[1] = sizeof(double) * (size_t)W * energy_ld(N);
+        return 2;
+    case FPCC_CC1B:
+        off[0] = l.F;
+        per[0] = spec;
+        off[1] = l.F2;
+        per[1] = spec;
+        return 2;
+    }
+    return 0;
+}
+
 // Bluestein: z = IDFT_N(mask DFT_N(x)) with both N-point DFTs as chirp
 // convolutions on the n-point grid (five two-pass stages through the scratch B).
 static int analytic_bluestein(const float *x, long long stride, int S, int W, int N, cf32 *Z, float *mx, cf32 *B,

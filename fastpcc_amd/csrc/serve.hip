@@ -22,20 +22,48 @@
 #include <cstring>
 #include <deque>
 #include <list>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 #include "../../include/fastpcc.h"
 #include "arena.hpp"
 #include "engine.hpp"
+#include "kernels.hpp"
 
 namespace fpcc {
 
 namespace {
 
 using clock_t_ = std::chrono::steady_clock;
+
+constexpr int kMaxRegions = 4;
+constexpr int kCopyStations = 128;  // stations per cache copy launch (offsets go by value)
+
+// A HIP event shared by the request that fills cache entries and the requests
+// that wait for them; destroyed with its last reference.
+struct Event {
+    hipEvent_t e = nullptr;
+    ~Event()
+    {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+
+// Station rows (every station-output region of one station) resident in the
+// cache arena.  ev: recorded after the fill, until the filling request is done.
+struct CacheEntry {
+    size_t off = 0, bytes = 0;
+    int pins = 0;
+    unsigned long long tick = 0;
+    std::shared_ptr<Event> ev;
+};
+
+typedef std::tuple<unsigned long long, int, unsigned, unsigned, size_t, size_t, double, double, unsigned, int, double>
+    CacheKey;
 
 struct Request {
     unsigned long long id = 0;
@@ -52,6 +80,19 @@ struct Request {
     int stream = -1;
     double t_submit = 0, t_start = 0, t_done = 0;
     unsigned long long pair_corr = 0;
+    // station-spectra cache (keyed requests)
+    std::vector<unsigned long long> keys;
+    int nreg = 0;
+    size_t off[kMaxRegions] = {}, per[kMaxRegions] = {}, rbase[kMaxRegions] = {}, entry_bytes = 0;
+    bool use_cache = false, fill_done = false;
+    std::vector<CacheEntry *> entries;   // per station
+    std::vector<int> misses;             // stations computed by this request
+    std::vector<CacheKey> new_keys;      // entries this request created
+    fpcc_job sub_job{};                  // the missing stations, gathered
+    size_t sub_off[kMaxRegions] = {};
+    size_t off_xs = 0, xs_bytes = 0, off_wss = 0, wss_bytes = 0, off_sxs = 0;
+    std::shared_ptr<Event> fill_ev;
+    std::vector<std::shared_ptr<Event>> waits;
 };
 
 struct Server;
@@ -64,8 +105,8 @@ struct DoneMsg {
 struct Server {
     fpcc_server_config cfg{};
     int device = 0;
-    void *dmem = nullptr, *hmem = nullptr;
-    Arena darena, harena;
+    void *dmem = nullptr, *hmem = nullptr, *cmem = nullptr;
+    Arena darena, harena, carena;
     std::vector<hipStream_t> streams;
     std::vector<int> stream_load;
     clock_t_::time_point t0;
@@ -75,13 +116,14 @@ struct Server {
     std::deque<std::shared_ptr<Request>> queue;
     std::deque<std::shared_ptr<Request>> finished;  // handed over by stream host functions
     std::unordered_map<unsigned long long, std::shared_ptr<Request>> all;
-    unsigned long long next_id = 1;
+    std::map<CacheKey, CacheEntry> cache;
+    unsigned long long next_id = 1, tick = 0;
     int inflight = 0, inflight_peak = 0;
     size_t dev_peak = 0;
     bool stop = false;
-    unsigned long long admitted = 0;
     // statistics
     unsigned long long n_submit = 0, n_done = 0, n_fail = 0, n_cancel = 0, n_reject = 0;
+    unsigned long long n_hit = 0, n_miss = 0;
     std::vector<double> lat_ms;
     double first_start = -1, last_done = 0;
     unsigned long long pair_corr_done = 0;
@@ -113,46 +155,310 @@ void stream_host_done(void *arg)
     delete m;
 }
 
+// ---- station rows between the cache and workspaces ----
+// Row block (station t, region k): src_base + src_off[k] + t*src_step[k] (+ eoff[t]
+// if src_entry), same for dst; per[k] bytes.  Grid: (chunks of 256 KiB, t*nreg + k).
+struct CopyArgs {
+    const char *src_base;
+    char *dst_base;
+    int nreg, nst, src_entry, dst_entry;
+    size_t src_off[kMaxRegions], dst_off[kMaxRegions], src_step[kMaxRegions], dst_step[kMaxRegions];
+    size_t per[kMaxRegions];
+    size_t eoff[kCopyStations];
+};
+
+constexpr size_t kCopyChunk = 256 << 10;
+
+__global__ void __launch_bounds__(256) k_copy_rows(CopyArgs a)
+{
+    const int t = blockIdx.y / a.nreg, k = blockIdx.y - t * a.nreg;
+    const size_t bytes = a.per[k];
+    const size_t c0 = (size_t)blockIdx.x * kCopyChunk;
+    if (c0 >= bytes) return;
+    const size_t c1 = bytes - c0 < kCopyChunk ? bytes : c0 + kCopyChunk;
+    const char *src = a.src_base + a.src_off[k] + t * a.src_step[k] + (a.src_entry ? a.eoff[t] : 0);
+    char *dst = a.dst_base + a.dst_off[k] + t * a.dst_step[k] + (a.dst_entry ? a.eoff[t] : 0);
+    if ((((size_t)src | (size_t)dst | c0 | c1) & 15) == 0) {
+        for (size_t i = c0 + 16 * threadIdx.x; i < c1; i += 16 * blockDim.x)
+            *reinterpret_cast<uint4 *>(dst + i) = *reinterpret_cast<const uint4 *>(src + i);
+    } else {  // every region holds 4-byte words
+        for (size_t i = c0 + 4 * threadIdx.x; i < c1; i += 4 * blockDim.x)
+            *reinterpret_cast<unsigned *>(dst + i) = *reinterpret_cast<const unsigned *>(src + i);
+    }
+}
+
+int copy_rows(CopyArgs a, const std::vector<size_t> &eoff, hipStream_t st)
+{
+    size_t maxb = 0;
+    for (int k = 0; k < a.nreg; ++k) maxb = std::max(maxb, a.per[k]);
+    const unsigned chunks = (unsigned)((maxb + kCopyChunk - 1) / kCopyChunk);
+    const int n = (int)eoff.size();
+    for (int t0 = 0; t0 < n; t0 += kCopyStations) {
+        a.nst = std::min(kCopyStations, n - t0);
+        for (int t = 0; t < a.nst; ++t) a.eoff[t] = eoff[t0 + t];
+        CopyArgs b = a;
+        for (int k = 0; k < a.nreg; ++k) {  // shift the stepped side to station t0
+            if (!a.src_entry) b.src_off[k] += (size_t)t0 * a.src_step[k];
+            if (!a.dst_entry) b.dst_off[k] += (size_t)t0 * a.dst_step[k];
+        }
+        hipLaunchKernelGGL(k_copy_rows, dim3(chunks, a.nst * a.nreg), dim3(256), 0, st, b);
+        if (hipGetLastError() != hipSuccess) return -1;
+    }
+    return 0;
+}
+
+// Async host->device copy of station rows `rows` of the caller's input into
+// consecutive rows of dst (runs of consecutive stations as one 2D copy).
+int copy_in_rows(Server *s, Request &r, const std::vector<int> &rows, float *dst, size_t off_stage, hipStream_t st)
+{
+    const fpcc_job &j = r.job;
+    const size_t row = (size_t)j.n_windows * j.win_len * sizeof(float);
+    if (!r.x_pinned) {
+        char *hx = (char *)s->hmem + off_stage;
+        for (size_t i = 0; i < rows.size(); ++i) memcpy(hx + i * row, r.x + (size_t)rows[i] * j.station_stride, row);
+        return hipMemcpyAsync(dst, hx, rows.size() * row, hipMemcpyHostToDevice, st) == hipSuccess ? 0 : -1;
+    }
+    for (size_t i = 0; i < rows.size();) {
+        size_t n = 1;
+        while (i + n < rows.size() && rows[i + n] == rows[i] + (int)n) ++n;
+        if (hipMemcpy2DAsync((char *)dst + i * row, row, r.x + (size_t)rows[i] * j.station_stride,
+                             j.station_stride * sizeof(float), row, n, hipMemcpyHostToDevice, st) != hipSuccess)
+            return -1;
+        i += n;
+    }
+    return 0;
+}
+
 // Launches one admitted request on its stream (scheduler thread, lock not held).
 int issue(Server *s, Request &r)
 {
     (void)hipSetDevice(s->device);
     hipStream_t st = s->streams[r.stream];
     char *dbase = (char *)s->dmem, *hbase = (char *)s->hmem;
-    float *dx = (float *)(dbase + r.off_x), *dy = (float *)(dbase + r.off_y);
+    float *dy = (float *)(dbase + r.off_y);
+    char *ws = dbase + r.off_ws;
     const fpcc_job &j = r.job;
     const size_t row = (size_t)j.n_windows * j.win_len * sizeof(float);
     if (s->cfg.fail_every > 0 && r.id % (unsigned long long)s->cfg.fail_every == 0) return -1;  // injected
-    if (r.x_pinned) {
-        if (hipMemcpy2DAsync(dx, row, r.x, j.station_stride * sizeof(float), row, j.n_stations,
-                             hipMemcpyHostToDevice, st) != hipSuccess)
-            return -1;
+    if (r.use_cache) {
+        if (!r.misses.empty()) {
+            std::vector<int> rows(r.misses);
+            if (rows.size() < 2) rows.push_back(rows[0]);  // a job has at least two stations
+            float *dxs = (float *)(dbase + r.off_xs);
+            if (copy_in_rows(s, r, rows, dxs, r.off_sxs, st)) return -1;
+            char *wss = dbase + r.off_wss;
+            int rc = fpcc_station_stage_device(&r.sub_job, dxs, wss, r.wss_bytes, st);
+            if (rc) return rc;
+            CopyArgs a{};
+            a.src_base = wss;
+            a.dst_base = (char *)s->cmem;
+            a.nreg = r.nreg;
+            a.dst_entry = 1;
+            for (int k = 0; k < r.nreg; ++k) {
+                a.src_off[k] = r.sub_off[k];
+                a.src_step[k] = r.per[k];
+                a.dst_off[k] = r.rbase[k];
+                a.per[k] = r.per[k];
+            }
+            std::vector<size_t> eoff;
+            for (int m : r.misses) eoff.push_back(r.entries[m]->off);
+            if (copy_rows(a, eoff, st)) return -1;
+            if (hipEventRecord(r.fill_ev->e, st) != hipSuccess) return -1;
+            r.fill_done = true;
+        }
+        for (auto &w : r.waits)
+            if (hipStreamWaitEvent(st, w->e, 0) != hipSuccess) return -1;
+        CopyArgs a{};
+        a.src_base = (const char *)s->cmem;
+        a.dst_base = ws;
+        a.nreg = r.nreg;
+        a.src_entry = 1;
+        for (int k = 0; k < r.nreg; ++k) {
+            a.src_off[k] = r.rbase[k];
+            a.dst_off[k] = r.off[k];
+            a.dst_step[k] = r.per[k];
+            a.per[k] = r.per[k];
+        }
+        std::vector<size_t> eoff;
+        for (CacheEntry *e : r.entries) eoff.push_back(e->off);
+        if (copy_rows(a, eoff, st)) return -1;
+        int rc = fpcc_pair_stage_device(&r.dev_job, ws, r.ws_bytes, dy, st);
+        if (rc) return rc;
     } else {
-        char *hx = hbase + r.off_sx;
-        for (unsigned i = 0; i < j.n_stations; ++i) memcpy(hx + i * row, r.x + (size_t)i * j.station_stride, row);
-        if (hipMemcpyAsync(dx, hx, r.x_bytes, hipMemcpyHostToDevice, st) != hipSuccess) return -1;
+        float *dx = (float *)(dbase + r.off_x);
+        if (r.x_pinned) {
+            if (hipMemcpy2DAsync(dx, row, r.x, j.station_stride * sizeof(float), row, j.n_stations,
+                                 hipMemcpyHostToDevice, st) != hipSuccess)
+                return -1;
+        } else {
+            char *hx = hbase + r.off_sx;
+            for (unsigned i = 0; i < j.n_stations; ++i)
+                memcpy(hx + i * row, r.x + (size_t)i * j.station_stride, row);
+            if (hipMemcpyAsync(dx, hx, r.x_bytes, hipMemcpyHostToDevice, st) != hipSuccess) return -1;
+        }
+        int rc = fpcc_allpairs_device(&r.dev_job, dx, dy, ws, r.ws_bytes, st);
+        if (rc) return rc;
     }
-    int rc = fpcc_allpairs_device(&r.dev_job, dx, dy, dbase + r.off_ws, r.ws_bytes, st);
-    if (rc) return rc;
     void *yh = r.y_pinned ? (void *)r.y : (void *)(hbase + r.off_sy);
     if (hipMemcpyAsync(yh, dy, r.y_bytes, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
     return 0;
 }
 
+CacheKey cache_key(const Request &r, unsigned long long station)
+{
+    const fpcc_job &j = r.job;
+    const bool w = j.kind == FPCC_WPCC;
+    return CacheKey(station, j.kind, j.win_len, j.n_windows, r.per[0], r.nreg > 1 ? r.per[1] : 0, w ? j.pmin : 0,
+                    w ? j.pmax : 0, w ? j.V : 0, w ? j.wtype : 0, w ? j.op1 : 0);
+}
+
 void release(Server *s, Request &r)
 {
-    s->darena.release(r.off_x, r.x_bytes);
+    if (r.use_cache) {
+        for (CacheEntry *e : r.entries)
+            if (e) e->pins--;
+        if (r.fill_done)
+            for (int m : r.misses)
+                if (r.entries[m] && r.entries[m]->ev == r.fill_ev) r.entries[m]->ev.reset();  // filled
+        if (!r.misses.empty()) {
+            s->darena.release(r.off_xs, r.xs_bytes);
+            s->darena.release(r.off_wss, r.wss_bytes);
+            if (!r.x_pinned) s->harena.release(r.off_sxs, r.xs_bytes);
+        }
+        r.entries.clear();
+        r.waits.clear();
+        r.fill_ev.reset();
+    } else {
+        s->darena.release(r.off_x, r.x_bytes);
+        if (!r.x_pinned) s->harena.release(r.off_sx, r.x_bytes);
+    }
     s->darena.release(r.off_y, r.y_bytes);
     s->darena.release(r.off_ws, r.ws_bytes);
-    if (!r.x_pinned) s->harena.release(r.off_sx, r.x_bytes);
     if (!r.y_pinned) s->harena.release(r.off_sy, r.y_bytes);
     s->stream_load[r.stream]--;
     s->inflight--;
 }
 
+// Evicts the least recently used unpinned entry (lock held); false if none.
+bool evict_one(Server *s)
+{
+    auto victim = s->cache.end();
+    for (auto it = s->cache.begin(); it != s->cache.end(); ++it)
+        if (!it->second.pins && (victim == s->cache.end() || it->second.tick < victim->second.tick)) victim = it;
+    if (victim == s->cache.end()) return false;
+    s->carena.release(victim->second.off, victim->second.bytes);
+    s->cache.erase(victim);
+    return true;
+}
+
+// Cached admission (lock held): 1 admitted, 0 not now, -1 cache not usable for
+// this request (too small; it then runs uncached).
+int reserve_cached(Server *s, Request &r)
+{
+    const unsigned S = r.job.n_stations;
+    r.entries.assign(S, nullptr);
+    r.misses.clear();
+    r.new_keys.clear();
+    r.waits.clear();
+    std::vector<int> dup(S, -1);  // a station repeating an earlier missing station's key
+    for (unsigned i = 0; i < S; ++i) {
+        auto it = s->cache.find(cache_key(r, r.keys[i]));
+        if (it != s->cache.end()) {
+            r.entries[i] = &it->second;
+            continue;
+        }
+        for (int m : r.misses)
+            if (r.keys[m] == r.keys[i]) {
+                dup[i] = m;
+                break;
+            }
+        if (dup[i] < 0) r.misses.push_back((int)i);
+    }
+    const int M = (int)r.misses.size();
+    const size_t row = (size_t)r.job.n_windows * r.job.win_len * sizeof(float);
+    r.xs_bytes = r.wss_bytes = 0;
+    if (M) {
+        r.sub_job = r.dev_job;
+        r.sub_job.n_stations = std::max(2, M);
+        r.xs_bytes = r.sub_job.n_stations * row;
+        r.wss_bytes = fpcc_workspace_size(&r.sub_job);
+        size_t so[kMaxRegions], sp[kMaxRegions];
+        if (!r.wss_bytes || job_station_regions(&r.sub_job, so, sp) != r.nreg) return -1;
+        for (int k = 0; k < r.nreg; ++k) r.sub_off[k] = so[k];
+    }
+    if ((size_t)M * r.entry_bytes > s->carena.size()) return -1;
+    const size_t need_d[4] = {r.y_bytes, r.ws_bytes, r.xs_bytes, r.wss_bytes};
+    const size_t need_h[2] = {r.x_pinned ? 0 : r.xs_bytes, r.y_pinned ? 0 : r.y_bytes};
+    if (!s->darena.fits(need_d) || !s->harena.fits(need_h)) return 0;
+    for (CacheEntry *e : r.entries)
+        if (e) e->pins++;
+    std::vector<size_t> noff;
+    bool ok = true;
+    for (int q = 0; q < M && ok; ++q) {
+        size_t o = 0;
+        while (!s->carena.alloc(r.entry_bytes, o))
+            if (!evict_one(s)) {
+                ok = false;
+                break;
+            }
+        if (ok) noff.push_back(o);
+    }
+    std::shared_ptr<Event> ev;
+    if (ok && M) {
+        ev = std::make_shared<Event>();
+        ok = hipEventCreateWithFlags(&ev->e, hipEventDisableTiming) == hipSuccess;
+    }
+    if (!ok) {  // pinned entries could not make room: run this one uncached
+        for (size_t o : noff) s->carena.release(o, r.entry_bytes);
+        for (CacheEntry *e : r.entries)
+            if (e) e->pins--;
+        r.entries.clear();
+        r.misses.clear();
+        return -1;
+    }
+    // commit
+    for (unsigned i = 0; i < S; ++i)
+        if (r.entries[i] && r.entries[i]->ev) r.waits.push_back(r.entries[i]->ev);  // still being filled
+    r.fill_ev = ev;
+    for (int q = 0; q < M; ++q) {
+        const CacheKey key = cache_key(r, r.keys[r.misses[q]]);
+        CacheEntry &e = s->cache[key];
+        e.off = noff[q];
+        e.bytes = r.entry_bytes;
+        e.pins = 1;
+        e.ev = ev;
+        r.entries[r.misses[q]] = &e;
+        r.new_keys.push_back(key);
+    }
+    for (unsigned i = 0; i < S; ++i)
+        if (dup[i] >= 0) {
+            r.entries[i] = r.entries[dup[i]];
+            r.entries[i]->pins++;
+        }
+    for (CacheEntry *e : r.entries) e->tick = ++s->tick;
+    s->darena.alloc(r.y_bytes, r.off_y);
+    s->darena.alloc(r.ws_bytes, r.off_ws);
+    if (M) {
+        s->darena.alloc(r.xs_bytes, r.off_xs);
+        s->darena.alloc(r.wss_bytes, r.off_wss);
+        if (!r.x_pinned) s->harena.alloc(r.xs_bytes, r.off_sxs);
+    }
+    if (!r.y_pinned) s->harena.alloc(r.y_bytes, r.off_sy);
+    s->n_hit += S - M;
+    s->n_miss += M;
+    r.use_cache = true;
+    r.fill_done = false;
+    return 1;
+}
+
 // Reserves the request's blocks if they all fit (lock held).
 bool try_reserve(Server *s, Request &r)
 {
+    if (s->cmem && !r.keys.empty() && r.nreg > 0) {
+        const int c = reserve_cached(s, r);
+        if (c >= 0) return c == 1;
+    }
+    r.use_cache = false;
     const size_t need_d[3] = {r.x_bytes, r.y_bytes, r.ws_bytes};
     if (!s->darena.fits(need_d)) return false;
     const size_t need_h[2] = {r.x_pinned ? 0 : r.x_bytes, r.y_pinned ? 0 : r.y_bytes};
@@ -201,14 +507,31 @@ void scheduler(Server *s)
         r.state = FPCC_REQ_RUNNING;
         r.t_start = s->now();
         if (s->first_start < 0) s->first_start = r.t_start;
-        s->admitted++;
         lk.unlock();
-        int rc = issue(s, r);
-        DoneMsg *m = new DoneMsg{s, pick};
-        hipStream_t st = s->streams[r.stream];
-        if (rc) r.rc = rc;
+        const int rc = issue(s, r);
+        lk.lock();
+        if (rc) {
+            r.rc = rc;
+            // entries whose fill never reached the stream must not be found
+            // (no admission ran since this request's, so nobody else holds them)
+            if (r.use_cache && !r.fill_done) {
+                for (const CacheKey &k : r.new_keys) {
+                    auto it = s->cache.find(k);
+                    if (it == s->cache.end()) continue;
+                    CacheEntry *e = &it->second;
+                    for (auto &p : r.entries)
+                        if (p == e) p = nullptr;
+                    s->carena.release(e->off, e->bytes);
+                    s->cache.erase(it);
+                }
+                r.new_keys.clear();
+            }
+        }
+        lk.unlock();
         // completion after everything already queued on the stream (also on failure:
         // earlier copies or kernels may still be reading the request's blocks)
+        DoneMsg *m = new DoneMsg{s, pick};
+        hipStream_t st = s->streams[r.stream];
         if (hipLaunchHostFunc(st, stream_host_done, m) != hipSuccess) {
             (void)hipStreamSynchronize(st);
             stream_host_done(m);
@@ -318,14 +641,22 @@ extern "C" int fpcc_server_create(const fpcc_server_config *cfg, fpcc_server **o
         delete h;
         return -1;
     }
+    if (s->cfg.cache_bytes && hipMalloc(&s->cmem, s->cfg.cache_bytes) != hipSuccess) {
+        (void)hipHostFree(s->hmem);
+        (void)hipFree(s->dmem);
+        delete h;
+        return -1;
+    }
     s->darena.reset(s->cfg.device_bytes);
     s->harena.reset(s->cfg.pinned_bytes);
+    s->carena.reset(s->cfg.cache_bytes);
     s->streams.resize(s->cfg.n_streams);
     s->stream_load.assign(s->cfg.n_streams, 0);
     for (auto &st : s->streams) {
         if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
             for (auto &t : s->streams)
                 if (t) (void)hipStreamDestroy(t);
+            if (s->cmem) (void)hipFree(s->cmem);
             (void)hipHostFree(s->hmem);
             (void)hipFree(s->dmem);
             delete h;
@@ -364,6 +695,16 @@ extern "C" int fpcc_server_submit(fpcc_server *h, const fpcc_request *req, unsig
     r->x_pinned = fpcc::is_pinned(req->x) && fpcc::is_pinned(req->x + last);
     r->y_pinned = fpcc::is_pinned(req->y) && fpcc::is_pinned(req->y + ylen - 1);
     r->pair_corr = (unsigned long long)j.n_stations * (j.n_stations - 1) / 2 * j.n_windows;
+    if (req->station_keys) {
+        r->keys.assign(req->station_keys, req->station_keys + j.n_stations);
+        r->nreg = fpcc::job_station_regions(&r->dev_job, r->off, r->per);
+        size_t base = 0;
+        for (int k = 0; k < r->nreg; ++k) {
+            r->rbase[k] = base;
+            base += (r->per[k] + 15) / 16 * 16;
+        }
+        r->entry_bytes = base;
+    }
     const size_t dfoot = fpcc::Arena::round(r->x_bytes) + fpcc::Arena::round(r->y_bytes) +
                          fpcc::Arena::round(r->ws_bytes);
     const size_t hfoot = (r->x_pinned ? 0 : fpcc::Arena::round(r->x_bytes)) +
@@ -462,6 +803,9 @@ extern "C" int fpcc_server_stats(fpcc_server *h, fpcc_serving_stats *out)
     out->failed = s->n_fail;
     out->cancelled = s->n_cancel;
     out->rejected = s->n_reject;
+    out->cache_hits = s->n_hit;
+    out->cache_misses = s->n_miss;
+    out->cache_used = s->carena.used();
     out->queued = s->queue.size();
     out->inflight = s->inflight;
     out->inflight_peak = s->inflight_peak;
@@ -502,6 +846,8 @@ extern "C" int fpcc_server_destroy(fpcc_server *h)
     s->compl_.join();
     (void)hipSetDevice(s->device);
     for (auto &st : s->streams) (void)hipStreamDestroy(st);
+    s->cache.clear();
+    if (s->cmem) (void)hipFree(s->cmem);
     (void)hipHostFree(s->hmem);
     (void)hipFree(s->dmem);
     delete h;

@@ -127,6 +127,14 @@ size_t wpcc_workspace(int S, int W, int N, int Nz, int nscales)
     return wpcc_layout(S, W, Nz, nscales).total;
 }
 
+// The pair stage reads only the per-scale spectra B (rows sw*S + scale).
+int wpcc_station_regions(int S, int W, int Nz, int nscales, size_t *off, size_t *per)
+{
+    off[0] = wpcc_layout(S, W, Nz, nscales).B;
+    per[0] = sizeof(cf32) * (size_t)W * nscales * Nz;
+    return 1;
+}
+
 int wpcc_station_stage(const float *d_x, long long station_stride, int S, int W, int N, const WFamily &f, char *ws,
                        hipStream_t st)
 {

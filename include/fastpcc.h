@@ -168,6 +168,16 @@ int fpcc_pair_stage_device(const fpcc_job *job, void *d_ws, size_t ws_bytes, flo
  * every fail_every-th request fail before its kernels launch (fault injection
  * for tests).
  *
+ * Resident station spectra: with cache_bytes > 0, a request that names its
+ * stations (station_keys, one 64-bit key per station row; equal keys promise
+ * equal samples in the job's windows) reuses the station-stage output (spectra,
+ * GNCC energies, 1-bit sign spectra, WPCC per-scale spectra) of earlier requests
+ * with the same kind, window length, window count, zero-pad length and filter
+ * parameters: only missing stations are computed (gathered into one job), their
+ * rows are kept in a separate cache arena of cache_bytes, and every station's
+ * rows are copied into the request's workspace before its pair stage.  Entries
+ * in use by running requests stay; others are evicted least recently used.
+ *
  * Return codes: 0 ok; -1 invalid argument or GPU failure; -3 |lag| >= N;
  * -4 the request's footprint exceeds the server's arenas (never admissible);
  * fpcc_server_wait returns 1 on timeout, fpcc_server_cancel 1 if the request had
@@ -185,12 +195,14 @@ typedef struct {
     int n_streams;        /* HIP streams (0: 4)                                   */
     int max_bypass;       /* times a waiting request may be passed (0: 8)          */
     int fail_every;       /* fault injection: fail every k-th request (0: off)    */
+    size_t cache_bytes;   /* resident station-spectra cache (0: off)              */
 } fpcc_server_config;
 
 typedef struct {
     fpcc_job job;   /* one all-pairs job (layout as fpcc_allpairs)        */
     const float *x; /* host input: n_stations rows of station_stride floats */
     float *y;       /* host output: fpcc_output_len(&job) floats            */
+    const unsigned long long *station_keys; /* per-station identity for the cache, or NULL */
 } fpcc_request;
 
 typedef struct {
@@ -201,6 +213,8 @@ typedef struct {
 
 typedef struct {
     unsigned long long submitted, completed, failed, cancelled, rejected;
+    unsigned long long cache_hits, cache_misses; /* stations found / computed for keyed requests */
+    size_t cache_used;                           /* bytes of resident station rows              */
     unsigned long long queued, inflight, inflight_peak;
     double p50_ms, p99_ms, mean_ms, max_ms; /* latency of completed requests          */
     double pair_correlations;              /* sum of pairs x windows delivered        */

@@ -106,3 +106,74 @@ def test_cancel_timeout_and_failures(fp):
         st = srv.stats()
         assert (st["completed"], st["failed"], st["cancelled"]) == (4, 1, 1) and st["device_used"] == 0
         assert srv.forget(ids[0]) == 0 and srv.forget(ids[0]) != 0
+
+
+# ---- resident station spectra ----
+
+def _network(S, T, seed):
+    x, _ = synth.make_stations(S, T, 300, seed=seed)
+    return x
+
+
+CACHE_CASES = [("pcc2", 3600, 500, "none", {}), ("pcc2", 2003, 500, "plain", {}), ("gncc", 5000, 800, "none", {}),
+               ("cc1b", 12000, 1000, "none", {}), ("pcc1", 2500, 200, "none", {}),
+               ("wpcc", 3000, 400, "none", dict(pmin=2.0, pmax=200.0, V=8, wtype=-1)),
+               ("pcc2", 2400, 300, "gram16", {})]
+
+
+@pytest.mark.parametrize("kind,N,lag,stack,kw", CACHE_CASES)
+def test_cache_reuse_matches_uncached(fp, kind, N, lag, stack, kw):
+    """Requests over overlapping station sets of one network: only unseen stations
+    are computed, and every result equals the request run alone."""
+    W = 2 if stack in ("plain", "gram16") else 1
+    net = _network(16, W * N, 11)
+    # overlap, a full repeat, single missing stations, and one station named twice
+    sets = [list(range(0, 8)), list(range(4, 12)), list(range(0, 8)), [3, 12, 5, 6], [13, 13, 9, 2]]
+    with fp.Server(device_bytes=4 << 30, pinned_bytes=256 << 20, max_inflight=4, n_streams=4,
+                   cache_bytes=2 << 30) as srv:
+        ys, ids, xs = [], [], []
+        for st in sets:
+            x = np.ascontiguousarray(net[st])
+            job = fp.make_job(kind, len(st), W, N, -lag, lag, stack=stack, station_stride=x.shape[1], **kw)
+            y = np.empty(int(np.prod(fp.output_shape(job))), np.float32)
+            ids.append(srv.submit(job, x, y, keys=np.array(st, dtype=np.uint64)))
+            ys.append(y)
+            xs.append(x)
+        for rid in ids:
+            assert srv.wait(rid)["state"] == 2
+        stat = srv.stats()
+        # stations computed: 0-7, 8-11, 12, 13 (once, though named twice)
+        print(f"{kind}/{stack}: hits {stat['cache_hits']} misses {stat['cache_misses']} "
+              f"cache {stat['cache_used'] / 2**20:.1f} MiB", flush=True)
+        assert stat["cache_misses"] == 14 and stat["cache_hits"] == 4 + 8 + 3 + 3
+        assert stat["device_used"] == 0
+    for st, x, y in zip(sets, xs, ys):
+        ref = fp.allpairs(x, W, N, -lag, lag, kind=kind, stack=stack, **kw).ravel()
+        assert np.array_equal(ref, y), f"{kind} stations {st}: max diff {np.abs(ref - y).max()}"
+
+
+def test_cache_eviction_small_budget(fp):
+    """A cache that holds about one request's stations: entries in use stay,
+    others are evicted, results stay exact and the cache never exceeds its budget."""
+    N, lag = 3600, 400
+    net = _network(24, N, 12)
+    entry = 8192 * 8  # one station's PCC2 spectrum row (Nz 8192, one window)
+    budget = 10 * entry
+    rng = np.random.default_rng(3)
+    sets = [sorted(rng.choice(24, size=8, replace=False).tolist()) for _ in range(12)]
+    with fp.Server(device_bytes=2 << 30, pinned_bytes=128 << 20, max_inflight=3, cache_bytes=budget) as srv:
+        ys, ids = [], []
+        for st in sets:
+            x = np.ascontiguousarray(net[st])
+            job = fp.make_job("pcc2", 8, 1, N, -lag, lag, stack="none", station_stride=N)
+            y = np.empty(28 * (2 * lag + 1), np.float32)
+            ids.append(srv.submit(job, x, y, keys=np.array(st, dtype=np.uint64)))
+            ys.append(y)
+        for rid in ids:
+            assert srv.wait(rid)["state"] == 2
+        stat = srv.stats()
+        assert stat["cache_used"] <= budget and stat["device_used"] == 0
+        print(f"eviction: hits {stat['cache_hits']} misses {stat['cache_misses']}")
+    for st, y in zip(sets, ys):
+        ref = fp.allpairs(np.ascontiguousarray(net[st]), 1, N, -lag, lag, kind="pcc2", stack="none").ravel()
+        assert np.array_equal(ref, y)
