@@ -92,19 +92,36 @@ def cpu_baseline(x, cfg, seconds):
 
 def serve_run(n_requests, rate):
     """Poisson stream through the request server after a short warm-up (plans and
-    filter banks of every zero-pad length are built on first use)."""
+    filter banks of every zero-pad length are built on first use); then the
+    network-day stream (station subsets of one shared window) with the resident
+    spectra cache on and off."""
     from fastpcc_amd import serving, Server
     net = serving.make_network()
+    kw = dict(device_bytes=96 << 30, pinned_bytes=4 << 30, max_inflight=16, n_streams=4)
     warm = serving.make_requests(48, rate, seed=99)
     reqs = serving.make_requests(n_requests, rate, seed=7)
-    with Server(device_bytes=96 << 30, pinned_bytes=4 << 30, max_inflight=16, n_streams=4) as srv:
-        serving.run(warm, net, server=srv, realtime=False)
-    with Server(device_bytes=96 << 30, pinned_bytes=4 << 30, max_inflight=16, n_streams=4) as srv:
-        stats, _, _ = serving.run(reqs, net, server=srv, realtime=True)
+    pool = serving.OutputPool(24, max(serving.out_len(r) for r in reqs + warm))
+    with Server(**kw) as srv:
+        serving.run(warm, net, server=srv, realtime=False, pool=pool, keep=False)
+    with Server(**kw) as srv:
+        stats, _, _ = serving.run(reqs, net, server=srv, realtime=True, pool=pool, keep=False)
     out = serving.summary(stats, reqs, rate)
     out["workload"] = ("Poisson stream, stations U{4..128}, one window of N log-uniform in [1800, 86400], "
                        "lags +-U{200..2000} (< N), PCC2/GNCC/1-bit/WPCC 50/20/10/20 (WPCC: Morlet, periods "
-                       "2-200 samples, 8 voices), inputs from a pinned 128 x 86400 network, pageable outputs")
+                       "2-200 samples, 8 voices), inputs from a pinned 128 x 86400 network, outputs into a "
+                       "pool of 24 pinned buffers (pageable when all are busy); no station keys (every "
+                       "request starts at its own time, so no station window repeats)")
+    del pool
+    day_rate = 200.0
+    day = serving.make_network_day_requests(max(100, n_requests // 4), day_rate)
+    out["network_day"] = {"workload": "Poisson 200/s, stations: random subsets of 8-64 of the 128, the same "
+                                      "21600-sample window (lags +-1000), PCC2/WPCC 50/50"}
+    for name, cb in (("cache", 16 << 30), ("no_cache", 0)):
+        with Server(cache_bytes=cb, **kw) as srv:
+            serving.run(day[:16], net, server=srv, realtime=False, keys=cb > 0)  # plans, filter banks
+        with Server(cache_bytes=cb, **kw) as srv:
+            st, _, _ = serving.run(day, net, server=srv, realtime=True, keys=cb > 0)
+        out["network_day"][name] = serving.summary(st, day, day_rate)
     return out
 
 

@@ -85,6 +85,7 @@ struct Request {
     int nreg = 0;
     size_t off[kMaxRegions] = {}, per[kMaxRegions] = {}, rbase[kMaxRegions] = {}, entry_bytes = 0;
     bool use_cache = false, fill_done = false;
+    bool direct = false;                 // every station missing: compute in place, copy out once
     std::vector<CacheEntry *> entries;   // per station
     std::vector<int> misses;             // stations computed by this request
     std::vector<CacheKey> new_keys;      // entries this request created
@@ -246,8 +247,10 @@ int issue(Server *s, Request &r)
             if (rows.size() < 2) rows.push_back(rows[0]);  // a job has at least two stations
             float *dxs = (float *)(dbase + r.off_xs);
             if (copy_in_rows(s, r, rows, dxs, r.off_sxs, st)) return -1;
-            char *wss = dbase + r.off_wss;
-            int rc = fpcc_station_stage_device(&r.sub_job, dxs, wss, r.wss_bytes, st);
+            // direct: the request's own station stage; else the gathered missing stations
+            char *wss = r.direct ? ws : dbase + r.off_wss;
+            int rc = r.direct ? fpcc_station_stage_device(&r.dev_job, dxs, ws, r.ws_bytes, st)
+                              : fpcc_station_stage_device(&r.sub_job, dxs, wss, r.wss_bytes, st);
             if (rc) return rc;
             CopyArgs a{};
             a.src_base = wss;
@@ -255,7 +258,7 @@ int issue(Server *s, Request &r)
             a.nreg = r.nreg;
             a.dst_entry = 1;
             for (int k = 0; k < r.nreg; ++k) {
-                a.src_off[k] = r.sub_off[k];
+                a.src_off[k] = r.direct ? r.off[k] : r.sub_off[k];
                 a.src_step[k] = r.per[k];
                 a.dst_off[k] = r.rbase[k];
                 a.per[k] = r.per[k];
@@ -268,6 +271,12 @@ int issue(Server *s, Request &r)
         }
         for (auto &w : r.waits)
             if (hipStreamWaitEvent(st, w->e, 0) != hipSuccess) return -1;
+        if (r.direct) {  // the workspace already holds every station
+            int rc = fpcc_pair_stage_device(&r.dev_job, ws, r.ws_bytes, dy, st);
+            if (rc) return rc;
+            void *yh = r.y_pinned ? (void *)r.y : (void *)(hbase + r.off_sy);
+            return hipMemcpyAsync(yh, dy, r.y_bytes, hipMemcpyDeviceToHost, st) == hipSuccess ? 0 : -1;
+        }
         CopyArgs a{};
         a.src_base = (const char *)s->cmem;
         a.dst_base = ws;
@@ -322,7 +331,7 @@ void release(Server *s, Request &r)
                 if (r.entries[m] && r.entries[m]->ev == r.fill_ev) r.entries[m]->ev.reset();  // filled
         if (!r.misses.empty()) {
             s->darena.release(r.off_xs, r.xs_bytes);
-            s->darena.release(r.off_wss, r.wss_bytes);
+            if (r.wss_bytes) s->darena.release(r.off_wss, r.wss_bytes);
             if (!r.x_pinned) s->harena.release(r.off_sxs, r.xs_bytes);
         }
         r.entries.clear();
@@ -377,7 +386,10 @@ int reserve_cached(Server *s, Request &r)
     const int M = (int)r.misses.size();
     const size_t row = (size_t)r.job.n_windows * r.job.win_len * sizeof(float);
     r.xs_bytes = r.wss_bytes = 0;
-    if (M) {
+    r.direct = M == (int)S;  // no hits and no repeated keys: the job itself is the gathered job
+    if (r.direct) {
+        r.xs_bytes = S * row;
+    } else if (M) {
         r.sub_job = r.dev_job;
         r.sub_job.n_stations = std::max(2, M);
         r.xs_bytes = r.sub_job.n_stations * row;
@@ -440,7 +452,7 @@ int reserve_cached(Server *s, Request &r)
     s->darena.alloc(r.ws_bytes, r.off_ws);
     if (M) {
         s->darena.alloc(r.xs_bytes, r.off_xs);
-        s->darena.alloc(r.wss_bytes, r.off_wss);
+        if (r.wss_bytes) s->darena.alloc(r.wss_bytes, r.off_wss);
         if (!r.x_pinned) s->harena.alloc(r.xs_bytes, r.off_sxs);
     }
     if (!r.y_pinned) s->harena.alloc(r.y_bytes, r.off_sy);

@@ -52,6 +52,24 @@ def make_requests(n, rate=50.0, n_net=128, length=86400, seed=7, s_range=(4, 128
     return out
 
 
+def make_network_day_requests(n, rate=200.0, n_net=128, N=21600, lag=1000, seed=11, s_range=(8, 64),
+                              mix=(("pcc2", 0.5), ("wpcc", 0.5))):
+    """Requests that share one window of the network (the same day, t0 = 0):
+    random station subsets, one kind each; their station spectra repeat across
+    requests, which the server's resident-spectra cache reuses."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    kinds = [k for k, _ in mix]
+    p = [w for _, w in mix]
+    out = []
+    for k in range(n):
+        S = int(rng.integers(s_range[0], s_range[1] + 1))
+        st = sorted(rng.choice(n_net, size=S, replace=False).tolist())
+        out.append(dict(t=float(t[k]), kind=kinds[int(rng.choice(len(kinds), p=p))], S=S, N=N, lag=lag,
+                        stations=st, t0=0))
+    return out
+
+
 def job_for(req, row_len):
     kw = WPCC_PARAMS if req["kind"] == "wpcc" else {}
     return make_job(req["kind"], req["S"], 1, req["N"], -req["lag"], req["lag"], stack="none",
@@ -61,21 +79,64 @@ def job_for(req, row_len):
 def request_input(net, req):
     """The request's input as a view into the network: rows of the network's
     length starting at (s0, t0); flattened from its first sample to the last
-    station's window end."""
+    station's window end.  Requests naming scattered stations get a gathered copy
+    (one row per station)."""
+    if "stations" in req:
+        return np.ascontiguousarray(net[req["stations"], req["t0"]:req["t0"] + req["N"]]).reshape(-1)
     flat = net.reshape(-1)
     start = req["s0"] * net.shape[1] + req["t0"]
     end = (req["s0"] + req["S"] - 1) * net.shape[1] + req["t0"] + req["N"]
     return flat[start:end]
 
 
-def run(reqs, net, server=None, realtime=True, **server_kw):
+def station_keys(req, row_len):
+    """64-bit identity of each station row of a request: station index, window
+    start and length (equal keys mean equal samples)."""
+    if "stations" in req:
+        st = np.asarray(req["stations"], dtype=np.uint64)
+    else:
+        st = np.arange(req["s0"], req["s0"] + req["S"], dtype=np.uint64)
+    return (st << np.uint64(40)) | (np.uint64(req["t0"]) << np.uint64(20)) | np.uint64(req["N"] & 0xFFFFF)
+
+
+class OutputPool:
+    """Pinned output buffers of one size, handed out per request and returned once
+    the client has consumed the result (results then arrive by DMA, without the
+    server's staging copy); None when all are in use."""
+
+    def __init__(self, n, floats):
+        self.floats = floats
+        self.free = [pinned_empty(floats) for _ in range(n)]
+        self.lock = threading.Lock()
+
+    def get(self, n):
+        if n > self.floats:
+            return None
+        with self.lock:
+            return self.free.pop() if self.free else None
+
+    def put(self, buf):
+        with self.lock:
+            self.free.append(buf)
+
+
+def out_len(req):
+    P = req["S"] * (req["S"] - 1) // 2
+    return P * (2 * req["lag"] + 1)
+
+
+def run(reqs, net, server=None, realtime=True, keys=False, pool=None, keep=True, **server_kw):
     """Submits every request at its arrival time (realtime) or all at once, waits
-    for all, and returns (server stats, per-request records, outputs)."""
+    for all, and returns (server stats, per-request records, outputs).  keys:
+    name each request's stations for the resident-spectra cache; pool: an
+    OutputPool for outputs (a pageable array when it is empty); keep: return the
+    outputs (pool buffers are recycled when False)."""
     own = server is None
     srv = server or Server(**server_kw)
     recs = [None] * len(reqs)
     ys = [None] * len(reqs)
     ids = [None] * len(reqs)
+    pooled = [None] * len(reqs)
     done = threading.Event()
 
     def waiter():
@@ -84,6 +145,12 @@ def run(reqs, net, server=None, realtime=True, **server_kw):
                 time.sleep(0.0002)
             st = srv.wait(ids[k])
             recs[k] = st
+            if pooled[k] is not None:
+                if keep:
+                    ys[k] = ys[k].copy()
+                else:
+                    ys[k] = None
+                pool.put(pooled[k])
         done.set()
 
     th = threading.Thread(target=waiter, daemon=True)
@@ -94,10 +161,17 @@ def run(reqs, net, server=None, realtime=True, **server_kw):
             dt = rq["t"] - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
-        job = job_for(rq, net.shape[1])
-        y = np.empty(int(np.prod(output_shape(job))), dtype=np.float32)
+        row_len = rq["N"] if "stations" in rq else net.shape[1]
+        job = job_for(rq, row_len)
+        n = int(np.prod(output_shape(job)))
+        buf = pool.get(n) if pool is not None else None
+        if buf is not None:
+            pooled[k] = buf
+            y = buf[:n]
+        else:
+            y = np.empty(n, dtype=np.float32)
         ys[k] = y
-        ids[k] = srv.submit(job, request_input(net, rq), y)
+        ids[k] = srv.submit(job, request_input(net, rq), y, keys=station_keys(rq, row_len) if keys else None)
     done.wait()
     wall = time.perf_counter() - t0
     stats = srv.stats()
@@ -110,7 +184,7 @@ def run(reqs, net, server=None, realtime=True, **server_kw):
 def summary(stats, reqs, rate):
     """JSON-ready summary of a serving run."""
     lat = stats
-    kinds = {k: sum(1 for r in reqs if r["kind"] == k) for k in KINDS}
+    kinds = {k: sum(1 for r in reqs if r["kind"] == k) for k in sorted({r["kind"] for r in reqs})}
     return {
         "requests": len(reqs), "arrival_rate_per_s": rate, "completed": int(lat["completed"]),
         "failed": int(lat["failed"]), "p50_ms": round(lat["p50_ms"], 3), "p99_ms": round(lat["p99_ms"], 3),
@@ -118,5 +192,6 @@ def summary(stats, reqs, rate):
         "pair_correlations_per_s": round(lat["pair_correlations"] / max(lat["wall_s"], 1e-9), 1),
         "pair_correlations": int(lat["pair_correlations"]), "wall_s": round(lat["wall_s"], 3),
         "inflight_peak": int(lat["inflight_peak"]), "device_peak_GB": round(lat["device_peak"] / 1e9, 3),
-        "mix": kinds,
+        "cache_hits": int(lat["cache_hits"]), "cache_misses": int(lat["cache_misses"]),
+        "mix": {k: v for k, v in kinds.items() if v},
     }
