@@ -112,6 +112,8 @@ def lib():
                                        ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_uint),
                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint), ctypes.c_float]
     L.cuda_tspcc2_set_freq.restype = i
+    L.AnalyticSignal.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), u]
+    L.AnalyticSignal.restype = i
     jp = ctypes.POINTER(FpccJob)
     L.fpcc_output_len.argtypes = [jp]
     L.fpcc_output_len.restype = ctypes.c_size_t

@@ -51,6 +51,17 @@ def _pair_set(name, x1, x2, lag1, lag2):
     return y
 
 
+def analytic_signal(x):
+    """FP64 analytic signal of one real trace (AnalyticSignal, src/FFTapps.c:45-80)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros(x.size, dtype=np.complex128)
+    r = _lib.lib().AnalyticSignal(ctypes.c_void_p(y.ctypes.data), x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                  x.size)
+    if r != 0:
+        raise FastPCCError(r, "AnalyticSignal")
+    return y
+
+
 def pcc2_set(x1, x2, lag1, lag2):
     """PCC2 of one station pair over Tr windows (src/FFTapps.c:785-881)."""
     return _pair_set("pcc2_set", x1, x2, lag1, lag2)

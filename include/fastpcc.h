@@ -34,6 +34,13 @@ extern "C" {
  * Reference entry points (src/FFTapps.h:6-16).  Host buffers in and out.
  * ------------------------------------------------------------------------- */
 
+/* FP64 analytic signal, replaces AnalyticSignal, src/FFTapps.c:45-80:
+ * y[n] = sum_k mask_k X_k e^{2 pi i k n/N}, X = DFT_N(x), mask = 1/N at DC, 2/N for
+ * 0 < k < (N+1)/2, 1/N at N/2 for even N, 0 above.  y and x hold N values (host
+ * memory); -1 if x or y is NULL or on a GPU failure.  Only dead branches of the
+ * reference call it (src/FFTapps.c:432-473, 743-779). */
+int AnalyticSignal(double _Complex *y, double *x, unsigned int N);
+
 /* PCC of power 2 (Ventosa et al. 2019), replaces pcc2_set, src/FFTapps.c:785-881.
  * FP32 like the reference's fftwf path. */
 int pcc2_set(float **const y, float **const x1, float **const x2, const unsigned int N,
