@@ -76,6 +76,7 @@ class ServingStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_ulonglong) for n in ("submitted", "completed", "failed", "cancelled", "rejected",
                                                   "cache_hits", "cache_misses")] + \
                [("cache_used", ctypes.c_size_t)] + \
+               [(n, ctypes.c_ulonglong) for n in ("batched_requests", "batch_launches")] + \
                [(n, ctypes.c_ulonglong) for n in ("queued", "inflight", "inflight_peak")] + \
                [(n, ctypes.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "pair_correlations",
                                                "busy_s")] + \
@@ -124,9 +125,18 @@ def lib():
     L.fpcc_allpairs_device.restype = i
     L.fpcc_allpairs.argtypes = [jp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
     L.fpcc_allpairs.restype = i
+    L.fpcc_station_stage_batch_device.argtypes = [jp, i, ctypes.POINTER(ctypes.c_void_p),
+                                                  ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t),
+                                                  ctypes.c_void_p]
+    L.fpcc_station_stage_batch_device.restype = i
+    L.fpcc_station_stage_device.argtypes = [jp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    L.fpcc_station_stage_device.restype = i
+    L.fpcc_pair_stage_device.argtypes = [jp, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]
+    L.fpcc_pair_stage_device.restype = i
     vp, ull = ctypes.c_void_p, ctypes.c_ulonglong
     L.fpcc_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
     L.fpcc_server_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ull)]
+    L.fpcc_server_submit_many.argtypes = [vp, ctypes.POINTER(Request), i, ctypes.POINTER(ull)]
     L.fpcc_server_poll.argtypes = [vp, ull, ctypes.POINTER(RequestStatus)]
     L.fpcc_server_wait.argtypes = [vp, ull, i, ctypes.POINTER(RequestStatus)]
     L.fpcc_server_cancel.argtypes = [vp, ull]
@@ -137,7 +147,7 @@ def lib():
     L.fpcc_host_alloc.restype = vp
     L.fpcc_host_free.argtypes = [vp]
     L.fpcc_host_free.restype = None
-    for name in ("create", "submit", "poll", "wait", "cancel", "forget", "stats", "destroy"):
+    for name in ("create", "submit", "submit_many", "poll", "wait", "cancel", "forget", "stats", "destroy"):
         getattr(L, "fpcc_server_" + name).restype = i
     _lib = L
     return L

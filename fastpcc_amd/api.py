@@ -315,6 +315,30 @@ class Server:
         self._keep[rid.value] = (x, y, keys)
         return rid.value
 
+    def submit_many(self, items):
+        """items: (job, x, y) or (job, x, y, keys) tuples entering the queue together.
+        Returns the request ids."""
+        reqs = (_lib.Request * len(items))()
+        keep = []
+        for k, it in enumerate(items):
+            job, x, y = it[:3]
+            keys = it[3] if len(it) > 3 else None
+            assert x.dtype == np.float32 and x.flags.c_contiguous and y.dtype == np.float32 and y.flags.c_contiguous
+            kp = None
+            if keys is not None:
+                keys = np.ascontiguousarray(keys, dtype=np.uint64)
+                kp = keys.ctypes.data_as(ctypes.POINTER(ctypes.c_ulonglong))
+            reqs[k] = _lib.Request(job, x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                   y.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), kp)
+            keep.append((x, y, keys))
+        ids = (ctypes.c_ulonglong * len(items))()
+        r = _lib.lib().fpcc_server_submit_many(self._h, reqs, len(items), ids)
+        if r != 0:
+            raise FastPCCError(r, "fpcc_server_submit_many")
+        for k in range(len(items)):
+            self._keep[ids[k]] = keep[k]
+        return list(ids)
+
     def _status(self, st):
         return {"state": st.state, "rc": st.rc, "t_submit": st.t_submit, "t_start": st.t_start,
                 "t_done": st.t_done}

@@ -3,6 +3,7 @@
 #include <map>
 #include <mutex>
 #include "bigfft.hpp"
+#include "kernels.hpp"
 #include "profile.hpp"
 
 namespace fpcc {
@@ -69,15 +70,36 @@ __device__ __forceinline__ void pair_of_big(int p, int S, int &i, int &j)
 // index of W_n^{c r} in the full table: c < N2 and r < N1, so c*r < n
 __device__ __forceinline__ int twidx(int c, int r) { return c * r; }
 
+// Job-table launch (BATCH): batch index b runs over the station-windows of several
+// jobs; job j (jb.b0[j] <= b < jb.b0[j+1]) supplies x, its stride, W, N and the
+// destination rows (COL_FWD_INPUT only).
+__device__ __forceinline__ int batch_job(const RealBatch &jb, int b)
+{
+    int j = 0;
+    while (j + 1 < jb.n && b >= jb.b0[j + 1]) ++j;
+    return j;
+}
+
+template <bool BATCH>
 __global__ void __launch_bounds__(256)
-k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, const cf32 *__restrict__ twN)
+k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, const cf32 *__restrict__ twN,
+           RealBatch jb)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int ld = N1 + 1;  // padded column stride (the loads write with stride ld)
     cf32 *buf0 = reinterpret_cast<cf32 *>(smem_raw);
     cf32 *buf1 = buf0 + kBigCols * ld;
     float *red = reinterpret_cast<float *>(buf1 + kBigCols * ld);
-    const int b = blockIdx.y;
+    int b = blockIdx.y;
+    if (BATCH) {
+        const int j = batch_job(jb, b);
+        b -= jb.b0[j];
+        a.x = jb.x[j];
+        a.x_stride = jb.stride[j];
+        a.W = jb.W[j];
+        a.N = jb.N[j];
+        a.dst = (cf32 *)jb.X[j];
+    }
     const int c0 = blockIdx.x * kBigCols;
     const long long base = (long long)b * a.bstride;
     const long long sbase = (long long)b * (a.src_bstride ? a.src_bstride : a.bstride);
@@ -203,13 +225,21 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
     }
 }
 
+// BATCH: the rows of job j's destination (in place, ROW_FWD_PLAIN only).
+template <bool BATCH>
 __global__ void __launch_bounds__(256)
-k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, const cf32 *__restrict__ twN)
+k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, const cf32 *__restrict__ twN,
+           RealBatch jb)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cf32 *buf0 = reinterpret_cast<cf32 *>(smem_raw);
     cf32 *buf1 = buf0 + kBigRows * N2;
-    const int b = blockIdx.y;
+    int b = blockIdx.y;
+    if (BATCH) {
+        const int j = batch_job(jb, b);
+        b -= jb.b0[j];
+        a.src = a.dst = (cf32 *)jb.X[j];
+    }
     const int r0 = blockIdx.x * kBigRows;
     const long long base = (long long)b * a.bstride;
     const long long n = (long long)N1 * N2;
@@ -295,12 +325,24 @@ k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, 
 }
 
 // One workgroup per station-window: P[n] = sum_{m<n} x[m]^2 in FP64, n in [0, N].
+// BATCH: the station-windows of the jobs of jb (x, stride, W, N, E, lde per job).
+template <bool BATCH>
 __global__ void __launch_bounds__(512)
 k_energy_prefix(const float *__restrict__ x, long long station_stride, int W, int N, double *__restrict__ E,
-                long long lde)
+                long long lde, RealBatch jb)
 {
     __shared__ double part[512];
-    const int sw = blockIdx.x;
+    int sw = blockIdx.x;
+    if (BATCH) {
+        const int j = batch_job(jb, sw);
+        sw -= jb.b0[j];
+        x = jb.x[j];
+        station_stride = jb.stride[j];
+        W = jb.W[j];
+        N = jb.N[j];
+        E = jb.E[j];
+        lde = jb.lde[j];
+    }
     const int s = sw / W, w = sw - s * W;
     const float *in = x + (long long)s * station_stride + (long long)w * N;
     const int T = blockDim.x;
@@ -336,7 +378,19 @@ int launch_energy_prefix(const float *d_x, long long station_stride, int S, int 
                          hipStream_t st)
 {
     ProfScope prof("energy_prefix", st);
-    hipLaunchKernelGGL(k_energy_prefix, dim3(S * W), dim3(512), 0, st, d_x, station_stride, W, N, d_E, lde);
+    RealBatch none{};
+    hipLaunchKernelGGL(k_energy_prefix<false>, dim3(S * W), dim3(512), 0, st, d_x, station_stride, W, N, d_E, lde,
+                       none);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_energy_prefix_batch(const RealBatch &jb, hipStream_t st)
+{
+    const int total = jb.b0[jb.n];
+    if (total <= 0) return 0;
+    ProfScope prof("energy_prefix_batch", st);
+    hipLaunchKernelGGL(k_energy_prefix<true>, dim3(total), dim3(512), 0, st, nullptr, 0LL, 1, 0, nullptr, 0LL, jb);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }
@@ -408,10 +462,25 @@ int run_cols(const BigPlan &bp, const ColArgs &a, int nbatch, hipStream_t st, co
         return -1;
     }
     const size_t lds = sizeof(cf32) * 2 * kBigCols * (size_t)(bp.n1 + 1) + sizeof(float) * 8;
-    allow_lds((const void *)k_big_cols, lds);
+    allow_lds((const void *)k_big_cols<false>, lds);
     ProfScope prof(name, st);
-    hipLaunchKernelGGL(k_big_cols, dim3((bp.n2 + kBigCols - 1) / kBigCols, nbatch), dim3(256), lds, st, a, bp.n1,
-                       bp.n2, bp.p1, bp.tw1, bp.twN);
+    RealBatch none{};
+    hipLaunchKernelGGL(k_big_cols<false>, dim3((bp.n2 + kBigCols - 1) / kBigCols, nbatch), dim3(256), lds, st, a,
+                       bp.n1, bp.n2, bp.p1, bp.tw1, bp.twN, none);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+int run_cols_batch(const BigPlan &bp, const ColArgs &a, const RealBatch &jb, hipStream_t st, const char *name)
+{
+    const int nbatch = jb.b0[jb.n];
+    if (nbatch <= 0) return 0;
+    if (!bp.ok || a.op != COL_FWD_INPUT || nbatch > 65535) return -1;
+    const size_t lds = sizeof(cf32) * 2 * kBigCols * (size_t)(bp.n1 + 1) + sizeof(float) * 8;
+    allow_lds((const void *)k_big_cols<true>, lds);
+    ProfScope prof(name, st);
+    hipLaunchKernelGGL(k_big_cols<true>, dim3((bp.n2 + kBigCols - 1) / kBigCols, nbatch), dim3(256), lds, st, a,
+                       bp.n1, bp.n2, bp.p1, bp.tw1, bp.twN, jb);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }
@@ -425,10 +494,25 @@ int run_rows(const BigPlan &bp, const RowArgs &a, int nbatch, hipStream_t st, co
         return -1;
     }
     const size_t lds = sizeof(cf32) * 2 * kBigRows * (size_t)bp.n2;
-    allow_lds((const void *)k_big_rows, lds);
+    allow_lds((const void *)k_big_rows<false>, lds);
     ProfScope prof(name, st);
-    hipLaunchKernelGGL(k_big_rows, dim3((bp.n1 + kBigRows - 1) / kBigRows, nbatch), dim3(256), lds, st, a, bp.n1,
-                       bp.n2, bp.p2, bp.tw2, bp.twN);
+    RealBatch none{};
+    hipLaunchKernelGGL(k_big_rows<false>, dim3((bp.n1 + kBigRows - 1) / kBigRows, nbatch), dim3(256), lds, st, a,
+                       bp.n1, bp.n2, bp.p2, bp.tw2, bp.twN, none);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+int run_rows_batch(const BigPlan &bp, const RowArgs &a, const RealBatch &jb, hipStream_t st, const char *name)
+{
+    const int nbatch = jb.b0[jb.n];
+    if (nbatch <= 0) return 0;
+    if (!bp.ok || a.op != ROW_FWD_PLAIN || nbatch > 65535) return -1;
+    const size_t lds = sizeof(cf32) * 2 * kBigRows * (size_t)bp.n2;
+    allow_lds((const void *)k_big_rows<true>, lds);
+    ProfScope prof(name, st);
+    hipLaunchKernelGGL(k_big_rows<true>, dim3((bp.n1 + kBigRows - 1) / kBigRows, nbatch), dim3(256), lds, st, a,
+                       bp.n1, bp.n2, bp.p2, bp.tw2, bp.twN, jb);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }

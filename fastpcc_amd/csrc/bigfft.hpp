@@ -103,7 +103,13 @@ int bluestein_n(int N);
 // T = FFT_n(b)/n in the two-pass (permuted) order, b = conj(w) wrapped circularly.
 int bluestein_tables(int N, int n, cf32 *chirp, cf32 *T, hipStream_t st);
 
+struct RealBatch;
 int run_cols(const BigPlan &bp, const ColArgs &a, int nbatch, hipStream_t st, const char *name);
+// COL_FWD_INPUT / ROW_FWD_PLAIN over the station-windows of several jobs (x, stride,
+// W, N and destination rows from the job table; the other fields from a).
+int run_cols_batch(const BigPlan &bp, const ColArgs &a, const RealBatch &jb, hipStream_t st, const char *name);
+int run_rows_batch(const BigPlan &bp, const RowArgs &a, const RealBatch &jb, hipStream_t st, const char *name);
+int launch_energy_prefix_batch(const RealBatch &jb, hipStream_t st);
 int run_rows(const BigPlan &bp, const RowArgs &a, int nbatch, hipStream_t st, const char *name);
 int fill_zero(float *p, long long n, hipStream_t st);
 // Exclusive prefix sums of x^2 in FP64, P[sw][0..N] (rows of lde), for the GNCC energies.

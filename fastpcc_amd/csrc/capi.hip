@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <vector>
 #include "../../include/fastpcc.h"
+#include "bigfft.hpp"
 #include "engine.hpp"
 #include "kernels.hpp"
 #include "wavelet.hpp"
@@ -461,6 +462,133 @@ extern "C" int fpcc_pair_stage_device(const fpcc_job *job, void *d_ws, size_t ws
     if (r) return r;
     if (!d_y || !d_ws || ws_bytes < layout(job, d).total) return -1;
     return pair_stage(job, d, (char *)d_ws, d_y, (hipStream_t)stream);
+}
+
+namespace {
+
+// Jobs whose station stages can share launches: GNCC and 1-bit (real-input
+// transforms of Nz points; any station count, window count, window length and
+// stride), grouped by kind, path and Nz.
+bool batch_key(const fpcc_job *job, const Dims &d, long long &key)
+{
+    if (job->kind != FPCC_GNCC && job->kind != FPCC_CC1B) return false;
+    key = ((long long)d.Nz << 8) | ((long long)d.large << 4) | job->kind;
+    return true;
+}
+
+// One group of batchable jobs (same key), at most kMaxBatchJobs.
+int station_stage_group(const std::vector<const fpcc_job *> &jobs, const std::vector<Dims> &D,
+                        const std::vector<const float *> &x, const std::vector<char *> &ws, hipStream_t st)
+{
+    const int n = (int)jobs.size();
+    const int kind = jobs[0]->kind, Nz = (int)D[0].Nz;
+    const bool large = D[0].large;
+    RealBatch jb{};
+    jb.n = n;
+    jb.b0[0] = 0;
+    for (int g = 0; g < n; ++g) {
+        jb.x[g] = x[g];
+        jb.stride[g] = (long long)jobs[g]->station_stride;
+        jb.W[g] = D[g].W;
+        jb.N[g] = D[g].N;
+        jb.lde[g] = energy_ld(D[g].N);
+        jb.b0[g + 1] = jb.b0[g] + D[g].S * D[g].W;
+    }
+    const int passes = kind == FPCC_CC1B ? 2 : 1;
+    int r = 0;
+    if (!large) {
+        for (int p = 0; p < passes; ++p) {
+            for (int g = 0; g < n; ++g) {
+                const Layout l = layout(jobs[g], D[g]);
+                jb.X[g] = ws[g] + (p ? l.b : l.a);
+                jb.E[g] = kind == FPCC_GNCC ? (double *)(ws[g] + l.b) : nullptr;
+            }
+            if ((r = launch_station_real_batch(jb, Nz, kind == FPCC_CC1B ? p + 1 : 0, kind == FPCC_GNCC, st)))
+                return r;
+        }
+        return 0;
+    }
+    if (jb.b0[n] > 65535) return -1;
+    const BigPlan &bz = big_plan(Nz);
+    for (int p = 0; p < passes; ++p) {
+        for (int g = 0; g < n; ++g) {
+            size_t off[4], per[4];
+            if (large_station_regions(kind, jobs[g]->stack, D[g].S, D[g].W, D[g].N, Nz, off, per) < 2) return -1;
+            const Layout l = layout(jobs[g], D[g]);
+            jb.X[g] = ws[g] + l.a + off[p];                                               // F, or F2 (1-bit)
+            jb.E[g] = kind == FPCC_GNCC ? (double *)(ws[g] + l.a + off[1]) : nullptr;  // GNCC energies
+        }
+        ColArgs ca;
+        ca.op = COL_FWD_INPUT;
+        ca.bstride = Nz;
+        ca.sign_rule = kind == FPCC_CC1B ? p + 1 : 0;
+        if ((r = run_cols_batch(bz, ca, jb, st, "real_fwd_cols_batch"))) return r;
+        RowArgs ra;
+        ra.op = ROW_FWD_PLAIN;
+        ra.bstride = Nz;
+        if ((r = run_rows_batch(bz, ra, jb, st, "real_fwd_rows_batch"))) return r;
+    }
+    if (kind == FPCC_GNCC) return launch_energy_prefix_batch(jb, st);
+    return 0;
+}
+
+}  // namespace
+
+namespace fpcc {
+bool job_batch_key(const fpcc_job *job, long long *key)
+{
+    Dims d;
+    if (resolve(job, d)) return false;
+    return batch_key(job, d, *key);
+}
+}  // namespace fpcc
+
+extern "C" int fpcc_station_stage_batch_device(const fpcc_job *jobs, int njobs, const float *const *d_x,
+                                               void *const *d_ws, const size_t *ws_bytes, void *stream)
+{
+    if (njobs < 0 || (njobs > 0 && (!jobs || !d_x || !d_ws || !ws_bytes))) return -1;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<Dims> D(njobs);
+    for (int j = 0; j < njobs; ++j) {
+        int r = resolve(&jobs[j], D[j]);
+        if (r) return r;
+        if (!d_x[j] || !d_ws[j] || ws_bytes[j] < layout(&jobs[j], D[j]).total) return -1;
+    }
+    std::vector<char> done(njobs, 0);
+    for (int i = 0; i < njobs; ++i) {
+        if (done[i]) continue;
+        long long ki;
+        std::vector<int> grp{i};
+        if (batch_key(&jobs[i], D[i], ki)) {
+            long long sw = (long long)D[i].S * D[i].W;
+            for (int j = i + 1; j < njobs && (int)grp.size() < kMaxBatchJobs; ++j) {
+                long long kj;
+                if (done[j] || !batch_key(&jobs[j], D[j], kj) || kj != ki) continue;
+                if (D[i].large && sw + (long long)D[j].S * D[j].W > 65535) continue;
+                sw += (long long)D[j].S * D[j].W;
+                grp.push_back(j);
+            }
+        }
+        for (int g : grp) done[g] = 1;
+        int r;
+        if (grp.size() == 1) {
+            r = station_stage(&jobs[i], D[i], d_x[i], (char *)d_ws[i], st);
+        } else {
+            std::vector<const fpcc_job *> jp;
+            std::vector<Dims> dg;
+            std::vector<const float *> xg;
+            std::vector<char *> wg;
+            for (int g : grp) {
+                jp.push_back(&jobs[g]);
+                dg.push_back(D[g]);
+                xg.push_back(d_x[g]);
+                wg.push_back((char *)d_ws[g]);
+            }
+            r = station_stage_group(jp, dg, xg, wg, st);
+        }
+        if (r) return r;
+    }
+    return 0;
 }
 
 extern "C" int fpcc_allpairs_device(const fpcc_job *job, const float *d_x, float *d_y, void *d_ws, size_t ws_bytes,

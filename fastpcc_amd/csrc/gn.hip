@@ -28,20 +28,17 @@ __device__ __forceinline__ T sample_val(float v, int sign_rule)
     return (T)v;
 }
 
+// One station-window: half spectrum of the zero-padded samples (or signs) into
+// out[0..M], and (P != nullptr) the exclusive FP64 prefix sums of x^2 into P[0..N].
 template <typename T>
-__global__ void __launch_bounds__(512)
-k_station_real(const float *__restrict__ x, long long station_stride, int W, int N, int Nz, int sign_rule,
-               FftPlan planM, const C2<T> *__restrict__ twM, const C2<T> *__restrict__ twc,
-               C2<T> *__restrict__ X, long long ldx, double *__restrict__ E, long long lde)
+__device__ void station_real_row(const float *__restrict__ in, int N, int Nz, int sign_rule, const FftPlan &planM,
+                                 const C2<T> *__restrict__ twM, const C2<T> *__restrict__ twc, C2<T> *__restrict__ out,
+                                 double *__restrict__ P, unsigned char *smem_raw)
 {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
     const int M = Nz >> 1;
     C2<T> *buf0 = reinterpret_cast<C2<T> *>(smem_raw);
     C2<T> *buf1 = buf0 + M;
     double *scan = reinterpret_cast<double *>(buf1 + M);  // blockDim.x partial sums
-    const int sw = blockIdx.x;
-    const int s = sw / W, w = sw - s * W;
-    const float *in = x + (long long)s * station_stride + (long long)w * N;
     const int T_ = blockDim.x;
     for (int m = threadIdx.x; m < M; m += T_) {
         const int n0 = 2 * m, n1 = 2 * m + 1;
@@ -51,7 +48,6 @@ k_station_real(const float *__restrict__ x, long long station_stride, int W, int
     }
     __syncthreads();
     C2<T> *Z = fft_lds<T, -1>(buf0, buf1, planM, twM);
-    C2<T> *out = X + (long long)sw * ldx;
     for (int k = threadIdx.x; k <= M; k += T_) {
         C2<T> zk = Z[k == M ? 0 : k];
         C2<T> zm = Z[k == 0 ? 0 : M - k];
@@ -62,7 +58,7 @@ k_station_real(const float *__restrict__ x, long long station_stride, int W, int
         C2<T> wk = (k == M) ? C2<T>{T(-1), T(0)} : C2<T>{twc[k].x, -twc[k].y};
         out[k] = ev + cmul(wk, od);
     }
-    if (E) {
+    if (P) {
         // exclusive prefix sums of x^2 (double), P[0..N]
         const int per = (N + T_ - 1) / T_;
         const int a = threadIdx.x * per, b = min(N, a + per);
@@ -83,7 +79,6 @@ k_station_real(const float *__restrict__ x, long long station_stride, int W, int
         }
         __syncthreads();
         double run = scan[threadIdx.x];
-        double *P = E + (long long)sw * lde;
         for (int n = a; n < b; ++n) {
             P[n] = run;
             double v = (double)in[n];
@@ -92,6 +87,38 @@ k_station_real(const float *__restrict__ x, long long station_stride, int W, int
         if (b == N && a < N) P[N] = run;
         if (N == 0 && threadIdx.x == 0) P[0] = 0.0;
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(512)
+k_station_real(const float *__restrict__ x, long long station_stride, int W, int N, int Nz, int sign_rule,
+               FftPlan planM, const C2<T> *__restrict__ twM, const C2<T> *__restrict__ twc,
+               C2<T> *__restrict__ X, long long ldx, double *__restrict__ E, long long lde)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int sw = blockIdx.x;
+    const int s = sw / W, w = sw - s * W;
+    const float *in = x + (long long)s * station_stride + (long long)w * N;
+    station_real_row<T>(in, N, Nz, sign_rule, planM, twM, twc, X + (long long)sw * ldx,
+                        E ? E + (long long)sw * lde : nullptr, smem_raw);
+}
+
+// The same for the station-windows of several jobs with one Nz (uneven station
+// counts, window counts and lengths) in one launch: block b belongs to job j with
+// b0[j] <= b < b0[j+1].
+template <typename T>
+__global__ void __launch_bounds__(512)
+k_station_real_batch(RealBatch jb, int Nz, int sign_rule, FftPlan planM, const C2<T> *__restrict__ twM,
+                     const C2<T> *__restrict__ twc, long long ldx)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int j = 0;
+    while (j + 1 < jb.n && (int)blockIdx.x >= jb.b0[j + 1]) ++j;
+    const int sw = blockIdx.x - jb.b0[j];
+    const int s = sw / jb.W[j], w = sw - s * jb.W[j];
+    const float *in = jb.x[j] + (long long)s * jb.stride[j] + (long long)w * jb.N[j];
+    station_real_row<T>(in, jb.N[j], Nz, sign_rule, planM, twM, twc, (C2<T> *)jb.X[j] + (long long)sw * ldx,
+                        jb.E[j] ? jb.E[j] + (long long)sw * jb.lde[j] : nullptr, smem_raw);
 }
 
 __device__ __forceinline__ void pair_of_gn(int p, int S, int &i, int &j)
@@ -213,6 +240,33 @@ int launch_station_real(const float *d_x, long long station_stride, int S, int W
         hipLaunchKernelGGL(k_station_real<float>, dim3(S * W), dim3(threads), lds, st, d_x, station_stride, W, N,
                            Nz, sign_rule, pM.plan, (const cf32 *)pM.tw, (const cf32 *)twc, (cf32 *)d_X, real_ld(Nz),
                            d_E, energy_ld(N));
+    }
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_station_real_batch(const RealBatch &jb, int Nz, int sign_rule, bool f64, hipStream_t st)
+{
+    if (jb.n < 1 || jb.n > kMaxBatchJobs) return -1;
+    const int M = Nz / 2;
+    const DevPlan &pM = get_plan(M, f64);
+    const void *twc = get_c2r_twiddles(Nz, f64);
+    if (!pM.ok || !twc) return -1;
+    const int threads = 512;
+    const size_t elem = f64 ? sizeof(cf64) : sizeof(cf32);
+    size_t lds = elem * 2 * (size_t)M + sizeof(double) * threads;
+    if (lds > 160 * 1024) return -1;
+    const int total = jb.b0[jb.n];
+    if (total <= 0) return 0;
+    ProfScope prof(f64 ? "station_real_f64_batch" : "station_real_f32_batch", st);
+    if (f64) {
+        allow_lds((const void *)k_station_real_batch<double>, lds);
+        hipLaunchKernelGGL(k_station_real_batch<double>, dim3(total), dim3(threads), lds, st, jb, Nz, sign_rule,
+                           pM.plan, (const cf64 *)pM.tw, (const cf64 *)twc, real_ld(Nz));
+    } else {
+        allow_lds((const void *)k_station_real_batch<float>, lds);
+        hipLaunchKernelGGL(k_station_real_batch<float>, dim3(total), dim3(threads), lds, st, jb, Nz, sign_rule,
+                           pM.plan, (const cf32 *)pM.tw, (const cf32 *)twc, real_ld(Nz));
     }
     FPCC_CHECK(hipGetLastError());
     return 0;

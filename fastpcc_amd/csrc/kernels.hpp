@@ -49,12 +49,29 @@ int fill_zero(float *p, long long n, hipStream_t st);
 int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int L, float scale, int count_norm,
                     float *d_y, hipStream_t st);
 
+// Station-windows of several jobs in one launch (uneven station counts, window
+// counts and lengths; one transform length): job j owns launch blocks
+// [b0[j], b0[j+1]), one per station-window, reading x[j] rows of stride[j] and
+// writing the rows of X[j] (and E[j] energy prefix sums of lde[j], if not null).
+constexpr int kMaxBatchJobs = 16;
+struct RealBatch {
+    int n;
+    int b0[kMaxBatchJobs + 1];
+    const float *x[kMaxBatchJobs];
+    long long stride[kMaxBatchJobs];
+    int W[kMaxBatchJobs], N[kMaxBatchJobs];
+    void *X[kMaxBatchJobs];
+    double *E[kMaxBatchJobs];
+    long long lde[kMaxBatchJobs];
+};
+
 // GNCC / 1-bit (gn.hip).  Half spectra rows hold real_ld(Nz) bins, energy rows
 // energy_ld(N) prefix sums.  sign_rule 0: samples, 1: x >= 0, 2: x > 0.
 long long real_ld(int Nz);
 long long energy_ld(int N);
 int launch_station_real(const float *d_x, long long station_stride, int S, int W, int N, int Nz, int sign_rule,
                         bool f64, void *d_X, double *d_E, hipStream_t st);
+int launch_station_real_batch(const RealBatch &jb, int Nz, int sign_rule, bool f64, hipStream_t st);
 // norm 0: sqrt(E1 E2) from prefix sums (GNCC); 1: overlap count (1-bit).
 // mode 0: y[p][w][L]; 1: mean over windows.
 int launch_pair_gn(const void *d_Xa, const void *d_Xb, const double *d_E, bool f64, int S, int W, int N, int Nz,
@@ -86,5 +103,8 @@ int wpcc_pair_stage(int S, int W, int N, const WFamily &f, int lag0, int L, bool
 // station's rows are the same in any job with the same kind, windows, N and Nz
 // (and, for WPCC, filter parameters).
 int job_station_regions(const fpcc_job *job, size_t *off, size_t *per);
+// Whether the job's station stage can share launches with other jobs of the same
+// key (fpcc_station_stage_batch_device).
+bool job_batch_key(const fpcc_job *job, long long *key);
 
 }  // namespace fpcc

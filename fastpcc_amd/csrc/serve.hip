@@ -80,6 +80,8 @@ struct Request {
     int stream = -1;
     double t_submit = 0, t_start = 0, t_done = 0;
     unsigned long long pair_corr = 0;
+    bool batchable = false;    // station stage may share launches (fpcc_station_stage_batch_device)
+    long long batch_key = 0;
     // station-spectra cache (keyed requests)
     std::vector<unsigned long long> keys;
     int nreg = 0;
@@ -124,7 +126,7 @@ struct Server {
     bool stop = false;
     // statistics
     unsigned long long n_submit = 0, n_done = 0, n_fail = 0, n_cancel = 0, n_reject = 0;
-    unsigned long long n_hit = 0, n_miss = 0;
+    unsigned long long n_hit = 0, n_miss = 0, n_batched = 0, n_batch_launch = 0;
     std::vector<double> lat_ms;
     double first_start = -1, last_done = 0;
     unsigned long long pair_corr_done = 0;
@@ -230,6 +232,34 @@ int copy_in_rows(Server *s, Request &r, const std::vector<int> &rows, float *dst
     return 0;
 }
 
+// H2D of an uncached request's whole input into its compact device rows.
+int stage_input(Server *s, Request &r)
+{
+    hipStream_t st = s->streams[r.stream];
+    const fpcc_job &j = r.job;
+    const size_t row = (size_t)j.n_windows * j.win_len * sizeof(float);
+    float *dx = (float *)((char *)s->dmem + r.off_x);
+    if (r.x_pinned)
+        return hipMemcpy2DAsync(dx, row, r.x, j.station_stride * sizeof(float), row, j.n_stations,
+                                hipMemcpyHostToDevice, st) == hipSuccess ? 0 : -1;
+    char *hx = (char *)s->hmem + r.off_sx;
+    for (unsigned i = 0; i < j.n_stations; ++i) memcpy(hx + i * row, r.x + (size_t)i * j.station_stride, row);
+    return hipMemcpyAsync(dx, hx, r.x_bytes, hipMemcpyHostToDevice, st) == hipSuccess ? 0 : -1;
+}
+
+// D2H of the output (into the caller's pinned y, or the staging block).
+int copy_out(Server *s, Request &r)
+{
+    void *yh = r.y_pinned ? (void *)r.y : (void *)((char *)s->hmem + r.off_sy);
+    return hipMemcpyAsync(yh, (char *)s->dmem + r.off_y, r.y_bytes, hipMemcpyDeviceToHost, s->streams[r.stream]) ==
+                   hipSuccess ? 0 : -1;
+}
+
+bool injected_failure(Server *s, const Request &r)
+{
+    return s->cfg.fail_every > 0 && r.id % (unsigned long long)s->cfg.fail_every == 0;
+}
+
 // Launches one admitted request on its stream (scheduler thread, lock not held).
 int issue(Server *s, Request &r)
 {
@@ -240,7 +270,7 @@ int issue(Server *s, Request &r)
     char *ws = dbase + r.off_ws;
     const fpcc_job &j = r.job;
     const size_t row = (size_t)j.n_windows * j.win_len * sizeof(float);
-    if (s->cfg.fail_every > 0 && r.id % (unsigned long long)s->cfg.fail_every == 0) return -1;  // injected
+    if (injected_failure(s, r)) return -1;
     if (r.use_cache) {
         if (!r.misses.empty()) {
             std::vector<int> rows(r.misses);
@@ -294,23 +324,66 @@ int issue(Server *s, Request &r)
         int rc = fpcc_pair_stage_device(&r.dev_job, ws, r.ws_bytes, dy, st);
         if (rc) return rc;
     } else {
-        float *dx = (float *)(dbase + r.off_x);
-        if (r.x_pinned) {
-            if (hipMemcpy2DAsync(dx, row, r.x, j.station_stride * sizeof(float), row, j.n_stations,
-                                 hipMemcpyHostToDevice, st) != hipSuccess)
-                return -1;
-        } else {
-            char *hx = hbase + r.off_sx;
-            for (unsigned i = 0; i < j.n_stations; ++i)
-                memcpy(hx + i * row, r.x + (size_t)i * j.station_stride, row);
-            if (hipMemcpyAsync(dx, hx, r.x_bytes, hipMemcpyHostToDevice, st) != hipSuccess) return -1;
-        }
-        int rc = fpcc_allpairs_device(&r.dev_job, dx, dy, ws, r.ws_bytes, st);
+        if (stage_input(s, r)) return -1;
+        int rc = fpcc_allpairs_device(&r.dev_job, (float *)(dbase + r.off_x), dy, ws, r.ws_bytes, st);
         if (rc) return rc;
     }
-    void *yh = r.y_pinned ? (void *)r.y : (void *)(hbase + r.off_sy);
-    if (hipMemcpyAsync(yh, dy, r.y_bytes, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
-    return 0;
+    (void)row;
+    return copy_out(s, r);
+}
+
+// A group of uncached requests whose station stages share launches: inputs on
+// their own streams, the grouped station stage on the first one's stream (joined
+// by events both ways), then each pair stage and output copy on its own stream.
+int issue_group(Server *s, std::vector<Request *> &g)
+{
+    (void)hipSetDevice(s->device);
+    char *dbase = (char *)s->dmem;
+    hipStream_t bst = s->streams[g[0]->stream];
+    std::vector<hipEvent_t> evs;
+    auto cleanup = [&] {
+        for (hipEvent_t e : evs) (void)hipEventDestroy(e);  // released once complete
+    };
+    auto event = [&](hipEvent_t &e) {
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return false;
+        evs.push_back(e);
+        return true;
+    };
+    std::vector<fpcc_job> jobs;
+    std::vector<const float *> xs;
+    std::vector<void *> wss;
+    std::vector<size_t> wsb;
+    for (Request *r : g) {
+        if (stage_input(s, *r)) {
+            cleanup();
+            return -1;
+        }
+        if (r != g[0]) {
+            hipEvent_t e;
+            if (!event(e) || hipEventRecord(e, s->streams[r->stream]) != hipSuccess ||
+                hipStreamWaitEvent(bst, e, 0) != hipSuccess) {
+                cleanup();
+                return -1;
+            }
+        }
+        jobs.push_back(r->dev_job);
+        xs.push_back((const float *)(dbase + r->off_x));
+        wss.push_back(dbase + r->off_ws);
+        wsb.push_back(r->ws_bytes);
+    }
+    int rc = fpcc_station_stage_batch_device(jobs.data(), (int)jobs.size(), xs.data(), wss.data(), wsb.data(), bst);
+    hipEvent_t done;
+    if (!rc && (!event(done) || hipEventRecord(done, bst) != hipSuccess)) rc = -1;
+    for (Request *r : g) {
+        if (rc) break;
+        hipStream_t st = s->streams[r->stream];
+        if (r != g[0] && hipStreamWaitEvent(st, done, 0) != hipSuccess) rc = -1;
+        if (!rc) rc = fpcc_pair_stage_device(&r->dev_job, dbase + r->off_ws, r->ws_bytes, (float *)(dbase + r->off_y),
+                                             st);
+        if (!rc) rc = copy_out(s, *r);
+    }
+    cleanup();
+    return rc;
 }
 
 CacheKey cache_key(const Request &r, unsigned long long station)
@@ -507,18 +580,55 @@ void scheduler(Server *s)
             s->cv_sched.wait(lk);
             continue;
         }
+        auto start = [&](Request &q) {
+            int best = 0;
+            for (int i = 1; i < (int)s->streams.size(); ++i)
+                if (s->stream_load[i] < s->stream_load[best]) best = i;
+            q.stream = best;
+            s->stream_load[best]++;
+            s->inflight++;
+            s->inflight_peak = std::max(s->inflight_peak, s->inflight);
+            s->dev_peak = std::max(s->dev_peak, s->darena.used());
+            q.state = FPCC_REQ_RUNNING;
+            q.t_start = s->now();
+            if (s->first_start < 0) s->first_start = q.t_start;
+        };
         Request &r = *pick;
-        int best = 0;
-        for (int i = 1; i < (int)s->streams.size(); ++i)
-            if (s->stream_load[i] < s->stream_load[best]) best = i;
-        r.stream = best;
-        s->stream_load[best]++;
-        s->inflight++;
-        s->inflight_peak = std::max(s->inflight_peak, s->inflight);
-        s->dev_peak = std::max(s->dev_peak, s->darena.used());
-        r.state = FPCC_REQ_RUNNING;
-        r.t_start = s->now();
-        if (s->first_start < 0) s->first_start = r.t_start;
+        start(r);
+        // the requests next in line whose station stages can share launches with it
+        std::vector<std::shared_ptr<Request>> group{pick};
+        if (!r.use_cache && r.batchable && !injected_failure(s, r)) {
+            while (!s->queue.empty() && (int)group.size() < kMaxBatchJobs && s->inflight < s->cfg.max_inflight) {
+                Request &q = *s->queue.front();
+                if (!q.batchable || q.batch_key != r.batch_key || injected_failure(s, q) || !try_reserve(s, q))
+                    break;
+                start(q);
+                group.push_back(s->queue.front());
+                s->queue.pop_front();
+            }
+        }
+        if (group.size() > 1) {
+            s->n_batched += group.size();
+            s->n_batch_launch++;
+            lk.unlock();
+            std::vector<Request *> g;
+            for (auto &q : group) g.push_back(q.get());
+            const int rc = issue_group(s, g);
+            lk.lock();
+            for (auto &q : group)
+                if (rc) q->rc = rc;
+            lk.unlock();
+            for (auto &q : group) {
+                DoneMsg *m = new DoneMsg{s, q};
+                hipStream_t st = s->streams[q->stream];
+                if (hipLaunchHostFunc(st, stream_host_done, m) != hipSuccess) {
+                    (void)hipStreamSynchronize(st);
+                    stream_host_done(m);
+                }
+            }
+            lk.lock();
+            continue;
+        }
         lk.unlock();
         const int rc = issue(s, r);
         lk.lock();
@@ -682,10 +792,51 @@ extern "C" int fpcc_server_create(const fpcc_server_config *cfg, fpcc_server **o
     return 0;
 }
 
+namespace fpcc {
+namespace {
+int make_request(Server *s, const fpcc_request *req, std::shared_ptr<Request> &out);
+int enqueue(Server *s, std::shared_ptr<Request> &r, unsigned long long *id);
+}  // namespace
+}  // namespace fpcc
+
 extern "C" int fpcc_server_submit(fpcc_server *h, const fpcc_request *req, unsigned long long *id)
 {
     if (!h || !req || !req->x || !req->y) return -1;
     Server *s = &h->impl;
+    std::shared_ptr<fpcc::Request> r;
+    int rc = fpcc::make_request(s, req, r);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(s->mu);
+    return fpcc::enqueue(s, r, id);
+}
+
+extern "C" int fpcc_server_submit_many(fpcc_server *h, const fpcc_request *reqs, int n, unsigned long long *ids)
+{
+    if (!h || n < 0 || (n && (!reqs || !ids))) return -1;
+    Server *s = &h->impl;
+    std::vector<std::shared_ptr<fpcc::Request>> rs(n);
+    for (int k = 0; k < n; ++k) {
+        if (!reqs[k].x || !reqs[k].y) return -1;
+        int rc = fpcc::make_request(s, &reqs[k], rs[k]);
+        if (rc) return rc;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);  // all enter the queue together
+    for (int k = 0; k < n; ++k) {
+        int rc = fpcc::enqueue(s, rs[k], &ids[k]);
+        if (rc) {
+            for (int q = k; q < n; ++q) ids[q] = 0;
+            return rc;
+        }
+    }
+    return 0;
+}
+
+namespace fpcc {
+namespace {
+
+int make_request(Server *s, const fpcc_request *req, std::shared_ptr<Request> &out)
+{
+    (void)s;
     const fpcc_job &j = req->job;
     const size_t ylen = fpcc_output_len(&j);
     if (!ylen) {
@@ -694,7 +845,7 @@ extern "C" int fpcc_server_submit(fpcc_server *h, const fpcc_request *req, unsig
         if (N >= 2 && ((unsigned)abs(j.lag1) >= N || (unsigned)abs(j.lag2) >= N)) return -3;
         return -1;
     }
-    auto r = std::make_shared<fpcc::Request>();
+    auto r = std::make_shared<Request>();
     r->job = j;
     r->dev_job = j;
     r->dev_job.station_stride = (unsigned long long)j.n_windows * j.win_len;
@@ -707,6 +858,7 @@ extern "C" int fpcc_server_submit(fpcc_server *h, const fpcc_request *req, unsig
     r->x_pinned = fpcc::is_pinned(req->x) && fpcc::is_pinned(req->x + last);
     r->y_pinned = fpcc::is_pinned(req->y) && fpcc::is_pinned(req->y + ylen - 1);
     r->pair_corr = (unsigned long long)j.n_stations * (j.n_stations - 1) / 2 * j.n_windows;
+    r->batchable = !req->station_keys && fpcc::job_batch_key(&r->dev_job, &r->batch_key);
     if (req->station_keys) {
         r->keys.assign(req->station_keys, req->station_keys + j.n_stations);
         r->nreg = fpcc::job_station_regions(&r->dev_job, r->off, r->per);
@@ -717,11 +869,14 @@ extern "C" int fpcc_server_submit(fpcc_server *h, const fpcc_request *req, unsig
         }
         r->entry_bytes = base;
     }
-    const size_t dfoot = fpcc::Arena::round(r->x_bytes) + fpcc::Arena::round(r->y_bytes) +
-                         fpcc::Arena::round(r->ws_bytes);
-    const size_t hfoot = (r->x_pinned ? 0 : fpcc::Arena::round(r->x_bytes)) +
-                         (r->y_pinned ? 0 : fpcc::Arena::round(r->y_bytes));
-    std::lock_guard<std::mutex> lk(s->mu);
+    out = r;
+    return 0;
+}
+
+int enqueue(Server *s, std::shared_ptr<Request> &r, unsigned long long *id)
+{
+    const size_t dfoot = Arena::round(r->x_bytes) + Arena::round(r->y_bytes) + Arena::round(r->ws_bytes);
+    const size_t hfoot = (r->x_pinned ? 0 : Arena::round(r->x_bytes)) + (r->y_pinned ? 0 : Arena::round(r->y_bytes));
     if (s->stop) return -1;
     if (dfoot > s->darena.size() || hfoot > s->harena.size()) {
         s->n_reject++;
@@ -736,6 +891,9 @@ extern "C" int fpcc_server_submit(fpcc_server *h, const fpcc_request *req, unsig
     s->cv_sched.notify_one();
     return 0;
 }
+
+}  // namespace
+}  // namespace fpcc
 
 extern "C" int fpcc_server_poll(fpcc_server *h, unsigned long long id, fpcc_request_status *st)
 {
@@ -818,6 +976,8 @@ extern "C" int fpcc_server_stats(fpcc_server *h, fpcc_serving_stats *out)
     out->cache_hits = s->n_hit;
     out->cache_misses = s->n_miss;
     out->cache_used = s->carena.used();
+    out->batched_requests = s->n_batched;
+    out->batch_launches = s->n_batch_launch;
     out->queued = s->queue.size();
     out->inflight = s->inflight;
     out->inflight_peak = s->inflight_peak;

@@ -156,6 +156,13 @@ int fpcc_allpairs(const fpcc_job *job, const float *h_x, float *h_y);
  * jobs with the same kind, stations, windows, N and lags. */
 int fpcc_station_stage_device(const fpcc_job *job, const float *d_x, void *d_ws, size_t ws_bytes, void *stream);
 int fpcc_pair_stage_device(const fpcc_job *job, void *d_ws, size_t ws_bytes, float *d_y, void *stream);
+/* Station stages of njobs jobs on one stream, each into its own workspace
+ * (d_x[j], d_ws[j], ws_bytes[j]).  Jobs of uneven station count, window count,
+ * window length and stride share kernel launches when their transforms agree
+ * (GNCC and 1-bit GNCC of equal Nz, up to 16 jobs per launch); the others run
+ * one by one.  The results equal fpcc_station_stage_device per job. */
+int fpcc_station_stage_batch_device(const fpcc_job *jobs, int njobs, const float *const *d_x, void *const *d_ws,
+                                    const size_t *ws_bytes, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Request server (no counterpart in the reference, whose driver correlates one
@@ -184,6 +191,11 @@ int fpcc_pair_stage_device(const fpcc_job *job, void *d_ws, size_t ws_bytes, flo
  * rows are kept in a separate cache arena of cache_bytes, and every station's
  * rows are copied into the request's workspace before its pair stage.  Entries
  * in use by running requests stay; others are evicted least recently used.
+ *
+ * Uneven jobs in one launch: when requests whose station stages can share
+ * kernels (fpcc_station_stage_batch_device: GNCC or 1-bit of one zero-pad length,
+ * without station keys) are next in line together, they are admitted as a group
+ * and their station stages run as one set of launches.
  *
  * Return codes: 0 ok; -1 invalid argument or GPU failure; -3 |lag| >= N;
  * -4 the request's footprint exceeds the server's arenas (never admissible);
@@ -222,6 +234,7 @@ typedef struct {
     unsigned long long submitted, completed, failed, cancelled, rejected;
     unsigned long long cache_hits, cache_misses; /* stations found / computed for keyed requests */
     size_t cache_used;                           /* bytes of resident station rows              */
+    unsigned long long batched_requests, batch_launches; /* requests whose station stages shared a launch; groups */
     unsigned long long queued, inflight, inflight_peak;
     double p50_ms, p99_ms, mean_ms, max_ms; /* latency of completed requests          */
     double pair_correlations;              /* sum of pairs x windows delivered        */
@@ -238,6 +251,9 @@ void fpcc_host_free(void *p);
 
 int fpcc_server_create(const fpcc_server_config *cfg, fpcc_server **server);
 int fpcc_server_submit(fpcc_server *server, const fpcc_request *req, unsigned long long *id);
+/* n requests entering the queue together (ids[k] for reqs[k]; on an error the
+ * requests from the failing one on are not queued and their ids are 0). */
+int fpcc_server_submit_many(fpcc_server *server, const fpcc_request *reqs, int n, unsigned long long *ids);
 int fpcc_server_poll(fpcc_server *server, unsigned long long id, fpcc_request_status *status);
 /* timeout_ms < 0 waits without limit */
 int fpcc_server_wait(fpcc_server *server, unsigned long long id, int timeout_ms, fpcc_request_status *status);
