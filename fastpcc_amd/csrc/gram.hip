@@ -342,37 +342,69 @@ k_pair_c2r(const cf32 *__restrict__ G, long long ldg, int Nz, FftPlan planM, con
 }
 
 // Specialisation for Nz = 8192 (M = 4096 = 16^3): compile-time in-place FFT, 35 KB
-// of LDS, four workgroups per CU.
+// of LDS, four persistent workgroups per CU, each walking pairs p, p + grid, ...
+// and loading the next pair's spectrum into registers while the current one is
+// folded, transformed and written.  The fold runs in place: Z[k] and Z[M-k] come
+// from the same two values G[k], G[M-k].
+__device__ __forceinline__ cf32 c2r_fold(cf32 a, cf32 c, cf32 w)
+{
+    const cf32 P = cf32{a.x + c.x, a.y - c.y};
+    const cf32 Q = cf32{a.x - c.x, a.y + c.y};
+    const cf32 wq = cmul(w, Q);
+    return cf32{P.x - wq.y, P.y + wq.x};
+}
+
 __global__ void __launch_bounds__(256)
-k_pair_c2r_4096(const cf32 *__restrict__ G, long long ldg, const cf32 *__restrict__ twM,
+k_pair_c2r_4096(const cf32 *__restrict__ G, long long ldg, int P, const cf32 *__restrict__ twM,
                 const cf32 *__restrict__ twc, int lag0, int L, float scale, int count_norm, int N,
                 float *__restrict__ y)
 {
     constexpr int M = 4096, Nz = 8192;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cf32 *lds = reinterpret_cast<cf32 *>(smem_raw);
-    const long long p = blockIdx.x;
-    const cf32 *g = G + p * ldg;
-    for (int k = threadIdx.x; k < M; k += 256) {
-        cf32 a = g[k], c = g[M - k];
-        if (k == 0) {
-            a.y = 0.f;
-            c.y = 0.f;
+    __shared__ cf32 gM;  // G[M] of the pair in LDS
+    const int t = threadIdx.x;
+    long long p = blockIdx.x;
+    if (p >= P) return;
+    cf32 pre[M / 256];
+    cf32 preM = cf32{0.f, 0.f};
+    auto fetch = [&](long long q) {
+        const cf32 *g = G + q * ldg;
+#pragma unroll
+        for (int i = 0; i < M / 256; ++i) pre[i] = g[t + 256 * i];
+        if (t == 0) preM = g[M];
+    };
+    fetch(p);
+    for (;;) {
+#pragma unroll
+        for (int i = 0; i < M / 256; ++i) lds[ct_pad(t + 256 * i)] = pre[i];
+        if (t == 0) gM = preM;
+        const long long pn = p + gridDim.x;
+        if (pn < P) fetch(pn);  // in flight during this pair's work
+        __syncthreads();
+        for (int k = t; k <= M / 2; k += 256) {
+            cf32 a = lds[ct_pad(k)];
+            cf32 c = k == 0 ? gM : lds[ct_pad(M - k)];
+            if (k == 0) {  // DC and Nyquist of a real signal's spectrum are real
+                a.y = 0.f;
+                c.y = 0.f;
+            }
+            lds[ct_pad(k)] = c2r_fold(a, c, twc[k]);
+            if (k != 0 && k != M / 2) lds[ct_pad(M - k)] = c2r_fold(c, a, twc[M - k]);
         }
-        cf32 P = cf32{a.x + c.x, a.y - c.y};
-        cf32 Q = cf32{a.x - c.x, a.y + c.y};
-        cf32 wq = cmul(twc[k], Q);
-        lds[ct_pad(k)] = cf32{P.x - wq.y, P.y + wq.x};
-    }
-    __syncthreads();
-    ct_fft<float, M, 256, +1, 16, 16, 16>(lds, twM);
-    float *dst = y + p * L;
-    for (int l = threadIdx.x; l < L; l += 256) {
-        const int n = (lag0 + l) & (Nz - 1);
-        cf32 v = lds[ct_pad(n >> 1)];
-        float r = scale * ((n & 1) ? v.y : v.x);
-        if (count_norm) r = (float)((double)r / (double)(N - abs(lag0 + l)));
-        dst[l] = r;
+        __syncthreads();
+        ct_fft<float, M, 256, +1, 16, 16, 16>(lds, twM);
+        float *dst = y + p * L;
+        for (int l = t; l < L; l += 256) {
+            const int n = (lag0 + l) & (Nz - 1);
+            cf32 v = lds[ct_pad(n >> 1)];
+            float r = scale * ((n & 1) ? v.y : v.x);
+            if (count_norm) r = (float)((double)r / (double)(N - abs(lag0 + l)));
+            dst[l] = r;
+        }
+        if (pn >= P) break;
+        __syncthreads();  // every lane has read this pair before the next one lands
+        p = pn;
     }
 }
 
@@ -474,8 +506,18 @@ int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int 
     if (M == 4096 && pM.plan.npass == 3 && pM.plan.radix[0] == 16 && pM.plan.radix[1] == 16 &&
         pM.plan.radix[2] == 16) {
         size_t lds = sizeof(cf32) * ct_lds_elems<4096>();
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+            return -1;
+        // one persistent workgroup per resident slot (registers allow two or more per CU)
+        static int per_cu = 0;
+        if (!per_cu && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pair_c2r_4096, 256,
+                                                                     lds) != hipSuccess || per_cu < 1))
+            per_cu = 1;
+        const int nwg = P < per_cu * ncu ? P : per_cu * ncu;
         ProfScope prof("pair_c2r", st);
-        hipLaunchKernelGGL(k_pair_c2r_4096, dim3(P), dim3(256), lds, st, d_G, gram_ld(Nz / 2 + 1),
+        hipLaunchKernelGGL(k_pair_c2r_4096, dim3(nwg), dim3(256), lds, st, d_G, gram_ld(Nz / 2 + 1), P,
                            (const cf32 *)pM.tw, (const cf32 *)twc, lag0, L, scale, count_norm, N, d_y);
         FPCC_CHECK(hipGetLastError());
         return 0;
