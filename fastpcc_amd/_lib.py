@@ -143,6 +143,12 @@ def lib():
     L.fpcc_server_forget.argtypes = [vp, ull]
     L.fpcc_server_stats.argtypes = [vp, ctypes.POINTER(ServingStats)]
     L.fpcc_server_destroy.argtypes = [vp]
+    L.fpcc_graph_create.argtypes = [jp, vp, vp, vp, ctypes.c_size_t, vp, ctypes.POINTER(vp)]
+    L.fpcc_graph_create.restype = i
+    L.fpcc_graph_launch.argtypes = [vp, vp]
+    L.fpcc_graph_launch.restype = i
+    L.fpcc_graph_destroy.argtypes = [vp]
+    L.fpcc_graph_destroy.restype = i
     L.fpcc_host_alloc.argtypes = [ctypes.c_size_t]
     L.fpcc_host_alloc.restype = vp
     L.fpcc_host_free.argtypes = [vp]

@@ -248,6 +248,36 @@ class DeviceRunner:
             raise FastPCCError(r, "fpcc_allpairs_device")
         return self.y.view(output_shape(self.job))
 
+    def capture(self, x_dev, stream):
+        """Captures the job on `stream` (a non-default torch stream) as a HIP graph
+        reading x_dev; replay() then recomputes y from x_dev's current contents."""
+        self.release_graph()
+        g = ctypes.c_void_p()
+        r = _lib.lib().fpcc_graph_create(ctypes.byref(self.job), ctypes.c_void_p(x_dev.data_ptr()),
+                                         ctypes.c_void_p(self.y.data_ptr()), ctypes.c_void_p(self.ws.data_ptr()),
+                                         self.ws_bytes, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(g))
+        if r != 0:
+            raise FastPCCError(r, "fpcc_graph_create")
+        self._graph, self._gx = g, x_dev
+        return self
+
+    def replay(self, stream):
+        r = _lib.lib().fpcc_graph_launch(self._graph, ctypes.c_void_p(stream.cuda_stream))
+        if r != 0:
+            raise FastPCCError(r, "fpcc_graph_launch")
+        return self.y.view(output_shape(self.job))
+
+    def release_graph(self):
+        if getattr(self, "_graph", None):
+            _lib.lib().fpcc_graph_destroy(self._graph)
+            self._graph = None
+
+    def __del__(self):
+        try:
+            self.release_graph()
+        except Exception:
+            pass
+
 
 _ERR[-4] = "request exceeds the server's memory budget"
 

@@ -165,6 +165,19 @@ int fpcc_station_stage_batch_device(const fpcc_job *jobs, int njobs, const float
                                     const size_t *ws_bytes, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Graph replay (no counterpart in the reference): one job's station and pair
+ * stages captured once as a HIP graph on `stream` (not the default stream) and
+ * replayed with one launch.  fpcc_graph_create runs the job eagerly first (so
+ * every plan and table exists), then captures it; each fpcc_graph_launch
+ * recomputes d_y from the current contents of d_x, with the same buffers.
+ * ------------------------------------------------------------------------- */
+typedef struct fpcc_graph fpcc_graph;
+int fpcc_graph_create(const fpcc_job *job, const float *d_x, float *d_y, void *d_ws, size_t ws_bytes, void *stream,
+                      fpcc_graph **graph);
+int fpcc_graph_launch(fpcc_graph *graph, void *stream);
+int fpcc_graph_destroy(fpcc_graph *graph);
+
+/* ---------------------------------------------------------------------------
  * Request server (no counterpart in the reference, whose driver correlates one
  * pair per process run, src/PCC_fullpair_1b.c:467-499): continuous batching of
  * all-pairs jobs on the current HIP device.

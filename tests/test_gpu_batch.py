@@ -13,32 +13,6 @@ from fastpcc_amd import synth
 pytestmark = pytest.mark.gpu
 
 
-class _Hip:
-    def __init__(self):
-        self.h = ctypes.CDLL("libamdhip64.so.7")
-        self.h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
-        self.h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-        self.h.hipFree.argtypes = [ctypes.c_void_p]
-        self.ptrs = []
-
-    def alloc(self, n):
-        p = ctypes.c_void_p()
-        assert self.h.hipMalloc(ctypes.byref(p), max(n, 1)) == 0
-        self.ptrs.append(p.value)
-        return p.value
-
-    def h2d(self, d, a):
-        assert self.h.hipMemcpy(d, a.ctypes.data, a.nbytes, 1) == 0
-
-    def d2h(self, a, d):
-        assert self.h.hipMemcpy(a.ctypes.data, d, a.nbytes, 2) == 0
-
-    def free(self):
-        for p in self.ptrs:
-            self.h.hipFree(p)
-        self.ptrs = []
-
-
 # (kind, S, W, N, lag, stack): equal Nz within each group
 GROUPS = [
     [("gncc", 5, 1, 3000, 800, "none"), ("gncc", 9, 2, 2500, 900, "plain"), ("gncc", 3, 1, 3900, 300, "none")],
@@ -48,10 +22,9 @@ GROUPS = [
 
 
 @pytest.mark.parametrize("group", range(len(GROUPS)))
-def test_station_stage_batch_exact(fp, group):
+def test_station_stage_batch_exact(fp, hip, group):
     L = fp.api._lib.lib()
-    hip = _Hip()
-    try:
+    if True:
         specs = GROUPS[group]
         jobs = (fp.api._lib.FpccJob * len(specs))()
         xs, dx, dws, wsb, refs = [], [], [], [], []
@@ -77,8 +50,6 @@ def test_station_stage_batch_exact(fp, group):
             hip.d2h(y, dy)
             print(f"{specs[k]}: max|batch - alone| = {np.abs(y - refs[k]).max():.3e}")
             assert np.array_equal(y, refs[k])
-    finally:
-        hip.free()
 
 
 def test_server_groups_uneven_jobs(fp):
