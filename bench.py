@@ -228,6 +228,20 @@ def main():
                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": b}
 
     variants = {}
+    # the same step replayed as one HIP graph (captured on its own stream)
+    gstream = torch.cuda.Stream(dev)
+    runner.capture(x_dev, gstream)
+    for _ in range(2):
+        runner.replay(gstream)
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    for _ in range(args.steps):
+        runner.replay(gstream)
+    torch.cuda.synchronize()
+    gms = (time.perf_counter() - ta) * 1e3 / args.steps
+    variants["graph_" + args.stack] = {"ms_per_step": gms, "value": P * W / (gms / 1e3), "steps": args.steps,
+                                       "note": "the headline step replayed as one captured HIP graph"}
+    runner.release_graph()
     for vs in [v for v in args.variants.split(",") if v and v != args.stack]:
         vjob = fastpcc_amd.make_job("pcc2", S, W, N, -lag, lag, stack=vs)
         vr = fastpcc_amd.DeviceRunner(vjob, dev)
