@@ -149,6 +149,11 @@ def lib():
     L.fpcc_graph_launch.restype = i
     L.fpcc_graph_destroy.argtypes = [vp]
     L.fpcc_graph_destroy.restype = i
+    ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+    L.fpcc_autotune_fft.argtypes = [i, i, i, ip, dp, ip, dp]
+    L.fpcc_autotune_fft.restype = i
+    L.fpcc_fft_split.argtypes = [i]
+    L.fpcc_fft_split.restype = i
     L.fpcc_host_alloc.argtypes = [ctypes.c_size_t]
     L.fpcc_host_alloc.restype = vp
     L.fpcc_host_free.argtypes = [vp]

@@ -2,6 +2,10 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <dlfcn.h>
+#include <cstdlib>
+#include <string>
+#include "../../include/fastpcc.h"
 #include "bigfft.hpp"
 #include "kernels.hpp"
 #include "profile.hpp"
@@ -9,30 +13,80 @@
 namespace fpcc {
 
 // Split n = n1*n2 with n1 <= 512 (column pass in LDS), n2 <= 2048 (row pass),
-// both 7-smooth, n1 as close to sqrt(n) as allowed and n2 >= 16 when possible.
-static BigPlan make_big_plan(int n)
+// both 7-smooth.  The split comes from the autotuning table when it lists n,
+// else n1 is the admissible divisor closest to sqrt(n), with n2 >= 16 when
+// possible.
+static bool valid_split(int n, int d)
 {
-    BigPlan b;
-    if (n < 64) return b;
+    if (d < 4 || d > 512 || d > n || n % d) return false;
+    const int e = n / d;
+    FftPlan t1, t2;
+    return e <= 2048 && factor_plan(d, t1) && factor_plan(e, t2);
+}
+
+static int heuristic_split(int n)
+{
     int best = 0;
     double bestscore = 1e300;
     for (int d = 4; d <= 512 && d <= n; ++d) {
-        if (n % d) continue;
-        const int e = n / d;
-        if (e > 2048) continue;
-        FftPlan t1, t2;
-        if (!factor_plan(d, t1) || !factor_plan(e, t2)) continue;
+        if (!valid_split(n, d)) continue;
         double score = std::fabs(std::log((double)d) - 0.5 * std::log((double)n));
-        if (e < kBigCols) score += 100;  // partial column tiles waste lanes
+        if (n / d < kBigCols) score += 100;  // partial column tiles waste lanes
         if (score < bestscore) {
             bestscore = score;
             best = d;
         }
     }
-    if (!best) return b;
+    return best;
+}
+
+// Tuning table: lines "n n1 [ms]" ('#' starts a comment), read once from
+// $FASTPCC_FFT_TUNING or tuning/fft_splits_<arch>.txt beside the library.
+static std::mutex tune_mu;
+static std::map<int, int> &tune_table()
+{
+    static std::map<int, int> t;
+    static bool loaded = false;
+    if (loaded) return t;
+    loaded = true;
+    std::string path;
+    if (const char *e = getenv("FASTPCC_FFT_TUNING")) {
+        path = e;
+    } else {
+        Dl_info info;
+        if (dladdr((const void *)&heuristic_split, &info) && info.dli_fname) {
+            std::string so = info.dli_fname;
+            const size_t slash = so.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : so.substr(0, slash)) + "/tuning/fft_splits_gfx950.txt";
+        }
+    }
+    if (FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "r")) {
+        char line[256];
+        while (fgets(line, sizeof line, f)) {
+            if (line[0] == '#') continue;
+            int n = 0, d = 0;
+            if (sscanf(line, "%d %d", &n, &d) == 2 && valid_split(n, d)) t[n] = d;
+        }
+        fclose(f);
+    }
+    return t;
+}
+
+static int chosen_split(int n)
+{
+    std::lock_guard<std::mutex> lk(tune_mu);
+    auto &t = tune_table();
+    auto it = t.find(n);
+    return it != t.end() ? it->second : heuristic_split(n);
+}
+
+static BigPlan build_big_plan(int n, int n1)
+{
+    BigPlan b;
+    if (n < 64 || !n1) return b;
     b.n = n;
-    b.n1 = best;
-    b.n2 = n / best;
+    b.n1 = n1;
+    b.n2 = n / n1;
     const DevPlan &a = get_plan(b.n1, false), &c = get_plan(b.n2, false);
     if (!a.ok || !c.ok) return b;
     b.p1 = a.plan;
@@ -43,6 +97,8 @@ static BigPlan make_big_plan(int n)
     b.ok = b.twN != nullptr;
     return b;
 }
+
+static BigPlan make_big_plan(int n) { return n < 64 ? BigPlan{} : build_big_plan(n, chosen_split(n)); }
 
 const BigPlan &big_plan(int n)
 {
@@ -517,6 +573,69 @@ int run_rows_batch(const BigPlan &bp, const RowArgs &a, const RealBatch &jb, hip
     return 0;
 }
 
+int tune_split(int n, int batch, int reps, int *best_n1, double *best_ms, int *heur_n1, double *heur_ms)
+{
+    if (n < 64 || batch < 1 || reps < 1) return -1;
+    std::vector<int> cand;
+    for (int d = 4; d <= 512 && d <= n; ++d)
+        if (valid_split(n, d)) cand.push_back(d);
+    if (cand.empty()) return -1;
+    void *buf = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = -1;
+    double best = 1e300, heur = 0;
+    int bestd = 0;
+    const int hd = heuristic_split(n);
+    if (hipMalloc(&buf, sizeof(cf32) * (size_t)n * batch) != hipSuccess) return -1;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+        hipEventCreate(&e1) == hipSuccess && hipMemsetAsync(buf, 0, sizeof(cf32) * (size_t)n * batch, st) == hipSuccess) {
+        rc = 0;
+        for (int d : cand) {
+            BigPlan bp = build_big_plan(n, d);
+            if (!bp.ok) continue;
+            ColArgs ca;
+            ca.op = COL_FWD_PLAIN;
+            ca.src = ca.dst = (cf32 *)buf;
+            ca.bstride = n;
+            RowArgs ra;
+            ra.op = ROW_FWD_PLAIN;
+            ra.src = ra.dst = (cf32 *)buf;
+            ra.bstride = n;
+            if (run_cols(bp, ca, batch, st, "tune_cols") || run_rows(bp, ra, batch, st, "tune_rows")) continue;
+            (void)hipEventRecord(e0, st);
+            for (int r = 0; r < reps; ++r)
+                if (run_cols(bp, ca, batch, st, "tune_cols") || run_rows(bp, ra, batch, st, "tune_rows")) break;
+            (void)hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                rc = -1;
+                break;
+            }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (d == hd) heur = ms / reps;
+            if (ms / reps < best) {
+                best = ms / reps;
+                bestd = d;
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(buf);
+    if (rc || !bestd) return -1;
+    {
+        std::lock_guard<std::mutex> lk(tune_mu);
+        tune_table()[n] = bestd;  // used by plans built from now on
+    }
+    if (best_n1) *best_n1 = bestd;
+    if (best_ms) *best_ms = best;
+    if (heur_n1) *heur_n1 = hd;
+    if (heur_ms) *heur_ms = heur;
+    return 0;
+}
+
 int fill_zero(float *p, long long n, hipStream_t st)
 {
     if (n <= 0) return 0;
@@ -526,3 +645,15 @@ int fill_zero(float *p, long long n, hipStream_t st)
 }
 
 }  // namespace fpcc
+
+extern "C" int fpcc_autotune_fft(int n, int batch, int reps, int *best_n1, double *best_ms, int *default_n1,
+                                 double *default_ms)
+{
+    return fpcc::tune_split(n, batch, reps, best_n1, best_ms, default_n1, default_ms);
+}
+
+extern "C" int fpcc_fft_split(int n)
+{
+    const fpcc::BigPlan &bp = fpcc::big_plan(n);
+    return bp.ok ? bp.n1 : 0;
+}

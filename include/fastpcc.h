@@ -277,6 +277,21 @@ int fpcc_server_stats(fpcc_server *server, fpcc_serving_stats *stats);
 int fpcc_server_destroy(fpcc_server *server);
 
 /* ---------------------------------------------------------------------------
+ * Autotuning (no counterpart in the reference).  Transforms longer than one
+ * workgroup's LDS run as two passes, n = n1 * n2 (n1 <= 512 column points, n2 <=
+ * 2048 row points, both 7-smooth).  The split comes from the table
+ * tuning/fft_splits_gfx950.txt beside the library (or $FASTPCC_FFT_TUNING), else
+ * from a heuristic.  fpcc_autotune_fft times every admissible split on `batch`
+ * forward transforms of n points (reps repetitions, HIP events), returns the
+ * fastest (and the heuristic's split and time, for comparison) and uses it for
+ * plans of n first built later in the process; fpcc_fft_split returns the split
+ * in use (0 if n has none).  Returns 0, or -1 if n has no admissible split.
+ * ------------------------------------------------------------------------- */
+int fpcc_autotune_fft(int n, int batch, int reps, int *best_n1, double *best_ms, int *default_n1,
+                      double *default_ms);
+int fpcc_fft_split(int n);
+
+/* ---------------------------------------------------------------------------
  * Kernel timing for benchmarks (no counterpart in the reference).  While
  * enabled, each kernel launch records a HIP event pair on its stream;
  * fpcc_profile_report waits for them and returns, per kernel name (written
