@@ -54,6 +54,12 @@ def kernel_flops(name, cfg, Nz):
     return P * (Nz // 2 + 1) * 2 * W * 8
 
 
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by
+# the same counters' readings for a 1 GiB copy (profiles/
+# pcc2_allpairs_256st_gram16_hbm_traffic.md), keyed by (config, stack, kernel).
+MEASURED_TRAFFIC = {("c3", "gram16", "gram_split16"): 1648.0e6}
+
+
 def kernel_bytes(name, cfg, Nz):
     """Algorithmic HBM bytes of one launch of `name`."""
     S, W, N, lag = cfg["S"], cfg["W"], cfg["N"], cfg["lag"]
@@ -63,8 +69,14 @@ def kernel_bytes(name, cfg, Nz):
         return S * W * N * 4 + spectra            # read records, write spectra
     if name.startswith("pair_pcc2"):
         return spectra + P * L * 4                # read spectra once, write stacked lags
+    nk = Nz // 2 + 1
     if name.startswith("gram"):
-        return spectra + P * (Nz // 2 + 1) * 8    # read spectra, write folded cross spectra
+        # read the operands (2W rows a bin: the W spectra and the W conjugated mirror
+        # bins, 8 bytes each: FP32 re/im, or FP16 hi+lo re/im), write the folded
+        # cross spectra
+        return S * 2 * W * nk * 8 + P * nk * 8
+    if name.startswith("spectra_to_u"):
+        return spectra + S * 2 * W * nk * 8       # read spectra, write the Gram operands
     if name.startswith("pair_c2r"):
         return P * (Nz // 2 + 1) * 8 + P * L * 4  # read folded cross spectra, write lags
     return None
@@ -140,6 +152,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=1000,
                     help="requests of the serving run (0: skip); 3000 is the full stream")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="mean arrivals per second")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay of the headline step")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="before the warm-up, one fill and one copy of this many MiB (the known-size kernels "
+                         "that scale FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path shards over station-pair tiles; "
@@ -165,6 +181,13 @@ def main():
     lib.fpcc_profile_report.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
 
+    if args.calib_copy_mb > 0:
+        ca = torch.empty(args.calib_copy_mb << 20, dtype=torch.uint8, device=dev)
+        cb = torch.empty_like(ca)
+        ca.fill_(1)
+        cb.copy_(ca)
+        torch.cuda.synchronize()
+        del ca, cb
     for _ in range(args.warmup):
         runner.run(x_dev, stream)
     torch.cuda.synchronize()
@@ -220,28 +243,31 @@ def main():
             ach = f / (kernels[dom]["avg_ms"] * 1e-3) / 1e12
             peak = MFMA_PEAK_TFLOPS[dom]
             roof = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
-                    "frac": round(ach / peak, 4), "traffic": None, "algorithmic_flops": f}
+                    "frac": round(ach / peak, 4), "traffic": MEASURED_TRAFFIC.get((args.config, args.stack, dom)),
+                    "algorithmic_flops": f}
         else:
             b = kernel_bytes(dom, cfg, Nz)
             ach = b / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
             roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": b}
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get((args.config, args.stack, dom)),
+                    "algorithmic_bytes": b}
 
     variants = {}
-    # the same step replayed as one HIP graph (captured on its own stream)
-    gstream = torch.cuda.Stream(dev)
-    runner.capture(x_dev, gstream)
-    for _ in range(2):
-        runner.replay(gstream)
-    torch.cuda.synchronize()
-    ta = time.perf_counter()
-    for _ in range(args.steps):
-        runner.replay(gstream)
-    torch.cuda.synchronize()
-    gms = (time.perf_counter() - ta) * 1e3 / args.steps
-    variants["graph_" + args.stack] = {"ms_per_step": gms, "value": P * W / (gms / 1e3), "steps": args.steps,
-                                       "note": "the headline step replayed as one captured HIP graph"}
-    runner.release_graph()
+    if not args.no_graph:
+        # the same step replayed as one HIP graph (captured on its own stream)
+        gstream = torch.cuda.Stream(dev)
+        runner.capture(x_dev, gstream)
+        for _ in range(2):
+            runner.replay(gstream)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        for _ in range(args.steps):
+            runner.replay(gstream)
+        torch.cuda.synchronize()
+        gms = (time.perf_counter() - ta) * 1e3 / args.steps
+        variants["graph_" + args.stack] = {"ms_per_step": gms, "value": P * W / (gms / 1e3), "steps": args.steps,
+                                           "note": "the headline step replayed as one captured HIP graph"}
+        runner.release_graph()
     for vs in [v for v in args.variants.split(",") if v and v != args.stack]:
         vjob = fastpcc_amd.make_job("pcc2", S, W, N, -lag, lag, stack=vs)
         vr = fastpcc_amd.DeviceRunner(vjob, dev)
