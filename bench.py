@@ -153,6 +153,7 @@ def main():
                     help="requests of the serving run (0: skip); 3000 is the full stream")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="mean arrivals per second")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay of the headline step")
+    ap.add_argument("--no-host", action="store_true", help="skip the PCIe-inclusive host-buffer timing")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="before the warm-up, one fill and one copy of this many MiB (the known-size kernels "
                          "that scale FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -287,6 +288,18 @@ def main():
             ent["max_abs_err_vs_fp64"] = float(np.abs(yv - y64).max())
         variants[vs] = ent
         del vr
+    if not args.no_host:
+        # PCIe-inclusive: host input in, host output out, through fpcc_allpairs
+        # (device buffers allocated per call, pageable host arrays)
+        fastpcc_amd.allpairs(x, W, N, -lag, lag, kind="pcc2", stack=args.stack)
+        ta = time.perf_counter()
+        for _ in range(3):
+            fastpcc_amd.allpairs(x, W, N, -lag, lag, kind="pcc2", stack=args.stack)
+        hms = (time.perf_counter() - ta) * 1e3 / 3
+        variants["host_buffers_" + args.stack] = {
+            "ms_per_step": hms, "value": P * W / (hms / 1e3), "steps": 3,
+            "note": "PCIe-inclusive: fpcc_allpairs from pageable host input to host output "
+                    f"({x.nbytes / 1e6:.1f} MB in, {P * L * 4 / 1e6:.1f} MB out, device buffers allocated per call)"}
     speedup = None
     if "plain" in variants:
         speedup = round(value / variants["plain"]["value"], 2)
