@@ -87,6 +87,18 @@ int orc_wpcc_family_export(unsigned N, int Lag1, int Lag2, double pmin, double p
                            double *out_s0_K0_op1, double *scale, unsigned *Ls, int *center,
                            double *wbuf, unsigned wbuf_len);
 
+/* ---- per-station preprocessing (src/PCC_fullpair_1b.c, src/ReadManySacs.c, src/rotlib.c) ---- */
+float orc_qabsmedian(const float *x, int n);                                 /* :1179-1214 */
+float orc_qmedian(const float *x, int n);                                    /* :1216-1251 */
+float orc_clip_threshold(const float *x, unsigned N);                        /* :885       */
+void orc_clipping(float *x, unsigned N);                                     /* :881-891   */
+float orc_trace_rms(const float *x, unsigned N);                             /* :355-363   */
+unsigned orc_white_nz(unsigned N);                                           /* :1072      */
+void orc_white_filter(double *H, double *HS, unsigned Nh, unsigned Nz, unsigned N, unsigned n1, unsigned n2);
+int orc_avewhite(float **x0, unsigned N, unsigned Tr, const double freq[2], double dt); /* :1065-1177 */
+int orc_preprocess(float *x, unsigned S, unsigned W, unsigned N, unsigned long long stride, int clip, float nstd,
+                   double f1, double f2, double dt, int common, const float *sign, unsigned char *valid, float *rms);
+
 #ifdef __cplusplus
 }
 #endif

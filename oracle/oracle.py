@@ -49,6 +49,21 @@ def lib():
         L.orc_wpcc_family_export.argtypes = [u, i, i, d, d, u, i, d, ctypes.POINTER(ctypes.c_uint),
                                              dp, dp, ctypes.POINTER(ctypes.c_uint),
                                              ctypes.POINTER(ctypes.c_int), dp, u]
+        L.orc_qabsmedian.argtypes = [fp, i]
+        L.orc_qabsmedian.restype = ctypes.c_float
+        L.orc_qmedian.argtypes = [fp, i]
+        L.orc_qmedian.restype = ctypes.c_float
+        L.orc_clip_threshold.argtypes = [fp, u]
+        L.orc_clip_threshold.restype = ctypes.c_float
+        L.orc_clipping.argtypes = [fp, u]
+        L.orc_trace_rms.argtypes = [fp, u]
+        L.orc_trace_rms.restype = ctypes.c_float
+        L.orc_white_nz.argtypes = [u]
+        L.orc_white_nz.restype = u
+        L.orc_white_filter.argtypes = [dp, dp, u, u, u, u, u]
+        L.orc_avewhite.argtypes = [ctypes.POINTER(fp), u, u, dp, d]
+        L.orc_preprocess.argtypes = [fp, u, u, u, ctypes.c_ulonglong, i, ctypes.c_float, d, d, d, i, fp,
+                                     ctypes.POINTER(ctypes.c_ubyte), fp]
         _lib = L
     return _lib
 
@@ -161,3 +176,68 @@ def wpcc_family(N, lag1, lag2, pmin, pmax, V, wtype=-1, op1=0.0):
         off += int(Ls[s])
     return dict(S=S, J=meta[1], V=meta[2], Nz=meta[3], Ls0=meta[4], s0=fl[0], K0=fl[1], op1=fl[2],
                 scale=scale, Ls=Ls, center=center, wc=waves)
+
+
+# ---- per-station preprocessing (src/PCC_fullpair_1b.c:881-925, 1065-1251) ----
+
+def qabsmedian(x):
+    x = _f32(x)
+    return float(lib().orc_qabsmedian(_ptr(x, ctypes.c_float), x.size))
+
+
+def qmedian(x):
+    x = _f32(x)
+    return float(lib().orc_qmedian(_ptr(x, ctypes.c_float), x.size))
+
+
+def clip_threshold(x):
+    x = _f32(x)
+    return np.float32(lib().orc_clip_threshold(_ptr(x, ctypes.c_float), x.size))
+
+
+def clipping(x):
+    y = _f32(x).copy()
+    lib().orc_clipping(_ptr(y, ctypes.c_float), y.size)
+    return y
+
+
+def trace_rms(x):
+    x = _f32(x)
+    return np.float32(lib().orc_trace_rms(_ptr(x, ctypes.c_float), x.size))
+
+
+def white_filter(H, Nz, N, n1, n2):
+    """(H after the filter design, HS) for an averaged amplitude spectrum H[Nz/2+1]."""
+    H = np.ascontiguousarray(H, dtype=np.float64).copy()
+    HS = np.zeros_like(H)
+    lib().orc_white_filter(_ptr(H, ctypes.c_double), _ptr(HS, ctypes.c_double), H.size, Nz, N, n1, n2)
+    return H, HS
+
+
+def avewhite(x, band, dt):
+    """AveWhite over the rows of x [Tr][N] (a whitened copy)."""
+    y = _f32(x).copy()
+    rows = (ctypes.POINTER(ctypes.c_float) * y.shape[0])(
+        *[y[t].ctypes.data_as(ctypes.POINTER(ctypes.c_float)) for t in range(y.shape[0])])
+    b = np.asarray(band, dtype=np.float64)
+    r = lib().orc_avewhite(rows, y.shape[1], y.shape[0], _ptr(b, ctypes.c_double), dt)
+    if r:
+        raise ValueError(f"oracle avewhite returned {r}")
+    return y
+
+
+def preprocess(x, W, N, clip=False, std=0.0, white=None, dt=1.0, common=False, sign=None):
+    """The station preprocessing of x [S][>= W*N] (a copy): returns (x, valid [S][W],
+    rms [S][W])."""
+    y = _f32(x).copy()
+    S = y.shape[0]
+    valid = np.zeros((S, W), dtype=np.uint8)
+    rms = np.zeros((S, W), dtype=np.float32)
+    sg = None if sign is None else _f32(sign).reshape(S, W)
+    f1, f2 = white if white else (0.0, 0.0)
+    r = lib().orc_preprocess(_ptr(y, ctypes.c_float), S, W, N, y.shape[1], int(clip), std, f1, f2, dt, int(common),
+                             None if sg is None else _ptr(sg, ctypes.c_float), _ptr(valid, ctypes.c_ubyte),
+                             _ptr(rms, ctypes.c_float))
+    if r:
+        raise ValueError(f"oracle preprocess returned {r}")
+    return y, valid.astype(bool), rms
