@@ -154,6 +154,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=50.0, help="mean arrivals per second")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay of the headline step")
     ap.add_argument("--no-host", action="store_true", help="skip the PCIe-inclusive host-buffer timing")
+    ap.add_argument("--no-pre", action="store_true", help="skip the preprocessing timing")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="before the warm-up, one fill and one copy of this many MiB (the known-size kernels "
                          "that scale FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -288,6 +289,42 @@ def main():
             ent["max_abs_err_vs_fp64"] = float(np.abs(yv - y64).max())
         variants[vs] = ent
         del vr
+    if not args.no_pre:
+        # per-station preprocessing of the same network: clipping, outlier windows
+        # (rms > 3 x median), FP64 whitening over 0.02-0.2 Hz
+        pp = fastpcc_amd.make_preproc(clip=True, std=3.0, white=(0.02, 0.2), dt=1.0)
+        pjob = fastpcc_amd.make_job("pcc2", S, W, N, 0, 0, station_stride=W * N)
+        pws = lib.fpcc_preprocess_workspace_size(ctypes.byref(pjob), ctypes.byref(pp))
+        xp = torch.empty_like(x_dev)
+        pvalid = torch.empty(S * W, dtype=torch.uint8, device=dev)
+        prms = torch.empty(S * W, dtype=torch.float32, device=dev)
+        pwsb = torch.empty(pws, dtype=torch.uint8, device=dev)
+
+        def pre_once():
+            xp.copy_(x_dev)
+            r = lib.fpcc_preprocess_device(ctypes.byref(pjob), ctypes.byref(pp), ctypes.c_void_p(xp.data_ptr()), None,
+                                           ctypes.c_void_p(pvalid.data_ptr()), ctypes.c_void_p(prms.data_ptr()),
+                                           ctypes.c_void_p(pwsb.data_ptr()), pws, ctypes.c_void_p(stream.cuda_stream))
+            if r:
+                raise RuntimeError(f"fpcc_preprocess_device returned {r}")
+
+        pre_once()
+        torch.cuda.synchronize()
+        lib.fpcc_profile_reset()
+        lib.fpcc_profile_enable(1)
+        for _ in range(5):
+            pre_once()
+        torch.cuda.synchronize()
+        lib.fpcc_profile_enable(0)
+        nk = lib.fpcc_profile_report(names, 4096, tot, cnt, 16)
+        pk = {nm: round(tot[k] / max(cnt[k], 1), 4) for k, nm in enumerate(names.value.decode().split("\n")[:nk])}
+        pms = sum(pk.values())
+        variants["preprocess"] = {
+            "ms_per_step": pms, "kernels_avg_ms": pk, "steps": 5, "kept_windows": int(pvalid.sum().item()),
+            "x_GBps": round(5 * x.nbytes / (pms * 1e-3) / 1e9, 1),
+            "note": "fpcc_preprocess_device on the same network: clipping, outlier windows (rms > 3 x median), "
+                    "FP64 whitening 0.02-0.2 Hz; ms = sum of kernel averages; x_GBps counts 5 passes over x"}
+        del xp, pwsb
     if not args.no_host:
         # PCIe-inclusive: host input in, host output out, through fpcc_allpairs
         # (device buffers allocated per call, pageable host arrays)

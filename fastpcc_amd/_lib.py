@@ -37,6 +37,17 @@ class FpccJob(ctypes.Structure):
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
 
 
+class Preproc(ctypes.Structure):
+    """Mirror of fpcc_preproc."""
+    _fields_ = [
+        ("clip", ctypes.c_int),
+        ("std_max", ctypes.c_float),
+        ("band", ctypes.c_double * 2),
+        ("dt", ctypes.c_double),
+        ("common_windows", ctypes.c_int),
+    ]
+
+
 class ServerConfig(ctypes.Structure):
     """Mirror of fpcc_server_config."""
     _fields_ = [
@@ -149,6 +160,14 @@ def lib():
     L.fpcc_graph_launch.restype = i
     L.fpcc_graph_destroy.argtypes = [vp]
     L.fpcc_graph_destroy.restype = i
+    pp = ctypes.POINTER(Preproc)
+    L.fpcc_preprocess_workspace_size.argtypes = [jp, pp]
+    L.fpcc_preprocess_workspace_size.restype = ctypes.c_size_t
+    L.fpcc_preprocess_device.argtypes = [jp, pp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp]
+    L.fpcc_preprocess_device.restype = i
+    L.fpcc_preprocess.argtypes = [jp, pp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                  ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(ctypes.c_float)]
+    L.fpcc_preprocess.restype = i
     ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
     L.fpcc_autotune_fft.argtypes = [i, i, i, ip, dp, ip, dp]
     L.fpcc_autotune_fft.restype = i

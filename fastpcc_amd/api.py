@@ -216,6 +216,38 @@ def allpairs(x, n_windows, win_len, lag1, lag2, kind="pcc2", stack="plain", **kw
     return y
 
 
+def make_preproc(clip=False, std=0.0, white=None, dt=1.0, common=False):
+    """fpcc_preproc: clip (bool), std (outlier factor, 0 off), white ((f1, f2) Hz
+    or None), dt (s), common (keep windows kept at every station)."""
+    p = _lib.Preproc()
+    p.clip = int(bool(clip))
+    p.std_max = float(std)
+    if white:
+        p.band[0], p.band[1] = float(white[0]), float(white[1])
+    p.dt = float(dt)
+    p.common_windows = int(bool(common))
+    return p
+
+
+def preprocess(x, n_windows, win_len, clip=False, std=0.0, white=None, dt=1.0, common=False, sign=None):
+    """Per-station preprocessing of host data x [S][>= W*N] (fpcc_preprocess): returns
+    (preprocessed copy of x, valid [S][W] bool, rms [S][W])."""
+    y = np.array(x, dtype=np.float32, order="C", copy=True)
+    S = y.shape[0]
+    job = make_job("pcc2", S, n_windows, win_len, 0, 0, station_stride=y.shape[1])
+    pp = make_preproc(clip, std, white, dt, common)
+    valid = np.zeros((S, n_windows), dtype=np.uint8)
+    rms = np.zeros((S, n_windows), dtype=np.float32)
+    sg = None if sign is None else np.ascontiguousarray(sign, dtype=np.float32).reshape(S, n_windows)
+    fp = ctypes.POINTER(ctypes.c_float)
+    r = _lib.lib().fpcc_preprocess(ctypes.byref(job), ctypes.byref(pp), y.ctypes.data_as(fp),
+                                   None if sg is None else sg.ctypes.data_as(fp),
+                                   valid.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), rms.ctypes.data_as(fp))
+    if r != 0:
+        raise FastPCCError(r, "fpcc_preprocess")
+    return y, valid.astype(bool), rms
+
+
 class DeviceRunner:
     """Device-resident all-pairs runs on torch-allocated HBM (torch is plumbing:
     allocation and the stream handle).  The workspace is allocated once."""

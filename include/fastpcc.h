@@ -165,6 +165,46 @@ int fpcc_station_stage_batch_device(const fpcc_job *jobs, int njobs, const float
                                     const size_t *ws_bytes, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Per-station preprocessing (the steps src/PCC_fullpair_1b.c runs on each
+ * station's traces before correlating them; here a trace is one window of the
+ * job's station rows, x[s*station_stride + w*win_len + n]).  In place on d_x, in
+ * this order:
+ *   sign       optional [S][W] polarity factors (+1/-1), the per-trace result of
+ *              CorrectRevesedPolarity (src/rotlib.c:14-34); every later step is
+ *              odd-symmetric, so applying it first changes no bit;
+ *   zero       windows of zeros are dropped (RemoveZeroTraces, src/ReadManySacs.c:529-559);
+ *   rms        [S][W] sqrt(sum of x^2 in FP64 / N), before clipping (:355-363);
+ *   clip       |x| > 4*median|x|/0.6745 -> +-that threshold (clipping, :881-891);
+ *   std_max    > 0: windows with rms > std_max * (median rms of the station's
+ *              non-zero windows) are dropped (RemoveOutlierTraces, :893-925);
+ *   common_windows != 0: a window is kept only where every station kept it (the
+ *              paired trace lists of a two-station run, :399-401);
+ *   dropped windows are zeroed; valid[S][W] = 1 for kept windows;
+ *   band, dt   band[0] > 0 and band[1] > band[0] (Hz, dt in s): FP64 spectral
+ *              whitening of each station's kept windows by their averaged
+ *              amplitude spectrum (AveWhite, :1065-1177); windows of 17 to 8192
+ *              samples (-2 otherwise).
+ * n_stations, n_windows (at most 8192), win_len and station_stride of the job
+ * are used; the kind, lags and stack are not.  Returns 0, -1 (invalid) or -2 (unsupported).
+ * No counterpart in the reference's library API: these are static functions
+ * of its driver.
+ * ------------------------------------------------------------------------- */
+typedef struct fpcc_preproc {
+    int clip;            /* clipping on/off                                    */
+    float std_max;       /* outlier factor (0: off)                            */
+    double band[2];      /* whitening band in Hz (0, 0: off)                   */
+    double dt;           /* sampling interval in s (whitening)                 */
+    int common_windows;  /* keep only windows kept at every station            */
+} fpcc_preproc;
+size_t fpcc_preprocess_workspace_size(const fpcc_job *job, const fpcc_preproc *pp);
+int fpcc_preprocess_device(const fpcc_job *job, const fpcc_preproc *pp, float *d_x, const float *d_sign,
+                           unsigned char *d_valid, float *d_rms, void *d_ws, size_t ws_bytes, void *stream);
+/* Host buffers: x is [n_stations][station_stride] floats (updated in place), sign
+ * [S][W] or NULL, valid [S][W] out, rms [S][W] out or NULL. */
+int fpcc_preprocess(const fpcc_job *job, const fpcc_preproc *pp, float *h_x, const float *h_sign,
+                    unsigned char *h_valid, float *h_rms);
+
+/* ---------------------------------------------------------------------------
  * Graph replay (no counterpart in the reference): one job's station and pair
  * stages captured once as a HIP graph on `stream` (not the default stream) and
  * replayed with one launch.  fpcc_graph_create runs the job eagerly first (so
