@@ -1,7 +1,10 @@
-"""Builds libfastpcc.so (HIP, gfx950) in-tree with hipcc.
+"""Builds libfastpcc.so (HIP, gfx950) in-tree with hipcc, and the C drivers.
 
 Every HIP source under csrc/ is compiled for gfx950 with packed-FP32 instruction
-selection disabled, then linked into fastpcc_amd/libfastpcc.so.
+selection disabled, every C++ host source under csrc/ with g++, and all are linked
+into fastpcc_amd/libfastpcc.so.  The C drivers under cli/ (the correlation
+driver and the SAC-list-to-MSACS1 converter) are built with gcc into
+fastpcc_amd/bin/, linked against the library.
 """
 import concurrent.futures as cf
 import os
@@ -12,6 +15,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfastpcc.so")
+CLI = os.path.join(HERE, "cli")
+BIN = os.path.join(HERE, "bin")
+CXX = os.environ.get("CXX", "g++")
+CC = os.environ.get("CC", "gcc")
+CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-I", os.path.join(HERE, "..", "include")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FASTPCC_ARCH", "gfx950")
 
@@ -24,12 +32,13 @@ CFLAGS = [
 
 
 def sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
 def _compile(src):
-    obj = os.path.join(BUILD, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
+    base, ext = os.path.splitext(os.path.basename(src))
+    obj = os.path.join(BUILD, base + (".o" if ext == ".hip" else ".host.o"))
+    cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj] if ext == ".hip" else [CXX, *CXXFLAGS, "-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{p.stdout}\n{p.stderr}")
@@ -52,7 +61,21 @@ def build(verbose=False):
     if p.returncode != 0:
         raise RuntimeError(f"link failed:\n{p.stdout}\n{p.stderr}")
     os.replace(tmp, LIB)
+    build_cli()
     return LIB
+
+
+def build_cli():
+    """gcc builds of the C drivers, linked against libfastpcc.so (rpath: ../)."""
+    os.makedirs(BIN, exist_ok=True)
+    for src in sorted(f for f in os.listdir(CLI) if f.endswith(".c")):
+        exe = os.path.join(BIN, src[:-2])
+        cmd = [CC, "-O2", "-std=gnu11", "-Wall", "-Wextra", "-I", os.path.join(HERE, "..", "include"),
+               os.path.join(CLI, src), "-o", exe + ".tmp", "-L", HERE, "-lfastpcc", "-Wl,-rpath,$ORIGIN/..", "-lm"]
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            raise RuntimeError(f"gcc failed on {src}:\n{p.stdout}\n{p.stderr}")
+        os.replace(exe + ".tmp", exe)
 
 
 if __name__ == "__main__":
