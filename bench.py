@@ -341,7 +341,7 @@ def main():
     if "plain" in variants:
         speedup = round(value / variants["plain"]["value"], 2)
         for name, ent in variants.items():
-            if name != "plain":
+            if name != "plain" and "value" in ent:
                 ent["speedup_vs_plain"] = round(ent["value"] / variants["plain"]["value"], 2)
 
     cpu = None

@@ -12,9 +12,11 @@
 //   Re G = sum_m ur_i ur_j + ui_i ui_j,   Im G = sum_m ur_i ui_j - ui_i ur_j.
 // Stages:
 //   k_spectra_to_u   F[s*W+w][k] -> U[k][s][m], k <= Nz/2  (LDS-tiled transpose)
-//   k_gram_*         U -> G[p][k] (pair-major rows of ld >= Nz/2+1 bins)
+//   k_gram, k_gram16_dma  U -> G[p][k] (pair-major rows of ld >= Nz/2+1 bins)
 //   k_pair_c2r       G[p] -> half-length complex IFFT -> lag window -> y[p][L]
+#include <algorithm>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <vector>
 #include "engine.hpp"
@@ -55,7 +57,7 @@ __device__ __forceinline__ uint2 u_element(cf32 v, float sigma)
 // signals, ldF >= Nz/2+1 bins per row): m in [0, W).
 __global__ void __launch_bounds__(256)
 k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz, int nk, int mirror, float sigma,
-               uint2 *__restrict__ U)
+               int grouped, uint2 *__restrict__ U)
 {
     __shared__ cf32 td[64][33];
     __shared__ cf32 tm[64][33];
@@ -86,8 +88,22 @@ k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz,
         if (k >= nk || sw >= SW) continue;
         const int s = sw / W, w = sw - s * W;
         uint2 *row = U + ((long long)k * S + s) * W2;
-        row[w] = u_element(td[r][kk], sigma);
-        if (mirror) row[W + w] = u_element(tm[r][kk], sigma);
+        if (grouped) {
+            // split elements in groups of four: 4 hi words, then their 4 lo words
+            unsigned *rw = reinterpret_cast<unsigned *>(row);
+            uint2 e = u_element(td[r][kk], sigma);
+            rw[8 * (w >> 2) + (w & 3)] = e.x;
+            rw[8 * (w >> 2) + 4 + (w & 3)] = e.y;
+            if (mirror) {
+                const int m = W + w;
+                e = u_element(tm[r][kk], sigma);
+                rw[8 * (m >> 2) + (m & 3)] = e.x;
+                rw[8 * (m >> 2) + 4 + (m & 3)] = e.y;
+            }
+        } else {
+            row[w] = u_element(td[r][kk], sigma);
+            if (mirror) row[W + w] = u_element(tm[r][kk], sigma);
+        }
     }
 }
 
@@ -305,6 +321,162 @@ k_gram(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, int W2
     }
 }
 
+// ---- split Gram with operands staged by direct global -> LDS loads ----
+// For rows of K = W2 split elements, K a multiple of 16, stored in groups of four
+// (4 hi words, then the 4 lo words): the same 64 x 64 blocks, chunks of kGF bins
+// and epilogue as k_gram<1>, but each bin's rows are copied by the memory unit
+// straight into LDS (buffer loads with the LDS flag, 16 bytes a lane, bounds-
+// checked by the buffer resource: out-of-range units read as zero) while the
+// MFMAs of the previous bin run; no registers are spent on staging.  An LDS row
+// holds the K/2 units of a station plus one or two pad units (an odd number of
+// 16-byte units), so the 16 rows of an operand read fall on distinct banks.  The
+// operand reads are inline ds_read_b128 that wait for themselves, which keeps the
+// compiler from waiting for the next bin's copies before every LDS read.
+__host__ __device__ constexpr int dma_row_units(int K) { return K / 2 + ((K / 2) % 2 ? 2 : 1); }
+
+__device__ __forceinline__ unsigned lds_offset(const void *p)
+{
+    return (unsigned)(size_t)((__attribute__((address_space(3))) const void *)p);
+}
+
+__global__ void __launch_bounds__(kGThreads)
+k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, int K, int nk,
+             const int2 *__restrict__ blocks, int nblk, int nchunks, float oscale, cf32 *__restrict__ G,
+             long long ldg)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int b = blockIdx.x, xcd = b & 7, slot = b >> 3;
+    const int cp = slot / nblk, blk = slot - cp * nblk;
+    const int chunk = ((cp >> 1) * 8 + xcd) * 2 + (cp & 1);
+    if (chunk >= nchunks) return;
+    const int bi = blocks[blk].x, bj = blocks[blk].y;
+    const bool same = bi == bj && Ua == Ub;
+    const int k0 = chunk * kGF;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int li = lane & 15, q = lane >> 4;
+    const int ti = wave & 3, cj = wave >> 2;
+    const bool active = !(bi == bj && 2 * cj + 1 < ti);
+    const int su = dma_row_units(K), upr = K / 2;
+    const int nchunk64 = (same ? kGB : 2 * kGB) * su / 64;  // 64-unit pieces per bin (rows are 64 per side)
+    const int bufb = 2 * kGB * su * 16;                     // bytes per staging buffer
+    // resources over this chunk's bins (bins past nk read as zero)
+    const long long binb = (long long)S * K * 8;
+    const long long availb = (long long)min(kGF, nk - k0) * binb;
+    const unsigned nrec = (unsigned)(availb < 0xFFFFFF00LL ? availb : 0xFFFFFF00LL);
+    const __amdgpu_buffer_rsrc_t ra =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(Ua + (long long)k0 * S * K), 0, nrec, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rb =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(Ub + (long long)k0 * S * K), 0, nrec, 0x00020000);
+    unsigned char *lds = smem_raw;
+
+    auto dma = [&](int f, int buf) {
+        const unsigned boff = (unsigned)(f * binb);
+        for (int j = 0; j < 9; ++j) {
+            const int c64 = j * 8 + wave;  // wave-uniform
+            if (c64 >= nchunk64) break;
+            const int side = c64 >= su;    // 64 rows x su units = su pieces per side
+            const int p = c64 * 64 + lane;
+            const int row = p / su, pu = p - row * su;
+            const int st = (side ? bj : bi) * kGB + (row & 63);
+            unsigned voff = 0xFFFFFFF0u;   // pad unit: past the end, reads zero
+            if (pu < upr) voff = boff + (unsigned)(((long long)st * K + 2 * pu) * 8);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(side ? rb : ra,
+                                                     (__attribute__((address_space(3))) void *)(lds + buf * bufb +
+                                                                                                c64 * 1024),
+                                                     16, voff, 0, 0, 0);
+        }
+    };
+
+    f4v acc[kGF][4];
+#pragma unroll
+    for (int f = 0; f < kGF; ++f)
+#pragma unroll
+        for (int a = 0; a < 4; ++a) acc[f][a] = f4v{0.f, 0.f, 0.f, 0.f};
+
+    dma(0, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    const int arow = 16 * ti + li;
+    const int brow = (same ? 0 : kGB) + 32 * cj + li;
+    const unsigned base = lds_offset(smem_raw);
+#pragma unroll
+    for (int f = 0; f < kGF; ++f) {
+        const int buf = f & 1;
+        if (f + 1 < kGF) dma(f + 1, buf ^ 1);
+        if (active) {
+            f4v r0 = acc[f][0], i0 = acc[f][1], r1 = acc[f][2], i1 = acc[f][3];
+            unsigned pa = base + buf * bufb + arow * su * 16 + 32 * q;
+            unsigned pb0 = base + buf * bufb + brow * su * 16 + 32 * q;
+            unsigned pb1 = pb0 + 16 * su * 16;
+            for (int stp = 0; stp < K / 16; ++stp) {
+                u4v ah, al, bh0, bl0, bh1, bl1;
+                asm volatile(
+                    "ds_read_b128 %0, %6\n\t"
+                    "ds_read_b128 %1, %6 offset:16\n\t"
+                    "ds_read_b128 %2, %7\n\t"
+                    "ds_read_b128 %3, %7 offset:16\n\t"
+                    "ds_read_b128 %4, %8\n\t"
+                    "ds_read_b128 %5, %8 offset:16\n\t"
+                    "s_waitcnt lgkmcnt(0)"
+                    : "=&v"(ah), "=&v"(al), "=&v"(bh0), "=&v"(bl0), "=&v"(bh1), "=&v"(bl1)
+                    : "v"(pa), "v"(pb0), "v"(pb1)
+                    : "memory");
+                pa += 128;
+                pb0 += 128;
+                pb1 += 128;
+                const h8v Ah = __builtin_bit_cast(h8v, ah), Al = __builtin_bit_cast(h8v, al);
+                const h8v Rh = __builtin_bit_cast(h8v, rot_cplx16(ah));
+                const h8v Rl = __builtin_bit_cast(h8v, rot_cplx16(al));
+                const h8v Bh0 = __builtin_bit_cast(h8v, bh0), Bl0 = __builtin_bit_cast(h8v, bl0);
+                const h8v Bh1 = __builtin_bit_cast(h8v, bh1), Bl1 = __builtin_bit_cast(h8v, bl1);
+                r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah, Bh0, r0, 0, 0, 0);
+                i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Rh, Bh0, i0, 0, 0, 0);
+                r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah, Bh1, r1, 0, 0, 0);
+                i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Rh, Bh1, i1, 0, 0, 0);
+                r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah, Bl0, r0, 0, 0, 0);
+                i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Rh, Bl0, i0, 0, 0, 0);
+                r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah, Bl1, r1, 0, 0, 0);
+                i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Rh, Bl1, i1, 0, 0, 0);
+                r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al, Bh0, r0, 0, 0, 0);
+                i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Rl, Bh0, i0, 0, 0, 0);
+                r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al, Bh1, r1, 0, 0, 0);
+                i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Rl, Bh1, i1, 0, 0, 0);
+            }
+            acc[f][0] = r0;
+            acc[f][1] = i0;
+            acc[f][2] = r1;
+            acc[f][3] = i1;
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // bin f + 1 has landed
+        __syncthreads();
+    }
+
+    float *out = reinterpret_cast<float *>(smem_raw);
+    for (int h = 0; h < 2; ++h) {
+        if (cj == h && active) {
+#pragma unroll
+            for (int f = 0; f < kGF; ++f)
+#pragma unroll
+                for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int il = 16 * ti + 4 * q + r, jl = 16 * tt + li;
+                        *reinterpret_cast<float2 *>(out + (il * 32 + jl) * kGOut + 2 * f) =
+                            float2{acc[f][2 * tt][r] * oscale, acc[f][2 * tt + 1][r] * oscale};
+                    }
+        }
+        __syncthreads();
+        for (int e = t; e < kGB * 32 * kGF; e += kGThreads) {
+            const int pl = e >> 3, f = e & 7;
+            const int i = bi * kGB + (pl >> 5), j = bj * kGB + 32 * h + (pl & 31);
+            if (i >= j || j >= S) continue;
+            const long long p = (long long)i * S - (long long)i * (i + 1) / 2 + (j - i - 1);
+            G[p * ldg + k0 + f] = *reinterpret_cast<const cf32 *>(out + pl * kGOut + 2 * f);
+        }
+        __syncthreads();
+    }
+}
+
 // ---- per pair: folded spectrum -> real lags ----
 // Z[k] = P + i e^{+2 pi i k/Nz} Q, P = G[k] + conj(G[M-k]), Q = G[k] - conj(G[M-k])
 __global__ void __launch_bounds__(512)
@@ -418,28 +590,31 @@ long long gram_ld(int nk) { return ((long long)nk + 15) / 16 * 16; }
 
 // Station blocks (64 x 64, bi <= bj) holding at least one pair i < j, as a device
 // table cached per device and station count.
-const int2 *gram_tiles(int S, int &ntiles)
+// Upper-triangle blocks (bi <= bj) of B stations, cached per device.
+static const int2 *block_list(int S, int B, int &ntiles)
 {
-    static std::map<std::pair<int, int>, std::pair<const int2 *, int>> cache;
+    static std::map<std::tuple<int, int, int>, std::pair<const int2 *, int>> cache;
     static std::mutex mu;
     int dev = 0;
     (void)hipGetDevice(&dev);
     std::lock_guard<std::mutex> lk(mu);
-    auto it = cache.find({dev, S});
+    auto it = cache.find({dev, S, B});
     if (it != cache.end()) {
         ntiles = it->second.second;
         return it->second.first;
     }
     std::vector<int2> t;
-    for (int bi = 0; bi * kGB < S; ++bi)
-        for (int bj = bi; bj * kGB < S; ++bj) t.push_back(int2{bi, bj});
+    for (int bi = 0; bi * B < S; ++bi)
+        for (int bj = bi; bj * B < S; ++bj) t.push_back(int2{bi, bj});
     void *d = nullptr;
     if (hipMalloc(&d, sizeof(int2) * t.size()) != hipSuccess) return nullptr;
     if (hipMemcpy(d, t.data(), sizeof(int2) * t.size(), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
-    cache[{dev, S}] = {(const int2 *)d, (int)t.size()};
+    cache[{dev, S, B}] = {(const int2 *)d, (int)t.size()};
     ntiles = (int)t.size();
     return (const int2 *)d;
 }
+
+const int2 *gram_tiles(int S, int &ntiles) { return block_list(S, kGB, ntiles); }
 
 float gram16_sigma(int bound)
 {
@@ -454,8 +629,9 @@ int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, in
     const int SW = S * W;
     dim3 grid((nk + 31) / 32, (SW + 63) / 64);
     ProfScope prof("spectra_to_u", st);
+    const int W2 = mirror ? 2 * W : W;
     hipLaunchKernelGGL(k_spectra_to_u, grid, dim3(256), 0, st, d_F, ldF, SW, W, Nz, nk, mirror, sigma,
-                       (uint2 *)d_U);
+                       (sigma > 0.f && gram16_grouped(W2)) ? 1 : 0, (uint2 *)d_U);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }
@@ -489,11 +665,30 @@ int launch_gram_f32(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int m0, i
     return launch_gram(false, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles, ntiles, 1.0f, d_G, ldg, st);
 }
 
+bool gram16_grouped(int W2) { return W2 % 16 == 0 && W2 <= kGK; }
+
 int launch_gram_split16(const void *d_Ua, const void *d_Ub, int S, int W2, int m0, int K, int nk,
                         const int2 *d_tiles, int ntiles, float sigma, cf32 *d_G, long long ldg, hipStream_t st)
 {
     if (!(sigma > 0.f)) return -1;
-    return launch_gram(true, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles, ntiles, 1.0f / (sigma * sigma), d_G, ldg, st);
+    if (!gram16_grouped(W2))
+        return launch_gram(true, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles, ntiles, 1.0f / (sigma * sigma), d_G, ldg,
+                           st);
+    // grouped rows (written so by launch_spectra_to_u): the whole row, DMA-staged
+    if (S < 2 || nk < 1 || m0 != 0 || K != W2) return -1;
+    const int nchunks = (nk + kGF - 1) / kGF;
+    if (ldg < (long long)nchunks * kGF) return -1;
+    if ((long long)S * K * 8 * kGF >= 0xFFFFFF00LL) return -1;
+    const long long slots = (long long)(nchunks + 15) / 16 * 2 * ntiles;
+    if (slots * 8 > 0x7fffffffLL) return -1;
+    static_assert(2 * 2 * kGB * dma_row_units(kGK) * 16 <= kGramLds, "DMA staging fits the Gram's LDS");
+    ProfScope prof("gram_split16", st);
+    allow_lds((const void *)k_gram16_dma, kGramLds);
+    hipLaunchKernelGGL(k_gram16_dma, dim3((unsigned)(slots * 8)), dim3(kGThreads), kGramLds, st,
+                       (const uint2 *)d_Ua, (const uint2 *)d_Ub, S, K, nk, d_tiles, ntiles, nchunks,
+                       1.0f / (sigma * sigma), d_G, ldg);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
 }
 
 int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int L, float scale, int count_norm,

@@ -39,6 +39,10 @@ int launch_gram_f32(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int m0, i
                     int ntiles, cf32 *d_G, long long ldg, hipStream_t st);
 // The same Gram on FP16 MFMA from split elements scaled by sigma (results are
 // divided by sigma^2).
+// Split U rows of W2 elements are stored in groups of four (4 hi words, then 4
+// lo words) when this holds, as launch_spectra_to_u writes them and
+// launch_gram_split16 reads them (which then takes m0 = 0, K = W2).
+bool gram16_grouped(int W2);
 int launch_gram_split16(const void *d_Ua, const void *d_Ub, int S, int W2, int m0, int K, int nk,
                         const int2 *d_tiles, int ntiles, float sigma, cf32 *d_G, long long ldg, hipStream_t st);
 // Power-of-two split scale for |component| <= bound: largest scaled value <= 2^14,
