@@ -471,7 +471,12 @@ namespace {
 // stride), grouped by kind, path and Nz.
 bool batch_key(const fpcc_job *job, const Dims &d, long long &key)
 {
-    if (job->kind != FPCC_GNCC && job->kind != FPCC_CC1B) return false;
+    if (job->kind == FPCC_PCC2) {
+        // the in-LDS station stage of any window length, except the specialised one
+        if (d.large || station_pcc_specialised(d.N, (int)d.Nz)) return false;
+    } else if (job->kind != FPCC_GNCC && job->kind != FPCC_CC1B) {
+        return false;
+    }
     key = ((long long)d.Nz << 8) | ((long long)d.large << 4) | job->kind;
     return true;
 }
@@ -494,8 +499,21 @@ int station_stage_group(const std::vector<const fpcc_job *> &jobs, const std::ve
         jb.lde[g] = energy_ld(D[g].N);
         jb.b0[g + 1] = jb.b0[g] + D[g].S * D[g].W;
     }
-    const int passes = kind == FPCC_CC1B ? 2 : 1;
     int r = 0;
+    if (kind == FPCC_PCC2) {
+        PccBatch pb{};
+        pb.n = n;
+        for (int g = 0; g < n; ++g) {
+            pb.b0[g] = jb.b0[g];
+            pb.x[g] = jb.x[g];
+            pb.stride[g] = jb.stride[g];
+            pb.W[g] = jb.W[g];
+            if ((r = station_pcc_batch_job(D[g].N, Nz, (cf32 *)(ws[g] + layout(jobs[g], D[g]).a), pb, g))) return r;
+        }
+        pb.b0[n] = jb.b0[n];
+        return launch_station_pcc_batch(pb, Nz, st);
+    }
+    const int passes = kind == FPCC_CC1B ? 2 : 1;
     if (!large) {
         for (int p = 0; p < passes; ++p) {
             for (int g = 0; g < n; ++g) {

@@ -69,6 +69,27 @@ struct RealBatch {
     long long lde[kMaxBatchJobs];
 };
 
+// Job table of a PCC2 station-stage launch over several jobs of one Nz <= 8192
+// (pcc2.hip): per job its samples, stride, windows, window length, that
+// length's FFT plan and twiddles, 1/N and the spectra rows.
+struct PccBatch {
+    int n;
+    int b0[kMaxBatchJobs + 1];
+    const float *x[kMaxBatchJobs];
+    long long stride[kMaxBatchJobs];
+    int W[kMaxBatchJobs], N[kMaxBatchJobs];
+    FftPlan planN[kMaxBatchJobs];
+    const cf32 *twN[kMaxBatchJobs];
+    float da1[kMaxBatchJobs];
+    cf32 *F[kMaxBatchJobs];
+};
+// True when (N, Nz) runs the compile-time specialised station kernel (those
+// jobs are not batched: the batch kernel is the generic one).
+bool station_pcc_specialised(int N, int Nz);
+// Fills job j's plan, twiddles, length, 1/N and output rows (-1: unsupported).
+int station_pcc_batch_job(int N, int Nz, cf32 *d_F, PccBatch &jb, int j);
+int launch_station_pcc_batch(const PccBatch &jb, int Nz, hipStream_t st);
+
 // GNCC / 1-bit (gn.hip).  Half spectra rows hold real_ld(Nz) bins, energy rows
 // energy_ld(N) prefix sums.  sign_rule 0: samples, 1: x >= 0, 2: x > 0.
 long long real_ld(int Nz);

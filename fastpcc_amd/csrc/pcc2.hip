@@ -21,18 +21,16 @@
 
 namespace fpcc {
 
-__global__ void __launch_bounds__(512)
-k_station_pcc(const float *__restrict__ x, long long station_stride, int W, int N, int Nz, FftPlan planN,
-              const cf32 *__restrict__ twN, FftPlan planZ, const cf32 *__restrict__ twZ, float da1,
-              cf32 *__restrict__ F, cf32 *__restrict__ phi)
+// One station-window: analytic signal, AmpNormf, zero-padding to Nz and FFT_Nz
+// into Frow (or, phirow != nullptr, the length-N phasors).
+__device__ void station_pcc_row(const float *__restrict__ in, int N, int Nz, const FftPlan &planN,
+                                const cf32 *__restrict__ twN, const FftPlan &planZ, const cf32 *__restrict__ twZ,
+                                float da1, cf32 *__restrict__ Frow, cf32 *__restrict__ phirow,
+                                unsigned char *smem_raw)
 {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
     cf32 *buf0 = reinterpret_cast<cf32 *>(smem_raw);
     cf32 *buf1 = buf0 + Nz;
     float *red = reinterpret_cast<float *>(buf1 + Nz);
-    const int sw = blockIdx.x;
-    const int s = sw / W, w = sw - s * W;
-    const float *in = x + (long long)s * station_stride + (long long)w * N;
     const int T = blockDim.x;
 
     for (int n = threadIdx.x; n < N; n += T) buf0[n] = cf32{in[n], 0.0f};
@@ -62,12 +60,11 @@ k_station_pcc(const float *__restrict__ x, long long station_stride, int W, int 
     }
     m = block_max(m, red);
     const float eps = (float)(1e-6 * sqrt((double)m));
-    if (phi) {  // phasor-only output (time-domain PCC1 / PCC-v)
-        cf32 *o = phi + (long long)sw * N;
+    if (phirow) {  // phasor-only output (time-domain PCC1 / PCC-v)
         for (int n = threadIdx.x; n < N; n += T) {
             cf32 v = z[n];
             float r = 1.0f / (sqrtf(v.x * v.x + v.y * v.y) + eps);
-            o[n] = __builtin_isnormal(r) ? cf32{v.x * r, v.y * r} : cf32{0.0f, 0.0f};
+            phirow[n] = __builtin_isnormal(r) ? cf32{v.x * r, v.y * r} : cf32{0.0f, 0.0f};
         }
         return;
     }
@@ -83,8 +80,36 @@ k_station_pcc(const float *__restrict__ x, long long station_stride, int W, int 
     }
     __syncthreads();
     cf32 *Fz = fft_lds<float, -1>(dst, z, planZ, twZ);
-    cf32 *out = F + (long long)sw * Nz;
-    for (int k = threadIdx.x; k < Nz; k += T) out[k] = Fz[k];
+    for (int k = threadIdx.x; k < Nz; k += T) Frow[k] = Fz[k];
+}
+
+__global__ void __launch_bounds__(512)
+k_station_pcc(const float *__restrict__ x, long long station_stride, int W, int N, int Nz, FftPlan planN,
+              const cf32 *__restrict__ twN, FftPlan planZ, const cf32 *__restrict__ twZ, float da1,
+              cf32 *__restrict__ F, cf32 *__restrict__ phi)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int sw = blockIdx.x;
+    const int s = sw / W, w = sw - s * W;
+    const float *in = x + (long long)s * station_stride + (long long)w * N;
+    station_pcc_row(in, N, Nz, planN, twN, planZ, twZ, da1, F ? F + (long long)sw * Nz : nullptr,
+                    phi ? phi + (long long)sw * N : nullptr, smem_raw);
+}
+
+// The station-windows of several PCC2 jobs with one Nz (uneven station counts,
+// window counts, lengths and strides) in one launch: block b belongs to job j
+// with b0[j] <= b < b0[j+1] and runs exactly k_station_pcc's code for it.
+__global__ void __launch_bounds__(512)
+k_station_pcc_batch(PccBatch jb, int Nz, FftPlan planZ, const cf32 *__restrict__ twZ)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int j = 0;
+    while (j + 1 < jb.n && (int)blockIdx.x >= jb.b0[j + 1]) ++j;
+    const int sw = blockIdx.x - jb.b0[j];
+    const int s = sw / jb.W[j], w = sw - s * jb.W[j];
+    const float *in = jb.x[j] + (long long)s * jb.stride[j] + (long long)w * jb.N[j];
+    station_pcc_row(in, jb.N[j], Nz, jb.planN[j], jb.twN[j], planZ, twZ, jb.da1[j], jb.F[j] + (long long)sw * Nz,
+                    nullptr, smem_raw);
 }
 
 // Station stage specialised for N = 3600, Nz = 8192 (one day at 1 Hz cut in
@@ -302,6 +327,41 @@ int launch_station_pcc(const float *d_x, long long station_stride, int S, int W,
     ProfScope prof("station_pcc", st);
     hipLaunchKernelGGL(k_station_pcc, dim3(S * W), dim3(threads), lds, st, d_x, station_stride, W, N, Nz,
                        pN.plan, (const cf32 *)pN.tw, pZ.plan, (const cf32 *)pZ.tw, da1, d_F, (cf32 *)nullptr);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+bool station_pcc_specialised(int N, int Nz)
+{
+    if (N != kSpN || Nz != kSpNz) return false;
+    const DevPlan &pN = get_plan(N, false);
+    const DevPlan &pZ = get_plan(Nz, false);
+    return pN.ok && pZ.ok && radices_are(pN.plan, {16, 9, 25}) && radices_are(pZ.plan, {16, 16, 16, 2});
+}
+
+int station_pcc_batch_job(int N, int Nz, cf32 *d_F, PccBatch &jb, int j)
+{
+    const DevPlan &pN = get_plan(N, false);
+    if (!pN.ok || Nz > 8192 || N > Nz) return -1;
+    jb.planN[j] = pN.plan;
+    jb.twN[j] = (const cf32 *)pN.tw;
+    jb.N[j] = N;
+    jb.da1[j] = (float)(1 / (double)N);
+    jb.F[j] = d_F;
+    return 0;
+}
+
+int launch_station_pcc_batch(const PccBatch &jb, int Nz, hipStream_t st)
+{
+    if (jb.n < 1 || jb.n > kMaxBatchJobs) return -1;
+    const DevPlan &pZ = get_plan(Nz, false);
+    if (!pZ.ok || Nz > 8192) return -1;
+    const int blocks = jb.b0[jb.n];
+    if (blocks <= 0) return 0;
+    const size_t lds = sizeof(cf32) * 2 * (size_t)Nz + sizeof(float) * 16;
+    allow_lds((const void *)k_station_pcc_batch, lds);
+    ProfScope prof("station_pcc_batch", st);
+    hipLaunchKernelGGL(k_station_pcc_batch, dim3(blocks), dim3(512), lds, st, jb, Nz, pZ.plan, (const cf32 *)pZ.tw);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }

@@ -159,8 +159,9 @@ int fpcc_pair_stage_device(const fpcc_job *job, void *d_ws, size_t ws_bytes, flo
 /* Station stages of njobs jobs on one stream, each into its own workspace
  * (d_x[j], d_ws[j], ws_bytes[j]).  Jobs of uneven station count, window count,
  * window length and stride share kernel launches when their transforms agree
- * (GNCC and 1-bit GNCC of equal Nz, up to 16 jobs per launch); the others run
- * one by one.  The results equal fpcc_station_stage_device per job. */
+ * (GNCC and 1-bit GNCC of equal Nz; PCC2 of equal Nz <= 8192, each job with its
+ * own window-length FFT; up to 16 jobs per launch); the others run one by one.
+ * The results equal fpcc_station_stage_device per job. */
 int fpcc_station_stage_batch_device(const fpcc_job *jobs, int njobs, const float *const *d_x, void *const *d_ws,
                                     const size_t *ws_bytes, void *stream);
 

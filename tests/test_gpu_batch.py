@@ -1,7 +1,7 @@
 """Uneven jobs in one launch: fpcc_station_stage_batch_device runs the station
-stages of GNCC and 1-bit jobs with different station counts, window counts,
-window lengths and strides (one Nz per group) as shared launches; each job's
-pair stage then gives exactly its standalone result.  Device memory comes from the
+stages of GNCC, 1-bit and PCC2 jobs with different station counts, window
+counts, window lengths and strides (one Nz per group) as shared launches; each
+job's pair stage then gives exactly its standalone result.  Device memory comes from the
 HIP runtime the library itself links."""
 import ctypes
 
@@ -18,6 +18,11 @@ GROUPS = [
     [("gncc", 5, 1, 3000, 800, "none"), ("gncc", 9, 2, 2500, 900, "plain"), ("gncc", 3, 1, 3900, 300, "none")],
     [("cc1b", 4, 1, 20000, 2000, "none"), ("cc1b", 7, 2, 12000, 1500, "plain"), ("cc1b", 2, 1, 16000, 900, "none")],
     [("gncc", 6, 1, 30000, 1000, "none"), ("gncc", 4, 3, 20000, 2000, "plain"), ("pcc2", 5, 1, 3600, 500, "none")],
+    # PCC2, Nz 4096: window lengths 3000, 2500, 3840, 2800 (each its own FFT plan) and all stacks
+    [("pcc2", 5, 1, 3000, 800, "none"), ("pcc2", 9, 2, 2500, 900, "plain"), ("pcc2", 3, 1, 3840, 200, "gram"),
+     ("pcc2", 4, 3, 2800, 1000, "gram16")],
+    # PCC2, Nz 8192: the specialised 3600-sample job runs alone, the others share a launch
+    [("pcc2", 4, 2, 3600, 1000, "none"), ("pcc2", 6, 1, 6000, 2000, "plain"), ("pcc2", 3, 2, 5000, 1500, "zsum")],
 ]
 
 
@@ -70,5 +75,26 @@ def test_server_groups_uneven_jobs(fp):
         st = srv.stats()
         print(f"batched {st['batched_requests']} in {st['batch_launches']} groups")
         assert st["batched_requests"] >= 2 and st["batch_launches"] >= 1 and st["device_used"] == 0
+    for (job, x, y), ref in zip(items, refs):
+        assert np.array_equal(y, ref)
+
+
+def test_server_groups_uneven_pcc2(fp):
+    """The same for a burst of PCC2 requests of different window lengths."""
+    specs = [(5, 3000, 800), (9, 2500, 900), (3, 3840, 200), (6, 2800, 1000), (4, 2000, 1500)]
+    items, refs = [], []
+    for k, (S, N, lag) in enumerate(specs):
+        x, _ = synth.make_stations(S, N, 200, seed=90 + k)
+        job = fp.make_job("pcc2", S, 1, N, -lag, lag, stack="none", station_stride=N)
+        y = np.empty(S * (S - 1) // 2 * (2 * lag + 1), np.float32)
+        items.append((job, x, y))
+        refs.append(fp.allpairs(x, 1, N, -lag, lag, kind="pcc2", stack="none").ravel())
+    with fp.Server(device_bytes=2 << 30, pinned_bytes=128 << 20, max_inflight=16) as srv:
+        ids = srv.submit_many(items)
+        for rid in ids:
+            assert srv.wait(rid)["state"] == 2
+        st = srv.stats()
+        print(f"batched {st['batched_requests']} in {st['batch_launches']} groups")
+        assert st["batched_requests"] >= 2 and st["batch_launches"] >= 1
     for (job, x, y), ref in zip(items, refs):
         assert np.array_equal(y, ref)
