@@ -130,11 +130,10 @@ constexpr int kGF = 8;
 constexpr int kGK = 64;
 constexpr int kGRow = 2 * kGK + 4;
 constexpr int kGBuf = 2 * kGB * kGRow;  // dwords per staging buffer (A rows, then B rows)
-constexpr int kGOut = 2 * kGF + 2;      // dwords per staged pair in the epilogue
-constexpr int kGOut4 = 2 * kGF + 4;     // the same, 16-byte aligned (k_gram16_dma)
+constexpr int kGOut4 = 2 * kGF + 4;     // dwords per staged pair in the epilogue (16-byte aligned)
 constexpr int kGThreads = 512;
 constexpr size_t kGramLds = (size_t)4 *
-    (2 * kGBuf > kGB * (kGB / 2) * kGOut ? 2 * kGBuf : kGB * (kGB / 2) * kGOut);
+    (2 * kGBuf > kGB * (kGB / 4) * kGOut4 ? 2 * kGBuf : kGB * (kGB / 4) * kGOut4);
 
 __device__ __forceinline__ u4v rot_cplx16(u4v v)
 {
@@ -295,28 +294,31 @@ k_gram(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, int W2
         }
     }
 
-    // epilogue: pair (il, jl) of a 64 x 32 half block at dwords [(il*32 + jl)*kGOut, +2*kGF)
-    float *out = reinterpret_cast<float *>(lds);
-    for (int h = 0; h < 2; ++h) {
-        if (cj == h && active) {
+    // epilogue in quarters (columns 16h..16h+15, held by the waves with cj = h/2,
+    // tt = h%2): pair (il, jl) at dwords (il*16 + jl)*kGOut4 in LDS, then 16 bytes
+    // (two bins) a lane, four lanes a pair: 64-byte segments of G rows
+    float *out = reinterpret_cast<float *>(smem_raw);
+    for (int h = 0; h < 4; ++h) {
+        const int tt = h & 1;
+        if (cj == (h >> 1) && active) {
 #pragma unroll
             for (int f = 0; f < kGF; ++f)
 #pragma unroll
-                for (int tt = 0; tt < 2; ++tt)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int il = 16 * ti + 4 * q + r, jl = 16 * tt + li;
-                        *reinterpret_cast<float2 *>(out + (il * 32 + jl) * kGOut + 2 * f) =
-                            float2{acc[f][2 * tt][r] * oscale, acc[f][2 * tt + 1][r] * oscale};
-                    }
+                for (int r = 0; r < 4; ++r) {
+                    const int il = 16 * ti + 4 * q + r, jl = li;
+                    const f4v re = tt ? acc[f][2] : acc[f][0], im = tt ? acc[f][3] : acc[f][1];
+                    *reinterpret_cast<float2 *>(out + (il * 16 + jl) * kGOut4 + 2 * f) =
+                        float2{re[r] * oscale, im[r] * oscale};
+                }
         }
         __syncthreads();
-        for (int e = t; e < kGB * 32 * kGF; e += kGThreads) {
-            const int pl = e >> 3, f = e & 7;
-            const int i = bi * kGB + (pl >> 5), j = bj * kGB + 32 * h + (pl & 31);
+        for (int e = t; e < kGB * 16 * 4; e += kGThreads) {
+            const int pl = e >> 2, qq = e & 3;
+            const int i = bi * kGB + (pl >> 4), j = bj * kGB + 16 * h + (pl & 15);
             if (i >= j || j >= S) continue;
             const long long p = (long long)i * S - (long long)i * (i + 1) / 2 + (j - i - 1);
-            G[p * ldg + k0 + f] = *reinterpret_cast<const cf32 *>(out + pl * kGOut + 2 * f);
+            *reinterpret_cast<f4v *>(G + p * ldg + k0 + 2 * qq) =
+                *reinterpret_cast<const f4v *>(out + pl * kGOut4 + 4 * qq);
         }
         __syncthreads();
     }
@@ -646,7 +648,7 @@ static int launch_gram(bool split, const void *d_Ua, const void *d_Ub, int S, in
 {
     if (S < 2 || K < 1 || nk < 1 || m0 < 0 || m0 + K > W2) return -1;
     const int nchunks = (nk + kGF - 1) / kGF;
-    if (ldg < (long long)nchunks * kGF) return -1;
+    if (ldg < (long long)nchunks * kGF || (ldg & 1)) return -1;  // 16-byte aligned stores
     const long long slots = (long long)(nchunks + 15) / 16 * 2 * ntiles;
     if (slots * 8 > 0x7fffffffLL) return -1;
     const void *fn = split ? (const void *)k_gram<1> : (const void *)k_gram<0>;
