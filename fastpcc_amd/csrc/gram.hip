@@ -82,6 +82,31 @@ k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz,
     // write: for each k of the tile, 64 station-windows (contiguous w runs per station)
     const int W2 = mirror ? 2 * W : W;
     const int S = SW / W;
+    if (grouped && (W & 3) == 0) {
+        // a thread converts 4 consecutive windows of one station (a group: never split
+        // across tiles or stations when 4 | W) and writes its 4 hi words, then 4 lo words
+        for (int idx = threadIdx.x; idx < 32 * 16; idx += 256) {
+            const int kk = idx >> 4, g = idx & 15;
+            const int k = k0 + kk, sw = sw0 + 4 * g;
+            if (k >= nk || sw >= SW) continue;
+            const int s = sw / W, w = sw - s * W;
+            uint4 *row = reinterpret_cast<uint4 *>(U + ((long long)k * S + s) * W2);
+            uint2 e0 = u_element(td[4 * g][kk], sigma), e1 = u_element(td[4 * g + 1][kk], sigma);
+            uint2 e2 = u_element(td[4 * g + 2][kk], sigma), e3 = u_element(td[4 * g + 3][kk], sigma);
+            row[2 * (w >> 2)] = uint4{e0.x, e1.x, e2.x, e3.x};
+            row[2 * (w >> 2) + 1] = uint4{e0.y, e1.y, e2.y, e3.y};
+            if (mirror) {
+                const int m = W + w;
+                e0 = u_element(tm[4 * g][kk], sigma);
+                e1 = u_element(tm[4 * g + 1][kk], sigma);
+                e2 = u_element(tm[4 * g + 2][kk], sigma);
+                e3 = u_element(tm[4 * g + 3][kk], sigma);
+                row[2 * (m >> 2)] = uint4{e0.x, e1.x, e2.x, e3.x};
+                row[2 * (m >> 2) + 1] = uint4{e0.y, e1.y, e2.y, e3.y};
+            }
+        }
+        return;
+    }
     for (int idx = threadIdx.x; idx < 32 * 64; idx += 256) {
         const int kk = idx >> 6, r = idx & 63;
         const int k = k0 + kk, sw = sw0 + r;
