@@ -173,6 +173,10 @@ def lib():
     L.fpcc_autotune_fft.restype = i
     L.fpcc_fft_split.argtypes = [i]
     L.fpcc_fft_split.restype = i
+    L.fpcc_autotune_radix.argtypes = [i, i, i, ip, ip, dp, dp]
+    L.fpcc_autotune_radix.restype = i
+    L.fpcc_fft_radices.argtypes = [i, ip]
+    L.fpcc_fft_radices.restype = i
     L.fpcc_host_alloc.argtypes = [ctypes.c_size_t]
     L.fpcc_host_alloc.restype = vp
     L.fpcc_host_free.argtypes = [vp]

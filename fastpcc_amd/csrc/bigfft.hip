@@ -2,7 +2,6 @@
 #include <cmath>
 #include <map>
 #include <mutex>
-#include <dlfcn.h>
 #include <cstdlib>
 #include <string>
 #include "../../include/fastpcc.h"
@@ -49,17 +48,8 @@ static std::map<int, int> &tune_table()
     static bool loaded = false;
     if (loaded) return t;
     loaded = true;
-    std::string path;
-    if (const char *e = getenv("FASTPCC_FFT_TUNING")) {
-        path = e;
-    } else {
-        Dl_info info;
-        if (dladdr((const void *)&heuristic_split, &info) && info.dli_fname) {
-            std::string so = info.dli_fname;
-            const size_t slash = so.rfind('/');
-            path = (slash == std::string::npos ? std::string(".") : so.substr(0, slash)) + "/tuning/fft_splits_gfx950.txt";
-        }
-    }
+    const char *e = getenv("FASTPCC_FFT_TUNING");
+    const std::string path = e ? std::string(e) : tuning_file("fft_splits_gfx950.txt");
     if (FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "r")) {
         char line[256];
         while (fgets(line, sizeof line, f)) {

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdio.h>
+#include <string>
+#include <vector>
 #include "fft_core.hpp"
 
 namespace fpcc {
@@ -36,6 +38,10 @@ const void *get_c2r_twiddles(int nz, bool f64);
 const void *get_full_twiddles(int n, bool f64);
 // Radix list for n (host); returns false if unsupported.
 bool factor_plan(int n, FftPlan &p);
+// Plan with the given pass radices (false unless valid for n).
+bool plan_from_radices(int n, const std::vector<int> &radices, FftPlan &p);
+// Path of a tuning table in tuning/ beside the library ("" if unknown).
+std::string tuning_file(const char *name);
 
 // Smallest power of two >= v
 static inline unsigned pow2_ceil(unsigned v)

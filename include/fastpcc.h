@@ -331,6 +331,17 @@ int fpcc_server_destroy(fpcc_server *server);
 int fpcc_autotune_fft(int n, int batch, int reps, int *best_n1, double *best_ms, int *default_n1,
                       double *default_ms);
 int fpcc_fft_split(int n);
+/* Radix order of the single-workgroup FP32 LDS FFTs (lengths up to 8192; 3600,
+ * 4096 and 8192 keep the order their compile-time kernels use): from the table
+ * tuning/fft_radix_gfx950.txt (or $FASTPCC_RADIX_TUNING), else the default
+ * factorisation.  fpcc_autotune_radix times every order of every multiset of
+ * supported radices (2, 3, 4, 5, 7, 8, 9, 16, 25) with product n, largest-first
+ * and smallest-first, on `batch` transforms (reps repetitions), returns the
+ * fastest in radices[npass] (and the default order's time) and uses it for plans
+ * of n first built later in the process; fpcc_fft_radices returns the pass count
+ * of the plan in use and its radices (0 if n has none). */
+int fpcc_autotune_radix(int n, int batch, int reps, int *radices, int *npass, double *best_ms, double *default_ms);
+int fpcc_fft_radices(int n, int *radices);
 
 /* ---------------------------------------------------------------------------
  * Kernel timing for benchmarks (no counterpart in the reference).  While
