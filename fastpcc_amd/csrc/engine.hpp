@@ -36,6 +36,9 @@ const DevPlan &get_plan(int n, bool f64);
 const void *get_c2r_twiddles(int nz, bool f64);
 // Table of exp(-2*pi*i*k/n), k in [0, n) (cached per device and precision).
 const void *get_full_twiddles(int n, bool f64);
+// Per-pass twiddles of an FP32 plan with the given radix order (the compile-time
+// FFT kernels fix their order; cached per device), nullptr if the order is invalid.
+const void *get_order_twiddles(int n, const std::vector<int> &radices);
 // Radix list for n (host); returns false if unsupported.
 bool factor_plan(int n, FftPlan &p);
 // Plan with the given pass radices (false unless valid for n).

@@ -216,6 +216,23 @@ const void *get_c2r_twiddles(int nz, bool f64)
     return d;
 }
 
+const void *get_order_twiddles(int n, const std::vector<int> &radices)
+{
+    static std::map<std::pair<int, std::vector<int>>, const void *> cache;
+    FftPlan p{};
+    if (!plan_from_radices(n, radices, p)) return nullptr;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_mu);
+    std::vector<int> key = radices;
+    key.push_back(dev);
+    auto it = cache.find({n, key});
+    if (it != cache.end()) return it->second;
+    const void *d = upload_table<float>(plan_twiddles(p));
+    cache.emplace(std::make_pair(n, key), d);
+    return d;
+}
+
 const void *get_full_twiddles(int n, bool f64)
 {
     static std::map<std::tuple<int, int, bool>, const void *> cache;

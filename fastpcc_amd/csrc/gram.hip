@@ -545,11 +545,11 @@ k_pair_c2r(const cf32 *__restrict__ G, long long ldg, int Nz, FftPlan planM, con
     }
 }
 
-// Specialisation for Nz = 8192 (M = 4096 = 16^3): compile-time in-place FFT, 35 KB
-// of LDS, four persistent workgroups per CU, each walking pairs p, p + grid, ...
-// and loading the next pair's spectrum into registers while the current one is
-// folded, transformed and written.  The fold runs in place: Z[k] and Z[M-k] come
-// from the same two values G[k], G[M-k].
+// Specialisation for Nz = 8192 (M = 4096): compile-time in-place FFT (radices Rs,
+// NT threads), 35 KB of LDS, persistent workgroups each walking pairs p, p + grid,
+// ... and loading the next pair's spectrum into registers while the current one
+// is folded, transformed and written.  The fold runs in place: Z[k] and Z[M-k]
+// come from the same two values G[k], G[M-k].
 __device__ __forceinline__ cf32 c2r_fold(cf32 a, cf32 c, cf32 w)
 {
     const cf32 P = cf32{a.x + c.x, a.y - c.y};
@@ -558,7 +558,8 @@ __device__ __forceinline__ cf32 c2r_fold(cf32 a, cf32 c, cf32 w)
     return cf32{P.x - wq.y, P.y + wq.x};
 }
 
-__global__ void __launch_bounds__(256)
+template <int NT, int... Rs>
+__global__ void __launch_bounds__(NT)
 k_pair_c2r_4096(const cf32 *__restrict__ G, long long ldg, int P, const cf32 *__restrict__ twM,
                 const cf32 *__restrict__ twc, int lag0, int L, float scale, int count_norm, int N,
                 float *__restrict__ y)
@@ -570,23 +571,23 @@ k_pair_c2r_4096(const cf32 *__restrict__ G, long long ldg, int P, const cf32 *__
     const int t = threadIdx.x;
     long long p = blockIdx.x;
     if (p >= P) return;
-    cf32 pre[M / 256];
+    cf32 pre[M / NT];
     cf32 preM = cf32{0.f, 0.f};
     auto fetch = [&](long long q) {
         const cf32 *g = G + q * ldg;
 #pragma unroll
-        for (int i = 0; i < M / 256; ++i) pre[i] = g[t + 256 * i];
+        for (int i = 0; i < M / NT; ++i) pre[i] = g[t + NT * i];
         if (t == 0) preM = g[M];
     };
     fetch(p);
     for (;;) {
 #pragma unroll
-        for (int i = 0; i < M / 256; ++i) lds[ct_pad(t + 256 * i)] = pre[i];
+        for (int i = 0; i < M / NT; ++i) lds[ct_pad(t + NT * i)] = pre[i];
         if (t == 0) gM = preM;
         const long long pn = p + gridDim.x;
         if (pn < P) fetch(pn);  // in flight during this pair's work
         __syncthreads();
-        for (int k = t; k <= M / 2; k += 256) {
+        for (int k = t; k <= M / 2; k += NT) {
             cf32 a = lds[ct_pad(k)];
             cf32 c = k == 0 ? gM : lds[ct_pad(M - k)];
             if (k == 0) {  // DC and Nyquist of a real signal's spectrum are real
@@ -597,9 +598,9 @@ k_pair_c2r_4096(const cf32 *__restrict__ G, long long ldg, int P, const cf32 *__
             if (k != 0 && k != M / 2) lds[ct_pad(M - k)] = c2r_fold(c, a, twc[M - k]);
         }
         __syncthreads();
-        ct_fft<float, M, 256, +1, 16, 16, 16>(lds, twM);
+        ct_fft<float, M, NT, +1, Rs...>(lds, twM);
         float *dst = y + p * L;
-        for (int l = t; l < L; l += 256) {
+        for (int l = t; l < L; l += NT) {
             const int n = (lag0 + l) & (Nz - 1);
             cf32 v = lds[ct_pad(n >> 1)];
             float r = scale * ((n & 1) ? v.y : v.x);
@@ -730,22 +731,27 @@ int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int 
     const DevPlan &pM = get_plan(M, false);
     const void *twc = get_c2r_twiddles(Nz, false);
     if (!pM.ok || !twc) return -1;
-    if (M == 4096 && pM.plan.npass == 3 && pM.plan.radix[0] == 16 && pM.plan.radix[1] == 16 &&
-        pM.plan.radix[2] == 16) {
+    if (M == 4096) {
+        // 512 threads, radix 8 x 8 x 8 x 8: eight values a thread per pass keep the
+        // kernel small enough in registers for four workgroups a CU
+        constexpr int kT = 512;
+        const auto fn = k_pair_c2r_4096<kT, 8, 8, 8, 8>;
+        const cf32 *tw8 = (const cf32 *)get_order_twiddles(4096, {8, 8, 8, 8});
+        if (!tw8) return -1;
         size_t lds = sizeof(cf32) * ct_lds_elems<4096>();
         int dev = 0, ncu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
             return -1;
-        // one persistent workgroup per resident slot (registers allow two or more per CU)
+        // one persistent workgroup per resident slot
         static int per_cu = 0;
-        if (!per_cu && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pair_c2r_4096, 256,
-                                                                     lds) != hipSuccess || per_cu < 1))
+        if (!per_cu && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, kT, lds) !=
+                            hipSuccess || per_cu < 1))
             per_cu = 1;
         const int nwg = P < per_cu * ncu ? P : per_cu * ncu;
         ProfScope prof("pair_c2r", st);
-        hipLaunchKernelGGL(k_pair_c2r_4096, dim3(nwg), dim3(256), lds, st, d_G, gram_ld(Nz / 2 + 1), P,
-                           (const cf32 *)pM.tw, (const cf32 *)twc, lag0, L, scale, count_norm, N, d_y);
+        hipLaunchKernelGGL(fn, dim3(nwg), dim3(kT), lds, st, d_G, gram_ld(Nz / 2 + 1), P, tw8, (const cf32 *)twc,
+                           lag0, L, scale, count_norm, N, d_y);
         FPCC_CHECK(hipGetLastError());
         return 0;
     }
