@@ -380,38 +380,48 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
     const int bi = blocks[blk].x, bj = blocks[blk].y;
     const bool same = bi == bj && Ua == Ub;
     const int k0 = chunk * kGF;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int t = threadIdx.x, lane = t & 63;
+    // the wave index is uniform: taking it through a scalar register keeps the
+    // buffer resources and LDS destinations of the copies scalar
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
     const int li = lane & 15, q = lane >> 4;
     const int ti = wave & 3, cj = wave >> 2;
     const bool active = !(bi == bj && 2 * cj + 1 < ti);
     const int su = dma_row_units(K), upr = K / 2;
     const int nchunk64 = (same ? kGB : 2 * kGB) * su / 64;  // 64-unit pieces per bin (rows are 64 per side)
     const int bufb = 2 * kGB * su * 16;                     // bytes per staging buffer
-    // resources over this chunk's bins (bins past nk read as zero)
+    // resources over this chunk's bins (bins past nk read as zero); the host keeps
+    // a chunk under 2^31 bytes, so a pad offset of 2^31 plus any bin's offset is
+    // still past the end
     const long long binb = (long long)S * K * 8;
-    const long long availb = (long long)min(kGF, nk - k0) * binb;
-    const unsigned nrec = (unsigned)(availb < 0xFFFFFF00LL ? availb : 0xFFFFFF00LL);
+    const unsigned nrec = (unsigned)((long long)min(kGF, nk - k0) * binb);
     const __amdgpu_buffer_rsrc_t ra =
         __builtin_amdgcn_make_buffer_rsrc((void *)(Ua + (long long)k0 * S * K), 0, nrec, 0x00020000);
     const __amdgpu_buffer_rsrc_t rb =
         __builtin_amdgcn_make_buffer_rsrc((void *)(Ub + (long long)k0 * S * K), 0, nrec, 0x00020000);
     unsigned char *lds = smem_raw;
+    // each lane's source offset within bin 0 for its (at most 9) copies of a bin
+    unsigned voff0[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const int c64 = j * 8 + wave;
+        const int side = c64 >= su;  // 64 rows x su units = su pieces per side
+        const int p = c64 * 64 + lane;
+        const int row = p / su, pu = p - row * su;
+        const int st = (side ? bj : bi) * kGB + (row & 63);
+        voff0[j] = pu < upr ? (unsigned)(((long long)st * K + 2 * pu) * 8) : 0x80000000u;
+    }
 
     auto dma = [&](int f, int buf) {
         const unsigned boff = (unsigned)(f * binb);
+#pragma unroll
         for (int j = 0; j < 9; ++j) {
-            const int c64 = j * 8 + wave;  // wave-uniform
+            const int c64 = j * 8 + wave;  // scalar
             if (c64 >= nchunk64) break;
-            const int side = c64 >= su;    // 64 rows x su units = su pieces per side
-            const int p = c64 * 64 + lane;
-            const int row = p / su, pu = p - row * su;
-            const int st = (side ? bj : bi) * kGB + (row & 63);
-            unsigned voff = 0xFFFFFFF0u;   // pad unit: past the end, reads zero
-            if (pu < upr) voff = boff + (unsigned)(((long long)st * K + 2 * pu) * 8);
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(side ? rb : ra,
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(c64 >= su ? rb : ra,
                                                      (__attribute__((address_space(3))) void *)(lds + buf * bufb +
                                                                                                 c64 * 1024),
-                                                     16, voff, 0, 0, 0);
+                                                     16, voff0[j] + boff, 0, 0, 0);
         }
     };
 
@@ -711,7 +721,7 @@ int launch_gram_split16(const void *d_Ua, const void *d_Ub, int S, int W2, int m
     if (S < 2 || nk < 1 || m0 != 0 || K != W2) return -1;
     const int nchunks = (nk + kGF - 1) / kGF;
     if (ldg < (long long)nchunks * kGF || (ldg & 1)) return -1;  // 16-byte aligned stores
-    if ((long long)S * K * 8 * kGF >= 0xFFFFFF00LL) return -1;
+    if ((long long)S * K * 8 * kGF >= 0x80000000LL) return -1;  // chunk offsets stay below the pad offset
     const long long slots = (long long)(nchunks + 15) / 16 * 2 * ntiles;
     if (slots * 8 > 0x7fffffffLL) return -1;
     static_assert(2 * 2 * kGB * dma_row_units(kGK) * 16 <= kGramLds, "DMA staging fits the Gram's LDS");

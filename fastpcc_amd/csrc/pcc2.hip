@@ -113,26 +113,33 @@ k_station_pcc_batch(PccBatch jb, int Nz, FftPlan planZ, const cf32 *__restrict__
 }
 
 // Station stage specialised for N = 3600, Nz = 8192 (one day at 1 Hz cut in
-// hourly windows): compile-time FFTs (radices 16*9*25 and 16*16*16*2), in place
-// in one padded LDS buffer, so two workgroups share a CU.
-constexpr int kSpN = 3600, kSpNz = 8192, kSpT = 256;
+// hourly windows): compile-time FFTs in one padded LDS buffer of 4096 values
+// (35 KB, so four workgroups share a CU).  The window occupies samples < 3600 <
+// 4096 of the zero-padded 8192, so the 8192-point spectrum splits into two
+// 4096-point transforms:
+//   F[2m] = FFT_4096(z)[m],  F[2m+1] = FFT_4096(z[n] e^{-2 pi i n/8192})[m];
+// the even bins wait in registers for the odd ones, and each lane writes (F[2m], F[2m+1]) as one 16-byte store.  Between the two
+// transforms the normalised window waits in the workgroup's own output row of F
+// (each thread reads back only what it wrote; the final store overwrites it),
+// which keeps the kernel within 128 VGPRs: four workgroups a CU.
+constexpr int kSpN = 3600, kSpNz = 8192, kSpM = 4096, kSpT = 256;
 
 __global__ void __launch_bounds__(kSpT)
 k_station_pcc_3600(const float *__restrict__ x, long long station_stride, int W, const cf32 *__restrict__ twN,
-                   const cf32 *__restrict__ twZ, float da1, cf32 *__restrict__ F)
+                   const cf32 *__restrict__ twM, const cf32 *__restrict__ twZ, float da1, cf32 *__restrict__ F)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cf32 *lds = reinterpret_cast<cf32 *>(smem_raw);
-    float *red = reinterpret_cast<float *>(lds + ct_lds_elems<kSpNz>());
-    const int sw = blockIdx.x;
+    float *red = reinterpret_cast<float *>(lds + ct_lds_elems<kSpM>());
+    const int sw = blockIdx.x, t = threadIdx.x;
     const int s = sw / W, w = sw - s * W;
     const float *in = x + (long long)s * station_stride + (long long)w * kSpN;
-    for (int n = threadIdx.x; n < kSpN; n += kSpT) lds[ct_pad(n)] = cf32{in[n], 0.0f};
+    for (int n = t; n < kSpN; n += kSpT) lds[ct_pad(n)] = cf32{in[n], 0.0f};
     __syncthreads();
     ct_fft<float, kSpN, kSpT, -1, 16, 9, 25>(lds, twN);
     constexpr int half = (kSpN + 1) >> 1;
     const float two_da1 = 2 * da1;
-    for (int k = threadIdx.x; k < kSpN; k += kSpT) {
+    for (int k = t; k < kSpN; k += kSpT) {
         const float g = k == 0 ? da1 : k < half ? two_da1 : k == (kSpN >> 1) ? da1 : 0.0f;
         cf32 v = lds[ct_pad(k)];
         lds[ct_pad(k)] = cf32{v.x * g, v.y * g};
@@ -140,34 +147,50 @@ k_station_pcc_3600(const float *__restrict__ x, long long station_stride, int W,
     __syncthreads();
     ct_fft<float, kSpN, kSpT, +1, 16, 9, 25>(lds, twN);
     float m = 0.0f;
-    for (int n = threadIdx.x; n < kSpN; n += kSpT) {
+    for (int n = t; n < kSpN; n += kSpT) {
         cf32 v = lds[ct_pad(n)];
         m = fmaxf(m, v.x * v.x + v.y * v.y);
     }
     m = block_max(m, red);
     const float eps = (float)(1e-6 * sqrt((double)m));
-    for (int n = threadIdx.x; n < kSpNz; n += kSpT) {
-        cf32 o = cf32{0.0f, 0.0f};
+    constexpr int kZ = (kSpN + kSpT - 1) / kSpT;  // 15 samples a thread (n = t + 256 i)
+    cf32 *row = F + (long long)sw * kSpNz;
+    cf32 z[kZ];
+#pragma unroll
+    for (int i = 0; i < kZ; ++i) {
+        const int n = t + kSpT * i;
+        z[i] = cf32{0.0f, 0.0f};
         if (n < kSpN) {
-            cf32 v = lds[ct_pad(n)];
-            float r = 1.0f / (sqrtf(v.x * v.x + v.y * v.y) + eps);
-            if (__builtin_isnormal(r)) o = cf32{v.x * r, v.y * r};
+            const cf32 v = lds[ct_pad(n)];
+            const float r = 1.0f / (sqrtf(v.x * v.x + v.y * v.y) + eps);
+            if (__builtin_isnormal(r)) z[i] = cf32{v.x * r, v.y * r};
         }
-        lds[ct_pad(n)] = o;
     }
     __syncthreads();
-    ct_fft<float, kSpNz, kSpT, -1, 16, 16, 16, 2>(lds, twZ);
-    cf32 *out = F + (long long)sw * kSpNz;
-    for (int k = threadIdx.x; k < kSpNz; k += kSpT) out[k] = lds[ct_pad(k)];
-}
-
-static bool radices_are(const FftPlan &p, std::initializer_list<int> rs)
-{
-    if (p.npass != (int)rs.size()) return false;
-    int i = 0;
-    for (int r : rs)
-        if (p.radix[i++] != r) return false;
-    return true;
+    // even bins
+#pragma unroll
+    for (int i = 0; i < kSpM / kSpT; ++i) lds[ct_pad(t + kSpT * i)] = i < kZ ? z[i] : cf32{0.0f, 0.0f};
+    __syncthreads();
+    ct_fft<float, kSpM, kSpT, -1, 16, 16, 16>(lds, twM);
+    cf32 ev[kSpM / kSpT];
+#pragma unroll
+    for (int i = 0; i < kSpM / kSpT; ++i) ev[i] = lds[ct_pad(t + kSpT * i)];
+    __syncthreads();
+    // odd bins
+#pragma unroll
+    for (int i = 0; i < kSpM / kSpT; ++i) {
+        const int n = t + kSpT * i;
+        lds[ct_pad(n)] = i < kZ && n < kSpN ? cmul(z[i], twZ[n]) : cf32{0.0f, 0.0f};
+    }
+    __syncthreads();
+    ct_fft<float, kSpM, kSpT, -1, 16, 16, 16>(lds, twM);
+    float4 *out = reinterpret_cast<float4 *>(row);
+#pragma unroll
+    for (int i = 0; i < kSpM / kSpT; ++i) {
+        const int k = t + kSpT * i;
+        const cf32 o = lds[ct_pad(k)];
+        out[k] = float4{ev[i].x, ev[i].y, o.x, o.y};
+    }
 }
 
 // AmpNormf (src/FFTapps.c:178-205) in place on rows of N complex values.
@@ -312,12 +335,15 @@ int launch_station_pcc(const float *d_x, long long station_stride, int S, int W,
         return -1;
     }
     const float da1 = (float)(1 / (double)N);
-    if (N == kSpN && Nz == kSpNz && radices_are(pN.plan, {16, 9, 25}) && radices_are(pZ.plan, {16, 16, 16, 2})) {
-        size_t lds = sizeof(cf32) * ct_lds_elems<kSpNz>() + sizeof(float) * 8;
+    if (station_pcc_specialised(N, Nz)) {
+        const void *twN = get_order_twiddles(kSpN, {16, 9, 25}), *twM = get_order_twiddles(kSpM, {16, 16, 16});
+        const void *twZ = get_full_twiddles(kSpNz, false);
+        if (!twN || !twM || !twZ) return -1;
+        size_t lds = sizeof(cf32) * ct_lds_elems<kSpM>() + sizeof(float) * 8;
         allow_lds((const void *)k_station_pcc_3600, lds);
         ProfScope prof("station_pcc", st);
         hipLaunchKernelGGL(k_station_pcc_3600, dim3(S * W), dim3(kSpT), lds, st, d_x, station_stride, W,
-                           (const cf32 *)pN.tw, (const cf32 *)pZ.tw, da1, d_F);
+                           (const cf32 *)twN, (const cf32 *)twM, (const cf32 *)twZ, da1, d_F);
         FPCC_CHECK(hipGetLastError());
         return 0;
     }
@@ -331,13 +357,8 @@ int launch_station_pcc(const float *d_x, long long station_stride, int S, int W,
     return 0;
 }
 
-bool station_pcc_specialised(int N, int Nz)
-{
-    if (N != kSpN || Nz != kSpNz) return false;
-    const DevPlan &pN = get_plan(N, false);
-    const DevPlan &pZ = get_plan(Nz, false);
-    return pN.ok && pZ.ok && radices_are(pN.plan, {16, 9, 25}) && radices_are(pZ.plan, {16, 16, 16, 2});
-}
+// The compile-time kernel takes its own twiddle tables, so only the lengths matter.
+bool station_pcc_specialised(int N, int Nz) { return N == kSpN && Nz == kSpNz; }
 
 int station_pcc_batch_job(int N, int Nz, cf32 *d_F, PccBatch &jb, int j)
 {
