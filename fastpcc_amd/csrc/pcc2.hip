@@ -114,14 +114,12 @@ k_station_pcc_batch(PccBatch jb, int Nz, FftPlan planZ, const cf32 *__restrict__
 
 // Station stage specialised for N = 3600, Nz = 8192 (one day at 1 Hz cut in
 // hourly windows): compile-time FFTs in one padded LDS buffer of 4096 values
-// (35 KB, so four workgroups share a CU).  The window occupies samples < 3600 <
-// 4096 of the zero-padded 8192, so the 8192-point spectrum splits into two
-// 4096-point transforms:
+// (35 KB; registers then allow three workgroups a CU).  The window occupies
+// samples < 3600 < 4096 of the zero-padded 8192, so the 8192-point spectrum
+// splits into two 4096-point transforms:
 //   F[2m] = FFT_4096(z)[m],  F[2m+1] = FFT_4096(z[n] e^{-2 pi i n/8192})[m];
-// the even bins wait in registers for the odd ones, and each lane writes (F[2m], F[2m+1]) as one 16-byte store.  Between the two
-// transforms the normalised window waits in the workgroup's own output row of F
-// (each thread reads back only what it wrote; the final store overwrites it),
-// which keeps the kernel within 128 VGPRs: four workgroups a CU.
+// the normalised window z waits in registers for the second one, the even bins
+// for the odd ones, and each lane writes (F[2m], F[2m+1]) as one 16-byte store.
 constexpr int kSpN = 3600, kSpNz = 8192, kSpM = 4096, kSpT = 256;
 
 __global__ void __launch_bounds__(kSpT)
