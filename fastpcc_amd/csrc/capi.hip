@@ -133,7 +133,7 @@ Layout layout(const fpcc_job *job, const Dims &d)
         l.a = take(l.total, sizeof(cf32) * SW * d.Nz);  // spectra
         if (is_gram(job->stack)) {
             l.b = take(l.total, sizeof(cf32) * K1 * SW * 2);                          // U
-            l.c = take(l.total, sizeof(cf32) * d.P * (size_t)gram_ld(d.Nz / 2 + 1));  // G
+            l.c = take(l.total, sizeof(cf32) * gram_grouped_elems(d.P, (int)K1));  // G
         }
         break;
     case FPCC_GNCC:
@@ -146,7 +146,7 @@ Layout layout(const fpcc_job *job, const Dims &d)
         if (is_gram(job->stack)) {
             l.c = take(l.total, sizeof(cf32) * K1 * SW);
             l.d = take(l.total, sizeof(cf32) * K1 * SW);
-            l.e = take(l.total, sizeof(cf32) * d.P * (size_t)gram_ld(d.Nz / 2 + 1));
+            l.e = take(l.total, sizeof(cf32) * gram_grouped_elems(d.P, (int)K1));
         }
         break;
     case FPCC_PCC1:
@@ -243,12 +243,13 @@ int pair_stage(const fpcc_job *job, const Dims &d, char *ws, float *y, hipStream
             const float sigma = s16 ? gram16_sigma(d.N) : 0.f;
             const int nk = d.Nz / 2 + 1;
             if ((r = launch_spectra_to_u(F, d.Nz, d.S, d.W, d.Nz, nk, 1, U, st, sigma))) return r;
-            r = s16 ? launch_gram_split16(U, U, d.S, 2 * d.W, 0, 2 * d.W, nk, tiles, ntiles, sigma, G, gram_ld(nk), st)
-                    : launch_gram_f32(U, U, d.S, 2 * d.W, 0, 2 * d.W, nk, tiles, ntiles, G, gram_ld(nk), st);
+            const GLayout gl = gram_grouped(d.P);
+            r = s16 ? launch_gram_split16(U, U, d.S, 2 * d.W, 0, 2 * d.W, nk, tiles, ntiles, sigma, G, gl, st)
+                    : launch_gram_f32(U, U, d.S, 2 * d.W, 0, 2 * d.W, nk, tiles, ntiles, G, gl, st);
             if (r) return r;
             // 1/2 of the Gram's Hermitian fold, the reference's 1/((float)Nz*(float)N), the mean over W
             const float scale = 0.5f * (1 / ((float)d.Nz * (float)d.N)) / (float)d.W;
-            return launch_pair_c2r(G, (int)d.P, d.Nz, d.N, d.W, d.lag0, d.L, scale, 0, y, st);
+            return launch_pair_c2r(G, gl, (int)d.P, d.Nz, d.N, d.W, d.lag0, d.L, scale, 0, y, st);
         }
         return launch_pair_pcc2((const cf32 *)(ws + l.a), d.S, d.W, d.N, d.Nz, d.lag0, d.L, job->stack, 0,
                                 (int)d.P, y, st);
@@ -270,12 +271,13 @@ int pair_stage(const fpcc_job *job, const Dims &d, char *ws, float *y, hipStream
                 return r;
             if ((r = launch_spectra_to_u((const cf32 *)(ws + l.b), ld, d.S, d.W, d.Nz, nk, 0, Ub, st, sigma)))
                 return r;
-            r = s16 ? launch_gram_split16(Ua, Ub, d.S, d.W, 0, d.W, nk, tiles, ntiles, sigma, G, gram_ld(nk), st)
-                    : launch_gram_f32(Ua, Ub, d.S, d.W, 0, d.W, nk, tiles, ntiles, G, gram_ld(nk), st);
+            const GLayout gl = gram_grouped(d.P);
+            r = s16 ? launch_gram_split16(Ua, Ub, d.S, d.W, 0, d.W, nk, tiles, ntiles, sigma, G, gl, st)
+                    : launch_gram_f32(Ua, Ub, d.S, d.W, 0, d.W, nk, tiles, ntiles, G, gl, st);
             if (r) return r;
             // ccf_lowlevel's 1/Nz, the mean over W, then 1/(N-|lag|) (gnf_lowlevel with +-1 data)
             const float scale = (float)(1.0 / (double)d.Nz) / (float)d.W;
-            return launch_pair_c2r(G, (int)d.P, d.Nz, d.N, d.W, d.lag0, d.L, scale, 1, y, st);
+            return launch_pair_c2r(G, gl, (int)d.P, d.Nz, d.N, d.W, d.lag0, d.L, scale, 1, y, st);
         }
         return launch_pair_gn(ws + l.a, ws + l.b, nullptr, false, d.S, d.W, d.N, d.Nz, d.lag0, d.L, 1, mode, y,
                               st);

@@ -169,10 +169,15 @@ __device__ __forceinline__ u4v rot_cplx16(u4v v)
     return r;
 }
 
+__device__ __forceinline__ long long gaddr(const GLayout &gl, long long p, int k)
+{
+    return (long long)(k >> gl.sh) * gl.gs + p * gl.pl + (k & ((1 << gl.sh) - 1));
+}
+
 template <int SPLIT>
 __global__ void __launch_bounds__(kGThreads)
 k_gram(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, int W2, int m0, int K, int nk,
-       const int2 *__restrict__ blocks, int nblk, int nchunks, float oscale, cf32 *__restrict__ G, long long ldg)
+       const int2 *__restrict__ blocks, int nblk, int nchunks, float oscale, cf32 *__restrict__ G, GLayout gl)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     unsigned *lds = reinterpret_cast<unsigned *>(smem_raw);
@@ -342,7 +347,7 @@ k_gram(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, int W2
             const int i = bi * kGB + (pl >> 4), j = bj * kGB + 16 * h + (pl & 15);
             if (i >= j || j >= S) continue;
             const long long p = (long long)i * S - (long long)i * (i + 1) / 2 + (j - i - 1);
-            *reinterpret_cast<f4v *>(G + p * ldg + k0 + 2 * qq) =
+            *reinterpret_cast<f4v *>(G + gaddr(gl, p, k0 + 2 * qq)) =
                 *reinterpret_cast<const f4v *>(out + pl * kGOut4 + 4 * qq);
         }
         __syncthreads();
@@ -370,7 +375,7 @@ __device__ __forceinline__ unsigned lds_offset(const void *p)
 __global__ void __launch_bounds__(kGThreads)
 k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, int K, int nk,
              const int2 *__restrict__ blocks, int nblk, int nchunks, float oscale, cf32 *__restrict__ G,
-             long long ldg)
+             GLayout gl)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int b = blockIdx.x, xcd = b & 7, slot = b >> 3;
@@ -512,7 +517,7 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
             const int i = bi * kGB + (pl >> 4), j = bj * kGB + 16 * h + (pl & 15);
             if (i >= j || j >= S) continue;
             const long long p = (long long)i * S - (long long)i * (i + 1) / 2 + (j - i - 1);
-            *reinterpret_cast<f4v *>(G + p * ldg + k0 + 2 * qq) =
+            *reinterpret_cast<f4v *>(G + gaddr(gl, p, k0 + 2 * qq)) =
                 *reinterpret_cast<const f4v *>(out + pl * kGOut4 + 4 * qq);
         }
         __syncthreads();
@@ -522,7 +527,7 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
 // ---- per pair: folded spectrum -> real lags ----
 // Z[k] = P + i e^{+2 pi i k/Nz} Q, P = G[k] + conj(G[M-k]), Q = G[k] - conj(G[M-k])
 __global__ void __launch_bounds__(512)
-k_pair_c2r(const cf32 *__restrict__ G, long long ldg, int Nz, FftPlan planM, const cf32 *__restrict__ twM,
+k_pair_c2r(const cf32 *__restrict__ G, GLayout gl, int Nz, FftPlan planM, const cf32 *__restrict__ twM,
            const cf32 *__restrict__ twc, int lag0, int L, float scale, int count_norm, int N,
            float *__restrict__ y)
 {
@@ -531,9 +536,8 @@ k_pair_c2r(const cf32 *__restrict__ G, long long ldg, int Nz, FftPlan planM, con
     cf32 *buf0 = reinterpret_cast<cf32 *>(smem_raw);
     cf32 *buf1 = buf0 + M;
     const long long p = blockIdx.x;
-    const cf32 *g = G + p * ldg;
     for (int k = threadIdx.x; k < M; k += blockDim.x) {
-        cf32 a = g[k], c = g[M - k];
+        cf32 a = G[gaddr(gl, p, k)], c = G[gaddr(gl, p, M - k)];
         if (k == 0) {  // DC and Nyquist of a real signal's spectrum are real
             a.y = 0.f;
             c.y = 0.f;
@@ -570,7 +574,7 @@ __device__ __forceinline__ cf32 c2r_fold(cf32 a, cf32 c, cf32 w)
 
 template <int NT, int... Rs>
 __global__ void __launch_bounds__(NT)
-k_pair_c2r_4096(const cf32 *__restrict__ G, long long ldg, int P, const cf32 *__restrict__ twM,
+k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__restrict__ twM,
                 const cf32 *__restrict__ twc, int lag0, int L, float scale, int count_norm, int N,
                 float *__restrict__ y)
 {
@@ -583,11 +587,16 @@ k_pair_c2r_4096(const cf32 *__restrict__ G, long long ldg, int P, const cf32 *__
     if (p >= P) return;
     cf32 pre[M / NT];
     cf32 preM = cf32{0.f, 0.f};
+    // bin t + NT i of pair q sits at q pl + ui(i) + vo: a scalar part and this
+    // thread's 32-bit offset (groups of at most NT bins, or rows; checked by the host)
+    const bool rows = gl.sh == 30;
+    const unsigned vo = rows ? (unsigned)t : (unsigned)((t >> gl.sh) * gl.gs + (t & ((1 << gl.sh) - 1)));
+    const long long ustep = rows ? NT : (long long)(NT >> gl.sh) * gl.gs;
     auto fetch = [&](long long q) {
-        const cf32 *g = G + q * ldg;
+        const cf32 *gq = G + q * gl.pl;
 #pragma unroll
-        for (int i = 0; i < M / NT; ++i) pre[i] = g[t + NT * i];
-        if (t == 0) preM = g[M];
+        for (int i = 0; i < M / NT; ++i) pre[i] = gq[i * ustep + vo];
+        if (t == 0) preM = G[gaddr(gl, q, M)];
     };
     fetch(p);
     for (;;) {
@@ -630,6 +639,27 @@ static void allow_lds(const void *fn, size_t bytes)
 }
 
 long long gram_ld(int nk) { return ((long long)nk + 15) / 16 * 16; }
+
+#ifndef FPCC_GSHIFT
+#define FPCC_GSHIFT 7
+#endif
+constexpr int kGGroup = 1 << FPCC_GSHIFT;  // bins per group of the grouped G
+// an odd number of pair slots per group keeps a pair's pieces off one memory channel
+GLayout gram_grouped(long long P) { return GLayout{(P | 1) * kGGroup, kGGroup, FPCC_GSHIFT}; }
+
+size_t gram_grouped_elems(long long P, int nk)
+{
+    const long long bins = ((long long)nk + kGF - 1) / kGF * kGF;  // the Gram writes whole chunks
+    return (size_t)((bins + kGGroup - 1) / kGGroup) * (size_t)(P | 1) * kGGroup;
+}
+
+// G takes bins [0, nchunks * kGF) of pairs [0, S(S-1)/2) with 16-byte stores
+static bool layout_ok(const GLayout &gl, int S, int nchunks)
+{
+    const long long P = (long long)S * (S - 1) / 2;
+    if (gl.sh == 30) return gl.gs == 0 && gl.pl >= (long long)nchunks * kGF && !(gl.pl & 1);
+    return gl.sh >= 3 && gl.sh < 30 && gl.pl == (1LL << gl.sh) && gl.gs >= P * gl.pl;
+}
 
 // Station blocks (64 x 64, bi <= bj) holding at least one pair i < j, as a device
 // table cached per device and station count.
@@ -680,11 +710,11 @@ int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, in
 }
 
 static int launch_gram(bool split, const void *d_Ua, const void *d_Ub, int S, int W2, int m0, int K, int nk,
-                       const int2 *d_tiles, int ntiles, float oscale, cf32 *d_G, long long ldg, hipStream_t st)
+                       const int2 *d_tiles, int ntiles, float oscale, cf32 *d_G, GLayout gl, hipStream_t st)
 {
     if (S < 2 || K < 1 || nk < 1 || m0 < 0 || m0 + K > W2) return -1;
     const int nchunks = (nk + kGF - 1) / kGF;
-    if (ldg < (long long)nchunks * kGF || (ldg & 1)) return -1;  // 16-byte aligned stores
+    if (!layout_ok(gl, S, nchunks)) return -1;
     const long long slots = (long long)(nchunks + 15) / 16 * 2 * ntiles;
     if (slots * 8 > 0x7fffffffLL) return -1;
     const void *fn = split ? (const void *)k_gram<1> : (const void *)k_gram<0>;
@@ -693,34 +723,34 @@ static int launch_gram(bool split, const void *d_Ua, const void *d_Ub, int S, in
     if (split)
         hipLaunchKernelGGL(k_gram<1>, dim3((unsigned)(slots * 8)), dim3(kGThreads), kGramLds, st,
                            (const uint2 *)d_Ua, (const uint2 *)d_Ub, S, W2, m0, K, nk, d_tiles, ntiles, nchunks,
-                           oscale, d_G, ldg);
+                           oscale, d_G, gl);
     else
         hipLaunchKernelGGL(k_gram<0>, dim3((unsigned)(slots * 8)), dim3(kGThreads), kGramLds, st,
                            (const uint2 *)d_Ua, (const uint2 *)d_Ub, S, W2, m0, K, nk, d_tiles, ntiles, nchunks,
-                           oscale, d_G, ldg);
+                           oscale, d_G, gl);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }
 
 int launch_gram_f32(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int m0, int K, int nk, const int2 *d_tiles,
-                    int ntiles, cf32 *d_G, long long ldg, hipStream_t st)
+                    int ntiles, cf32 *d_G, GLayout gl, hipStream_t st)
 {
-    return launch_gram(false, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles, ntiles, 1.0f, d_G, ldg, st);
+    return launch_gram(false, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles, ntiles, 1.0f, d_G, gl, st);
 }
 
 bool gram16_grouped(int W2) { return W2 % 16 == 0 && W2 <= kGK; }
 
 int launch_gram_split16(const void *d_Ua, const void *d_Ub, int S, int W2, int m0, int K, int nk,
-                        const int2 *d_tiles, int ntiles, float sigma, cf32 *d_G, long long ldg, hipStream_t st)
+                        const int2 *d_tiles, int ntiles, float sigma, cf32 *d_G, GLayout gl, hipStream_t st)
 {
     if (!(sigma > 0.f)) return -1;
     if (!gram16_grouped(W2))
-        return launch_gram(true, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles, ntiles, 1.0f / (sigma * sigma), d_G, ldg,
+        return launch_gram(true, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles, ntiles, 1.0f / (sigma * sigma), d_G, gl,
                            st);
     // grouped rows (written so by launch_spectra_to_u): the whole row, DMA-staged
     if (S < 2 || nk < 1 || m0 != 0 || K != W2) return -1;
     const int nchunks = (nk + kGF - 1) / kGF;
-    if (ldg < (long long)nchunks * kGF || (ldg & 1)) return -1;  // 16-byte aligned stores
+    if (!layout_ok(gl, S, nchunks)) return -1;
     if ((long long)S * K * 8 * kGF >= 0x80000000LL) return -1;  // chunk offsets stay below the pad offset
     const long long slots = (long long)(nchunks + 15) / 16 * 2 * ntiles;
     if (slots * 8 > 0x7fffffffLL) return -1;
@@ -729,22 +759,24 @@ int launch_gram_split16(const void *d_Ua, const void *d_Ub, int S, int W2, int m
     allow_lds((const void *)k_gram16_dma, kGramLds);
     hipLaunchKernelGGL(k_gram16_dma, dim3((unsigned)(slots * 8)), dim3(kGThreads), kGramLds, st,
                        (const uint2 *)d_Ua, (const uint2 *)d_Ub, S, K, nk, d_tiles, ntiles, nchunks,
-                       1.0f / (sigma * sigma), d_G, ldg);
+                       1.0f / (sigma * sigma), d_G, gl);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }
 
-int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int L, float scale, int count_norm,
-                    float *d_y, hipStream_t st)
+int launch_pair_c2r(const cf32 *d_G, GLayout gl, int P, int Nz, int N, int W, int lag0, int L, float scale,
+                    int count_norm, float *d_y, hipStream_t st)
 {
     const int M = Nz / 2;
     const DevPlan &pM = get_plan(M, false);
     const void *twc = get_c2r_twiddles(Nz, false);
     if (!pM.ok || !twc) return -1;
-    if (M == 4096) {
+    constexpr int kT = 512;
+    // k_pair_c2r_4096 splits a bin's address into a scalar part and a 32-bit offset
+    const bool split_ok = gl.sh == 30 || (gl.sh <= 9 && (((kT - 1) >> gl.sh) + 1) * gl.gs < (1LL << 29));
+    if (M == 4096 && split_ok) {
         // 512 threads, radix 8 x 8 x 8 x 8: eight values a thread per pass keep the
         // kernel small enough in registers for four workgroups a CU
-        constexpr int kT = 512;
         const auto fn = k_pair_c2r_4096<kT, 8, 8, 8, 8>;
         const cf32 *tw8 = (const cf32 *)get_order_twiddles(4096, {8, 8, 8, 8});
         if (!tw8) return -1;
@@ -760,7 +792,7 @@ int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int 
             per_cu = 1;
         const int nwg = P < per_cu * ncu ? P : per_cu * ncu;
         ProfScope prof("pair_c2r", st);
-        hipLaunchKernelGGL(fn, dim3(nwg), dim3(kT), lds, st, d_G, gram_ld(Nz / 2 + 1), P, tw8, (const cf32 *)twc,
+        hipLaunchKernelGGL(fn, dim3(nwg), dim3(kT), lds, st, d_G, gl, P, tw8, (const cf32 *)twc,
                            lag0, L, scale, count_norm, N, d_y);
         FPCC_CHECK(hipGetLastError());
         return 0;
@@ -768,7 +800,7 @@ int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int 
     size_t lds = sizeof(cf32) * 2 * (size_t)M;
     allow_lds((const void *)k_pair_c2r, lds);
     ProfScope prof("pair_c2r", st);
-    hipLaunchKernelGGL(k_pair_c2r, dim3(P), dim3(512), lds, st, d_G, gram_ld(Nz / 2 + 1), Nz, pM.plan,
+    hipLaunchKernelGGL(k_pair_c2r, dim3(P), dim3(512), lds, st, d_G, gl, Nz, pM.plan,
                        (const cf32 *)pM.tw, (const cf32 *)twc, lag0, L, scale, count_norm, N, d_y);
     FPCC_CHECK(hipGetLastError());
     return 0;

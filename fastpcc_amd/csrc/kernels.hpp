@@ -26,6 +26,18 @@ int launch_pair_pcc2(const cf32 *d_F, int S, int W, int N, int Nz, int lag0, int
 
 // Stack over windows as a per-frequency Gram (gram.hip).
 long long gram_ld(int nk);                       // row length of G for nk bins
+// Layout of G: bin k of pair p at (k >> sh) * gs + p * pl + (k & (2^sh - 1)).
+// Rows (pair-major, ld bins): {0, ld, 30}.  Grouped: bins in groups of 128, each
+// group holding 128 bins of every pair ({(P | 1) * 128, 128, 7}; an odd number
+// of pair slots a group), so a Gram chunk of bins writes pairs 1 KB apart and a
+// pair's spectrum is read in 1 KB pieces.
+struct GLayout {
+    long long gs, pl;
+    int sh;
+};
+inline GLayout gram_rows(long long ld) { return GLayout{0, ld, 30}; }
+GLayout gram_grouped(long long P);
+size_t gram_grouped_elems(long long P, int nk);  // complex values of a grouped G
 const int2 *gram_tiles(int S, int &ntiles);      // cached device tile table
 // U[k][s][m] for k < nk from rows F[s*W+w] of ldF bins; mirror 1 appends the
 // conjugated bin Nz-k as m = W+w.  sigma = 0: FP32 complex elements; sigma > 0:
@@ -34,9 +46,9 @@ const int2 *gram_tiles(int S, int &ntiles);      // cached device tile table
 int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, int nk, int mirror, void *d_U,
                         hipStream_t st, float sigma = 0.f);
 // G[p][k] = sum_{m0 <= m < m0+K} conj(Ua[k][i][m]) Ub[k][j][m] for pairs i < j, k < nk
-// (U rows hold W2 values per station); bins nk .. ldg-1 of the last chunk are zeroed.
+// (U rows hold W2 values per station); bins nk .. of the last chunk of 8 are zeroed.
 int launch_gram_f32(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int m0, int K, int nk, const int2 *d_tiles,
-                    int ntiles, cf32 *d_G, long long ldg, hipStream_t st);
+                    int ntiles, cf32 *d_G, GLayout gl, hipStream_t st);
 // The same Gram on FP16 MFMA from split elements scaled by sigma (results are
 // divided by sigma^2).
 // Split U rows of W2 elements are stored in groups of four (4 hi words, then 4
@@ -44,14 +56,14 @@ int launch_gram_f32(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int m0, i
 // launch_gram_split16 reads them (which then takes m0 = 0, K = W2).
 bool gram16_grouped(int W2);
 int launch_gram_split16(const void *d_Ua, const void *d_Ub, int S, int W2, int m0, int K, int nk,
-                        const int2 *d_tiles, int ntiles, float sigma, cf32 *d_G, long long ldg, hipStream_t st);
+                        const int2 *d_tiles, int ntiles, float sigma, cf32 *d_G, GLayout gl, hipStream_t st);
 // Power-of-two split scale for |component| <= bound: largest scaled value <= 2^14,
 // inside FP16's range with lo parts normal down to 2^-8 of it.
 float gram16_sigma(int bound);
 int fill_zero(float *p, long long n, hipStream_t st);
 // y[p][l] = scale * c2r(G[p])[(lag0+l) mod Nz], divided by N-|lag| if count_norm
-int launch_pair_c2r(const cf32 *d_G, int P, int Nz, int N, int W, int lag0, int L, float scale, int count_norm,
-                    float *d_y, hipStream_t st);
+int launch_pair_c2r(const cf32 *d_G, GLayout gl, int P, int Nz, int N, int W, int lag0, int L, float scale,
+                    int count_norm, float *d_y, hipStream_t st);
 
 // Station-windows of several jobs in one launch (uneven station counts, window
 // counts and lengths; one transform length): job j owns launch blocks

@@ -325,8 +325,8 @@ int large_pair_stage(int kind, int stack, int S, int W, int N, int Nz, int lag0,
         const long long ldg = gram_ld(Nz);
         if ((r = launch_spectra_to_u(F, Nz, S, W, Nz, Nz, 0, U, st, sigma))) return r;
         if (kind == FPCC_CC1B && (r = launch_spectra_to_u(F2, Nz, S, W, Nz, Nz, 0, U2, st, sigma))) return r;
-        r = s16 ? launch_gram_split16(U, U2, S, W, 0, W, Nz, tiles, ntiles, sigma, G, ldg, st)
-                : launch_gram_f32(U, U2, S, W, 0, W, Nz, tiles, ntiles, G, ldg, st);
+        r = s16 ? launch_gram_split16(U, U2, S, W, 0, W, Nz, tiles, ntiles, sigma, G, gram_rows(ldg), st)
+                : launch_gram_f32(U, U2, S, W, 0, W, Nz, tiles, ntiles, G, gram_rows(ldg), st);
         if (r) return r;
         RowArgs ra;
         ra.op = ROW_INV_PLAIN;

@@ -231,7 +231,7 @@ int wpcc_pair_stage(int S, int W, int N, const WFamily &f, int lag0, int L, bool
     const int nrounds = per_window ? W : 1;
     for (int w = 0; w < nrounds; ++w) {
         const int m0 = per_window ? w * nsc : 0, K = per_window ? nsc : W * nsc;
-        if ((r = launch_gram_f32(U, U, S, W * nsc, m0, K, Nz, tiles, ntiles, G, ldg, st))) return r;
+        if ((r = launch_gram_f32(U, U, S, W * nsc, m0, K, Nz, tiles, ntiles, G, gram_rows(ldg), st))) return r;
         RowArgs ra{};
         ra.op = ROW_INV_PLAIN;
         ra.src = G;
