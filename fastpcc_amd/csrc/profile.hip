@@ -79,6 +79,13 @@ extern "C" void fpcc_profile_enable(int on)
 {
     std::lock_guard<std::mutex> lk(mu);
     enabled = on != 0;
+    // events made up front: a timed run then only records them (creating events
+    // between launches delays the launches)
+    while (enabled && pool.size() < 2048) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) break;
+        pool.push_back(e);
+    }
 }
 
 extern "C" void fpcc_profile_reset(void)
