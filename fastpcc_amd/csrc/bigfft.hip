@@ -126,16 +126,98 @@ __device__ __forceinline__ int batch_job(const RealBatch &jb, int b)
     return j;
 }
 
-template <bool BATCH>
+// Compile-time batched transforms for the common splits (2^17 .. 2^19 as 256 x
+// 512 .. 256 x 2048): NTR transforms of NPT points transform in place in one LDS
+// buffer, transform t at t * LDT, point i at ct_pad(i) (one pad slot per 16
+// values), 256 threads, each pass reading all of a thread's butterflies before
+// writing.  Columns: 16 x 256 points, radix 16 x 16, stride 274; rows: 4 x 512
+// (8 x 8 x 8, stride 544) or 4 x 1024 (16 x 8 x 8, stride 1088).  The strides keep
+// the loads, passes and stores within 1.5x of conflict-free LDS access.
+constexpr int kCtN1 = 256, kCtLD = 274;
+
+template <int NPT, int NTR, int LDT, int DIR, int R, int NS>
+__device__ __forceinline__ void ctb_pass(cf32 *lds, const cf32 *__restrict__ tw)
+{
+    constexpr int NBF = NPT / R, TOT = NTR * NBF, BPT = (TOT + 255) / 256;
+    cf32 v[BPT][R];
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+        const int bb = threadIdx.x + 256 * i;
+        if (TOT % 256 == 0 || bb < TOT) {
+            const int t = bb / NBF, b = bb % NBF;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[i][r] = lds[t * LDT + ct_pad(b + r * NBF)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+        const int bb = threadIdx.x + 256 * i;
+        if (TOT % 256 == 0 || bb < TOT) {
+            const int t = bb / NBF, b = bb % NBF;
+            const int k = b % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    cf32 w = tw[(r - 1) * NS + k];
+                    if (DIR > 0) w.y = -w.y;
+                    v[i][r] = cmul(v[i][r], w);
+                }
+            }
+            Dft<R, DIR>::run(v[i]);
+            const int base = (b - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[t * LDT + ct_pad(base + r * NS)] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int NPT, int NTR, int LDT, int DIR, int NS, int OFF, int R, int... Rest>
+__device__ __forceinline__ void ctb_passes(cf32 *lds, const cf32 *__restrict__ tw)
+{
+    ctb_pass<NPT, NTR, LDT, DIR, R, NS>(lds, tw + OFF);
+    if constexpr (sizeof...(Rest) > 0) ctb_passes<NPT, NTR, LDT, DIR, NS * R, OFF + (R - 1) * NS, Rest...>(lds, tw);
+}
+
+// tw: the per-pass twiddles of the plan with radices Rs (in that order)
+template <int NPT, int NTR, int LDT, int DIR, int... Rs>
+__device__ __forceinline__ cf32 *ctb_fft(cf32 *lds, const cf32 *__restrict__ tw)
+{
+    static_assert((Rs * ... * 1) == NPT, "radices must multiply to NPT");
+    ctb_passes<NPT, NTR, LDT, DIR, 1, 0, Rs...>(lds, tw);
+    return lds;
+}
+
+template <int DIR>
+__device__ __forceinline__ cf32 *ct_cols_fft(cf32 *lds, const cf32 *__restrict__ tw)
+{
+    return ctb_fft<kCtN1, kBigCols, kCtLD, DIR, 16, 16>(lds, tw);
+}
+
+// rows: stride of the in-place layout for N2 (0: not specialised)
+__host__ __device__ constexpr int ct_row_ld(int n2) { return n2 == 512 ? 544 : n2 == 1024 ? 1088 : 0; }
+
+template <int CTN2, int DIR>
+__device__ __forceinline__ cf32 *ct_rows_fft(cf32 *lds, const cf32 *__restrict__ tw)
+{
+    if constexpr (CTN2 == 512) return ctb_fft<512, kBigRows, ct_row_ld(512), DIR, 8, 8, 8>(lds, tw);
+    else return ctb_fft<1024, kBigRows, ct_row_ld(1024), DIR, 16, 8, 8>(lds, tw);
+}
+
+template <bool BATCH, bool CT>
 __global__ void __launch_bounds__(256)
 k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, const cf32 *__restrict__ twN,
            RealBatch jb)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int ld = N1 + 1;  // padded column stride (the loads write with stride ld)
+    // column t, point r at cix(t, r): padded column stride N1 + 1 with ping-pong
+    // buffers, or the in-place layout of ct_cols_fft (N1 = 256, tw1 its twiddles)
+    const int ld = CT ? kCtLD : N1 + 1;
+    auto cix = [&](int t, int r) { return CT ? t * kCtLD + ct_pad(r) : t * ld + r; };
     cf32 *buf0 = reinterpret_cast<cf32 *>(smem_raw);
     cf32 *buf1 = buf0 + kBigCols * ld;
-    float *red = reinterpret_cast<float *>(buf1 + kBigCols * ld);
+    float *red = reinterpret_cast<float *>(CT ? buf1 : buf1 + kBigCols * ld);
     int b = blockIdx.y;
     if (BATCH) {
         const int j = batch_job(jb, b);
@@ -190,12 +272,15 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
                 v = a.src[sbase + g];
             }
         }
-        buf0[t * ld + r] = v;
+        buf0[cix(t, r)] = v;
     }
     __syncthreads();
     const bool inv = op == COL_INV_MAX || op == COL_INV_LAGWIN || op == COL_CHIRP_MID || op == COL_INV_CHIRP_MAX;
-    cf32 *res = inv ? fft_lds_batch<float, +1>(buf0, buf1, p1, tw1, kBigCols, ld)
-                    : fft_lds_batch<float, -1>(buf0, buf1, p1, tw1, kBigCols, ld);
+    cf32 *res;
+    if (CT) res = inv ? ct_cols_fft<+1>(buf0, tw1) : ct_cols_fft<-1>(buf0, tw1);
+    else
+        res = inv ? fft_lds_batch<float, +1>(buf0, buf1, p1, tw1, kBigCols, ld)
+                  : fft_lds_batch<float, -1>(buf0, buf1, p1, tw1, kBigCols, ld);
     if (op == COL_CHIRP_MID) {
         // second transform's chirped input: conj(mask X) w = mask conj(conv) (|w| = 1)
         const int half = (a.N + 1) >> 1;
@@ -208,11 +293,11 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
                 else if (g < half) gm = 2 * a.da1;
                 else if (!(a.N & 1) && g == (a.N >> 1)) gm = a.da1;
             }
-            const cf32 v = res[t * ld + r];
-            res[t * ld + r] = cf32{gm * v.x, -gm * v.y};
+            const cf32 v = res[cix(t, r)];
+            res[cix(t, r)] = cf32{gm * v.x, -gm * v.y};
         }
         __syncthreads();
-        res = fft_lds_batch<float, -1>(res, res == buf0 ? buf1 : buf0, p1, tw1, kBigCols, ld);
+        res = CT ? ct_cols_fft<-1>(res, tw1) : fft_lds_batch<float, -1>(res, res == buf0 ? buf1 : buf0, p1, tw1, kBigCols, ld);
     }
     const bool fwd_out = !inv || op == COL_CHIRP_MID;
     float m = 0.f;
@@ -229,7 +314,7 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
         const int c = c0 + t;
         if (c >= N2) continue;
         const long long g = (long long)r * N2 + c;
-        cf32 v = res[t * ld + r];
+        cf32 v = res[cix(t, r)];
         if (fwd_out) {
             a.dst[base + g] = cmul(v, twN[twidx(c, r)]);  // W_n^{n2 k1}
         } else if (op == COL_INV_CHIRP_MAX) {
@@ -272,12 +357,16 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
 }
 
 // BATCH: the rows of job j's destination (in place, ROW_FWD_PLAIN only).
-template <bool BATCH>
+// CTN2 = 512 or 1024: N2 = CTN2 transforms in place (ct_rows_fft, tw2 its
+// twiddles); 0: the runtime plan with ping-pong buffers.
+template <bool BATCH, int CTN2>
 __global__ void __launch_bounds__(256)
 k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, const cf32 *__restrict__ twN,
            RealBatch jb)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    // row rr, point c at rix(rr, c)
+    auto rix = [&](int rr, int c) { return CTN2 ? rr * ct_row_ld(CTN2) + ct_pad(c) : rr * N2 + c; };
     cf32 *buf0 = reinterpret_cast<cf32 *>(smem_raw);
     cf32 *buf1 = buf0 + kBigRows * N2;
     int b = blockIdx.y;
@@ -321,21 +410,25 @@ k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, 
                 v = a.src[base + g];
             }
         }
-        buf0[idx] = v;
+        buf0[rix(rr, c)] = v;
     }
     __syncthreads();
     const bool inv = op == ROW_INV_MUL || op == ROW_INV_PLAIN || op == ROW_INV_CROSS;
-    cf32 *res = (inv ? fft_lds_batch<float, +1>(buf0, buf1, p2, tw2, kBigRows, N2)
-                     : fft_lds_batch<float, -1>(buf0, buf1, p2, tw2, kBigRows, N2));
+    cf32 *res;
+    if constexpr (CTN2 != 0) res = inv ? ct_rows_fft<CTN2, +1>(buf0, tw2) : ct_rows_fft<CTN2, -1>(buf0, tw2);
+    else
+        res = (inv ? fft_lds_batch<float, +1>(buf0, buf1, p2, tw2, kBigRows, N2)
+                   : fft_lds_batch<float, -1>(buf0, buf1, p2, tw2, kBigRows, N2));
     const float sc = (op == ROW_FWD_SCALE) ? a.wscale[b % a.S] : 1.0f;
     const int half = (a.N + 1) >> 1;
     if (op == ROW_MUL_INV) {
         for (int idx = threadIdx.x; idx < kBigRows * N2; idx += blockDim.x) {
             const int rr = idx / N2, c = idx - rr * N2;
-            if (r0 + rr < N1) res[idx] = cmul(res[idx], a.T[(long long)(r0 + rr) * N2 + c]);
+            if (r0 + rr < N1) res[rix(rr, c)] = cmul(res[rix(rr, c)], a.T[(long long)(r0 + rr) * N2 + c]);
         }
         __syncthreads();
-        res = fft_lds_batch<float, +1>(res, res == buf0 ? buf1 : buf0, p2, tw2, kBigRows, N2);
+        if constexpr (CTN2 != 0) res = ct_rows_fft<CTN2, +1>(res, tw2);
+        else res = fft_lds_batch<float, +1>(res, res == buf0 ? buf1 : buf0, p2, tw2, kBigRows, N2);
     }
     if (op == ROW_FWD_MASK || op == ROW_MASK_INV) {
         for (int idx = threadIdx.x; idx < kBigRows * N2; idx += blockDim.x) {
@@ -346,12 +439,12 @@ k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, 
             else if (k < half) gm = 2 * a.da1;
             else if (!(a.N & 1) && k == (a.N >> 1)) gm = a.da1;
             else gm = 0.f;
-            res[idx] = cf32{res[idx].x * gm, res[idx].y * gm};
+            res[rix(rr, c)] = cf32{res[rix(rr, c)].x * gm, res[rix(rr, c)].y * gm};
         }
         if (op == ROW_MASK_INV) {
             __syncthreads();
-            cf32 *other = (res == buf0) ? buf1 : buf0;
-            res = fft_lds_batch<float, +1>(res, other, p2, tw2, kBigRows, N2);
+            if constexpr (CTN2 != 0) res = ct_rows_fft<CTN2, +1>(res, tw2);
+            else res = fft_lds_batch<float, +1>(res, (res == buf0) ? buf1 : buf0, p2, tw2, kBigRows, N2);
         }
     }
     const bool inv_out = inv || op == ROW_MASK_INV || op == ROW_MUL_INV;
@@ -360,7 +453,7 @@ k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, 
         const int r = r0 + rr;
         if (r >= N1) continue;
         const long long g = (long long)r * N2 + c;
-        cf32 v = res[idx];
+        cf32 v = res[rix(rr, c)];
         if (inv_out) {
             v = cmulc(twN[twidx(c, r)], v);  // W_n^{-n2 k1}
         } else if (op == ROW_FWD_SCALE) {
@@ -499,6 +592,29 @@ int bluestein_tables(int N, int n, cf32 *chirp, cf32 *T, hipStream_t st)
     return run_rows(bp, ra, 1, st, "bluestein_table_rows");
 }
 
+template <bool BATCH>
+static int launch_cols(const BigPlan &bp, const ColArgs &a, const RealBatch &jb, int nbatch, hipStream_t st,
+                       const char *name)
+{
+    const dim3 grid((bp.n2 + kBigCols - 1) / kBigCols, nbatch);
+    if (bp.n1 == kCtN1) {  // compile-time in-place column transform, one LDS buffer
+        const cf32 *twc = (const cf32 *)get_order_twiddles(kCtN1, {16, 16});
+        if (!twc) return -1;
+        const size_t lds = sizeof(cf32) * kBigCols * (size_t)kCtLD + sizeof(float) * 8;
+        ProfScope prof(name, st);
+        hipLaunchKernelGGL((k_big_cols<BATCH, true>), grid, dim3(256), lds, st, a, bp.n1, bp.n2, bp.p1, twc, bp.twN,
+                           jb);
+    } else {
+        const size_t lds = sizeof(cf32) * 2 * kBigCols * (size_t)(bp.n1 + 1) + sizeof(float) * 8;
+        allow_lds((const void *)k_big_cols<BATCH, false>, lds);
+        ProfScope prof(name, st);
+        hipLaunchKernelGGL((k_big_cols<BATCH, false>), grid, dim3(256), lds, st, a, bp.n1, bp.n2, bp.p1, bp.tw1,
+                           bp.twN, jb);
+    }
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
 int run_cols(const BigPlan &bp, const ColArgs &a, int nbatch, hipStream_t st, const char *name)
 {
     if (nbatch <= 0) return 0;
@@ -507,14 +623,8 @@ int run_cols(const BigPlan &bp, const ColArgs &a, int nbatch, hipStream_t st, co
         fprintf(stderr, "fastpcc: %d transforms exceed one launch\n", nbatch);
         return -1;
     }
-    const size_t lds = sizeof(cf32) * 2 * kBigCols * (size_t)(bp.n1 + 1) + sizeof(float) * 8;
-    allow_lds((const void *)k_big_cols<false>, lds);
-    ProfScope prof(name, st);
     RealBatch none{};
-    hipLaunchKernelGGL(k_big_cols<false>, dim3((bp.n2 + kBigCols - 1) / kBigCols, nbatch), dim3(256), lds, st, a,
-                       bp.n1, bp.n2, bp.p1, bp.tw1, bp.twN, none);
-    FPCC_CHECK(hipGetLastError());
-    return 0;
+    return launch_cols<false>(bp, a, none, nbatch, st, name);
 }
 
 int run_cols_batch(const BigPlan &bp, const ColArgs &a, const RealBatch &jb, hipStream_t st, const char *name)
@@ -522,11 +632,37 @@ int run_cols_batch(const BigPlan &bp, const ColArgs &a, const RealBatch &jb, hip
     const int nbatch = jb.b0[jb.n];
     if (nbatch <= 0) return 0;
     if (!bp.ok || a.op != COL_FWD_INPUT || nbatch > 65535) return -1;
-    const size_t lds = sizeof(cf32) * 2 * kBigCols * (size_t)(bp.n1 + 1) + sizeof(float) * 8;
-    allow_lds((const void *)k_big_cols<true>, lds);
+    return launch_cols<true>(bp, a, jb, nbatch, st, name);
+}
+
+template <bool BATCH, int CTN2>
+static int launch_rows_ct(const BigPlan &bp, const RowArgs &a, const RealBatch &jb, int nbatch, hipStream_t st,
+                          const char *name, const cf32 *twc)
+{
+    const size_t lds = sizeof(cf32) * kBigRows * (size_t)ct_row_ld(CTN2);
     ProfScope prof(name, st);
-    hipLaunchKernelGGL(k_big_cols<true>, dim3((bp.n2 + kBigCols - 1) / kBigCols, nbatch), dim3(256), lds, st, a,
-                       bp.n1, bp.n2, bp.p1, bp.tw1, bp.twN, jb);
+    hipLaunchKernelGGL((k_big_rows<BATCH, CTN2>), dim3((bp.n1 + kBigRows - 1) / kBigRows, nbatch), dim3(256), lds,
+                       st, a, bp.n1, bp.n2, bp.p2, twc, bp.twN, jb);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <bool BATCH>
+static int launch_rows(const BigPlan &bp, const RowArgs &a, const RealBatch &jb, int nbatch, hipStream_t st,
+                       const char *name)
+{
+    if (bp.n2 == 512 || bp.n2 == 1024) {  // compile-time in-place row transforms, one LDS buffer
+        const cf32 *twc = (const cf32 *)(bp.n2 == 512 ? get_order_twiddles(512, {8, 8, 8})
+                                                      : get_order_twiddles(1024, {16, 8, 8}));
+        if (!twc) return -1;
+        return bp.n2 == 512 ? launch_rows_ct<BATCH, 512>(bp, a, jb, nbatch, st, name, twc)
+                            : launch_rows_ct<BATCH, 1024>(bp, a, jb, nbatch, st, name, twc);
+    }
+    const size_t lds = sizeof(cf32) * 2 * kBigRows * (size_t)bp.n2;
+    allow_lds((const void *)k_big_rows<BATCH, 0>, lds);
+    ProfScope prof(name, st);
+    hipLaunchKernelGGL((k_big_rows<BATCH, 0>), dim3((bp.n1 + kBigRows - 1) / kBigRows, nbatch), dim3(256), lds, st,
+                       a, bp.n1, bp.n2, bp.p2, bp.tw2, bp.twN, jb);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }
@@ -539,14 +675,8 @@ int run_rows(const BigPlan &bp, const RowArgs &a, int nbatch, hipStream_t st, co
         fprintf(stderr, "fastpcc: %d transforms exceed one launch\n", nbatch);
         return -1;
     }
-    const size_t lds = sizeof(cf32) * 2 * kBigRows * (size_t)bp.n2;
-    allow_lds((const void *)k_big_rows<false>, lds);
-    ProfScope prof(name, st);
     RealBatch none{};
-    hipLaunchKernelGGL(k_big_rows<false>, dim3((bp.n1 + kBigRows - 1) / kBigRows, nbatch), dim3(256), lds, st, a,
-                       bp.n1, bp.n2, bp.p2, bp.tw2, bp.twN, none);
-    FPCC_CHECK(hipGetLastError());
-    return 0;
+    return launch_rows<false>(bp, a, none, nbatch, st, name);
 }
 
 int run_rows_batch(const BigPlan &bp, const RowArgs &a, const RealBatch &jb, hipStream_t st, const char *name)
@@ -554,13 +684,7 @@ int run_rows_batch(const BigPlan &bp, const RowArgs &a, const RealBatch &jb, hip
     const int nbatch = jb.b0[jb.n];
     if (nbatch <= 0) return 0;
     if (!bp.ok || a.op != ROW_FWD_PLAIN || nbatch > 65535) return -1;
-    const size_t lds = sizeof(cf32) * 2 * kBigRows * (size_t)bp.n2;
-    allow_lds((const void *)k_big_rows<true>, lds);
-    ProfScope prof(name, st);
-    hipLaunchKernelGGL(k_big_rows<true>, dim3((bp.n1 + kBigRows - 1) / kBigRows, nbatch), dim3(256), lds, st, a,
-                       bp.n1, bp.n2, bp.p2, bp.tw2, bp.twN, jb);
-    FPCC_CHECK(hipGetLastError());
-    return 0;
+    return launch_rows<true>(bp, a, jb, nbatch, st, name);
 }
 
 int tune_split(int n, int batch, int reps, int *best_n1, double *best_ms, int *heur_n1, double *heur_ms)

@@ -194,7 +194,9 @@ k_gram(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, int W2
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int li = lane & 15, q = lane >> 4;
     const int ti = wave & 3, cj = wave >> 2;
-    const bool active = !(bi == bj && 2 * cj + 1 < ti);  // diagonal block: skip sub-blocks below it
+    // skip sub-blocks below a diagonal block's diagonal, and those whose rows or
+    // columns are all past the last station
+    const bool active = !(bi == bj && 2 * cj + 1 < ti) && bi * kGB + 16 * ti < S && bj * kGB + 32 * cj < S;
     const int nkc = (K + kGK - 1) / kGK;
     const int nsteps = kGF * nkc;
     const int nper = same ? kGB * kGK / kGThreads : 2 * kGB * kGK / kGThreads;  // elements per thread
@@ -391,7 +393,7 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
     const int li = lane & 15, q = lane >> 4;
     const int ti = wave & 3, cj = wave >> 2;
-    const bool active = !(bi == bj && 2 * cj + 1 < ti);
+    const bool active = !(bi == bj && 2 * cj + 1 < ti) && bi * kGB + 16 * ti < S && bj * kGB + 32 * cj < S;
     const int su = dma_row_units(K), upr = K / 2;
     const int nchunk64 = (same ? kGB : 2 * kGB) * su / 64;  // 64-unit pieces per bin (rows are 64 per side)
     const int bufb = 2 * kGB * su * 16;                     // bytes per staging buffer
