@@ -83,7 +83,7 @@ static BigPlan build_big_plan(int n, int n1)
     b.p2 = c.plan;
     b.tw1 = (const cf32 *)a.tw;
     b.tw2 = (const cf32 *)c.tw;
-    b.twN = (const cf32 *)get_full_twiddles(n, false);
+    b.twN = (const cf32 *)get_split_twiddles(n);
     b.ok = b.twN != nullptr;
     return b;
 }
@@ -114,7 +114,13 @@ __device__ __forceinline__ void pair_of_big(int p, int S, int &i, int &j)
 }
 
 // index of W_n^{c r} in the full table: c < N2 and r < N1, so c*r < n
-__device__ __forceinline__ int twidx(int c, int r) { return c * r; }
+// W_n^{c r} from the two-factor table of get_split_twiddles (c r < n): two small
+// cached lookups instead of a gather over an n-entry table
+__device__ __forceinline__ cf32 tw_cr(const cf32 *__restrict__ twN, int c, int r)
+{
+    const unsigned m = (unsigned)c * (unsigned)r;
+    return cmul(twN[kTwLo + m / kTwLo], twN[m % kTwLo]);
+}
 
 // Job-table launch (BATCH): batch index b runs over the station-windows of several
 // jobs; job j (jb.b0[j] <= b < jb.b0[j+1]) supplies x, its stride, W, N and the
@@ -316,7 +322,7 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
         const long long g = (long long)r * N2 + c;
         cf32 v = res[cix(t, r)];
         if (fwd_out) {
-            a.dst[base + g] = cmul(v, twN[twidx(c, r)]);  // W_n^{n2 k1}
+            a.dst[base + g] = cmul(v, tw_cr(twN, c, r));  // W_n^{n2 k1}
         } else if (op == COL_INV_CHIRP_MAX) {
             if (g < a.N) {
                 const cf32 d = cmul(a.chirp[g], v);
@@ -455,7 +461,7 @@ k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, 
         const long long g = (long long)r * N2 + c;
         cf32 v = res[rix(rr, c)];
         if (inv_out) {
-            v = cmulc(twN[twidx(c, r)], v);  // W_n^{-n2 k1}
+            v = cmulc(tw_cr(twN, c, r), v);  // W_n^{-n2 k1}
         } else if (op == ROW_FWD_SCALE) {
             v = cf32{v.x * sc, v.y * sc};
         }

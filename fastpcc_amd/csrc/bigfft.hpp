@@ -17,7 +17,7 @@ constexpr int kBigRows = 4;   // rows per workgroup
 struct BigPlan {
     int n = 0, n1 = 0, n2 = 0;
     FftPlan p1{}, p2{};
-    const cf32 *tw1 = nullptr, *tw2 = nullptr, *twN = nullptr;
+    const cf32 *tw1 = nullptr, *tw2 = nullptr, *twN = nullptr;  // twN: get_split_twiddles(n)
     bool ok = false;
 };
 

@@ -36,6 +36,11 @@ const DevPlan &get_plan(int n, bool f64);
 const void *get_c2r_twiddles(int nz, bool f64);
 // Table of exp(-2*pi*i*k/n), k in [0, n) (cached per device and precision).
 const void *get_full_twiddles(int n, bool f64);
+// FP32 exp(-2*pi*i*m/n) for m in [0, n) in two factors: entries [0, kTwLo) hold
+// W^i, entries kTwLo + j hold W^{kTwLo j}; W^m = T[kTwLo + m / kTwLo] T[m % kTwLo]
+// (cached per device).
+constexpr int kTwLo = 512;
+const void *get_split_twiddles(int n);
 // Per-pass twiddles of an FP32 plan with the given radix order (the compile-time
 // FFT kernels fix their order; cached per device), nullptr if the order is invalid.
 const void *get_order_twiddles(int n, const std::vector<int> &radices);

@@ -233,6 +233,34 @@ const void *get_order_twiddles(int n, const std::vector<int> &radices)
     return d;
 }
 
+const void *get_split_twiddles(int n)
+{
+    static std::map<std::pair<int, int>, const void *> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = cache.find({dev, n});
+    if (it != cache.end()) return it->second;
+    std::vector<double> t;
+    const int hi = (n + kTwLo - 1) / kTwLo;
+    t.reserve(2 * (size_t)(kTwLo + hi));
+    for (int k = 0; k < kTwLo; ++k) {
+        double c, s;
+        tw_fwd(k, n, c, s);
+        t.push_back(c);
+        t.push_back(s);
+    }
+    for (int j = 0; j < hi; ++j) {
+        double c, s;
+        tw_fwd((long)kTwLo * j, n, c, s);
+        t.push_back(c);
+        t.push_back(s);
+    }
+    const void *d = upload_table<float>(t);
+    cache.emplace(std::make_pair(dev, n), d);
+    return d;
+}
+
 const void *get_full_twiddles(int n, bool f64)
 {
     static std::map<std::tuple<int, int, bool>, const void *> cache;
