@@ -157,6 +157,10 @@ def main():
     ap.add_argument("--no-host", action="store_true", help="skip the PCIe-inclusive host-buffer timing")
     ap.add_argument("--no-pre", action="store_true", help="skip the preprocessing timing")
     ap.add_argument("--no-wpcc", action="store_true", help="skip the WPCC all-pairs timing (configuration 4)")
+    ap.add_argument("--no-kinds", action="store_true",
+                    help="skip the other correlation kinds at configuration 2 (64 stations)")
+    ap.add_argument("--no-paper", action="store_true",
+                    help="skip the reference's published workload (one pair, 649 day windows of 21600)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="before the warm-up, one fill and one copy of this many MiB (the known-size kernels "
                          "that scale FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -366,9 +370,92 @@ def main():
                                              "reference), one core"}
         variants["wpcc_c4"] = ent
         del wr, xwd
+    if not args.no_kinds:
+        # configuration 2: 64 stations x 24 windows of 3600, lags +-1000, every kind, mean stack
+        # (PCC-v with v = 1.5: the reference routes v = 1 and 2 to PCC1 and PCC2)
+        S2, W2c, N2c, l2 = 64, 24, 3600, 1000
+        x2, _ = synth.make_stations(S2, W2c * N2c, 1000, seed=2024)
+        x2d = torch.from_numpy(x2).to(dev)
+        P2 = S2 * (S2 - 1) // 2
+        kinds = {}
+        for kind, kst, kv, nst in (("pcc2", "gram16", 2.0, 10), ("gncc", "plain", 2.0, 10), ("cc1b", "gram16", 2.0, 10),
+                                   ("pcc1", "plain", 2.0, 2), ("pccv", "plain", 1.5, 1)):
+            kj = fastpcc_amd.make_job(kind, S2, W2c, N2c, -l2, l2, stack=kst, v=kv)
+            kr = fastpcc_amd.DeviceRunner(kj, dev)
+            kr.run(x2d, stream)
+            torch.cuda.synchronize()
+            ta = time.perf_counter()
+            for _ in range(nst):
+                kr.run(x2d, stream)
+            torch.cuda.synchronize()
+            kms = (time.perf_counter() - ta) * 1e3 / nst
+            ent = {"stack": kst, "ms_per_step": round(kms, 4), "value": P2 * W2c / (kms / 1e3), "steps": nst}
+            if kind == "pccv":
+                ent["v"] = kv
+            if not args.no_check:
+                import oracle
+                yk = kr.y.view(P2, 2 * l2 + 1).cpu().numpy()[:2]
+                y64 = oracle.allpairs(kind, x2, W2c, N2c, -l2, l2, stacked=True, pair_end=2, f64=True, v=kv)
+                ent["max_abs_err_vs_fp64"] = float(np.abs(yk - y64).max())
+            kinds[kind] = ent
+            del kr
+        variants["kinds_c2"] = {"unit": "pair-correlations/s", "kinds": kinds,
+                                "workload": "64 stations x 86400 samples, 24 windows of 3600, nfft 8192, lags "
+                                            "+-1000, all 2016 pairs, mean stack"}
+        del x2d
+    if not args.no_paper:
+        # the reference's published benchmark (SRL 2019, doc/Ventosa_S_2019_j-srl_tow_pldvPCC.pdf p.4): 649
+        # day-long windows of one station pair, N = 21600 (dt = 4 s), lags +-12000 s = +-3000 samples, per-window
+        # output; PCC1 took 71.65 s on a 4-core i5-4690K and 0.70 s GPU-enhanced (GTX 1060), PCC2 and 1-bit
+        # GNCC on the CPU 1.79x and 4.17x faster than that GPU run.  Here through the reference's own entry
+        # points (host buffers in and out; the library keeps their staging between calls), and
+        # device-resident (inputs already in HBM, the pair's 649 windows as one job).
+        Tw, Nw, lp = 649, 21600, 3000
+        xa, _ = synth.make_stations(2, Tw * Nw, 1000, seed=649)
+        x1p, x2p = xa[0].reshape(Tw, Nw), xa[1].reshape(Tw, Nw)
+        published = {"pcc1": {"gpu_gtx1060_s": 0.70, "cpu_i5_4690k_s": 71.65},
+                     "pcc2": {"cpu_i5_4690k_s": round(0.70 / 1.79, 3)}, "cc1b": {"cpu_i5_4690k_s": round(0.70 / 4.17, 3)}}
+        paper = {}
+        for kind, fn in (("pcc1", fastpcc_amd.pcc1_set), ("pcc2", fastpcc_amd.pcc2_set), ("cc1b", fastpcc_amd.cc1b_set)):
+            fn(x1p[:8], x2p[:8], -lp, lp)  # plans and code objects
+            torch.cuda.synchronize()
+            ta = time.perf_counter()
+            fn(x1p, x2p, -lp, lp)  # first call at this size: grows the entry points' kept buffers
+            first_s = time.perf_counter() - ta
+            ta = time.perf_counter()
+            yp = fn(x1p, x2p, -lp, lp)
+            host_s = time.perf_counter() - ta
+            pj = fastpcc_amd.make_job(kind, 2, Tw, Nw, -lp, lp, stack="none")
+            pr = fastpcc_amd.DeviceRunner(pj, dev)
+            xpd = torch.from_numpy(xa).to(dev)
+            pr.run(xpd, stream)
+            torch.cuda.synchronize()
+            ta = time.perf_counter()
+            for _ in range(3):
+                pr.run(xpd, stream)
+            torch.cuda.synchronize()
+            dev_s = (time.perf_counter() - ta) / 3
+            ent = {"entry_point_s": round(host_s, 4), "first_call_s": round(first_s, 4),
+                   "device_resident_s": round(dev_s, 5), "published": published[kind]}
+            for k, ref_s in published[kind].items():
+                ent["speedup_vs_" + k.rsplit("_", 1)[0]] = round(ref_s / host_s, 1)
+            if not args.no_check:
+                import oracle
+                yr = oracle.run(kind, x1p[:2], x2p[:2], -lp, lp)
+                ent["max_abs_err_vs_oracle"] = float(np.abs(np.asarray(yp)[:2] - yr).max())
+                ent["max_abs_diff_vs_device_resident"] = float(
+                    np.abs(np.asarray(yp) - pr.y.view(Tw, 2 * lp + 1).cpu().numpy()).max())
+            paper[kind] = ent
+            del pr, xpd
+        variants["paper_649_windows"] = {
+            "unit": "s", "paper": paper,
+            "workload": "one station pair, 649 windows of 21600 samples (synthetic, the published run's shape), "
+                        "lags +-3000, per-window output; entry_point_s = pcc1_set / pcc2_set / cc1b_set from host "
+                        "buffers to host output, a repeated call; first_call_s = the first call at this size, which "
+                        "also allocates the pinned and device buffers the entry points keep"}
     if not args.no_host:
         # PCIe-inclusive: host input in, host output out, through fpcc_allpairs
-        # (device buffers allocated per call, pageable host arrays)
+        # (pageable host arrays)
         fastpcc_amd.allpairs(x, W, N, -lag, lag, kind="pcc2", stack=args.stack)
         ta = time.perf_counter()
         for _ in range(3):
@@ -377,7 +464,7 @@ def main():
         variants["host_buffers_" + args.stack] = {
             "ms_per_step": hms, "value": P * W / (hms / 1e3), "steps": 3,
             "note": "PCIe-inclusive: fpcc_allpairs from pageable host input to host output "
-                    f"({x.nbytes / 1e6:.1f} MB in, {P * L * 4 / 1e6:.1f} MB out, device buffers allocated per call)"}
+                    f"({x.nbytes / 1e6:.1f} MB in, {P * L * 4 / 1e6:.1f} MB out)"}
     speedup = None
     if "plain" in variants:
         speedup = round(value / variants["plain"]["value"], 2)

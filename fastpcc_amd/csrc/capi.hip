@@ -4,6 +4,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <vector>
 #include "../../include/fastpcc.h"
 #include "bigfft.hpp"
@@ -31,6 +34,122 @@ struct HostBuf {
         if (p) (void)hipHostFree(p);
     }
 };
+
+// Buffers of the host entry points (the reference-style *_set calls): pinned
+// input and output staging, device input, output and workspace.  Pinned staging
+// of at most kKeepHost bytes and device buffers of at most kKeepDevice bytes are
+// kept per device between calls (grown to the largest such call, held until the
+// process ends; one call at a time per device); larger ones are allocated and
+// freed by the call.
+constexpr size_t kKeepHost = (size_t)1 << 30, kKeepDevice = (size_t)8 << 30;
+
+struct Kept {
+    std::mutex mu;
+    void *hx = nullptr, *hy = nullptr, *dx = nullptr, *dy = nullptr, *dws = nullptr;
+    size_t nhx = 0, nhy = 0, ndx = 0, ndy = 0, ndws = 0;
+};
+
+Kept &kept_for_device()
+{
+    static std::mutex mu;
+    static std::map<int, std::unique_ptr<Kept>> m;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(mu);
+    auto &k = m[dev];
+    if (!k) k.reset(new Kept);
+    return *k;
+}
+
+bool grow(void *&p, size_t &cap, size_t need, bool host)
+{
+    if (need <= cap) return true;
+    if (p) (void)(host ? hipHostFree(p) : hipFree(p));
+    p = nullptr;
+    cap = 0;
+    if ((host ? hipHostMalloc(&p, need, hipHostMallocDefault) : hipMalloc(&p, need)) != hipSuccess) {
+        p = nullptr;
+        return false;
+    }
+    cap = need;
+    return true;
+}
+
+class EntryBufs {
+public:
+    void *hx = nullptr, *hy = nullptr, *dx = nullptr, *dy = nullptr, *dws = nullptr;
+
+    // sizes in bytes (0: not needed); false if an allocation fails
+    bool get(size_t bhx, size_t bhy, size_t bdx, size_t bdy, size_t bdws)
+    {
+        const bool keep_h = bhx + bhy <= kKeepHost, keep_d = bdx + bdy + bdws <= kKeepDevice;
+        if (keep_h || keep_d) {
+            k_ = &kept_for_device();
+            lk_ = std::unique_lock<std::mutex>(k_->mu);
+        }
+        if (keep_h) {
+            if (!grow(k_->hx, k_->nhx, bhx, true) || !grow(k_->hy, k_->nhy, bhy, true)) return false;
+            hx = bhx ? k_->hx : nullptr;
+            hy = bhy ? k_->hy : nullptr;
+        } else {
+            if ((bhx && hipHostMalloc(&ohx_.p, bhx, hipHostMallocDefault) != hipSuccess) ||
+                (bhy && hipHostMalloc(&ohy_.p, bhy, hipHostMallocDefault) != hipSuccess))
+                return false;
+            hx = ohx_.p;
+            hy = ohy_.p;
+        }
+        if (keep_d) {
+            if (!grow(k_->dx, k_->ndx, bdx, false) || !grow(k_->dy, k_->ndy, bdy, false) ||
+                !grow(k_->dws, k_->ndws, bdws, false))
+                return false;
+            dx = bdx ? k_->dx : nullptr;
+            dy = bdy ? k_->dy : nullptr;
+            dws = bdws ? k_->dws : nullptr;
+        } else {
+            if ((bdx && hipMalloc(&odx_.p, bdx) != hipSuccess) || (bdy && hipMalloc(&ody_.p, bdy) != hipSuccess) ||
+                (bdws && hipMalloc(&odws_.p, bdws) != hipSuccess))
+                return false;
+            dx = odx_.p;
+            dy = ody_.p;
+            dws = odws_.p;
+        }
+        return true;
+    }
+
+private:
+    Kept *k_ = nullptr;
+    std::unique_lock<std::mutex> lk_;
+    HostBuf ohx_, ohy_;
+    DeviceBuf odx_, ody_, odws_;
+};
+
+// Rows of two stations (x1 rows, then x2 rows, each of `row` floats) gathered
+// into pinned staging and copied to the device in pieces of about 8 MB on the
+// null stream, so the gather of one piece overlaps the copy of the last.
+// `before(tr)` runs before row tr of either station is read.  -1 on a null row.
+template <typename Before>
+int upload_rows(const float *const *x1, const float *const *x2, unsigned Tr, size_t row, float *hx, float *dx,
+                Before before)
+{
+    for (unsigned tr = 0; tr < Tr; ++tr)
+        if (!x1[tr] || !x2[tr]) return -1;
+    const size_t piece = std::max<size_t>(row, ((size_t)8 << 20) / sizeof(float));
+    size_t done = 0, filled = 0;
+    for (unsigned k = 0; k < 2 * Tr; ++k) {
+        const unsigned tr = k < Tr ? k : k - Tr;
+        if (k < Tr) before(tr);
+        const float *src = k < Tr ? x1[tr] : x2[tr];
+        if (!src) return -1;
+        memcpy(hx + filled, src, sizeof(float) * row);
+        filled += row;
+        if (filled - done >= piece || k + 1 == 2 * Tr) {
+            FPCC_CHECK(hipMemcpyAsync(dx + done, hx + done, sizeof(float) * (filled - done), hipMemcpyHostToDevice,
+                                      nullptr));
+            done = filled;
+        }
+    }
+    return 0;
+}
 
 // Resolved sizes of a job.
 struct Dims {
@@ -298,25 +417,18 @@ int wpcc_pair_host(const WFamily &f, float **y, float **x1, float **x2, unsigned
     if (!y || !x1 || !x2) return -1;
     const size_t xlen = (size_t)2 * Tr * N, ylen = (size_t)Tr * L;
     const size_t wsb = wpcc_workspace(2, (int)Tr, (int)N, (int)f.Nz, (int)f.S);
-    HostBuf hx, hy;
-    DeviceBuf dx, dy, dws;
-    FPCC_CHECK(hipHostMalloc(&hx.p, sizeof(float) * xlen, hipHostMallocDefault));
-    FPCC_CHECK(hipHostMalloc(&hy.p, sizeof(float) * ylen, hipHostMallocDefault));
-    for (unsigned tr = 0; tr < Tr; ++tr) {
+    for (unsigned tr = 0; tr < Tr; ++tr)
         if (!x1[tr] || !x2[tr] || !y[tr]) return -1;
-        memcpy((float *)hx.p + (size_t)tr * N, x1[tr], sizeof(float) * N);
-        memcpy((float *)hx.p + (size_t)(Tr + tr) * N, x2[tr], sizeof(float) * N);
-    }
-    FPCC_CHECK(hipMalloc(&dx.p, sizeof(float) * xlen));
-    FPCC_CHECK(hipMalloc(&dy.p, sizeof(float) * ylen));
-    FPCC_CHECK(hipMalloc(&dws.p, wsb));
-    FPCC_CHECK(hipMemcpy(dx.p, hx.p, sizeof(float) * xlen, hipMemcpyHostToDevice));
-    int r = wpcc_station_stage((const float *)dx.p, (long long)Tr * N, 2, (int)Tr, (int)N, f, (char *)dws.p, nullptr);
+    EntryBufs b;
+    if (!b.get(sizeof(float) * xlen, sizeof(float) * ylen, sizeof(float) * xlen, sizeof(float) * ylen, wsb)) return -1;
+    int r = upload_rows(x1, x2, Tr, N, (float *)b.hx, (float *)b.dx, [](unsigned) {});
     if (r) return r;
-    if ((r = wpcc_pair_stage(2, (int)Tr, (int)N, f, lag0, L, true, (char *)dws.p, (float *)dy.p, nullptr)))
+    r = wpcc_station_stage((const float *)b.dx, (long long)Tr * N, 2, (int)Tr, (int)N, f, (char *)b.dws, nullptr);
+    if (r) return r;
+    if ((r = wpcc_pair_stage(2, (int)Tr, (int)N, f, lag0, L, true, (char *)b.dws, (float *)b.dy, nullptr)))
         return r;
-    FPCC_CHECK(hipMemcpy(hy.p, dy.p, sizeof(float) * ylen, hipMemcpyDeviceToHost));
-    for (unsigned tr = 0; tr < Tr; ++tr) memcpy(y[tr], (float *)hy.p + (size_t)tr * L, sizeof(float) * L);
+    FPCC_CHECK(hipMemcpy(b.hy, b.dy, sizeof(float) * ylen, hipMemcpyDeviceToHost));
+    for (unsigned tr = 0; tr < Tr; ++tr) memcpy(y[tr], (float *)b.hy + (size_t)tr * L, sizeof(float) * L);
     return 0;
 }
 
@@ -339,48 +451,46 @@ int pcc_time_host(int pcc1, float **y, const float *const *x1, const float *cons
     const int l1 = (Lag1 >= -N) ? 0 : -(Lag1 + N);
     const size_t cw = complex_in ? 2 : 1;
     const size_t xlen = (size_t)2 * Tr * N * cw, ylen = (size_t)Tr * L;
-    HostBuf hx, hy;
-    DeviceBuf dx, dy, dphi;
-    FPCC_CHECK(hipHostMalloc(&hx.p, sizeof(float) * xlen, hipHostMallocDefault));
-    FPCC_CHECK(hipHostMalloc(&hy.p, sizeof(float) * ylen, hipHostMallocDefault));
-    for (unsigned tr = 0; tr < Tr; ++tr) {
-        if (anok) sem_wait(&anok[tr]);
+    for (unsigned tr = 0; tr < Tr; ++tr)
         if (!x1[tr] || !x2[tr] || !y[tr]) return -1;
-        memcpy((float *)hx.p + (size_t)tr * N * cw, x1[tr], sizeof(float) * N * cw);
-        memcpy((float *)hx.p + (size_t)(Tr + tr) * N * cw, x2[tr], sizeof(float) * N * cw);
-    }
-    FPCC_CHECK(hipMalloc(&dx.p, sizeof(float) * xlen));
-    FPCC_CHECK(hipMalloc(&dy.p, sizeof(float) * ylen));
-    FPCC_CHECK(hipMemcpy(dx.p, hx.p, sizeof(float) * xlen, hipMemcpyHostToDevice));
-    FPCC_CHECK(hipMemset(dy.p, 0, sizeof(float) * ylen));
-    int r;
-    const cf32 *phi;
     FftPlan fpl;
-    if (complex_in) {
-        if ((r = launch_ampnormf_rows((cf32 *)dx.p, 2 * (int)Tr, N, nullptr))) return r;
-        phi = (const cf32 *)dx.p;
-    } else if (N <= 8192 && factor_plan(N, fpl)) {
-        FPCC_CHECK(hipMalloc(&dphi.p, sizeof(cf32) * 2 * Tr * (size_t)N));
-        if ((r = launch_station_phasor((const float *)dx.p, (long long)Tr * N, 2, Tr, N, (cf32 *)dphi.p, nullptr)))
-            return r;
-        phi = (const cf32 *)dphi.p;
-    } else {
-        // the phasors lead the large-window workspace (rows of N complex values)
-        if (!large_supported(FPCC_PCC1, N, N)) {
-            fprintf(stderr, "fastpcc: no analytic-signal plan for N=%d\n", N);
-            return -1;
-        }
-        FPCC_CHECK(hipMalloc(&dphi.p, large_workspace(FPCC_PCC1, FPCC_STACK_NONE, 2, (int)Tr, N, N)));
-        if ((r = large_station_stage(FPCC_PCC1, (const float *)dx.p, (long long)Tr * N, 2, (int)Tr, N, N,
-                                     FPCC_STACK_NONE, (char *)dphi.p, nullptr)))
-            return r;
-        phi = (const cf32 *)dphi.p;
+    const bool small = N <= 8192 && factor_plan(N, fpl);
+    if (!complex_in && !small && !large_supported(FPCC_PCC1, N, N)) {
+        fprintf(stderr, "fastpcc: no analytic-signal plan for N=%d\n", N);
+        return -1;
     }
-    if ((r = launch_pcc_time(phi, 2, Tr, N, Lag1 + l1, Lp - l1, v, pcc1, 0, (float *)dy.p, L, l1, nullptr)))
+    // phasors: in place (analytic input), rows of N complex values, or leading the
+    // large-window workspace
+    const size_t bphi = complex_in ? 0
+                        : small    ? sizeof(cf32) * 2 * Tr * (size_t)N
+                                   : large_workspace(FPCC_PCC1, FPCC_STACK_NONE, 2, (int)Tr, N, N);
+    EntryBufs b;
+    if (!b.get(sizeof(float) * xlen, sizeof(float) * ylen, sizeof(float) * xlen, sizeof(float) * ylen, bphi))
+        return -1;
+    int r = upload_rows(x1, x2, Tr, (size_t)N * cw, (float *)b.hx, (float *)b.dx, [&](unsigned tr) {
+        if (anok) sem_wait(&anok[tr]);
+    });
+    if (r) return r;
+    FPCC_CHECK(hipMemsetAsync(b.dy, 0, sizeof(float) * ylen, nullptr));
+    const cf32 *phi;
+    if (complex_in) {
+        if ((r = launch_ampnormf_rows((cf32 *)b.dx, 2 * (int)Tr, N, nullptr))) return r;
+        phi = (const cf32 *)b.dx;
+    } else if (small) {
+        if ((r = launch_station_phasor((const float *)b.dx, (long long)Tr * N, 2, Tr, N, (cf32 *)b.dws, nullptr)))
+            return r;
+        phi = (const cf32 *)b.dws;
+    } else {
+        if ((r = large_station_stage(FPCC_PCC1, (const float *)b.dx, (long long)Tr * N, 2, (int)Tr, N, N,
+                                     FPCC_STACK_NONE, (char *)b.dws, nullptr)))
+            return r;
+        phi = (const cf32 *)b.dws;
+    }
+    if ((r = launch_pcc_time(phi, 2, Tr, N, Lag1 + l1, Lp - l1, v, pcc1, 0, (float *)b.dy, L, l1, nullptr)))
         return r;
-    FPCC_CHECK(hipMemcpy(hy.p, dy.p, sizeof(float) * ylen, hipMemcpyDeviceToHost));
+    FPCC_CHECK(hipMemcpy(b.hy, b.dy, sizeof(float) * ylen, hipMemcpyDeviceToHost));
     for (unsigned tr = 0; tr < Tr; ++tr) {
-        const float *src = (float *)hy.p + (size_t)tr * L;
+        const float *src = (float *)b.hy + (size_t)tr * L;
         if (zero_rows) memcpy(y[tr], src, sizeof(float) * L);
         else if (Lp > l1) memcpy(y[tr] + l1, src + l1, sizeof(float) * (Lp - l1));
     }
@@ -410,24 +520,16 @@ int pair_set_host(int kind, float **y, float **x1, float **x2, unsigned N, unsig
     int r = resolve(&job, d);
     if (r) return r;
     const size_t xlen = (size_t)2 * Tr * N, ylen = (size_t)Tr * d.L;
-    HostBuf hx, hy;
-    DeviceBuf dx, dy, dws;
-    FPCC_CHECK(hipHostMalloc(&hx.p, sizeof(float) * xlen, hipHostMallocDefault));
-    FPCC_CHECK(hipHostMalloc(&hy.p, sizeof(float) * ylen, hipHostMallocDefault));
-    for (unsigned tr = 0; tr < Tr; ++tr) {
+    for (unsigned tr = 0; tr < Tr; ++tr)
         if (!x1[tr] || !x2[tr] || !y[tr]) return -1;
-        memcpy((float *)hx.p + (size_t)tr * N, x1[tr], sizeof(float) * N);
-        memcpy((float *)hx.p + (size_t)(Tr + tr) * N, x2[tr], sizeof(float) * N);
-    }
     const size_t wsb = layout(&job, d).total;
-    FPCC_CHECK(hipMalloc(&dx.p, sizeof(float) * xlen));
-    FPCC_CHECK(hipMalloc(&dy.p, sizeof(float) * ylen));
-    FPCC_CHECK(hipMalloc(&dws.p, wsb));
-    FPCC_CHECK(hipMemcpy(dx.p, hx.p, sizeof(float) * xlen, hipMemcpyHostToDevice));
-    if ((r = station_stage(&job, d, (const float *)dx.p, (char *)dws.p, nullptr))) return r;
-    if ((r = pair_stage(&job, d, (char *)dws.p, (float *)dy.p, nullptr))) return r;
-    FPCC_CHECK(hipMemcpy(hy.p, dy.p, sizeof(float) * ylen, hipMemcpyDeviceToHost));
-    for (unsigned tr = 0; tr < Tr; ++tr) memcpy(y[tr], (float *)hy.p + (size_t)tr * d.L, sizeof(float) * d.L);
+    EntryBufs b;
+    if (!b.get(sizeof(float) * xlen, sizeof(float) * ylen, sizeof(float) * xlen, sizeof(float) * ylen, wsb)) return -1;
+    if ((r = upload_rows(x1, x2, Tr, N, (float *)b.hx, (float *)b.dx, [](unsigned) {}))) return r;
+    if ((r = station_stage(&job, d, (const float *)b.dx, (char *)b.dws, nullptr))) return r;
+    if ((r = pair_stage(&job, d, (char *)b.dws, (float *)b.dy, nullptr))) return r;
+    FPCC_CHECK(hipMemcpy(b.hy, b.dy, sizeof(float) * ylen, hipMemcpyDeviceToHost));
+    for (unsigned tr = 0; tr < Tr; ++tr) memcpy(y[tr], (float *)b.hy + (size_t)tr * d.L, sizeof(float) * d.L);
     return 0;
 }
 
@@ -627,14 +729,12 @@ extern "C" int fpcc_allpairs(const fpcc_job *job, const float *h_x, float *h_y)
     if (!h_x || !h_y) return -1;
     const size_t ylen = fpcc_output_len(job), ws = layout(job, d).total;
     const size_t xlen = (size_t)(job->n_stations - 1) * job->station_stride + (size_t)job->n_windows * job->win_len;
-    DeviceBuf dx, dy, dws;
-    FPCC_CHECK(hipMalloc(&dx.p, sizeof(float) * xlen));
-    FPCC_CHECK(hipMalloc(&dy.p, sizeof(float) * ylen));
-    FPCC_CHECK(hipMalloc(&dws.p, ws));
-    FPCC_CHECK(hipMemcpy(dx.p, h_x, sizeof(float) * xlen, hipMemcpyHostToDevice));
-    r = fpcc_allpairs_device(job, (const float *)dx.p, (float *)dy.p, dws.p, ws, nullptr);
+    EntryBufs b;
+    if (!b.get(0, 0, sizeof(float) * xlen, sizeof(float) * ylen, ws)) return -1;
+    FPCC_CHECK(hipMemcpy(b.dx, h_x, sizeof(float) * xlen, hipMemcpyHostToDevice));
+    r = fpcc_allpairs_device(job, (const float *)b.dx, (float *)b.dy, b.dws, ws, nullptr);
     if (r) return r;
-    FPCC_CHECK(hipMemcpy(h_y, dy.p, sizeof(float) * ylen, hipMemcpyDeviceToHost));
+    FPCC_CHECK(hipMemcpy(h_y, b.dy, sizeof(float) * ylen, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -70,3 +70,20 @@ def test_allpairs_large(fp, orc, kind, stack):
     err = np.abs(y - yr).max()
     print(f"{kind}/{stack} N={N} max|gpu-oracle|={err:.3e}")
     assert y.shape == yr.shape and err <= TOL
+
+
+@pytest.mark.parametrize("kind,N,lag", [("pcc2", 140000, 3000), ("gncc", 140000, 3000), ("cc1b", 140000, 3000),
+                                        ("gncc", 86400, 2000), ("pcc2", 300000, 1000)])
+def test_compile_time_four_step_splits(fp, orc, kind, N, lag):
+    """Transform lengths whose splits take the in-place column (256 points) and
+    row (512 / 1024 points) passes: PCC2 at Nz 262144 (256 x 1024) and 524288
+    (256 x 2048, generic rows), GNCC / 1-bit half lengths 131072 (256 x 512) and
+    65536 (128 x 512)."""
+    x1, x2 = _rand(1, N, N + lag)
+    fn = {"pcc2": fp.pcc2_set, "gncc": fp.ccgn_set, "cc1b": fp.cc1b_set}[kind]
+    print(f"{kind} N={N} lags +-{lag}", flush=True)
+    y = fn(x1, x2, -lag, lag)
+    yr = orc.run(kind, x1, x2, -lag, lag)
+    err = np.abs(y - yr).max()
+    print(f"  max|gpu-oracle|={err:.3e}")
+    assert err <= TOL
