@@ -33,6 +33,17 @@ __device__ __forceinline__ void pair_of_t(int p, int S, int &i, int &j)
 }
 
 // y rows start at out_off within rows of out_ld floats.
+// Hardware square root and power (v_sqrt_f32; v_log_f32 and v_exp_f32, i.e.
+// 2^(V log2 x)), about 1 ulp and a few ulp per term, as the reference's CUDA
+// engine compiles its terms (fast math); the correctly rounded library versions
+// cost several times more instructions.  Sums over N terms of either sign keep
+// the output within 1e-8 of the correctly rounded terms.  x^V for V <= 0 keeps powf.
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float hw_pow(float x, float V)
+{
+    return __builtin_amdgcn_exp2f(V * __builtin_amdgcn_logf(x));
+}
+
 __global__ void __launch_bounds__(kPtLags)
 k_pcc_time(const cf32 *__restrict__ phi, int S, int W, int N, int lag_first, int L, int pcc1, float V,
            double div_pow, int mode, float *__restrict__ y, int out_ld, int out_off)
@@ -78,18 +89,31 @@ k_pcc_time(const cf32 *__restrict__ phi, int S, int W, int N, int lag_first, int
             if (!valid) continue;
             const int lo = max(n_lo, n0) - n0, hi = min(n_hi, n0 + kPtTile) - n0;
             float part = 0.0f;
+            // full tiles (the common case) as a fixed-count loop the compiler
+            // unrolls, so the LDS reads of several terms are in flight together
+            auto run = [&](auto term) {
+                if (lo == 0 && hi == kPtTile) {
+#pragma unroll 16
+                    for (int t = 0; t < kPtTile; ++t) part += term(s2[t + tid], s1[t]);
+                } else {
+                    for (int t = lo; t < hi; ++t) part += term(s2[t + tid], s1[t]);
+                }
+            };
             if (pcc1) {
-                for (int t = lo; t < hi; ++t) {
-                    const cf32 a = s2[t + tid], b = s1[t];
+                run([](cf32 a, cf32 b) {
                     const float sx = a.x + b.x, sy = a.y + b.y, dx = a.x - b.x, dy = a.y - b.y;
-                    part += sqrtf(sx * sx + sy * sy) - sqrtf(dx * dx + dy * dy);
-                }
+                    return hw_sqrt(sx * sx + sy * sy) - hw_sqrt(dx * dx + dy * dy);
+                });
+            } else if (V > 0.f) {
+                run([V](cf32 a, cf32 b) {
+                    const float sx = a.x + b.x, sy = a.y + b.y, dx = a.x - b.x, dy = a.y - b.y;
+                    return hw_pow(sx * sx + sy * sy, V) - hw_pow(dx * dx + dy * dy, V);
+                });
             } else {
-                for (int t = lo; t < hi; ++t) {
-                    const cf32 a = s2[t + tid], b = s1[t];
+                run([V](cf32 a, cf32 b) {
                     const float sx = a.x + b.x, sy = a.y + b.y, dx = a.x - b.x, dy = a.y - b.y;
-                    part += powf(sx * sx + sy * sy, V) - powf(dx * dx + dy * dy, V);
-                }
+                    return powf(sx * sx + sy * sy, V) - powf(dx * dx + dy * dy, V);
+                });
             }
             acc += (double)part;
         }
