@@ -77,6 +77,13 @@ struct ColArgs {
     const double *E = nullptr;
     long long lde = 0;
     int Nsig = 0;
+    // lags whose overlap m = Nsig - |lag| is below K: the correlation as an exact
+    // FP64 sum over the overlap from the first and last K samples of each window
+    // (edge[(station * Wall + window) * 2K + ...], head then tail); esum: the
+    // output sums all Wall windows (else window w); norm 2 takes the samples'
+    // signs (x1 >= 0, x2 > 0)
+    const float *edge = nullptr;
+    int K = 0, Wall = 1, esum = 0;
 };
 
 struct RowArgs {

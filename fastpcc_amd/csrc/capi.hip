@@ -650,6 +650,13 @@ int station_stage_group(const std::vector<const fpcc_job *> &jobs, const std::ve
         ra.bstride = Nz;
         if ((r = run_rows_batch(bz, ra, jb, st, "real_fwd_rows_batch"))) return r;
     }
+    for (int g = 0; g < n; ++g) {  // window edges for the exact short-overlap lags
+        size_t off[4], per[4];
+        if (large_station_regions(kind, jobs[g]->stack, D[g].S, D[g].W, D[g].N, Nz, off, per) < 3) return -1;
+        if ((r = large_edge_copy(jb.x[g], jb.stride[g], D[g].S, D[g].W, D[g].N,
+                                 (float *)(ws[g] + layout(jobs[g], D[g]).a + off[2]), st)))
+            return r;
+    }
     if (kind == FPCC_GNCC) return launch_energy_prefix_batch(jb, st);
     return 0;
 }

@@ -124,6 +124,9 @@ int large_pair_stage(int kind, int stack, int S, int W, int N, int Nz, int lag0,
 // Workspace regions holding the station stage's per-station output (what the pair
 // stage reads): station s at off[k] + s*per[k]; returns the region count.
 int large_station_regions(int kind, int stack, int S, int W, int N, int Nz, size_t *off, size_t *per);
+// GNCC / 1-bit on two-pass transforms: the first and last edge samples of every
+// window into the region large_station_regions lists third.
+int large_edge_copy(const float *x, long long stride, int S, int W, int N, float *edge, hipStream_t st);
 
 // WPCC (wpcc.hip): workspace bytes and the two stages for S stations x W windows.
 struct WFamily;

@@ -21,11 +21,37 @@
 
 namespace fpcc {
 
+// GNCC and 1-bit on two-pass transforms (FP32): lags whose overlap is shorter than
+// edge_len(N) samples are summed exactly from the first and last edge_len(N)
+// samples of each window, kept beside the spectra (k_big_cols, COL_INV_LAGWIN).
+// Below N/64 samples of overlap the transform's absolute error, divided by the
+// overlap's energy, could exceed 1e-6; above it the error is at most 64 times its
+// full-overlap size (a few 1e-9 measured).
+int edge_len(int N) { return std::min(N, std::max(64, (N + 63) / 64)); }
+
+__global__ void __launch_bounds__(256)
+k_edge_copy(const float *__restrict__ x, long long stride, int W, int N, int K, float *__restrict__ edge)
+{
+    const int sw = blockIdx.x, s = sw / W, w = sw - s * W;
+    const float *in = x + (long long)s * stride + (long long)w * N;
+    float *out = edge + (long long)sw * 2 * K;
+    for (int i = threadIdx.x; i < 2 * K; i += blockDim.x) out[i] = in[i < K ? i : N - 2 * K + i];
+}
+
+int large_edge_copy(const float *x, long long stride, int S, int W, int N, float *edge, hipStream_t st)
+{
+    if (S * W <= 0) return 0;
+    ProfScope prof("edge_copy", st);
+    hipLaunchKernelGGL(k_edge_copy, dim3(S * W), dim3(256), 0, st, x, stride, W, N, edge_len(N), edge);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
 namespace {
 size_t al(size_t v) { return (v + 255) / 256 * 256; }
 
 struct LargeLayout {
-    size_t Z = 0, F = 0, F2 = 0, mx = 0, E = 0, C = 0, U = 0, U2 = 0, G = 0, B = 0, total = 0;
+    size_t Z = 0, F = 0, F2 = 0, mx = 0, E = 0, X = 0, C = 0, U = 0, U2 = 0, G = 0, B = 0, total = 0;
 };
 
 // Analytic signals of lengths without a two-pass split run as Bluestein
@@ -85,10 +111,12 @@ LargeLayout large_layout(int kind, int stack, int S, int W, int N, int Nz)
     case FPCC_GNCC:
         l.F = take(t, sizeof(cf32) * SW * Nz);
         l.E = take(t, sizeof(double) * SW * (size_t)energy_ld(N));
+        l.X = take(t, sizeof(float) * SW * 2 * (size_t)edge_len(N));
         break;
     case FPCC_CC1B:
         l.F = take(t, sizeof(cf32) * SW * Nz);
         l.F2 = take(t, sizeof(cf32) * SW * Nz);
+        l.X = take(t, sizeof(float) * SW * 2 * (size_t)edge_len(N));
         break;
     }
     if (kind == FPCC_PCC2 || kind == FPCC_GNCC || kind == FPCC_CC1B) {
@@ -133,13 +161,17 @@ int large_station_regions(int kind, int stack, int S, int W, int N, int Nz, size
         per[0] = spec;
         off[1] = l.E;
         per[1] = sizeof(double) * (size_t)W * energy_ld(N);
-        return 2;
+        off[2] = l.X;
+        per[2] = sizeof(float) * (size_t)W * 2 * edge_len(N);
+        return 3;
     case FPCC_CC1B:
         off[0] = l.F;
         per[0] = spec;
         off[1] = l.F2;
         per[1] = spec;
-        return 2;
+        off[2] = l.X;
+        per[2] = sizeof(float) * (size_t)W * 2 * edge_len(N);
+        return 3;
     }
     return 0;
 }
@@ -288,6 +320,7 @@ int large_station_stage(int kind, const float *x, long long stride, int S, int W
             ra.bstride = Nz;
             if ((r = run_rows(bz, ra, SW, st, "real_fwd_rows"))) return r;
         }
+        if ((r = large_edge_copy(x, stride, S, W, N, (float *)(ws + l.X), st))) return r;
         if (kind == FPCC_GNCC)
             return launch_energy_prefix(x, stride, S, W, N, (double *)(ws + l.E), energy_ld(N), st);
         return 0;
@@ -342,12 +375,20 @@ int large_pair_stage(int kind, int stack, int S, int W, int N, int Nz, int lag0,
             ca.op = COL_INV_LAGWIN;
             ca.src = G + (long long)p0 * ldg;
             ca.bstride = ldg;
-            ca.y = y + (long long)p0 * L;
+            ca.y = y;  // pair p0 + b at y[(p0 + b) L]
             ca.lag0 = lag0;
             ca.L = L;
             ca.scale = base_scale / (float)W;
             ca.norm = norm;
             ca.Nsig = N;
+            ca.S = S;
+            ca.pair0 = p0;
+            if (kind == FPCC_CC1B) {
+                ca.edge = (const float *)(ws + l.X);
+                ca.K = edge_len(N);
+                ca.Wall = W;
+                ca.esum = 1;
+            }
             if ((r = run_cols(bz, ca, np, st, "pair_inv_cols"))) return r;
         }
         return 0;
@@ -394,6 +435,12 @@ int large_pair_stage(int kind, int stack, int S, int W, int N, int Nz, int lag0,
             ca.E = (const double *)(ws + l.E);
             ca.lde = energy_ld(N);
             if (rounds) ca.Wfull = W;  // energies are indexed by (station, window)
+            if (kind != FPCC_PCC2) {
+                ca.edge = (const float *)(ws + l.X);
+                ca.K = edge_len(N);
+                ca.Wall = W;
+                ca.esum = wsum;
+            }
             const int pair0 = (int)(per_window ? b0 / W : b0);
             ra.pair0 = pair0;
             ca.pair0 = pair0;

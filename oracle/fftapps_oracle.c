@@ -380,6 +380,11 @@ static double norm_f(const float *x, unsigned n1, unsigned n2)
 /* PCC2 (src/FFTapps.c:785-881)                                              */
 /* ------------------------------------------------------------------------- */
 
+/* Lag window of L values starting at lag.  The reference copies the window's
+ * negative lags with `for (n=0; n<-lag; n++)` (src/FFTapps.c:350, :367 and the
+ * same loop in pcc2_set and tspcc2_set), which writes past y's L values when the
+ * whole window lies below -L; every copy here also stops at n < L.  The L values
+ * written are the reference's. */
 static int lag_setup(int Lag1, int Lag2, int *L, int *lag)
 {
     if (Lag2 >= Lag1) { *L = Lag2 - Lag1 + 1; *lag = Lag1; }
@@ -424,7 +429,7 @@ int orc_pcc2_set(float **y, float **x1, float **x2, unsigned N, unsigned Tr, int
                 out[n] = (float complex)(conj((double complex)xa1[n]) * (double complex)xa2[n]);
             offt_execute_ws_f(bZ, out, out, ws);
             int n;
-            for (n = 0; n < -lag; n++) y[tr][n] = crealf(fa1 * out[n + Nz + lag]);
+            for (n = 0; n < -lag && n < L; n++) y[tr][n] = crealf(fa1 * out[n + Nz + lag]);
             for (; n < L; n++) y[tr][n] = crealf(fa1 * out[n + lag]);
         }
         free(xa1); free(xa2); free(out); free(ws);
@@ -469,7 +474,7 @@ static int orc64_pcc2(double **y, float **x1, float **x2, unsigned N, unsigned T
             for (unsigned n = 0; n < Nz; n++) out[n] = conj(xa1[n]) * xa2[n];
             offt_execute_ws_d(bZ, out, out, ws);
             int n;
-            for (n = 0; n < -lag; n++) y[tr][n] = fa1 * creal(out[n + Nz + lag]);
+            for (n = 0; n < -lag && n < L; n++) y[tr][n] = fa1 * creal(out[n + Nz + lag]);
             for (; n < L; n++) y[tr][n] = fa1 * creal(out[n + lag]);
         }
         free(xd); free(xa1); free(xa2); free(out); free(ws);
@@ -593,7 +598,7 @@ static int ccgn_core(float **yf, double **yd, float **x1, float **x2, unsigned N
             for (unsigned n = 0; n < Nh; n++) fout[n] = conj(fin1[n]) * fin2[n];
             c2r_d(out, fout, Nz, bZ, full, ws);
             int n;
-            for (n = 0; n < -lag; n++) ybuf[n] = da1 * out[n + Nz + lag];
+            for (n = 0; n < -lag && n < L; n++) ybuf[n] = da1 * out[n + Nz + lag];
             for (; n < L; n++) ybuf[n] = da1 * out[n + lag];
             double norm1 = norm_d(in1, n11, n12), norm2 = norm_d(in2, n21, n22);
             gn_lowlevel_d(ybuf, in1, in2, norm1, norm2, N, L, lag);
@@ -650,7 +655,7 @@ int orc_cc1b_set(float **y, float **x1, float **x2, unsigned N, unsigned Tr, int
             for (unsigned n = 0; n < Nh; n++) fout[n] = conjf(fin1[n]) * fin2[n];
             c2r_f(out, fout, Nz, bZ, full, ws);
             int n;
-            for (n = 0; n < -lag; n++) y[tr][n] = da1 * out[n + Nz + lag];
+            for (n = 0; n < -lag && n < L; n++) y[tr][n] = da1 * out[n + Nz + lag];
             for (; n < L; n++) y[tr][n] = da1 * out[n + lag];
             double norm1 = norm_f(in1, n11, n12), norm2 = norm_f(in2, n21, n22);
             gn_lowlevel_f(y[tr], in1, in2, norm1, norm2, N, L, lag);
@@ -694,7 +699,7 @@ static int orc64_cc1b(double **y, float **x1, float **x2, unsigned N, unsigned T
             for (unsigned n = 0; n < Nh; n++) fout[n] = conj(fin1[n]) * fin2[n];
             c2r_d(out, fout, Nz, bZ, full, ws);
             int n;
-            for (n = 0; n < -lag; n++) y[tr][n] = da1 * out[n + Nz + lag];
+            for (n = 0; n < -lag && n < L; n++) y[tr][n] = da1 * out[n + Nz + lag];
             for (; n < L; n++) y[tr][n] = da1 * out[n + lag];
             double norm1 = norm_d(in1, n11, n12), norm2 = norm_d(in2, n21, n22);
             gn_lowlevel_d(y[tr], in1, in2, norm1, norm2, N, L, lag);
@@ -771,7 +776,7 @@ static int tspcc2_core(float **yf, double **yd, float **x1, float **x2, unsigned
             }
             offt_execute_ws_d(bZ, ywt, ywt, ws);
             int n;
-            for (n = 0; n < -lag; n++) {
+            for (n = 0; n < -lag && n < L; n++) {
                 double v = C * creal(ywt[n + Nz + lag]);
                 if (yf) yf[tr][n] = (float)v; else yd[tr][n] = v;
             }

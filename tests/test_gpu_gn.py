@@ -81,3 +81,19 @@ def test_golden_config1(fp, kind, stack):
     err = np.abs(y - gold).max()
     print(f"{kind}/{stack} vs golden (FP64): {err:.3e}")
     assert err <= TOL
+
+
+@pytest.mark.parametrize("kind", ["pcc2", "gncc", "cc1b"])
+@pytest.mark.parametrize("N,lag1,lag2", [(2000, -1500, -1200), (2000, 1200, 1500), (21600, -15000, -14000),
+                                         (21600, 14000, 15000)])
+def test_one_sided_lag_windows(fp, orc, kind, N, lag1, lag2):
+    """Lag windows wholly on one side of zero and beyond -L (the reference's copy
+    loop writes past y there; the window's values are the same), in-LDS and
+    two-pass transform lengths."""
+    x1, x2 = _rand(2, N, N + lag1)
+    fn = {"pcc2": fp.pcc2_set, "gncc": fp.ccgn_set, "cc1b": fp.cc1b_set}[kind]
+    y = fn(x1, x2, lag1, lag2)
+    yr = orc.run(kind, x1, x2, lag1, lag2)
+    err = np.abs(y - yr).max()
+    print(f"{kind} N={N} lags ({lag1},{lag2}) max|gpu-oracle|={err:.3e}")
+    assert y.shape == yr.shape and err <= TOL

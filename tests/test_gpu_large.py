@@ -87,3 +87,38 @@ def test_compile_time_four_step_splits(fp, orc, kind, N, lag):
     err = np.abs(y - yr).max()
     print(f"  max|gpu-oracle|={err:.3e}")
     assert err <= TOL
+
+
+@pytest.mark.parametrize("N", [8000, 10007, 21600])
+@pytest.mark.parametrize("kind", ["gncc", "cc1b"])
+def test_extreme_lags_two_pass(fp, orc, kind, N):
+    """Lags up to N-1 on the two-pass path (Nz = 2^ceil(log2(2N-1)) > 8192): the
+    normalisation divides by the energy (or count) of a short overlap, where the
+    FP32 transform's error would exceed 1e-5; those lags are summed exactly.  GNCC
+    against the oracle (FP64 like the reference); 1-bit against the FP64 oracle
+    (the reference's FP32 transform itself is 1e-5 off at the edges)."""
+    x1, x2 = _rand(2, N, N + 7)
+    fn = {"gncc": fp.ccgn_set, "cc1b": fp.cc1b_set}[kind]
+    for l1, l2 in ((-(N - 1), -(N - 400)), (N - 400, N - 1), (-(N - 1), N - 1)):
+        y = fn(x1, x2, l1, l2)
+        yr = orc.run(kind, x1, x2, l1, l2, f64=(kind == "cc1b"))
+        err = np.abs(y - yr).max()
+        print(f"{kind} N={N} lags ({l1},{l2}) max|gpu-{'fp64' if kind == 'cc1b' else 'oracle'}|={err:.3e}")
+        assert err <= TOL
+
+
+@pytest.mark.parametrize("kind,stack", [("gncc", "none"), ("gncc", "plain"), ("cc1b", "none"), ("cc1b", "plain"),
+                                        ("cc1b", "gram"), ("cc1b", "gram16")])
+def test_extreme_lags_allpairs(fp, orc, kind, stack):
+    """All pairs of 4 stations x 3 windows of 9000 samples, lags -(N-1) .. -(N-300)
+    and the far positive end: the exact short-overlap sums per window, per window
+    then averaged (GNCC), and summed over windows (1-bit plain and Gram)."""
+    S, W, N = 4, 3, 9000
+    x, _ = synth.make_stations(S, W * N, 300, seed=77)
+    for l1, l2 in ((-(N - 1), -(N - 300)), (N - 300, N - 1)):
+        y = fp.allpairs(x, W, N, l1, l2, kind=kind, stack=stack)
+        yr = orc.allpairs(kind, x, W, N, l1, l2, stacked=(stack != "none"), f64=True)
+        err = np.abs(y.reshape(yr.shape) - yr).max()
+        print(f"{kind}/{stack} lags ({l1},{l2}) max|gpu-fp64|={err:.3e}")
+        tol = 1e-4 if stack == "gram16" else TOL
+        assert err <= tol

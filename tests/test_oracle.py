@@ -362,3 +362,26 @@ def test_wpcc_vs_direct_time_domain(orc, wtype):
     assert np.abs(y64 - ref).max() < 1e-12
     y = orc.run("wpcc", x1, x2, l1, l2, pmin=4.0, pmax=16.0, V=2, wtype=wtype)[0]
     assert np.abs(y - ref).max() < 1e-6
+
+
+# ---------------------------------------------------------------- one-sided lag windows (a11)
+
+@pytest.mark.parametrize("kind", ["pcc2", "gncc", "cc1b", "wpcc"])
+@pytest.mark.parametrize("f64", [False, True])
+def test_one_sided_lag_windows_are_slices(orc, kind, f64):
+    """Windows wholly below -L or above +L (the reference's negative-lag copy loop
+    would write past y for the first) equal the slices of the widest window with
+    the same Nz: the oracle copies exactly the window's L values."""
+    N = 400
+    rng = np.random.default_rng(11)
+    x1 = rng.standard_normal((2, N)).astype(np.float32)
+    x2 = rng.standard_normal((2, N)).astype(np.float32)
+    kw = dict(pmin=4.0, pmax=20.0, V=2) if kind == "wpcc" else {}
+    full = orc.run(kind, x1, x2, -(N - 1), N - 1, f64=f64, **kw)
+    for l1, l2 in ((-(N - 1), -(N - 60)), (N - 60, N - 1), (-(N - 1), -(N - 1)), (-300, -250)):
+        y = orc.run(kind, x1, x2, l1, l2, f64=f64, **kw)
+        ref = full[:, l1 + N - 1:l2 + N]
+        if kind == "wpcc":  # tspcc2_set's L = |Lag2-Lag1+1| and its Nz depend only on max |lag|: same here
+            ref = full[:, l1 + N - 1:l1 + N - 1 + y.shape[1]]
+        assert y.shape == ref.shape
+        np.testing.assert_allclose(y, ref, rtol=0, atol=1e-9 if f64 else 1e-6)
