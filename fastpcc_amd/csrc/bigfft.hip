@@ -6,6 +6,7 @@
 #include <string>
 #include "../../include/fastpcc.h"
 #include "bigfft.hpp"
+#include "edge.hpp"
 #include "kernels.hpp"
 #include "profile.hpp"
 
@@ -347,24 +348,8 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
                     // the overlap's small energy, would dominate; sum it exactly
                     int ei = i, ej = j;
                     if (a.norm != 1) pair_of_big(p, a.S, ei, ej);
-                    const int K2 = 2 * a.K;
                     const int w0 = a.esum ? 0 : w, w1 = a.esum ? a.Wall : w + 1;
-                    double c = 0.0;
-                    for (int ww = w0; ww < w1; ++ww) {
-                        const float *e1 = a.edge + (long long)(ei * a.Wall + ww) * K2;
-                        const float *e2 = a.edge + (long long)(ej * a.Wall + ww) * K2;
-                        // lag >= 0: x1[0, m) x2[N-m, N); lag < 0: x1[N-m, N) x2[0, m)
-                        const float *u = lag >= 0 ? e1 : e1 + K2 - m;
-                        const float *q2 = lag >= 0 ? e2 + K2 - m : e2;
-                        for (int t2 = 0; t2 < m; ++t2) {
-                            double p1 = u[t2], p2 = q2[t2];
-                            if (a.norm == 2) {
-                                p1 = p1 >= 0.0 ? 1.0 : -1.0;
-                                p2 = p2 > 0.0 ? 1.0 : -1.0;
-                            }
-                            c += p1 * p2;
-                        }
-                    }
+                    const double c = edge_corr(a.edge, a.K, a.Wall, ei, ej, w0, w1, lag, m, a.norm == 2);
                     o = c * (double)a.scale * (double)n;  // the transform's scale: v.x is n c
                 }
                 if (a.norm == 1) {

@@ -227,7 +227,7 @@ int resolve(const fpcc_job *job, Dims &d)
 
 // Workspace regions of a job (byte offsets, 256-B aligned).
 struct Layout {
-    size_t a = 0, b = 0, c = 0, d = 0, e = 0, total = 0;
+    size_t a = 0, b = 0, c = 0, d = 0, e = 0, f = 0, total = 0;
 };
 
 bool is_gram(int stack) { return stack == FPCC_STACK_GRAM || stack == FPCC_STACK_GRAM16; }
@@ -262,6 +262,7 @@ Layout layout(const fpcc_job *job, const Dims &d)
     case FPCC_CC1B:
         l.a = take(l.total, sizeof(cf32) * SW * (size_t)real_ld(d.Nz));  // x >= 0 signs
         l.b = take(l.total, sizeof(cf32) * SW * (size_t)real_ld(d.Nz));  // x > 0 signs
+        l.f = take(l.total, sizeof(float) * SW * 2 * (size_t)edge_len(d.N));  // window edges (edge.hpp)
         if (is_gram(job->stack)) {
             l.c = take(l.total, sizeof(cf32) * K1 * SW);
             l.d = take(l.total, sizeof(cf32) * K1 * SW);
@@ -305,7 +306,9 @@ int job_station_regions(const fpcc_job *job, size_t *off, size_t *per)
         off[0] = l.a;
         off[1] = l.b;
         per[0] = per[1] = sizeof(cf32) * W * (size_t)real_ld(d.Nz);
-        return 2;
+        off[2] = l.f;
+        per[2] = sizeof(float) * W * 2 * (size_t)edge_len(d.N);
+        return 3;
     case FPCC_PCC1:
     case FPCC_PCCV: off[0] = l.a; per[0] = sizeof(cf32) * W * d.N; return 1;
     case FPCC_WPCC: {
@@ -332,7 +335,8 @@ int station_stage(const fpcc_job *job, const Dims &d, const float *x, char *ws, 
     case FPCC_CC1B: {
         int r = launch_station_real(x, stride, d.S, d.W, d.N, d.Nz, 1, false, ws + l.a, nullptr, st);
         if (r) return r;
-        return launch_station_real(x, stride, d.S, d.W, d.N, d.Nz, 2, false, ws + l.b, nullptr, st);
+        if ((r = launch_station_real(x, stride, d.S, d.W, d.N, d.Nz, 2, false, ws + l.b, nullptr, st))) return r;
+        return large_edge_copy(x, stride, d.S, d.W, d.N, (float *)(ws + l.f), st);
     }
     case FPCC_PCC1:
     case FPCC_PCCV: return launch_station_phasor(x, stride, d.S, d.W, d.N, (cf32 *)(ws + l.a), st);
@@ -396,10 +400,11 @@ int pair_stage(const fpcc_job *job, const Dims &d, char *ws, float *y, hipStream
             if (r) return r;
             // ccf_lowlevel's 1/Nz, the mean over W, then 1/(N-|lag|) (gnf_lowlevel with +-1 data)
             const float scale = (float)(1.0 / (double)d.Nz) / (float)d.W;
-            return launch_pair_c2r(G, gl, (int)d.P, d.Nz, d.N, d.W, d.lag0, d.L, scale, 1, y, st);
+            return launch_pair_c2r(G, gl, (int)d.P, d.Nz, d.N, d.W, d.lag0, d.L, scale, 1, y, st,
+                                   (const float *)(ws + l.f), d.S);
         }
         return launch_pair_gn(ws + l.a, ws + l.b, nullptr, false, d.S, d.W, d.N, d.Nz, d.lag0, d.L, 1, mode, y,
-                              st);
+                              st, (const float *)(ws + l.f));
     case FPCC_PCC1:
     case FPCC_PCCV:
         return launch_pcc_time((const cf32 *)(ws + l.a), d.S, d.W, d.N, d.lag0, d.L,
@@ -628,6 +633,11 @@ int station_stage_group(const std::vector<const fpcc_job *> &jobs, const std::ve
             if ((r = launch_station_real_batch(jb, Nz, kind == FPCC_CC1B ? p + 1 : 0, kind == FPCC_GNCC, st)))
                 return r;
         }
+        if (kind == FPCC_CC1B)  // window edges for the exact short-overlap lags
+            for (int g = 0; g < n; ++g)
+                if ((r = large_edge_copy(jb.x[g], jb.stride[g], D[g].S, D[g].W, D[g].N,
+                                         (float *)(ws[g] + layout(jobs[g], D[g]).f), st)))
+                    return r;
         return 0;
     }
     if (jb.b0[n] > 65535) return -1;

@@ -17,6 +17,7 @@
 #include "engine.hpp"
 #include "kernels.hpp"
 #include "profile.hpp"
+#include "edge.hpp"
 
 namespace fpcc {
 
@@ -138,7 +139,7 @@ __global__ void __launch_bounds__(512)
 k_pair_gn(const C2<T> *__restrict__ Xa, const C2<T> *__restrict__ Xb, long long ldx,
           const double *__restrict__ E, long long lde, int S, int W, int N, int Nz, FftPlan planM,
           const C2<T> *__restrict__ twM, const C2<T> *__restrict__ twc, int lag0, int L, int norm, int mode,
-          float *__restrict__ y)
+          float *__restrict__ y, const float *__restrict__ edge, int K)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int M = Nz >> 1;
@@ -184,6 +185,8 @@ k_pair_gn(const C2<T> *__restrict__ Xa, const C2<T> *__restrict__ Xb, long long 
             const int n = lag & (Nz - 1);
             C2<T> v = z[n >> 1];
             double c = (double)(((n & 1) ? v.y : v.x) * inv_nz);
+            const int m = N - abs(lag);
+            if (edge && m > 0 && m < K) c = edge_corr(edge, K, W, i, j, w, w + 1, lag, m, norm != 0);
             double den;
             if (norm == 0) {
                 const int a1 = max(0, -lag), b1 = min(N, N - lag);
@@ -273,8 +276,9 @@ int launch_station_real_batch(const RealBatch &jb, int Nz, int sign_rule, bool f
 }
 
 int launch_pair_gn(const void *d_Xa, const void *d_Xb, const double *d_E, bool f64, int S, int W, int N, int Nz,
-                   int lag0, int L, int norm, int mode, float *d_y, hipStream_t st)
+                   int lag0, int L, int norm, int mode, float *d_y, hipStream_t st, const float *d_edge)
 {
+    const int K = d_edge ? edge_len(N) : 0;
     const int M = Nz / 2;
     const DevPlan &pM = get_plan(M, f64);
     const void *twc = get_c2r_twiddles(Nz, f64);
@@ -293,12 +297,12 @@ int launch_pair_gn(const void *d_Xa, const void *d_Xb, const double *d_E, bool f
         allow_lds((const void *)k_pair_gn<double>, lds);
         hipLaunchKernelGGL(k_pair_gn<double>, dim3(blocks), dim3(512), lds, st, (const cf64 *)d_Xa,
                            (const cf64 *)d_Xb, real_ld(Nz), d_E, energy_ld(N), S, W, N, Nz, pM.plan,
-                           (const cf64 *)pM.tw, (const cf64 *)twc, lag0, L, norm, mode, d_y);
+                           (const cf64 *)pM.tw, (const cf64 *)twc, lag0, L, norm, mode, d_y, d_edge, K);
     } else {
         allow_lds((const void *)k_pair_gn<float>, lds);
         hipLaunchKernelGGL(k_pair_gn<float>, dim3(blocks), dim3(512), lds, st, (const cf32 *)d_Xa,
                            (const cf32 *)d_Xb, real_ld(Nz), d_E, energy_ld(N), S, W, N, Nz, pM.plan,
-                           (const cf32 *)pM.tw, (const cf32 *)twc, lag0, L, norm, mode, d_y);
+                           (const cf32 *)pM.tw, (const cf32 *)twc, lag0, L, norm, mode, d_y, d_edge, K);
     }
     FPCC_CHECK(hipGetLastError());
     return 0;

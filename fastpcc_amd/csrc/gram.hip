@@ -22,6 +22,7 @@
 #include "engine.hpp"
 #include "kernels.hpp"
 #include "profile.hpp"
+#include "edge.hpp"
 
 namespace fpcc {
 
@@ -527,11 +528,29 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
 }
 
 // ---- per pair: folded spectrum -> real lags ----
+// 1-bit: window edge samples (edge.hpp) for the exact short-overlap lags
+struct C2rEdge {
+    const float *edge = nullptr;
+    int K = 0, S = 0, W = 0;
+};
+
+// one output lag: r (scaled transform value), then for counts (1-bit) the exact
+// sum over a short overlap and the division by the overlap count
+__device__ __forceinline__ float c2r_value(float r, float scale, int count_norm, int N, int Nz, int lag,
+                                           const C2rEdge &ed, int pi, int pj)
+{
+    if (!count_norm) return r;
+    const int m = N - abs(lag);
+    double d = (double)r;
+    if (ed.edge && m > 0 && m < ed.K)
+        d = edge_corr(ed.edge, ed.K, ed.W, pi, pj, 0, ed.W, lag, m, true) * (double)scale * (double)Nz;
+    return (float)(d / (double)m);
+}
 // Z[k] = P + i e^{+2 pi i k/Nz} Q, P = G[k] + conj(G[M-k]), Q = G[k] - conj(G[M-k])
 __global__ void __launch_bounds__(512)
 k_pair_c2r(const cf32 *__restrict__ G, GLayout gl, int Nz, FftPlan planM, const cf32 *__restrict__ twM,
            const cf32 *__restrict__ twc, int lag0, int L, float scale, int count_norm, int N,
-           float *__restrict__ y)
+           float *__restrict__ y, C2rEdge ed)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int M = Nz >> 1;
@@ -551,13 +570,13 @@ k_pair_c2r(const cf32 *__restrict__ G, GLayout gl, int Nz, FftPlan planM, const 
     }
     __syncthreads();
     cf32 *z = fft_lds<float, +1>(buf0, buf1, planM, twM);
+    int pi = 0, pj = 0;
+    if (ed.edge) edge_pair((int)p, ed.S, pi, pj);
     float *dst = y + p * L;
     for (int l = threadIdx.x; l < L; l += blockDim.x) {
         const int n = (lag0 + l) & (Nz - 1);
         cf32 v = z[n >> 1];
-        float r = scale * ((n & 1) ? v.y : v.x);
-        if (count_norm) r = (float)((double)r / (double)(N - abs(lag0 + l)));
-        dst[l] = r;
+        dst[l] = c2r_value(scale * ((n & 1) ? v.y : v.x), scale, count_norm, N, Nz, lag0 + l, ed, pi, pj);
     }
 }
 
@@ -578,7 +597,7 @@ template <int NT, int... Rs>
 __global__ void __launch_bounds__(NT)
 k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__restrict__ twM,
                 const cf32 *__restrict__ twc, int lag0, int L, float scale, int count_norm, int N,
-                float *__restrict__ y)
+                float *__restrict__ y, C2rEdge ed)
 {
     constexpr int M = 4096, Nz = 8192;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -620,13 +639,13 @@ k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__res
         }
         __syncthreads();
         ct_fft<float, M, NT, +1, Rs...>(lds, twM);
+        int pi = 0, pj = 0;
+        if (ed.edge) edge_pair((int)p, ed.S, pi, pj);
         float *dst = y + p * L;
         for (int l = t; l < L; l += NT) {
             const int n = (lag0 + l) & (Nz - 1);
             cf32 v = lds[ct_pad(n >> 1)];
-            float r = scale * ((n & 1) ? v.y : v.x);
-            if (count_norm) r = (float)((double)r / (double)(N - abs(lag0 + l)));
-            dst[l] = r;
+            dst[l] = c2r_value(scale * ((n & 1) ? v.y : v.x), scale, count_norm, N, Nz, lag0 + l, ed, pi, pj);
         }
         if (pn >= P) break;
         __syncthreads();  // every lane has read this pair before the next one lands
@@ -767,8 +786,15 @@ int launch_gram_split16(const void *d_Ua, const void *d_Ub, int S, int W2, int m
 }
 
 int launch_pair_c2r(const cf32 *d_G, GLayout gl, int P, int Nz, int N, int W, int lag0, int L, float scale,
-                    int count_norm, float *d_y, hipStream_t st)
+                    int count_norm, float *d_y, hipStream_t st, const float *d_edge, int S)
 {
+    C2rEdge ed;
+    if (d_edge && count_norm) {
+        ed.edge = d_edge;
+        ed.K = edge_len(N);
+        ed.S = S;
+        ed.W = W;
+    }
     const int M = Nz / 2;
     const DevPlan &pM = get_plan(M, false);
     const void *twc = get_c2r_twiddles(Nz, false);
@@ -795,7 +821,7 @@ int launch_pair_c2r(const cf32 *d_G, GLayout gl, int P, int Nz, int N, int W, in
         const int nwg = P < per_cu * ncu ? P : per_cu * ncu;
         ProfScope prof("pair_c2r", st);
         hipLaunchKernelGGL(fn, dim3(nwg), dim3(kT), lds, st, d_G, gl, P, tw8, (const cf32 *)twc,
-                           lag0, L, scale, count_norm, N, d_y);
+                           lag0, L, scale, count_norm, N, d_y, ed);
         FPCC_CHECK(hipGetLastError());
         return 0;
     }
@@ -803,7 +829,7 @@ int launch_pair_c2r(const cf32 *d_G, GLayout gl, int P, int Nz, int N, int W, in
     allow_lds((const void *)k_pair_c2r, lds);
     ProfScope prof("pair_c2r", st);
     hipLaunchKernelGGL(k_pair_c2r, dim3(P), dim3(512), lds, st, d_G, gl, Nz, pM.plan,
-                       (const cf32 *)pM.tw, (const cf32 *)twc, lag0, L, scale, count_norm, N, d_y);
+                       (const cf32 *)pM.tw, (const cf32 *)twc, lag0, L, scale, count_norm, N, d_y, ed);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }

@@ -62,8 +62,10 @@ int launch_gram_split16(const void *d_Ua, const void *d_Ub, int S, int W2, int m
 float gram16_sigma(int bound);
 int fill_zero(float *p, long long n, hipStream_t st);
 // y[p][l] = scale * c2r(G[p])[(lag0+l) mod Nz], divided by N-|lag| if count_norm
+// d_edge (1-bit): window edge samples of edge_len(N) (see edge.hpp) for the exact
+// short-overlap lags; S stations
 int launch_pair_c2r(const cf32 *d_G, GLayout gl, int P, int Nz, int N, int W, int lag0, int L, float scale,
-                    int count_norm, float *d_y, hipStream_t st);
+                    int count_norm, float *d_y, hipStream_t st, const float *d_edge = nullptr, int S = 0);
 
 // Station-windows of several jobs in one launch (uneven station counts, window
 // counts and lengths; one transform length): job j owns launch blocks
@@ -112,7 +114,7 @@ int launch_station_real_batch(const RealBatch &jb, int Nz, int sign_rule, bool f
 // norm 0: sqrt(E1 E2) from prefix sums (GNCC); 1: overlap count (1-bit).
 // mode 0: y[p][w][L]; 1: mean over windows.
 int launch_pair_gn(const void *d_Xa, const void *d_Xb, const double *d_E, bool f64, int S, int W, int N, int Nz,
-                   int lag0, int L, int norm, int mode, float *d_y, hipStream_t st);
+                   int lag0, int L, int norm, int mode, float *d_y, hipStream_t st, const float *d_edge = nullptr);
 
 // Windows longer than one workgroup's LDS (large.hip).
 bool large_supported(int kind, int N, int Nz);
@@ -127,6 +129,8 @@ int large_station_regions(int kind, int stack, int S, int W, int N, int Nz, size
 // GNCC / 1-bit on two-pass transforms: the first and last edge samples of every
 // window into the region large_station_regions lists third.
 int large_edge_copy(const float *x, long long stride, int S, int W, int N, float *edge, hipStream_t st);
+// edge samples kept per window for the exact short-overlap lags (edge.hpp)
+int edge_len(int N);
 
 // WPCC (wpcc.hip): workspace bytes and the two stages for S stations x W windows.
 struct WFamily;

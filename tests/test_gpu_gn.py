@@ -97,3 +97,35 @@ def test_one_sided_lag_windows(fp, orc, kind, N, lag1, lag2):
     err = np.abs(y - yr).max()
     print(f"{kind} N={N} lags ({lag1},{lag2}) max|gpu-oracle|={err:.3e}")
     assert y.shape == yr.shape and err <= TOL
+
+
+@pytest.mark.parametrize("N", [100, 3000, 4000, 4096])
+def test_cc1b_extreme_lags(fp, orc, N):
+    """Lags up to N-1 on the in-LDS path (Nz <= 8192): the 1-bit count at an
+    overlap of m samples is a small integer divided by m, and the FP32 transform's
+    error there would exceed 1e-5; those lags are counted exactly from the window
+    edges.  Against the FP64 oracle (the reference's own FP32 transform is 1e-5
+    off at these lags)."""
+    x1, x2 = _rand(3, N, N + 11)
+    for l1, l2 in ((-(N - 1), -(N - N // 10)), (N - N // 10, N - 1), (-(N - 1), N - 1)):
+        y = fp.cc1b_set(x1, x2, l1, l2)
+        yr = orc.run("cc1b", x1, x2, l1, l2, f64=True)
+        err = np.abs(y - yr).max()
+        print(f"cc1b N={N} lags ({l1},{l2}) max|gpu-fp64|={err:.3e}", flush=True)
+        assert y.shape == yr.shape and err <= TOL
+
+
+@pytest.mark.parametrize("stack", ["none", "plain", "gram", "gram16"])
+def test_cc1b_extreme_lags_allpairs(fp, orc, stack):
+    """All pairs of 5 stations x 3 windows of 3000 samples at the far lags, one
+    record zero in places: per window (no stack) and summed over windows (plain,
+    FP32 Gram and split-FP16 Gram stacks) against the FP64 oracle."""
+    S, W, N = 5, 3, 3000
+    x = np.random.default_rng(9).standard_normal((S, W * N)).astype(np.float32)
+    x[2, 100:700] = 0.0
+    for l1, l2 in ((-(N - 1), -(N - 300)), (N - 300, N - 1)):
+        y = fp.allpairs(x, W, N, l1, l2, kind="cc1b", stack=stack)
+        yr = orc.allpairs("cc1b", x, W, N, l1, l2, stacked=(stack != "none"), f64=True)
+        err = np.abs(y.reshape(yr.shape) - yr).max()
+        print(f"cc1b/{stack} lags ({l1},{l2}) max|gpu-fp64|={err:.3e}", flush=True)
+        assert err <= (1e-4 if stack == "gram16" else TOL)
