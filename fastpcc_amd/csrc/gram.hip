@@ -581,10 +581,11 @@ k_pair_c2r(const cf32 *__restrict__ G, GLayout gl, int Nz, FftPlan planM, const 
 }
 
 // Specialisation for Nz = 8192 (M = 4096): compile-time in-place FFT (radices Rs,
-// NT threads), 35 KB of LDS, persistent workgroups each walking pairs p, p + grid,
-// ... and loading the next pair's spectrum into registers while the current one
-// is folded, transformed and written.  The fold runs in place: Z[k] and Z[M-k]
-// come from the same two values G[k], G[M-k].
+// NT threads), persistent workgroups each walking pairs p, p + grid, ...  Thread t
+// loads bins k = t + NT i (k < M/2) and their mirrors M - k, so the fold into the
+// half-length spectrum Z runs in registers as the spectrum is written to LDS
+// (Z[k] and Z[M-k] both come from G[k] and G[M-k]); the next pair's bins load
+// while the current pair is transformed.
 __device__ __forceinline__ cf32 c2r_fold(cf32 a, cf32 c, cf32 w)
 {
     const cf32 P = cf32{a.x + c.x, a.y - c.y};
@@ -593,59 +594,79 @@ __device__ __forceinline__ cf32 c2r_fold(cf32 a, cf32 c, cf32 w)
     return cf32{P.x - wq.y, P.y + wq.x};
 }
 
-template <int NT, int... Rs>
-__global__ void __launch_bounds__(NT)
+template <int NT, bool COUNT, int... Rs>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
 k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__restrict__ twM,
-                const cf32 *__restrict__ twc, int lag0, int L, float scale, int count_norm, int N,
-                float *__restrict__ y, C2rEdge ed)
+                const cf32 *__restrict__ twc, int lag0, int L, float scale, int N, float *__restrict__ y,
+                C2rEdge ed)
 {
     constexpr int M = 4096, Nz = 8192;
+    constexpr int H = M / 2 / NT;    // bins k < M/2 a thread
+    static_assert(M / 2 % NT == 0, "whole bins a thread");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cf32 *lds = reinterpret_cast<cf32 *>(smem_raw);
-    __shared__ cf32 gM;  // G[M] of the pair in LDS
     const int t = threadIdx.x;
     long long p = blockIdx.x;
     if (p >= P) return;
-    cf32 pre[M / NT];
-    cf32 preM = cf32{0.f, 0.f};
-    // bin t + NT i of pair q sits at q pl + ui(i) + vo: a scalar part and this
-    // thread's 32-bit offset (groups of at most NT bins, or rows; checked by the host)
+    // bin j NT + t of pair q sits at q pl + pos(j NT) + vo, bin j NT - t at
+    // q pl + pos((j - 1) NT) + vm (NT is a multiple of the group size, checked by the host)
     const bool rows = gl.sh == 30;
-    const unsigned vo = rows ? (unsigned)t : (unsigned)((t >> gl.sh) * gl.gs + (t & ((1 << gl.sh) - 1)));
-    const long long ustep = rows ? NT : (long long)(NT >> gl.sh) * gl.gs;
+    const int mask = rows ? 0 : (1 << gl.sh) - 1;
+    const unsigned vo = rows ? (unsigned)t : (unsigned)((t >> gl.sh) * gl.gs + (t & mask));
+    const unsigned vm = rows ? (unsigned)(NT - t)
+                             : (unsigned)(((NT >> gl.sh) - ((t + mask) >> gl.sh)) * gl.gs + ((-t) & mask));
+    auto pos = [&](int k) -> long long { return rows ? (long long)k : (long long)(k >> gl.sh) * gl.gs; };
+    cf32 pa[H], pc[H], ph = cf32{0.f, 0.f};
+    const cf32 w0 = twc[t];  // twc[k] = exp(2 pi i k / Nz); twc[t + NT i] = w0 twc[NT i]
     auto fetch = [&](long long q) {
         const cf32 *gq = G + q * gl.pl;
 #pragma unroll
-        for (int i = 0; i < M / NT; ++i) pre[i] = gq[i * ustep + vo];
-        if (t == 0) preM = G[gaddr(gl, q, M)];
+        for (int i = 0; i < H; ++i) {
+            pa[i] = gq[pos(NT * i) + vo];
+            pc[i] = gq[pos(M - NT * (i + 1)) + vm];  // bin M - t - NT i
+        }
+        if (t == 0) ph = gq[pos(M / 2)];
     };
     fetch(p);
     for (;;) {
 #pragma unroll
-        for (int i = 0; i < M / NT; ++i) lds[ct_pad(t + NT * i)] = pre[i];
-        if (t == 0) gM = preM;
-        const long long pn = p + gridDim.x;
-        if (pn < P) fetch(pn);  // in flight during this pair's work
-        __syncthreads();
-        for (int k = t; k <= M / 2; k += NT) {
-            cf32 a = lds[ct_pad(k)];
-            cf32 c = k == 0 ? gM : lds[ct_pad(M - k)];
-            if (k == 0) {  // DC and Nyquist of a real signal's spectrum are real
+        for (int i = 0; i < H; ++i) {
+            cf32 a = pa[i], c = pc[i];
+            if (i == 0 && t == 0) {  // DC and Nyquist of a real signal's spectrum are real
                 a.y = 0.f;
                 c.y = 0.f;
             }
-            lds[ct_pad(k)] = c2r_fold(a, c, twc[k]);
-            if (k != 0 && k != M / 2) lds[ct_pad(M - k)] = c2r_fold(c, a, twc[M - k]);
+            const cf32 w = i == 0 ? w0 : cmul(w0, twc[NT * i]);
+            lds[ct_pad(t + NT * i)] = c2r_fold(a, c, w);
+            // twc[M - k] = -conj(twc[k]); Z[M] is Z[0] (not written again)
+            if (i > 0 || t > 0) lds[ct_pad(M - t - NT * i)] = c2r_fold(c, a, cf32{-w.x, w.y});
         }
+        if (t == 0) lds[ct_pad(M / 2)] = cf32{2.f * ph.x, -2.f * ph.y};  // the fold at k = M/2 (twiddle i)
+        const long long pn = p + gridDim.x;
+        if (pn < P) fetch(pn);  // in flight during this pair's work
         __syncthreads();
         ct_fft<float, M, NT, +1, Rs...>(lds, twM);
-        int pi = 0, pj = 0;
-        if (ed.edge) edge_pair((int)p, ed.S, pi, pj);
         float *dst = y + p * L;
+        // lags whose overlap is under ed.K samples come from the edge sums below
         for (int l = t; l < L; l += NT) {
             const int n = (lag0 + l) & (Nz - 1);
-            cf32 v = lds[ct_pad(n >> 1)];
-            dst[l] = c2r_value(scale * ((n & 1) ? v.y : v.x), scale, count_norm, N, Nz, lag0 + l, ed, pi, pj);
+            const cf32 v = lds[ct_pad(n >> 1)];
+            const float r = scale * ((n & 1) ? v.y : v.x);
+            if (!COUNT) {
+                dst[l] = r;
+            } else {
+                const int m = N - abs(lag0 + l);
+                if (m >= (ed.edge ? ed.K : 1)) dst[l] = (float)((double)r / (double)m);
+            }
+        }
+        if (COUNT && ed.edge) {  // lags -(N-1) .. -(N-K+1) and N-K+1 .. N-1
+            int pi = 0, pj = 0;
+            edge_pair((int)p, ed.S, pi, pj);
+            for (int e = t; e < 2 * (ed.K - 1); e += NT) {
+                const int lag = e < ed.K - 1 ? e - (N - 1) : N - ed.K + 1 + (e - (ed.K - 1));
+                const int l = lag - lag0;
+                if (l >= 0 && l < L) dst[l] = c2r_value(0.f, scale, 1, N, Nz, lag, ed, pi, pj);
+            }
         }
         if (pn >= P) break;
         __syncthreads();  // every lane has read this pair before the next one lands
@@ -682,9 +703,8 @@ static bool layout_ok(const GLayout &gl, int S, int nchunks)
     return gl.sh >= 3 && gl.sh < 30 && gl.pl == (1LL << gl.sh) && gl.gs >= P * gl.pl;
 }
 
-// Station blocks (64 x 64, bi <= bj) holding at least one pair i < j, as a device
-// table cached per device and station count.
-// Upper-triangle blocks (bi <= bj) of B stations, cached per device.
+// Upper-triangle blocks (bi <= bj) of B stations, each holding at least one pair
+// i < j, as a device table cached per device and station count.
 static const int2 *block_list(int S, int B, int &ntiles)
 {
     static std::map<std::tuple<int, int, int>, std::pair<const int2 *, int>> cache;
@@ -785,6 +805,34 @@ int launch_gram_split16(const void *d_Ua, const void *d_Ub, int S, int W2, int m
     return 0;
 }
 
+// 512 threads, radix 8 x 8 x 8 x 8: eight values a thread per pass and the next
+// pair's bins in flight keep the kernel within 128 registers, two workgroups a CU
+// (256 threads with radix 16 x 16 x 16 and no prefetch spill and run 19% slower)
+constexpr int kC2rThreads = 512;
+
+template <bool COUNT>
+static int launch_c2r_4096(const cf32 *d_G, const GLayout &gl, int P, const cf32 *twr, const cf32 *twc, int lag0,
+                           int L, float scale, int N, float *d_y, const C2rEdge &ed, hipStream_t st)
+{
+    constexpr int kT = kC2rThreads;
+    const auto fn = k_pair_c2r_4096<kT, COUNT, 8, 8, 8, 8>;
+    const size_t lds = sizeof(cf32) * ct_lds_elems<4096>();
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        return -1;
+    // one persistent workgroup per resident slot
+    static int per_cu = 0;
+    if (!per_cu && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, kT, lds) != hipSuccess ||
+                    per_cu < 1))
+        per_cu = 1;
+    const int nwg = P < per_cu * ncu ? P : per_cu * ncu;
+    ProfScope prof("pair_c2r", st);
+    hipLaunchKernelGGL(fn, dim3(nwg), dim3(kT), lds, st, d_G, gl, P, twr, twc, lag0, L, scale, N, d_y, ed);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
 int launch_pair_c2r(const cf32 *d_G, GLayout gl, int P, int Nz, int N, int W, int lag0, int L, float scale,
                     int count_norm, float *d_y, hipStream_t st, const float *d_edge, int S)
 {
@@ -799,31 +847,15 @@ int launch_pair_c2r(const cf32 *d_G, GLayout gl, int P, int Nz, int N, int W, in
     const DevPlan &pM = get_plan(M, false);
     const void *twc = get_c2r_twiddles(Nz, false);
     if (!pM.ok || !twc) return -1;
-    constexpr int kT = 512;
     // k_pair_c2r_4096 splits a bin's address into a scalar part and a 32-bit offset
-    const bool split_ok = gl.sh == 30 || (gl.sh <= 9 && (((kT - 1) >> gl.sh) + 1) * gl.gs < (1LL << 29));
+    // below (kT >> sh) groups
+    constexpr int kT = kC2rThreads;
+    const bool split_ok = gl.sh == 30 || ((1 << gl.sh) <= kT && (long long)(kT >> gl.sh) * gl.gs + kT < (1LL << 29));
     if (M == 4096 && split_ok) {
-        // 512 threads, radix 8 x 8 x 8 x 8: eight values a thread per pass keep the
-        // kernel small enough in registers for four workgroups a CU
-        const auto fn = k_pair_c2r_4096<kT, 8, 8, 8, 8>;
-        const cf32 *tw8 = (const cf32 *)get_order_twiddles(4096, {8, 8, 8, 8});
-        if (!tw8) return -1;
-        size_t lds = sizeof(cf32) * ct_lds_elems<4096>();
-        int dev = 0, ncu = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
-            return -1;
-        // one persistent workgroup per resident slot
-        static int per_cu = 0;
-        if (!per_cu && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, kT, lds) !=
-                            hipSuccess || per_cu < 1))
-            per_cu = 1;
-        const int nwg = P < per_cu * ncu ? P : per_cu * ncu;
-        ProfScope prof("pair_c2r", st);
-        hipLaunchKernelGGL(fn, dim3(nwg), dim3(kT), lds, st, d_G, gl, P, tw8, (const cf32 *)twc,
-                           lag0, L, scale, count_norm, N, d_y, ed);
-        FPCC_CHECK(hipGetLastError());
-        return 0;
+        const cf32 *twr = (const cf32 *)get_order_twiddles(4096, {8, 8, 8, 8});
+        if (!twr) return -1;
+        return count_norm ? launch_c2r_4096<true>(d_G, gl, P, twr, (const cf32 *)twc, lag0, L, scale, N, d_y, ed, st)
+                          : launch_c2r_4096<false>(d_G, gl, P, twr, (const cf32 *)twc, lag0, L, scale, N, d_y, ed, st);
     }
     size_t lds = sizeof(cf32) * 2 * (size_t)M;
     allow_lds((const void *)k_pair_c2r, lds);
