@@ -334,7 +334,18 @@ constexpr int ct_lds_elems()
     return N + N / 16 + 1;
 }
 
-template <typename T, int N, int NT, int DIR, int R, int NS>
+// LDS layouts of a stage: 0 identity, 1 ct_pad, 2 the low five bits of the index
+// XOR-ed with bits 3..7 (a permutation of every aligned block of 256)
+template <int LAY>
+__host__ __device__ constexpr int ct_lay(int i)
+{
+    if constexpr (LAY == 1) return ct_pad(i);
+    else if constexpr (LAY == 2) return i ^ ((i >> 3) & 31);
+    else return i;
+}
+
+// One pass reading its inputs in layout LIN and writing its outputs in layout LOUT.
+template <typename T, int N, int NT, int DIR, int R, int NS, int LIN = 1, int LOUT = 1>
 __device__ __forceinline__ void ct_pass(C2<T> *lds, const C2<T> *__restrict__ tw)
 {
     constexpr int NB = N / R;
@@ -345,7 +356,7 @@ __device__ __forceinline__ void ct_pass(C2<T> *lds, const C2<T> *__restrict__ tw
         const int b = threadIdx.x + i * NT;
         if (NB % NT == 0 || b < NB) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[i][r] = lds[ct_pad(b + r * NB)];
+            for (int r = 0; r < R; ++r) v[i][r] = lds[ct_lay<LIN>(b + r * NB)];
         }
     }
     __syncthreads();
@@ -365,7 +376,7 @@ __device__ __forceinline__ void ct_pass(C2<T> *lds, const C2<T> *__restrict__ tw
             Dft<R, DIR>::run(v[i]);
             const int base = (b - k) * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) lds[ct_pad(base + r * NS)] = v[i][r];
+            for (int r = 0; r < R; ++r) lds[ct_lay<LOUT>(base + r * NS)] = v[i][r];
         }
     }
     __syncthreads();

@@ -580,7 +580,7 @@ k_pair_c2r(const cf32 *__restrict__ G, GLayout gl, int Nz, FftPlan planM, const 
     }
 }
 
-// Specialisation for Nz = 8192 (M = 4096): compile-time in-place FFT (radices Rs,
+// Specialisation for Nz = 8192 (M = 4096): compile-time in-place FFT (radix 8^4,
 // NT threads), persistent workgroups each walking pairs p, p + grid, ...  Thread t
 // loads bins k = t + NT i (k < M/2) and their mirrors M - k, so the fold into the
 // half-length spectrum Z runs in registers as the spectrum is written to LDS
@@ -594,7 +594,7 @@ __device__ __forceinline__ cf32 c2r_fold(cf32 a, cf32 c, cf32 w)
     return cf32{P.x - wq.y, P.y + wq.x};
 }
 
-template <int NT, bool COUNT, int... Rs>
+template <int NT, bool COUNT>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
 k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__restrict__ twM,
                 const cf32 *__restrict__ twc, int lag0, int L, float scale, int N, float *__restrict__ y,
@@ -602,7 +602,7 @@ k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__res
 {
     constexpr int M = 4096, Nz = 8192;
     constexpr int H = M / 2 / NT;    // bins k < M/2 a thread
-    static_assert(M / 2 % NT == 0, "whole bins a thread");
+    static_assert(NT == 512, "one radix-8 butterfly a thread per pass");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cf32 *lds = reinterpret_cast<cf32 *>(smem_raw);
     const int t = threadIdx.x;
@@ -637,20 +637,25 @@ k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__res
                 c.y = 0.f;
             }
             const cf32 w = i == 0 ? w0 : cmul(w0, twc[NT * i]);
-            lds[ct_pad(t + NT * i)] = c2r_fold(a, c, w);
+            lds[t + NT * i] = c2r_fold(a, c, w);
             // twc[M - k] = -conj(twc[k]); Z[M] is Z[0] (not written again)
-            if (i > 0 || t > 0) lds[ct_pad(M - t - NT * i)] = c2r_fold(c, a, cf32{-w.x, w.y});
+            if (i > 0 || t > 0) lds[M - t - NT * i] = c2r_fold(c, a, cf32{-w.x, w.y});
         }
-        if (t == 0) lds[ct_pad(M / 2)] = cf32{2.f * ph.x, -2.f * ph.y};  // the fold at k = M/2 (twiddle i)
+        if (t == 0) lds[M / 2] = cf32{2.f * ph.x, -2.f * ph.y};  // the fold at k = M/2 (twiddle i)
         const long long pn = p + gridDim.x;
         if (pn < P) fetch(pn);  // in flight during this pair's work
         __syncthreads();
-        ct_fft<float, M, NT, +1, Rs...>(lds, twM);
+        // radix 8 x 8 x 8 x 8; the stages between passes 1-2 and 2-3 are XOR-swizzled,
+        // the others plain: every LDS access of a wave is then free of bank conflicts
+        ct_pass<float, M, NT, +1, 8, 1, 0, 2>(lds, twM);
+        ct_pass<float, M, NT, +1, 8, 8, 2, 2>(lds, twM + 7);
+        ct_pass<float, M, NT, +1, 8, 64, 2, 0>(lds, twM + 7 + 56);
+        ct_pass<float, M, NT, +1, 8, 512, 0, 0>(lds, twM + 7 + 56 + 448);
         float *dst = y + p * L;
         // lags whose overlap is under ed.K samples come from the edge sums below
         for (int l = t; l < L; l += NT) {
             const int n = (lag0 + l) & (Nz - 1);
-            const cf32 v = lds[ct_pad(n >> 1)];
+            const cf32 v = lds[n >> 1];
             const float r = scale * ((n & 1) ? v.y : v.x);
             if (!COUNT) {
                 dst[l] = r;
@@ -815,8 +820,8 @@ static int launch_c2r_4096(const cf32 *d_G, const GLayout &gl, int P, const cf32
                            int L, float scale, int N, float *d_y, const C2rEdge &ed, hipStream_t st)
 {
     constexpr int kT = kC2rThreads;
-    const auto fn = k_pair_c2r_4096<kT, COUNT, 8, 8, 8, 8>;
-    const size_t lds = sizeof(cf32) * ct_lds_elems<4096>();
+    const auto fn = k_pair_c2r_4096<kT, COUNT>;
+    const size_t lds = sizeof(cf32) * 4096;
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
