@@ -57,8 +57,8 @@ def kernel_flops(name, cfg, Nz):
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by
 # the same counters' readings for a 1 GiB copy (profiles/
 # pcc2_allpairs_256st_gram16_hbm_traffic.md), keyed by (config, stack, kernel).
-MEASURED_TRAFFIC = {("c3", "gram16", "gram_split16"): 1636.0e6, ("c3", "gram16", "pair_c2r"): 1336.5e6,
-                    ("c3", "gram16", "station_pcc"): 495.9e6, ("c3", "gram16", "spectra_to_u"): 907.1e6}
+MEASURED_TRAFFIC = {("c3", "gram16", "gram_split16"): 1635.7e6, ("c3", "gram16", "pair_c2r"): 1340.5e6,
+                    ("c3", "gram16", "station_pcc"): 495.8e6, ("c3", "gram16", "spectra_to_u"): 907.2e6}
 
 
 def kernel_bytes(name, cfg, Nz):
