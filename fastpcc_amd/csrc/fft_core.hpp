@@ -382,6 +382,38 @@ __device__ __forceinline__ void ct_pass(C2<T> *lds, const C2<T> *__restrict__ tw
     __syncthreads();
 }
 
+// The same pass out of place: inputs from `in` (layout LIN), outputs to `out`
+// (layout LOUT), one barrier at the end (no lane writes what another still reads).
+template <typename T, int N, int NT, int DIR, int R, int NS, int LIN, int LOUT>
+__device__ __forceinline__ void ct_pass_oop(const C2<T> *in, C2<T> *out, const C2<T> *__restrict__ tw)
+{
+    constexpr int NB = N / R;
+    constexpr int BPT = (NB + NT - 1) / NT;
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+        const int b = threadIdx.x + i * NT;
+        if (NB % NT == 0 || b < NB) {
+            C2<T> v[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = in[ct_lay<LIN>(b + r * NB)];
+            const int k = b % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    C2<T> w = tw[(r - 1) * NS + k];
+                    if (DIR > 0) w.y = -w.y;
+                    v[r] = cmul(v[r], w);
+                }
+            }
+            Dft<R, DIR>::run(v);
+            const int base = (b - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) out[ct_lay<LOUT>(base + r * NS)] = v[r];
+        }
+    }
+    __syncthreads();
+}
+
 template <typename T, int N, int NT, int DIR, int NS, int OFF, int R, int... Rest>
 __device__ __forceinline__ void ct_passes(C2<T> *lds, const C2<T> *__restrict__ tw)
 {

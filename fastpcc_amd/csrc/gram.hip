@@ -604,7 +604,9 @@ k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__res
     constexpr int H = M / 2 / NT;    // bins k < M/2 a thread
     static_assert(NT == 512, "one radix-8 butterfly a thread per pass");
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    cf32 *lds = reinterpret_cast<cf32 *>(smem_raw);
+    // two buffers: each pass reads one and writes the other; the next pair's fold
+    // goes to the buffer its predecessor's last pass read
+    cf32 *lds = reinterpret_cast<cf32 *>(smem_raw), *alt = lds + M;
     const int t = threadIdx.x;
     long long p = blockIdx.x;
     if (p >= P) return;
@@ -647,10 +649,10 @@ k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__res
         __syncthreads();
         // radix 8 x 8 x 8 x 8; the stages between passes 1-2 and 2-3 are XOR-swizzled,
         // the others plain: every LDS access of a wave is then free of bank conflicts
-        ct_pass<float, M, NT, +1, 8, 1, 0, 2>(lds, twM);
-        ct_pass<float, M, NT, +1, 8, 8, 2, 2>(lds, twM + 7);
-        ct_pass<float, M, NT, +1, 8, 64, 2, 0>(lds, twM + 7 + 56);
-        ct_pass<float, M, NT, +1, 8, 512, 0, 0>(lds, twM + 7 + 56 + 448);
+        ct_pass_oop<float, M, NT, +1, 8, 1, 0, 2>(lds, alt, twM);
+        ct_pass_oop<float, M, NT, +1, 8, 8, 2, 2>(alt, lds, twM + 7);
+        ct_pass_oop<float, M, NT, +1, 8, 64, 2, 0>(lds, alt, twM + 7 + 56);
+        ct_pass_oop<float, M, NT, +1, 8, 512, 0, 0>(alt, lds, twM + 7 + 56 + 448);
         float *dst = y + p * L;
         // lags whose overlap is under ed.K samples come from the edge sums below
         for (int l = t; l < L; l += NT) {
@@ -674,7 +676,12 @@ k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__res
             }
         }
         if (pn >= P) break;
-        __syncthreads();  // every lane has read this pair before the next one lands
+        // the next fold writes the buffer the last pass read (every lane is past
+        // that pass's barrier); this pair's outputs are read before the next
+        // fold's barrier, ahead of the next first pass writing them over
+        cf32 *const tmp = lds;
+        lds = alt;
+        alt = tmp;
         p = pn;
     }
 }
@@ -821,7 +828,7 @@ static int launch_c2r_4096(const cf32 *d_G, const GLayout &gl, int P, const cf32
 {
     constexpr int kT = kC2rThreads;
     const auto fn = k_pair_c2r_4096<kT, COUNT>;
-    const size_t lds = sizeof(cf32) * 4096;
+    const size_t lds = sizeof(cf32) * 2 * 4096;
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
