@@ -9,6 +9,7 @@
 #include "edge.hpp"
 #include "kernels.hpp"
 #include "profile.hpp"
+#include "pairs.hpp"
 
 namespace fpcc {
 
@@ -101,17 +102,6 @@ const BigPlan &big_plan(int n)
     auto it = cache.find({dev, n});
     if (it != cache.end()) return it->second;
     return cache.emplace(std::make_pair(dev, n), make_big_plan(n)).first->second;
-}
-
-__device__ __forceinline__ void pair_of_big(int p, int S, int &i, int &j)
-{
-    double b = 2.0 * S - 1.0;
-    int ii = (int)floor((b - sqrt(b * b - 8.0 * p)) * 0.5);
-    if (ii < 0) ii = 0;
-    while (ii > 0 && ii * S - ii * (ii + 1) / 2 > p) --ii;
-    while ((ii + 1) * S - (ii + 1) * (ii + 2) / 2 <= p) ++ii;
-    i = ii;
-    j = p - (ii * S - ii * (ii + 1) / 2) + ii + 1;
 }
 
 // index of W_n^{c r} in the full table: c < N2 and r < N1, so c*r < n
@@ -314,7 +304,7 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
         const int q = b / a.Wb;
         w = a.wsel + (b - q * a.Wb);
         p = a.pair0 + q;
-        if (a.norm == 1) pair_of_big(p, a.S, i, j);
+        if (a.norm == 1) pair_index(p, a.S, i, j);
     }
     for (int idx = threadIdx.x; idx < kBigCols * N1; idx += blockDim.x) {
         const int t = idx % kBigCols, r = idx / kBigCols;
@@ -347,7 +337,7 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
                     // short overlap: the FP32 transform's absolute error, divided by
                     // the overlap's small energy, would dominate; sum it exactly
                     int ei = i, ej = j;
-                    if (a.norm != 1) pair_of_big(p, a.S, ei, ej);
+                    if (a.norm != 1) pair_index(p, a.S, ei, ej);
                     const int w0 = a.esum ? 0 : w, w1 = a.esum ? a.Wall : w + 1;
                     const double c = edge_corr(a.edge, a.K, a.Wall, ei, ej, w0, w1, lag, m, a.norm == 2);
                     o = c * (double)a.scale * (double)n;  // the transform's scale: v.x is n c
@@ -406,7 +396,7 @@ k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, 
     const cf32 *F2 = a.F2 ? a.F2 : a.F;
     if (op == ROW_INV_CROSS) {
         const int q = b / a.Wb;
-        pair_of_big(a.pair0 + q, a.Sst, ci, cj);
+        pair_index(a.pair0 + q, a.Sst, ci, cj);
         cw0 = a.wsum ? 0 : a.wsel + (b - q * a.Wb);
         cw1 = a.wsum ? a.Wfull : cw0 + 1;
     }

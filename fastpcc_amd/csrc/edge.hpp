@@ -8,20 +8,9 @@
 // x[N - 2K + t].
 #pragma once
 #include <hip/hip_runtime.h>
+#include "pairs.hpp"
 
 namespace fpcc {
-
-// pair p of S stations -> (i, j), i < j, row-major upper triangle
-__device__ __forceinline__ void edge_pair(int p, int S, int &i, int &j)
-{
-    const double b = 2.0 * S - 1.0;
-    int ii = (int)floor((b - sqrt(b * b - 8.0 * p)) * 0.5);
-    if (ii < 0) ii = 0;
-    while (ii > 0 && ii * S - ii * (ii + 1) / 2 > p) --ii;
-    while ((ii + 1) * S - (ii + 1) * (ii + 2) / 2 <= p) ++ii;
-    i = ii;
-    j = p - (ii * S - ii * (ii + 1) / 2) + ii + 1;
-}
 
 // sum over windows [w0, w1) of sum_n x1[n] x2[n + lag] (m = N - |lag| terms),
 // x1 of station i, x2 of station j; signs: x1 >= 0 and x2 > 0 as +-1 (1-bit)

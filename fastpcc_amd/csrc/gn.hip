@@ -18,6 +18,7 @@
 #include "kernels.hpp"
 #include "profile.hpp"
 #include "edge.hpp"
+#include "pairs.hpp"
 
 namespace fpcc {
 
@@ -122,17 +123,6 @@ k_station_real_batch(RealBatch jb, int Nz, int sign_rule, FftPlan planM, const C
                         jb.E[j] ? jb.E[j] + (long long)sw * jb.lde[j] : nullptr, smem_raw);
 }
 
-__device__ __forceinline__ void pair_of_gn(int p, int S, int &i, int &j)
-{
-    double b = 2.0 * S - 1.0;
-    int ii = (int)floor((b - sqrt(b * b - 8.0 * p)) * 0.5);
-    if (ii < 0) ii = 0;
-    while (ii > 0 && ii * S - ii * (ii + 1) / 2 > p) --ii;
-    while ((ii + 1) * S - (ii + 1) * (ii + 2) / 2 <= p) ++ii;
-    i = ii;
-    j = p - (ii * S - ii * (ii + 1) / 2) + ii + 1;
-}
-
 // mode 0: y[p][w][L]; 1: mean over windows.  norm 0: energies (GNCC); 1: overlap count.
 template <typename T>
 __global__ void __launch_bounds__(512)
@@ -157,7 +147,7 @@ k_pair_gn(const C2<T> *__restrict__ Xa, const C2<T> *__restrict__ Xb, long long 
         w1 = W;
     }
     int i, j;
-    pair_of_gn(p, S, i, j);
+    pair_index(p, S, i, j);
     const int T_ = blockDim.x;
     const T inv_nz = T(1) / (T)Nz;
     for (int l = threadIdx.x; l < L; l += T_) acc[l] = 0.0f;

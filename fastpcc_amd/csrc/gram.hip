@@ -571,7 +571,7 @@ k_pair_c2r(const cf32 *__restrict__ G, GLayout gl, int Nz, FftPlan planM, const 
     __syncthreads();
     cf32 *z = fft_lds<float, +1>(buf0, buf1, planM, twM);
     int pi = 0, pj = 0;
-    if (ed.edge) edge_pair((int)p, ed.S, pi, pj);
+    if (ed.edge) pair_index((int)p, ed.S, pi, pj);
     float *dst = y + p * L;
     for (int l = threadIdx.x; l < L; l += blockDim.x) {
         const int n = (lag0 + l) & (Nz - 1);
@@ -668,7 +668,7 @@ k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__res
         }
         if (COUNT && ed.edge) {  // lags -(N-1) .. -(N-K+1) and N-K+1 .. N-1
             int pi = 0, pj = 0;
-            edge_pair((int)p, ed.S, pi, pj);
+            pair_index((int)p, ed.S, pi, pj);
             for (int e = t; e < 2 * (ed.K - 1); e += NT) {
                 const int lag = e < ed.K - 1 ? e - (N - 1) : N - ed.K + 1 + (e - (ed.K - 1));
                 const int l = lag - lag0;

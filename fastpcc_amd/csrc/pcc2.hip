@@ -18,6 +18,7 @@
 #include "engine.hpp"
 #include "kernels.hpp"
 #include "profile.hpp"
+#include "pairs.hpp"
 
 namespace fpcc {
 
@@ -210,18 +211,6 @@ __global__ void __launch_bounds__(256) k_ampnormf_rows(cf32 *__restrict__ z, int
     }
 }
 
-__device__ __forceinline__ void pair_of(int p, int S, int &i, int &j)
-{
-    // p = i*S - i*(i+1)/2 + (j-i-1), i<j
-    double b = 2.0 * S - 1.0;
-    int ii = (int)floor((b - sqrt(b * b - 8.0 * p)) * 0.5);
-    if (ii < 0) ii = 0;
-    while (ii > 0 && ii * S - ii * (ii + 1) / 2 > p) --ii;
-    while ((ii + 1) * S - (ii + 1) * (ii + 2) / 2 <= p) ++ii;
-    i = ii;
-    j = p - (ii * S - ii * (ii + 1) / 2) + ii + 1;
-}
-
 // Fold conj(F1)F2 into the half-length spectrum Z (see file header) in LDS.
 __device__ __forceinline__ void fold_cross(const cf32 *__restrict__ F1, const cf32 *__restrict__ F2, int Nz,
                                            const cf32 *__restrict__ twc, cf32 *Z, bool accumulate)
@@ -276,7 +265,7 @@ k_pair_pcc2(const cf32 *__restrict__ F, int S, int W, int Nz, FftPlan planM, con
         w1 = W;
     }
     int i, j;
-    pair_of(p, S, i, j);
+    pair_index(p, S, i, j);
     const int T = blockDim.x;
     if (mode == 2) {
         for (int w = w0; w < w1; ++w) {

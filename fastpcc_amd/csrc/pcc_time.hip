@@ -15,22 +15,12 @@
 #include "engine.hpp"
 #include "kernels.hpp"
 #include "profile.hpp"
+#include "pairs.hpp"
 
 namespace fpcc {
 
 constexpr int kPtLags = 256;
 constexpr int kPtTile = 256;
-
-__device__ __forceinline__ void pair_of_t(int p, int S, int &i, int &j)
-{
-    double b = 2.0 * S - 1.0;
-    int ii = (int)floor((b - sqrt(b * b - 8.0 * p)) * 0.5);
-    if (ii < 0) ii = 0;
-    while (ii > 0 && ii * S - ii * (ii + 1) / 2 > p) --ii;
-    while ((ii + 1) * S - (ii + 1) * (ii + 2) / 2 <= p) ++ii;
-    i = ii;
-    j = p - (ii * S - ii * (ii + 1) / 2) + ii + 1;
-}
 
 // y rows start at out_off within rows of out_ld floats.
 // Hardware square root and power (v_sqrt_f32; v_log_f32 and v_exp_f32, i.e.
@@ -62,7 +52,7 @@ k_pcc_time(const cf32 *__restrict__ phi, int S, int W, int N, int lag_first, int
         w1 = W;
     }
     int i, j;
-    pair_of_t(p, S, i, j);
+    pair_index(p, S, i, j);
     const int tid = threadIdx.x;
     const int lagA = lag_first + lb * kPtLags;
     const int lagB = min(lagA + kPtLags - 1, lag_first + L - 1);
