@@ -113,6 +113,10 @@ orc_wfamily *orc_wpcc_family(unsigned N, int Lag1, int Lag2, double pmin, double
     w->s0 = s0;
     w->op1 = op1;
     const unsigned S = w->S;
+    if (S == 0) { /* V = 0: no scales */
+        free(w);
+        return NULL;
+    }
     w->scale = malloc(sizeof(double) * S);
     w->Ls = malloc(sizeof(unsigned) * S);
     w->center = malloc(sizeof(int) * S);
