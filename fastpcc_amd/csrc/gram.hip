@@ -15,6 +15,7 @@
 //   k_gram, k_gram16_dma  U -> G[p][k] (pair-major rows of ld >= Nz/2+1 bins)
 //   k_pair_c2r       G[p] -> half-length complex IFFT -> lag window -> y[p][L]
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <tuple>
 #include <mutex>
@@ -833,11 +834,15 @@ static int launch_c2r_4096(const cf32 *d_G, const GLayout &gl, int P, const cf32
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
         return -1;
-    // one persistent workgroup per resident slot
-    static int per_cu = 0;
-    if (!per_cu && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, kT, lds) != hipSuccess ||
-                    per_cu < 1))
-        per_cu = 1;
+    // one persistent workgroup per resident slot (the query is the same on every
+    // call and device of this build; concurrent first calls store the same value)
+    static std::atomic<int> per_cu_cached{0};
+    int per_cu = per_cu_cached.load(std::memory_order_relaxed);
+    if (!per_cu) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, kT, lds) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        per_cu_cached.store(per_cu, std::memory_order_relaxed);
+    }
     const int nwg = P < per_cu * ncu ? P : per_cu * ncu;
     ProfScope prof("pair_c2r", st);
     hipLaunchKernelGGL(fn, dim3(nwg), dim3(kT), lds, st, d_G, gl, P, twr, twc, lag0, L, scale, N, d_y, ed);
