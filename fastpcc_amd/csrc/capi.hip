@@ -6,7 +6,9 @@
 #include <algorithm>
 #include <map>
 #include <memory>
+#include <atomic>
 #include <mutex>
+#include <thread>
 #include <vector>
 #include "../../include/fastpcc.h"
 #include "bigfft.hpp"
@@ -123,30 +125,108 @@ private:
     DeviceBuf odx_, ody_, odws_;
 };
 
+// Host threads for the row gathers of the host entry points (pageable rows into
+// pinned staging and back): one thread copies at a fraction of host memory
+// bandwidth.  Inputs under two pieces stay on the calling thread.
+constexpr unsigned kGatherThreads = 8;
+constexpr size_t kPieceBytes = (size_t)8 << 20;
+
+static unsigned gather_threads(size_t bytes)
+{
+    if (bytes < 2 * kPieceBytes) return 1;
+    const unsigned hw = std::thread::hardware_concurrency();
+    return std::max(1u, std::min(kGatherThreads, hw ? hw : 1u));
+}
+
+// fn(k) for k in [0, n) spread over up to kGatherThreads threads (the caller's
+// included) when the rows hold at least 4 MB; returns once every call is done.
+template <typename Fn>
+static void rows_parallel(size_t n, size_t bytes, Fn fn)
+{
+    unsigned nt = bytes >= ((size_t)4 << 20) ? gather_threads(2 * kPieceBytes) : 1;
+    if (nt > n) nt = (unsigned)std::max<size_t>(n, 1);
+    std::vector<std::thread> pool;
+    try {
+        for (unsigned w = 1; w < nt; ++w)
+            pool.emplace_back([&, w] {
+                for (size_t k = w; k < n; k += nt) fn(k);
+            });
+    } catch (...) {  // the rows of threads that did not start are copied below
+    }
+    for (size_t k = 0; k < n; k += nt) fn(k);
+    for (unsigned w = (unsigned)pool.size() + 1; w < nt; ++w)
+        for (size_t k = w; k < n; k += nt) fn(k);
+    for (auto &t : pool) t.join();
+}
+
 // Rows of two stations (x1 rows, then x2 rows, each of `row` floats) gathered
 // into pinned staging and copied to the device in pieces of about 8 MB on the
-// null stream, so the gather of one piece overlaps the copy of the last.
-// `before(tr)` runs before row tr of either station is read.  -1 on a null row.
+// null stream, so the gather of one piece overlaps the copy of the last; the
+// rows of a piece are spread over the gather threads.  `before(tr)` runs (on the
+// calling thread) before row tr of either station is read.  -1 on a null row.
 template <typename Before>
 int upload_rows(const float *const *x1, const float *const *x2, unsigned Tr, size_t row, float *hx, float *dx,
                 Before before)
 {
     for (unsigned tr = 0; tr < Tr; ++tr)
         if (!x1[tr] || !x2[tr]) return -1;
-    const size_t piece = std::max<size_t>(row, ((size_t)8 << 20) / sizeof(float));
-    size_t done = 0, filled = 0;
-    for (unsigned k = 0; k < 2 * Tr; ++k) {
-        const unsigned tr = k < Tr ? k : k - Tr;
-        if (k < Tr) before(tr);
-        const float *src = k < Tr ? x1[tr] : x2[tr];
-        if (!src) return -1;
-        memcpy(hx + filled, src, sizeof(float) * row);
-        filled += row;
-        if (filled - done >= piece || k + 1 == 2 * Tr) {
-            FPCC_CHECK(hipMemcpyAsync(dx + done, hx + done, sizeof(float) * (filled - done), hipMemcpyHostToDevice,
-                                      nullptr));
-            done = filled;
+    const size_t nrows = 2 * (size_t)Tr;
+    const size_t prow = std::max<size_t>(1, kPieceBytes / (sizeof(float) * std::max<size_t>(row, 1)));
+    const size_t npieces = (nrows + prow - 1) / prow;
+    auto src_of = [&](size_t k) { return k < Tr ? x1[k] : x2[k - Tr]; };
+    auto copy_piece_rows = [&](size_t pc, unsigned w, unsigned nw) {
+        for (size_t k = pc * prow + w; k < std::min(nrows, (pc + 1) * prow); k += nw)
+            memcpy(hx + k * row, src_of(k), sizeof(float) * row);
+    };
+    auto send = [&](size_t pc) {
+        const size_t k0 = pc * prow, k1 = std::min(nrows, (pc + 1) * prow);
+        return hipMemcpyAsync(dx + k0 * row, hx + k0 * row, sizeof(float) * (k1 - k0) * row,
+                              hipMemcpyHostToDevice, nullptr);
+    };
+    const unsigned nt = gather_threads(sizeof(float) * nrows * row);
+    if (nt > 1) {
+        // pieces are released in order once their before() calls have returned;
+        // of nw copiers (the calling thread is copier 0), copier w takes rows
+        // w, w + nw, ... of each released piece
+        std::atomic<size_t> released{0};
+        std::atomic<unsigned> nw{0};
+        std::vector<std::atomic<unsigned>> finished(npieces);
+        for (auto &f : finished) f.store(0);
+        std::vector<std::thread> pool;
+        try {
+            for (unsigned w = 1; w < nt; ++w)
+                pool.emplace_back([&, w] {
+                    for (size_t pc = 0; pc < npieces; ++pc) {
+                        while (released.load(std::memory_order_acquire) <= pc) std::this_thread::yield();
+                        copy_piece_rows(pc, w, nw.load(std::memory_order_relaxed));
+                        finished[pc].fetch_add(1, std::memory_order_release);
+                    }
+                });
+        } catch (...) {  // fewer threads than asked for: those that started share the rows
         }
+        const unsigned n = (unsigned)pool.size() + 1;
+        nw.store(n, std::memory_order_relaxed);  // published to the copiers by the first release
+        hipError_t err = hipSuccess;
+        for (size_t pc = 0; pc < npieces; ++pc) {
+            for (size_t k = pc * prow; k < std::min<size_t>(Tr, (pc + 1) * prow); ++k) before((unsigned)k);
+            released.store(pc + 1, std::memory_order_release);
+            copy_piece_rows(pc, 0, n);
+            while (finished[pc].load(std::memory_order_acquire) < n - 1) std::this_thread::yield();
+            if (err == hipSuccess) err = send(pc);
+        }
+        for (auto &t : pool) t.join();
+        if (err != hipSuccess) {
+            fprintf(stderr, "fastpcc: %s\n", hipGetErrorString(err));
+            return -1;
+        }
+        return 0;
+    }
+    for (size_t pc = 0; pc < npieces; ++pc) {
+        for (size_t k = pc * prow; k < std::min(nrows, (pc + 1) * prow); ++k) {
+            if (k < Tr) before((unsigned)k);
+            memcpy(hx + k * row, src_of(k), sizeof(float) * row);
+        }
+        FPCC_CHECK(send(pc));
     }
     return 0;
 }
@@ -433,7 +513,8 @@ int wpcc_pair_host(const WFamily &f, float **y, float **x1, float **x2, unsigned
     if ((r = wpcc_pair_stage(2, (int)Tr, (int)N, f, lag0, L, true, (char *)b.dws, (float *)b.dy, nullptr)))
         return r;
     FPCC_CHECK(hipMemcpy(b.hy, b.dy, sizeof(float) * ylen, hipMemcpyDeviceToHost));
-    for (unsigned tr = 0; tr < Tr; ++tr) memcpy(y[tr], (float *)b.hy + (size_t)tr * L, sizeof(float) * L);
+    rows_parallel(Tr, sizeof(float) * ylen,
+                  [&](size_t tr) { memcpy(y[tr], (float *)b.hy + tr * L, sizeof(float) * L); });
     return 0;
 }
 
@@ -494,11 +575,11 @@ int pcc_time_host(int pcc1, float **y, const float *const *x1, const float *cons
     if ((r = launch_pcc_time(phi, 2, Tr, N, Lag1 + l1, Lp - l1, v, pcc1, 0, (float *)b.dy, L, l1, nullptr)))
         return r;
     FPCC_CHECK(hipMemcpy(b.hy, b.dy, sizeof(float) * ylen, hipMemcpyDeviceToHost));
-    for (unsigned tr = 0; tr < Tr; ++tr) {
-        const float *src = (float *)b.hy + (size_t)tr * L;
+    rows_parallel(Tr, sizeof(float) * ylen, [&](size_t tr) {
+        const float *src = (float *)b.hy + tr * L;
         if (zero_rows) memcpy(y[tr], src, sizeof(float) * L);
         else if (Lp > l1) memcpy(y[tr] + l1, src + l1, sizeof(float) * (Lp - l1));
-    }
+    });
     return 0;
 }
 
@@ -534,7 +615,8 @@ int pair_set_host(int kind, float **y, float **x1, float **x2, unsigned N, unsig
     if ((r = station_stage(&job, d, (const float *)b.dx, (char *)b.dws, nullptr))) return r;
     if ((r = pair_stage(&job, d, (char *)b.dws, (float *)b.dy, nullptr))) return r;
     FPCC_CHECK(hipMemcpy(b.hy, b.dy, sizeof(float) * ylen, hipMemcpyDeviceToHost));
-    for (unsigned tr = 0; tr < Tr; ++tr) memcpy(y[tr], (float *)b.hy + (size_t)tr * d.L, sizeof(float) * d.L);
+    rows_parallel(Tr, sizeof(float) * (size_t)Tr * d.L,
+                  [&](size_t tr) { memcpy(y[tr], (float *)b.hy + tr * d.L, sizeof(float) * d.L); });
     return 0;
 }
 

@@ -89,3 +89,18 @@ def test_gram_stack_matches_oracle(fp, orc, S, W, N, lag):
     print(f"  max|gram-oracle|={err:.3e} max|gram-plain| (all pairs)={np.abs(y - yp).max():.3e}")
     assert err <= TOL
     assert np.abs(y - yp).max() <= TOL
+
+
+def test_pcc2_set_many_rows(fp, orc):
+    """256 windows of 21600 samples (44 MB of input rows): the host entry gathers
+    the rows on several threads in 8 MB pieces and copies the output rows back
+    on several threads; every row against the oracle."""
+    rng = np.random.default_rng(21600)
+    x1 = rng.standard_normal((256, 21600)).astype(np.float32)
+    x2 = rng.standard_normal((256, 21600)).astype(np.float32)
+    print("pcc2_set Tr=256 N=21600 lags (-3000,3000)", flush=True)
+    y = fp.pcc2_set(x1, x2, -3000, 3000)
+    yr = orc.run("pcc2", x1, x2, -3000, 3000)
+    err = np.abs(y - yr).max()
+    print(f"  max|gpu-oracle|={err:.3e}")
+    assert y.shape == yr.shape and err <= TOL
