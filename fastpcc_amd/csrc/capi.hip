@@ -159,31 +159,31 @@ static void rows_parallel(size_t n, size_t bytes, Fn fn)
     for (auto &t : pool) t.join();
 }
 
-// Rows of two stations (x1 rows, then x2 rows, each of `row` floats) gathered
-// into pinned staging and copied to the device in pieces of about 8 MB on the
-// null stream, so the gather of one piece overlaps the copy of the last; the
-// rows of a piece are spread over the gather threads.  `before(tr)` runs (on the
-// calling thread) before row tr of either station is read.  -1 on a null row.
-template <typename Before>
-int upload_rows(const float *const *x1, const float *const *x2, unsigned Tr, size_t row, float *hx, float *dx,
-                Before before)
+// `total` floats in rows of `row` (the last row may be shorter; src_of(k) is
+// row k) gathered into pinned staging and copied to the device in pieces of
+// about 8 MB on the null stream, so the gather of one piece overlaps the copy of
+// the last; the rows of a piece are spread over the gather threads.
+// `before(k)` runs (on the calling thread) before row k < nbefore is read.
+template <typename SrcOf, typename Before>
+int upload_gather(size_t total, size_t row, SrcOf src_of, size_t nbefore, Before before, float *hx, float *dx)
 {
-    for (unsigned tr = 0; tr < Tr; ++tr)
-        if (!x1[tr] || !x2[tr]) return -1;
-    const size_t nrows = 2 * (size_t)Tr;
-    const size_t prow = std::max<size_t>(1, kPieceBytes / (sizeof(float) * std::max<size_t>(row, 1)));
+    if (!total) return 0;
+    row = std::max<size_t>(row, 1);
+    const size_t nrows = (total + row - 1) / row;
+    const size_t prow = std::max<size_t>(1, kPieceBytes / (sizeof(float) * row));
     const size_t npieces = (nrows + prow - 1) / prow;
-    auto src_of = [&](size_t k) { return k < Tr ? x1[k] : x2[k - Tr]; };
     auto copy_piece_rows = [&](size_t pc, unsigned w, unsigned nw) {
         for (size_t k = pc * prow + w; k < std::min(nrows, (pc + 1) * prow); k += nw)
-            memcpy(hx + k * row, src_of(k), sizeof(float) * row);
+            memcpy(hx + k * row, src_of(k), sizeof(float) * std::min(row, total - k * row));
     };
     auto send = [&](size_t pc) {
-        const size_t k0 = pc * prow, k1 = std::min(nrows, (pc + 1) * prow);
-        return hipMemcpyAsync(dx + k0 * row, hx + k0 * row, sizeof(float) * (k1 - k0) * row,
-                              hipMemcpyHostToDevice, nullptr);
+        const size_t f0 = pc * prow * row, f1 = std::min(total, (pc + 1) * prow * row);
+        return hipMemcpyAsync(dx + f0, hx + f0, sizeof(float) * (f1 - f0), hipMemcpyHostToDevice, nullptr);
     };
-    const unsigned nt = gather_threads(sizeof(float) * nrows * row);
+    auto befores = [&](size_t pc) {
+        for (size_t k = pc * prow; k < std::min(nbefore, (pc + 1) * prow); ++k) before(k);
+    };
+    const unsigned nt = gather_threads(sizeof(float) * total);
     if (nt > 1) {
         // pieces are released in order once their before() calls have returned;
         // of nw copiers (the calling thread is copier 0), copier w takes rows
@@ -208,7 +208,7 @@ int upload_rows(const float *const *x1, const float *const *x2, unsigned Tr, siz
         nw.store(n, std::memory_order_relaxed);  // published to the copiers by the first release
         hipError_t err = hipSuccess;
         for (size_t pc = 0; pc < npieces; ++pc) {
-            for (size_t k = pc * prow; k < std::min<size_t>(Tr, (pc + 1) * prow); ++k) before((unsigned)k);
+            befores(pc);
             released.store(pc + 1, std::memory_order_release);
             copy_piece_rows(pc, 0, n);
             while (finished[pc].load(std::memory_order_acquire) < n - 1) std::this_thread::yield();
@@ -222,11 +222,94 @@ int upload_rows(const float *const *x1, const float *const *x2, unsigned Tr, siz
         return 0;
     }
     for (size_t pc = 0; pc < npieces; ++pc) {
-        for (size_t k = pc * prow; k < std::min(nrows, (pc + 1) * prow); ++k) {
-            if (k < Tr) before((unsigned)k);
-            memcpy(hx + k * row, src_of(k), sizeof(float) * row);
-        }
+        befores(pc);
+        copy_piece_rows(pc, 0, 1);
         FPCC_CHECK(send(pc));
+    }
+    return 0;
+}
+
+// Rows of two stations (x1 rows, then x2 rows, each of `row` floats), the
+// reference's row-pointer inputs.  `before(tr)` runs before row tr of either
+// station is read.  -1 on a null row.
+template <typename Before>
+int upload_rows(const float *const *x1, const float *const *x2, unsigned Tr, size_t row, float *hx, float *dx,
+                Before before)
+{
+    for (unsigned tr = 0; tr < Tr; ++tr)
+        if (!x1[tr] || !x2[tr]) return -1;
+    return upload_gather(
+        2 * (size_t)Tr * row, row, [&](size_t k) { return k < Tr ? x1[k] : x2[k - Tr]; }, Tr,
+        [&](size_t k) { before((unsigned)k); }, hx, dx);
+}
+
+// `total` floats from the device (dy) to the caller's pageable h_y through
+// pinned staging (hy): the copies to staging go in pieces of about 8 MB on the
+// null stream, and the gather threads move each piece on as soon as it lands.
+static int download_scatter(size_t total, const float *dy, float *hy, float *h_y)
+{
+    if (!total) return 0;
+    const size_t pf = kPieceBytes / sizeof(float), npieces = (total + pf - 1) / pf;
+    const unsigned nt = gather_threads(sizeof(float) * total);
+    std::vector<hipEvent_t> ev(npieces, nullptr);
+    bool events = nt > 1;
+    for (size_t pc = 0; events && pc < npieces; ++pc)
+        events = hipEventCreateWithFlags(&ev[pc], hipEventDisableTiming) == hipSuccess;
+    auto drop_events = [&] {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    };
+    if (!events) {
+        drop_events();
+        FPCC_CHECK(hipMemcpy(hy, dy, sizeof(float) * total, hipMemcpyDeviceToHost));
+        memcpy(h_y, hy, sizeof(float) * total);
+        return 0;
+    }
+    hipError_t err = hipSuccess;
+    for (size_t pc = 0; pc < npieces && err == hipSuccess; ++pc) {
+        const size_t f0 = pc * pf, f1 = std::min(total, f0 + pf);
+        err = hipMemcpyAsync(hy + f0, dy + f0, sizeof(float) * (f1 - f0), hipMemcpyDeviceToHost, nullptr);
+        if (err == hipSuccess) err = hipEventRecord(ev[pc], nullptr);
+    }
+    if (err != hipSuccess) {
+        (void)hipDeviceSynchronize();
+        drop_events();
+        fprintf(stderr, "fastpcc: %s\n", hipGetErrorString(err));
+        return -1;
+    }
+    // of nw copiers (the calling thread is copier 0), copier w moves the
+    // quarter-megabyte blocks w, w + nw, ... of each piece that has landed
+    constexpr size_t kBlock = (size_t)1 << 16;  // floats
+    auto move_piece = [&](size_t pc, unsigned w, unsigned nw) {
+        const size_t f0 = pc * pf, f1 = std::min(total, f0 + pf);
+        for (size_t b0 = f0 + w * kBlock; b0 < f1; b0 += (size_t)nw * kBlock)
+            memcpy(h_y + b0, hy + b0, sizeof(float) * std::min(kBlock, f1 - b0));
+    };
+    std::atomic<size_t> landed{0};
+    std::atomic<unsigned> nw{0};
+    std::vector<std::thread> pool;
+    try {
+        for (unsigned w = 1; w < nt; ++w)
+            pool.emplace_back([&, w] {
+                for (size_t pc = 0; pc < npieces; ++pc) {
+                    while (landed.load(std::memory_order_acquire) <= pc) std::this_thread::yield();
+                    move_piece(pc, w, nw.load(std::memory_order_relaxed));
+                }
+            });
+    } catch (...) {  // fewer threads than asked for: those that started share the blocks
+    }
+    const unsigned n = (unsigned)pool.size() + 1;
+    nw.store(n, std::memory_order_relaxed);  // published to the copiers by the first landing
+    for (size_t pc = 0; pc < npieces; ++pc) {
+        if (err == hipSuccess) err = hipEventSynchronize(ev[pc]);
+        landed.store(pc + 1, std::memory_order_release);  // on a failure the copiers still finish
+        move_piece(pc, 0, n);
+    }
+    for (auto &t : pool) t.join();
+    drop_events();
+    if (err != hipSuccess) {
+        fprintf(stderr, "fastpcc: %s\n", hipGetErrorString(err));
+        return -1;
     }
     return 0;
 }
@@ -829,12 +912,16 @@ extern "C" int fpcc_allpairs(const fpcc_job *job, const float *h_x, float *h_y)
     const size_t ylen = fpcc_output_len(job), ws = layout(job, d).total;
     const size_t xlen = (size_t)(job->n_stations - 1) * job->station_stride + (size_t)job->n_windows * job->win_len;
     EntryBufs b;
-    if (!b.get(0, 0, sizeof(float) * xlen, sizeof(float) * ylen, ws)) return -1;
-    FPCC_CHECK(hipMemcpy(b.dx, h_x, sizeof(float) * xlen, hipMemcpyHostToDevice));
+    if (!b.get(sizeof(float) * xlen, sizeof(float) * ylen, sizeof(float) * xlen, sizeof(float) * ylen, ws))
+        return -1;
+    constexpr size_t kRow = (size_t)1 << 18;  // floats a gathered piece of the input
+    if ((r = upload_gather(
+             xlen, kRow, [&](size_t k) { return h_x + k * kRow; }, 0, [](size_t) {}, (float *)b.hx,
+             (float *)b.dx)))
+        return r;
     r = fpcc_allpairs_device(job, (const float *)b.dx, (float *)b.dy, b.dws, ws, nullptr);
     if (r) return r;
-    FPCC_CHECK(hipMemcpy(h_y, b.dy, sizeof(float) * ylen, hipMemcpyDeviceToHost));
-    return 0;
+    return download_scatter(ylen, (const float *)b.dy, (float *)b.hy, h_y);
 }
 
 // ---- reference-compatible entries (src/FFTapps.h:11-16) ----

@@ -104,3 +104,26 @@ def test_pcc2_set_many_rows(fp, orc):
     err = np.abs(y - yr).max()
     print(f"  max|gpu-oracle|={err:.3e}")
     assert y.shape == yr.shape and err <= TOL
+
+
+def test_allpairs_host_buffers_large(fp, hip):
+    """fpcc_allpairs from host buffers big enough for the threaded pinned staging
+    (64 stations x 86400 samples = 22 MB in, 2016 pairs x 2001 lags = 16 MB out)
+    gives bit for bit the device-buffer run of the same job."""
+    import ctypes
+    S, W, N = 64, 24, 3600
+    x, _ = synth.make_stations(S, W * N, 300, seed=64)
+    y_host = fp.allpairs(x, W, N, -1000, 1000, kind="pcc2", stack="gram16").ravel()
+    L = fp.api._lib.lib()
+    job = fp.make_job("pcc2", S, W, N, -1000, 1000, stack="gram16")
+    ws_bytes = L.fpcc_workspace_size(ctypes.byref(job))
+    ny = L.fpcc_output_len(ctypes.byref(job))
+    st = hip.stream()
+    dx, dy, dws = hip.alloc(x.nbytes), hip.alloc(4 * ny), hip.alloc(ws_bytes)
+    hip.h2d(dx, x)
+    assert L.fpcc_allpairs_device(ctypes.byref(job), dx, dy, dws, ws_bytes, st) == 0
+    hip.sync(st)
+    y_dev = np.empty(ny, np.float32)
+    hip.d2h(y_dev, dy)
+    print(f"host vs device buffers: {np.abs(y_host - y_dev).max():.3e}", flush=True)
+    assert np.array_equal(y_host, y_dev)
