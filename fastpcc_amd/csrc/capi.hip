@@ -911,17 +911,28 @@ extern "C" int fpcc_allpairs(const fpcc_job *job, const float *h_x, float *h_y)
     if (!h_x || !h_y) return -1;
     const size_t ylen = fpcc_output_len(job), ws = layout(job, d).total;
     const size_t xlen = (size_t)(job->n_stations - 1) * job->station_stride + (size_t)job->n_windows * job->win_len;
+    {
+        // through pinned staging when it can be had
+        EntryBufs b;
+        if (b.get(sizeof(float) * xlen, sizeof(float) * ylen, sizeof(float) * xlen, sizeof(float) * ylen, ws)) {
+            constexpr size_t kRow = (size_t)1 << 18;  // floats a gathered piece of the input
+            if ((r = upload_gather(
+                     xlen, kRow, [&](size_t k) { return h_x + k * kRow; }, 0, [](size_t) {}, (float *)b.hx,
+                     (float *)b.dx)))
+                return r;
+            r = fpcc_allpairs_device(job, (const float *)b.dx, (float *)b.dy, b.dws, ws, nullptr);
+            if (r) return r;
+            return download_scatter(ylen, (const float *)b.dy, (float *)b.hy, h_y);
+        }
+    }
+    // else (pinned host memory refused) straight from and to the pageable buffers
     EntryBufs b;
-    if (!b.get(sizeof(float) * xlen, sizeof(float) * ylen, sizeof(float) * xlen, sizeof(float) * ylen, ws))
-        return -1;
-    constexpr size_t kRow = (size_t)1 << 18;  // floats a gathered piece of the input
-    if ((r = upload_gather(
-             xlen, kRow, [&](size_t k) { return h_x + k * kRow; }, 0, [](size_t) {}, (float *)b.hx,
-             (float *)b.dx)))
-        return r;
+    if (!b.get(0, 0, sizeof(float) * xlen, sizeof(float) * ylen, ws)) return -1;
+    FPCC_CHECK(hipMemcpy(b.dx, h_x, sizeof(float) * xlen, hipMemcpyHostToDevice));
     r = fpcc_allpairs_device(job, (const float *)b.dx, (float *)b.dy, b.dws, ws, nullptr);
     if (r) return r;
-    return download_scatter(ylen, (const float *)b.dy, (float *)b.hy, h_y);
+    FPCC_CHECK(hipMemcpy(h_y, b.dy, sizeof(float) * ylen, hipMemcpyDeviceToHost));
+    return 0;
 }
 
 // ---- reference-compatible entries (src/FFTapps.h:11-16) ----
