@@ -148,11 +148,12 @@ size_t fpcc_workspace_size(const fpcc_job *job);
  * stream); no allocation, no synchronisation.  Returns 0 or an error code. */
 int fpcc_allpairs_device(const fpcc_job *job, const float *d_x, float *d_y, void *d_ws, size_t ws_bytes,
                          void *stream);
-/* Host buffers: copies x in, runs, copies y back.  Device buffers of up to 8 GiB
- * (and the pinned staging of the reference-style host entries, up to 1 GiB) are
- * kept per device between calls of the host entries, grown to the largest call;
- * larger calls allocate and free their own.  Host-entry calls on one device run
- * one at a time. */
+/* Host buffers: copies x in, runs, copies y back, through pinned staging moved
+ * on up to 8 host threads (pageable copies if pinned memory is refused).  Device
+ * buffers of up to 8 GiB and the pinned staging of the host entries (this one and
+ * the reference-style ones), up to 1 GiB, are kept per device between calls,
+ * grown to the largest call; larger calls allocate and free their own.
+ * Host-entry calls on one device run one at a time. */
 int fpcc_allpairs(const fpcc_job *job, const float *h_x, float *h_y);
 /* The two halves of fpcc_allpairs_device.  The station stage writes every
  * (station, window) spectrum into the workspace; the pair stage reads them and
