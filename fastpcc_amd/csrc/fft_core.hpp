@@ -335,12 +335,15 @@ constexpr int ct_lds_elems()
 }
 
 // LDS layouts of a stage: 0 identity, 1 ct_pad, 2 the low five bits of the index
-// XOR-ed with bits 3..7 (a permutation of every aligned block of 256)
+// XOR-ed with bits 3..7 (a permutation of every aligned block of 256), 3 the low
+// four bits XOR-ed with bits 5..8 (a permutation of every aligned block of 16, so
+// any multiple of 16 is a valid length)
 template <int LAY>
 __host__ __device__ constexpr int ct_lay(int i)
 {
     if constexpr (LAY == 1) return ct_pad(i);
     else if constexpr (LAY == 2) return i ^ ((i >> 3) & 31);
+    else if constexpr (LAY == 3) return i ^ ((i >> 5) & 15);
     else return i;
 }
 
