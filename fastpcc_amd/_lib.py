@@ -181,6 +181,10 @@ def lib():
     L.fpcc_host_alloc.restype = vp
     L.fpcc_host_free.argtypes = [vp]
     L.fpcc_host_free.restype = None
+    L.fpcc_release_buffers.argtypes = []
+    L.fpcc_release_buffers.restype = i
+    L.fpcc_set_kept_limits.argtypes = [ctypes.c_size_t, ctypes.c_size_t]
+    L.fpcc_set_kept_limits.restype = None
     for name in ("create", "submit", "submit_many", "poll", "wait", "cancel", "forget", "stats", "destroy"):
         getattr(L, "fpcc_server_" + name).restype = i
     _lib = L

@@ -83,20 +83,45 @@ def pcc_set(x1, x2, v, lag1, lag2):
     return _time_set("pcc_set", x1, x2, lag1, lag2, v=v)
 
 
-def pcc1_highlevel(z1, z2, lag1, lag2, y=None):
+def pcc1_highlevel(z1, z2, lag1, lag2, y=None, anok=None):
     """Analytic-signal entry of the reference's GPU engine (src/ccs_cuda.h:10):
     z1, z2 are [Tr][N] complex64 analytic signals; returns y [Tr][L] (only the
-    computable lags are written into y, which starts as zeros or the given array)."""
-    return _time_set("pcc1_highlevel", z1, z2, lag1, lag2, complex_in=True, y=y)
+    computable lags are written into y, which starts as zeros or the given array).
+    anok: None, or the address of an array of Tr sem_t (row tr is read only after
+    sem_wait(&anok[tr]) returns).  With anok, z1 and z2 must already be C-contiguous
+    complex64 arrays: the library reads them in place while a producer fills them."""
+    return _time_set("pcc1_highlevel", z1, z2, lag1, lag2, complex_in=True, y=y, anok=anok)
 
 
-def pcc_highlevel(z1, z2, v, lag1, lag2, y=None):
-    """PCC-v analytic-signal entry (src/ccs_cuda.h:9)."""
-    return _time_set("pcc_highlevel", z1, z2, lag1, lag2, v=v, complex_in=True, y=y)
+def pcc_highlevel(z1, z2, v, lag1, lag2, y=None, anok=None):
+    """PCC-v analytic-signal entry (src/ccs_cuda.h:9); anok as pcc1_highlevel."""
+    return _time_set("pcc_highlevel", z1, z2, lag1, lag2, v=v, complex_in=True, y=y, anok=anok)
 
 
-def _time_set(name, x1, x2, lag1, lag2, v=None, complex_in=False, y=None):
+def pcc1_highlevel2(x1, x2, lag1, lag2, y=None):
+    """Real-input PCC1 entry of the reference's GPU engine (src/ccs_cuda.h:11,
+    src/ccs_cuda.cu:796-870): returns nothing in C (errors are printed); only the
+    computable lags of y are written."""
+    return _time_set("pcc1_highlevel2", x1, x2, lag1, lag2, y=y)
+
+
+def release_buffers():
+    """Frees the buffers the host entries keep between calls; returns the number
+    of calls still holding theirs."""
+    return _lib.lib().fpcc_release_buffers()
+
+
+def set_kept_limits(host_bytes, device_bytes):
+    """Caps of the pinned host and device bytes kept per device between calls."""
+    _lib.lib().fpcc_set_kept_limits(host_bytes, device_bytes)
+
+
+def _time_set(name, x1, x2, lag1, lag2, v=None, complex_in=False, y=None, anok=None):
     dt = np.complex64 if complex_in else np.float32
+    if anok is not None:
+        for a in (x1, x2):
+            if not (isinstance(a, np.ndarray) and a.dtype == dt and a.ndim == 2 and a.flags.c_contiguous):
+                raise ValueError("with anok the inputs must be C-contiguous [Tr][N] arrays read in place")
     x1 = np.ascontiguousarray(np.atleast_2d(x1), dtype=dt)
     x2 = np.ascontiguousarray(np.atleast_2d(x2), dtype=dt)
     if x1.shape != x2.shape:
@@ -115,9 +140,12 @@ def _time_set(name, x1, x2, lag1, lag2, v=None, complex_in=False, y=None):
     if name == "pcc_set":
         r = fn(py, p1, p2, N, Tr, v, lag1, lag2)
     elif name == "pcc_highlevel":
-        r = fn(py, p1, p2, N, Tr, v, lag1, lag2, None)
+        r = fn(py, p1, p2, N, Tr, v, lag1, lag2, anok)
     elif name == "pcc1_highlevel":
-        r = fn(py, p1, p2, N, Tr, lag1, lag2, None)
+        r = fn(py, p1, p2, N, Tr, lag1, lag2, anok)
+    elif name == "pcc1_highlevel2":
+        fn(py, p1, p2, N, Tr, lag1, lag2)
+        r = 0
     else:
         r = fn(py, p1, p2, N, Tr, lag1, lag2)
     if r != 0:

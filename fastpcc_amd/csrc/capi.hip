@@ -1,4 +1,5 @@
 // C ABI of libfastpcc (declared in include/fastpcc.h).
+#include <errno.h>
 #include <math.h>
 #include <semaphore.h>
 #include <stdlib.h>
@@ -7,6 +8,8 @@
 #include <map>
 #include <memory>
 #include <atomic>
+#include <condition_variable>
+#include <sched.h>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -37,106 +40,229 @@ struct HostBuf {
     }
 };
 
-// Buffers of the host entry points (the reference-style *_set calls): pinned
-// input and output staging, device input, output and workspace.  Pinned staging
-// of at most kKeepHost bytes and device buffers of at most kKeepDevice bytes are
-// kept per device between calls (grown to the largest such call, held until the
-// process ends; one call at a time per device); larger ones are allocated and
-// freed by the call.
-constexpr size_t kKeepHost = (size_t)1 << 30, kKeepDevice = (size_t)8 << 30;
+// Buffers of the host entry points (the reference-style *_set calls and
+// fpcc_allpairs): pinned input and output staging, device input, output and
+// workspace.  Each call takes one buffer set from its device's pool (a set no other
+// call holds, or a new one), so concurrent callers never wait for each other.  A
+// set's buffers are kept between calls, grown to the largest call that used the
+// set, as long as the device's kept total stays within the limits (default 1 GiB
+// pinned, 8 GiB device; fpcc_set_kept_limits); a call whose buffers would pass
+// them allocates its own and frees them on return.  fpcc_release_buffers frees
+// every kept buffer.
+std::atomic<size_t> g_keep_host{(size_t)1 << 30}, g_keep_device{(size_t)8 << 30};
 
-struct Kept {
-    std::mutex mu;
-    void *hx = nullptr, *hy = nullptr, *dx = nullptr, *dy = nullptr, *dws = nullptr;
-    size_t nhx = 0, nhy = 0, ndx = 0, ndy = 0, ndws = 0;
+struct BufSet {
+    void *p[5] = {};  // hx, hy (pinned host), dx, dy, dws (device)
+    size_t n[5] = {};
+    unsigned gen = 0;
 };
+constexpr bool kSetHost[5] = {true, true, false, false, false};
 
-Kept &kept_for_device()
+void free_set(BufSet &s)
 {
-    static std::mutex mu;
-    static std::map<int, std::unique_ptr<Kept>> m;
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    std::lock_guard<std::mutex> lk(mu);
-    auto &k = m[dev];
-    if (!k) k.reset(new Kept);
-    return *k;
+    for (int i = 0; i < 5; ++i) {
+        if (s.p[i]) (void)(kSetHost[i] ? hipHostFree(s.p[i]) : hipFree(s.p[i]));
+        s.p[i] = nullptr;
+        s.n[i] = 0;
+    }
 }
 
-bool grow(void *&p, size_t &cap, size_t need, bool host)
+struct DevicePool {
+    std::mutex mu;
+    std::vector<std::unique_ptr<BufSet>> idle;
+    size_t kept_h = 0, kept_d = 0;  // bytes held by every set of this device, idle or in use
+    unsigned gen = 0;               // bumped by fpcc_release_buffers: sets of older generations are freed
+    int busy = 0;
+};
+
+std::mutex g_pools_mu;
+std::map<int, std::unique_ptr<DevicePool>> g_pools;
+
+DevicePool &pool_for_device(int dev)
 {
-    if (need <= cap) return true;
-    if (p) (void)(host ? hipHostFree(p) : hipFree(p));
-    p = nullptr;
-    cap = 0;
-    if ((host ? hipHostMalloc(&p, need, hipHostMallocDefault) : hipMalloc(&p, need)) != hipSuccess) {
-        p = nullptr;
-        return false;
-    }
-    cap = need;
-    return true;
+    std::lock_guard<std::mutex> lk(g_pools_mu);
+    auto &p = g_pools[dev];
+    if (!p) p.reset(new DevicePool);
+    return *p;
 }
 
 class EntryBufs {
 public:
     void *hx = nullptr, *hy = nullptr, *dx = nullptr, *dy = nullptr, *dws = nullptr;
 
+    ~EntryBufs()
+    {
+        for (int i = 0; i < 5; ++i)
+            if (own_[i]) (void)(kSetHost[i] ? hipHostFree(own_[i]) : hipFree(own_[i]));
+        if (!set_) return;
+        std::lock_guard<std::mutex> lk(pool_->mu);
+        --pool_->busy;
+        if (set_->gen != pool_->gen) {  // released while this call held it
+            for (int i = 0; i < 5; ++i) (kSetHost[i] ? pool_->kept_h : pool_->kept_d) -= set_->n[i];
+            free_set(*set_);
+            return;
+        }
+        pool_->idle.push_back(std::move(set_));
+    }
+
     // sizes in bytes (0: not needed); false if an allocation fails
     bool get(size_t bhx, size_t bhy, size_t bdx, size_t bdy, size_t bdws)
     {
-        const bool keep_h = bhx + bhy <= kKeepHost, keep_d = bdx + bdy + bdws <= kKeepDevice;
-        if (keep_h || keep_d) {
-            k_ = &kept_for_device();
-            lk_ = std::unique_lock<std::mutex>(k_->mu);
+        const size_t need[5] = {bhx, bhy, bdx, bdy, bdws};
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        pool_ = &pool_for_device(dev);
+        {
+            std::lock_guard<std::mutex> lk(pool_->mu);
+            // the idle set that needs the least growth
+            size_t best = ~(size_t)0;
+            int bi = -1;
+            for (int i = 0; i < (int)pool_->idle.size(); ++i) {
+                size_t grow = 0;
+                for (int k = 0; k < 5; ++k) grow += need[k] > pool_->idle[i]->n[k] ? need[k] : 0;
+                if (grow < best) best = grow, bi = i;
+            }
+            if (bi >= 0) {
+                set_ = std::move(pool_->idle[bi]);
+                pool_->idle.erase(pool_->idle.begin() + bi);
+            } else {
+                set_.reset(new BufSet);
+                set_->gen = pool_->gen;
+            }
+            ++pool_->busy;
+            // the growth this set may take, within the device's limits (reserved now,
+            // corrected to what was allocated below)
+            for (int k = 0; k < 5; ++k)
+                if (need[k] > set_->n[k]) (kSetHost[k] ? res_h_ : res_d_) += need[k] - set_->n[k];
+            keep_h_ = pool_->kept_h + res_h_ <= g_keep_host.load();
+            keep_d_ = pool_->kept_d + res_d_ <= g_keep_device.load();
+            if (!keep_h_) res_h_ = 0;
+            if (!keep_d_) res_d_ = 0;
+            pool_->kept_h += res_h_;
+            pool_->kept_d += res_d_;
         }
-        if (keep_h) {
-            if (!grow(k_->hx, k_->nhx, bhx, true) || !grow(k_->hy, k_->nhy, bhy, true)) return false;
-            hx = bhx ? k_->hx : nullptr;
-            hy = bhy ? k_->hy : nullptr;
-        } else {
-            if ((bhx && hipHostMalloc(&ohx_.p, bhx, hipHostMallocDefault) != hipSuccess) ||
-                (bhy && hipHostMalloc(&ohy_.p, bhy, hipHostMallocDefault) != hipSuccess))
-                return false;
-            hx = ohx_.p;
-            hy = ohy_.p;
-        }
-        if (keep_d) {
-            if (!grow(k_->dx, k_->ndx, bdx, false) || !grow(k_->dy, k_->ndy, bdy, false) ||
-                !grow(k_->dws, k_->ndws, bdws, false))
-                return false;
-            dx = bdx ? k_->dx : nullptr;
-            dy = bdy ? k_->dy : nullptr;
-            dws = bdws ? k_->dws : nullptr;
-        } else {
-            if ((bdx && hipMalloc(&odx_.p, bdx) != hipSuccess) || (bdy && hipMalloc(&ody_.p, bdy) != hipSuccess) ||
-                (bdws && hipMalloc(&odws_.p, bdws) != hipSuccess))
-                return false;
-            dx = odx_.p;
-            dy = ody_.p;
-            dws = odws_.p;
-        }
-        return true;
+        size_t before_h = 0, before_d = 0;
+        for (int k = 0; k < 5; ++k) (kSetHost[k] ? before_h : before_d) += set_->n[k];
+        const bool ok = fill(need);
+        size_t after_h = 0, after_d = 0;
+        for (int k = 0; k < 5; ++k) (kSetHost[k] ? after_h : after_d) += set_->n[k];
+        std::lock_guard<std::mutex> lk(pool_->mu);
+        pool_->kept_h = pool_->kept_h - res_h_ + after_h - before_h;
+        pool_->kept_d = pool_->kept_d - res_d_ + after_d - before_d;
+        return ok;
     }
 
 private:
-    Kept *k_ = nullptr;
-    std::unique_lock<std::mutex> lk_;
-    HostBuf ohx_, ohy_;
-    DeviceBuf odx_, ody_, odws_;
+    static bool alloc(int k, size_t bytes, void *&p)
+    {
+        p = nullptr;
+        const hipError_t e = kSetHost[k] ? hipHostMalloc(&p, bytes, hipHostMallocDefault) : hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        return true;
+    }
+    // kept buffers grown in place (set_), the others allocated for this call (own_)
+    bool fill(const size_t *need)
+    {
+        void **out[5] = {&hx, &hy, &dx, &dy, &dws};
+        for (int k = 0; k < 5; ++k) {
+            if (!need[k]) continue;
+            if (kSetHost[k] ? keep_h_ : keep_d_) {
+                if (need[k] > set_->n[k]) {
+                    if (set_->p[k]) (void)(kSetHost[k] ? hipHostFree(set_->p[k]) : hipFree(set_->p[k]));
+                    set_->p[k] = nullptr;
+                    set_->n[k] = 0;
+                    if (!alloc(k, need[k], set_->p[k])) return false;
+                    set_->n[k] = need[k];
+                }
+                *out[k] = set_->p[k];
+            } else {
+                if (!alloc(k, need[k], own_[k])) return false;
+                *out[k] = own_[k];
+            }
+        }
+        return true;
+    }
+    DevicePool *pool_ = nullptr;
+    std::unique_ptr<BufSet> set_;
+    bool keep_h_ = false, keep_d_ = false;
+    size_t res_h_ = 0, res_d_ = 0;
+    void *own_[5] = {};
 };
+
+// Frees the idle kept sets of every device and marks the sets in use to be freed
+// when their calls return.  Returns the number of sets still in use.
+int release_kept()
+{
+    std::lock_guard<std::mutex> lk(g_pools_mu);
+    int dev0 = 0, busy = 0;
+    (void)hipGetDevice(&dev0);
+    for (auto &kv : g_pools) {
+        DevicePool &p = *kv.second;
+        std::lock_guard<std::mutex> lp(p.mu);
+        if (p.idle.empty() && !p.busy) continue;
+        (void)hipSetDevice(kv.first);
+        for (auto &s : p.idle) {
+            for (int i = 0; i < 5; ++i) (kSetHost[i] ? p.kept_h : p.kept_d) -= s->n[i];
+            free_set(*s);
+        }
+        p.idle.clear();
+        ++p.gen;
+        busy += p.busy;
+    }
+    (void)hipSetDevice(dev0);
+    return busy;
+}
 
 // Host threads for the row gathers of the host entry points (pageable rows into
 // pinned staging and back): one thread copies at a fraction of host memory
-// bandwidth.  Inputs under two pieces stay on the calling thread.
+// bandwidth.  Inputs under two pieces stay on the calling thread.  The count is
+// capped by the CPUs this process may run on (affinity mask, cgroup cpusets).
 constexpr unsigned kGatherThreads = 8;
 constexpr size_t kPieceBytes = (size_t)8 << 20;
+
+static unsigned usable_cpus()
+{
+    static const unsigned n = [] {
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        if (sched_getaffinity(0, sizeof(set), &set) == 0) return (unsigned)std::max(1, CPU_COUNT(&set));
+        const unsigned hw = std::thread::hardware_concurrency();
+        return hw ? hw : 1u;
+    }();
+    return n;
+}
 
 static unsigned gather_threads(size_t bytes)
 {
     if (bytes < 2 * kPieceBytes) return 1;
-    const unsigned hw = std::thread::hardware_concurrency();
-    return std::max(1u, std::min(kGatherThreads, hw ? hw : 1u));
+    return std::max(1u, std::min(kGatherThreads, usable_cpus()));
 }
+
+// A counter that threads block on (no spinning) until it reaches a value.
+class Gate {
+public:
+    void add(size_t d = 1)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            v_ += d;
+        }
+        cv_.notify_all();
+    }
+    void wait_for(size_t target)
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return v_ >= target; });
+    }
+
+private:
+    std::mutex mu_;
+    std::condition_variable cv_;
+    size_t v_ = 0;
+};
 
 // fn(k) for k in [0, n) spread over up to kGatherThreads threads (the caller's
 // included) when the rows hold at least 4 MB; returns once every call is done.
@@ -162,19 +288,26 @@ static void rows_parallel(size_t n, size_t bytes, Fn fn)
 // `total` floats in rows of `row` (the last row may be shorter; src_of(k) is
 // row k) gathered into pinned staging and copied to the device in pieces of
 // about 8 MB on the null stream, so the gather of one piece overlaps the copy of
-// the last; the rows of a piece are spread over the gather threads.
-// `before(k)` runs (on the calling thread) before row k < nbefore is read.
+// the last; the rows of a piece are spread over the gather threads, which block
+// (not spin) while they wait for a piece.  With `blocking_before`, before(k) may
+// block (a semaphore wait): the rows are then copied on the calling thread alone,
+// each one right after its before(k) returns, so no helper threads compete with
+// the caller's producers for cores.  before(k) runs (on the calling thread) before
+// row k < nbefore is read.
 template <typename SrcOf, typename Before>
-int upload_gather(size_t total, size_t row, SrcOf src_of, size_t nbefore, Before before, float *hx, float *dx)
+int upload_gather(size_t total, size_t row, SrcOf src_of, size_t nbefore, Before before, float *hx, float *dx,
+                  bool blocking_before = false)
 {
     if (!total) return 0;
     row = std::max<size_t>(row, 1);
     const size_t nrows = (total + row - 1) / row;
     const size_t prow = std::max<size_t>(1, kPieceBytes / (sizeof(float) * row));
     const size_t npieces = (nrows + prow - 1) / prow;
+    auto copy_row = [&](size_t k) {
+        memcpy(hx + k * row, src_of(k), sizeof(float) * std::min(row, total - k * row));
+    };
     auto copy_piece_rows = [&](size_t pc, unsigned w, unsigned nw) {
-        for (size_t k = pc * prow + w; k < std::min(nrows, (pc + 1) * prow); k += nw)
-            memcpy(hx + k * row, src_of(k), sizeof(float) * std::min(row, total - k * row));
+        for (size_t k = pc * prow + w; k < std::min(nrows, (pc + 1) * prow); k += nw) copy_row(k);
     };
     auto send = [&](size_t pc) {
         const size_t f0 = pc * prow * row, f1 = std::min(total, (pc + 1) * prow * row);
@@ -183,35 +316,35 @@ int upload_gather(size_t total, size_t row, SrcOf src_of, size_t nbefore, Before
     auto befores = [&](size_t pc) {
         for (size_t k = pc * prow; k < std::min(nbefore, (pc + 1) * prow); ++k) before(k);
     };
-    const unsigned nt = gather_threads(sizeof(float) * total);
+    const unsigned nt = blocking_before ? 1 : gather_threads(sizeof(float) * total);
     if (nt > 1) {
         // pieces are released in order once their before() calls have returned;
         // of nw copiers (the calling thread is copier 0), copier w takes rows
         // w, w + nw, ... of each released piece
-        std::atomic<size_t> released{0};
+        Gate released, finished;  // pieces released; (piece, helper) copies done
         std::atomic<unsigned> nw{0};
-        std::vector<std::atomic<unsigned>> finished(npieces);
-        for (auto &f : finished) f.store(0);
         std::vector<std::thread> pool;
         try {
             for (unsigned w = 1; w < nt; ++w)
                 pool.emplace_back([&, w] {
                     for (size_t pc = 0; pc < npieces; ++pc) {
-                        while (released.load(std::memory_order_acquire) <= pc) std::this_thread::yield();
+                        released.wait_for(pc + 1);
                         copy_piece_rows(pc, w, nw.load(std::memory_order_relaxed));
-                        finished[pc].fetch_add(1, std::memory_order_release);
+                        finished.add();
                     }
                 });
         } catch (...) {  // fewer threads than asked for: those that started share the rows
         }
         const unsigned n = (unsigned)pool.size() + 1;
-        nw.store(n, std::memory_order_relaxed);  // published to the copiers by the first release
+        nw.store(n, std::memory_order_relaxed);  // published to the helpers by the first release
         hipError_t err = hipSuccess;
         for (size_t pc = 0; pc < npieces; ++pc) {
             befores(pc);
-            released.store(pc + 1, std::memory_order_release);
+            released.add();
             copy_piece_rows(pc, 0, n);
-            while (finished[pc].load(std::memory_order_acquire) < n - 1) std::this_thread::yield();
+            // helpers take the pieces in order and piece pc + 1 is not released
+            // yet, so (pc + 1)(n - 1) finished copies mean piece pc is complete
+            finished.wait_for((pc + 1) * (n - 1));
             if (err == hipSuccess) err = send(pc);
         }
         for (auto &t : pool) t.join();
@@ -222,8 +355,10 @@ int upload_gather(size_t total, size_t row, SrcOf src_of, size_t nbefore, Before
         return 0;
     }
     for (size_t pc = 0; pc < npieces; ++pc) {
-        befores(pc);
-        copy_piece_rows(pc, 0, 1);
+        for (size_t k = pc * prow; k < std::min(nrows, (pc + 1) * prow); ++k) {
+            if (k < nbefore) before(k);
+            copy_row(k);
+        }
         FPCC_CHECK(send(pc));
     }
     return 0;
@@ -231,16 +366,18 @@ int upload_gather(size_t total, size_t row, SrcOf src_of, size_t nbefore, Before
 
 // Rows of two stations (x1 rows, then x2 rows, each of `row` floats), the
 // reference's row-pointer inputs.  `before(tr)` runs before row tr of either
-// station is read.  -1 on a null row.
+// station is read (it may block if blocking_before).  -1 on a null row.
 template <typename Before>
 int upload_rows(const float *const *x1, const float *const *x2, unsigned Tr, size_t row, float *hx, float *dx,
-                Before before)
+                Before before, bool blocking_before = false)
 {
     for (unsigned tr = 0; tr < Tr; ++tr)
         if (!x1[tr] || !x2[tr]) return -1;
+    // staging row k < Tr is x1[k], waited for by before(k); rows Tr + k (x2[k])
+    // come after every before() has returned
     return upload_gather(
         2 * (size_t)Tr * row, row, [&](size_t k) { return k < Tr ? x1[k] : x2[k - Tr]; }, Tr,
-        [&](size_t k) { before((unsigned)k); }, hx, dx);
+        [&](size_t k) { before((unsigned)k); }, hx, dx, blocking_before);
 }
 
 // `total` floats from the device (dy) to the caller's pageable h_y through
@@ -285,14 +422,14 @@ static int download_scatter(size_t total, const float *dy, float *hy, float *h_y
         for (size_t b0 = f0 + w * kBlock; b0 < f1; b0 += (size_t)nw * kBlock)
             memcpy(h_y + b0, hy + b0, sizeof(float) * std::min(kBlock, f1 - b0));
     };
-    std::atomic<size_t> landed{0};
+    Gate landed;
     std::atomic<unsigned> nw{0};
     std::vector<std::thread> pool;
     try {
         for (unsigned w = 1; w < nt; ++w)
             pool.emplace_back([&, w] {
                 for (size_t pc = 0; pc < npieces; ++pc) {
-                    while (landed.load(std::memory_order_acquire) <= pc) std::this_thread::yield();
+                    landed.wait_for(pc + 1);
                     move_piece(pc, w, nw.load(std::memory_order_relaxed));
                 }
             });
@@ -302,7 +439,7 @@ static int download_scatter(size_t total, const float *dy, float *hy, float *h_y
     nw.store(n, std::memory_order_relaxed);  // published to the copiers by the first landing
     for (size_t pc = 0; pc < npieces; ++pc) {
         if (err == hipSuccess) err = hipEventSynchronize(ev[pc]);
-        landed.store(pc + 1, std::memory_order_release);  // on a failure the copiers still finish
+        landed.add();  // on a failure the copiers still finish
         move_piece(pc, 0, n);
     }
     for (auto &t : pool) t.join();
@@ -636,9 +773,14 @@ int pcc_time_host(int pcc1, float **y, const float *const *x1, const float *cons
     EntryBufs b;
     if (!b.get(sizeof(float) * xlen, sizeof(float) * ylen, sizeof(float) * xlen, sizeof(float) * ylen, bphi))
         return -1;
-    int r = upload_rows(x1, x2, Tr, (size_t)N * cw, (float *)b.hx, (float *)b.dx, [&](unsigned tr) {
-        if (anok) sem_wait(&anok[tr]);
-    });
+    int r = upload_rows(
+        x1, x2, Tr, (size_t)N * cw, (float *)b.hx, (float *)b.dx,
+        [&](unsigned tr) {
+            if (anok)
+                while (sem_wait(&anok[tr]) != 0 && errno == EINTR) {
+                }
+        },
+        anok != nullptr);
     if (r) return r;
     FPCC_CHECK(hipMemsetAsync(b.dy, 0, sizeof(float) * ylen, nullptr));
     const cf32 *phi;
@@ -933,6 +1075,14 @@ extern "C" int fpcc_allpairs(const fpcc_job *job, const float *h_x, float *h_y)
     if (r) return r;
     FPCC_CHECK(hipMemcpy(h_y, b.dy, sizeof(float) * ylen, hipMemcpyDeviceToHost));
     return 0;
+}
+
+extern "C" int fpcc_release_buffers(void) { return release_kept(); }
+
+extern "C" void fpcc_set_kept_limits(size_t host_bytes, size_t device_bytes)
+{
+    g_keep_host.store(host_bytes);
+    g_keep_device.store(device_bytes);
 }
 
 // ---- reference-compatible entries (src/FFTapps.h:11-16) ----

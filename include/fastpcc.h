@@ -149,12 +149,22 @@ size_t fpcc_workspace_size(const fpcc_job *job);
 int fpcc_allpairs_device(const fpcc_job *job, const float *d_x, float *d_y, void *d_ws, size_t ws_bytes,
                          void *stream);
 /* Host buffers: copies x in, runs, copies y back, through pinned staging moved
- * on up to 8 host threads (pageable copies if pinned memory is refused).  Device
- * buffers of up to 8 GiB and the pinned staging of the host entries (this one and
- * the reference-style ones), up to 1 GiB, are kept per device between calls,
- * grown to the largest call; larger calls allocate and free their own.
- * Host-entry calls on one device run one at a time. */
+ * on up to 8 host threads (no more than the CPUs the process may run on; pageable
+ * copies if pinned memory is refused).  Each host-entry call (this one and the
+ * reference-style ones) takes a buffer set of its own, so concurrent calls do not
+ * wait for each other; the sets' pinned staging and device buffers are kept per
+ * device between calls while the device's kept totals stay within the limits
+ * below, and calls beyond them allocate and free their own. */
 int fpcc_allpairs(const fpcc_job *job, const float *h_x, float *h_y);
+/* Limits of the pinned host and device bytes the host entries keep per device
+ * between calls (defaults 1 GiB and 8 GiB; 0 keeps nothing).  Buffers already
+ * kept stay until fpcc_release_buffers. */
+void fpcc_set_kept_limits(size_t host_bytes, size_t device_bytes);
+/* Frees every kept buffer on every device (the reference frees its buffers at the
+ * end of each call, src/ccs_cuda.cu:600-603; here that happens on request).
+ * Buffers held by calls still running are freed when those calls return.
+ * Returns the number of such calls. */
+int fpcc_release_buffers(void);
 /* The two halves of fpcc_allpairs_device.  The station stage writes every
  * (station, window) spectrum into the workspace; the pair stage reads them and
  * writes y.  Spectra left in the workspace may be reused by later pair stages of

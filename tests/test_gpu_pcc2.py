@@ -108,10 +108,13 @@ def test_pcc2_set_many_rows(fp, orc):
 
 def test_allpairs_host_buffers_large(fp, hip):
     """fpcc_allpairs from host buffers big enough for the threaded pinned staging
-    (64 stations x 86400 samples = 22 MB in, 2016 pairs x 2001 lags = 16 MB out)
-    gives bit for bit the device-buffer run of the same job."""
+    in both directions (72 stations x 86400 samples = 24.9 MB in, 2556 pairs x 2001
+    lags = 20.5 MB out; the threaded copies start at two 8 MiB pieces) gives bit for
+    bit the device-buffer run of the same job."""
     import ctypes
-    S, W, N = 64, 24, 3600
+    S, W, N = 72, 24, 3600
+    piece = 8 << 20
+    assert 4 * S * W * N >= 2 * piece and 4 * (S * (S - 1) // 2) * 2001 >= 2 * piece
     x, _ = synth.make_stations(S, W * N, 300, seed=64)
     y_host = fp.allpairs(x, W, N, -1000, 1000, kind="pcc2", stack="gram16").ravel()
     L = fp.api._lib.lib()
