@@ -101,6 +101,8 @@ const BigPlan &big_plan(int n)
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find({dev, n});
     if (it != cache.end()) return it->second;
+    static const BigPlan none{};
+    if (!table_builds_allowed()) return none;
     return cache.emplace(std::make_pair(dev, n), make_big_plan(n)).first->second;
 }
 

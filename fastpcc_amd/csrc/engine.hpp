@@ -51,6 +51,18 @@ bool plan_from_radices(int n, const std::vector<int> &radices, FftPlan &p);
 // Path of a tuning table in tuning/ beside the library ("" if unknown).
 std::string tuning_file(const char *name);
 
+// Constant device tables (FFT plans and twiddles, Gram tile lists, filter banks)
+// are built on first use, once per device, and kept.  Inside a NoBuildScope (on
+// the constructing thread) a table that does not exist yet is not built: the
+// lookup fails as an unsupported case would, and tables_missing() turns true.
+struct NoBuildScope {
+    NoBuildScope();
+    ~NoBuildScope();
+};
+// True outside a NoBuildScope; inside one, false (and the miss is recorded).
+bool table_builds_allowed();
+bool tables_missing();
+
 // Smallest power of two >= v
 static inline unsigned pow2_ceil(unsigned v)
 {

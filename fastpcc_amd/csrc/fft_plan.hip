@@ -151,6 +151,26 @@ static const void *upload_table(const std::vector<double> &re_im)
 
 static std::mutex g_mu;
 
+namespace {
+thread_local int t_nobuild = 0;
+thread_local bool t_missing = false;
+}  // namespace
+
+NoBuildScope::NoBuildScope()
+{
+    if (t_nobuild++ == 0) t_missing = false;
+}
+NoBuildScope::~NoBuildScope() { --t_nobuild; }
+
+bool table_builds_allowed()
+{
+    if (!t_nobuild) return true;
+    t_missing = true;
+    return false;
+}
+
+bool tables_missing() { return t_missing; }
+
 // Per-pass twiddles W_{Ns R}^{r k}, r < R, k < Ns, in pass order.
 static std::vector<double> plan_twiddles(const FftPlan &plan)
 {
@@ -179,6 +199,8 @@ const DevPlan &get_plan(int n, bool f64)
     auto key = std::make_tuple(dev, n, f64);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
+    static const DevPlan none{};
+    if (!table_builds_allowed()) return none;
     DevPlan dp{};
     dp.ok = factor_plan(n, dp.plan);
     if (dp.ok && !f64) {
@@ -204,6 +226,7 @@ const void *get_c2r_twiddles(int nz, bool f64)
     auto key = std::make_tuple(dev, nz, f64);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
+    if (!table_builds_allowed()) return nullptr;
     std::vector<double> t;
     for (int k = 0; k < nz / 2; ++k) {
         double c, s;
@@ -228,6 +251,7 @@ const void *get_order_twiddles(int n, const std::vector<int> &radices)
     key.push_back(dev);
     auto it = cache.find({n, key});
     if (it != cache.end()) return it->second;
+    if (!table_builds_allowed()) return nullptr;
     const void *d = upload_table<float>(plan_twiddles(p));
     cache.emplace(std::make_pair(n, key), d);
     return d;
@@ -241,6 +265,7 @@ const void *get_split_twiddles(int n)
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = cache.find({dev, n});
     if (it != cache.end()) return it->second;
+    if (!table_builds_allowed()) return nullptr;
     std::vector<double> t;
     const int hi = (n + kTwLo - 1) / kTwLo;
     t.reserve(2 * (size_t)(kTwLo + hi));
@@ -270,6 +295,7 @@ const void *get_full_twiddles(int n, bool f64)
     auto key = std::make_tuple(dev, n, f64);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
+    if (!table_builds_allowed()) return nullptr;
     std::vector<double> t;
     t.reserve(2 * (size_t)n);
     for (int k = 0; k < n; ++k) {

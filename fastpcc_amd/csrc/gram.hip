@@ -730,6 +730,7 @@ static const int2 *block_list(int S, int B, int &ntiles)
         ntiles = it->second.second;
         return it->second.first;
     }
+    if (!table_builds_allowed()) return nullptr;
     std::vector<int2> t;
     for (int bi = 0; bi * B < S; ++bi)
         for (int bj = bi; bj * B < S; ++bj) t.push_back(int2{bi, bj});

@@ -325,7 +325,7 @@ int launch_station_pcc(const float *d_x, long long station_stride, int S, int W,
     const DevPlan &pN = get_plan(N, false);
     const DevPlan &pZ = get_plan(Nz, false);
     if (!pN.ok || !pZ.ok) {
-        fprintf(stderr, "fastpcc: unsupported FFT length (N=%d, Nz=%d)\n", N, Nz);
+        if (!tables_missing()) fprintf(stderr, "fastpcc: unsupported FFT length (N=%d, Nz=%d)\n", N, Nz);
         return -1;
     }
     if (Nz > 8192) {

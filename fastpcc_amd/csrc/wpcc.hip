@@ -55,6 +55,7 @@ static int filter_bank(const WFamily &f, hipStream_t st, FilterBank &out)
             return 0;
         }
     }
+    if (!table_builds_allowed()) return -1;
     const BigPlan &bp = big_plan((int)f.Nz);
     if (!bp.ok) return -1;
     const size_t Nz = f.Nz;

@@ -148,6 +148,21 @@ size_t fpcc_workspace_size(const fpcc_job *job);
  * stream); no allocation, no synchronisation.  Returns 0 or an error code. */
 int fpcc_allpairs_device(const fpcc_job *job, const float *d_x, float *d_y, void *d_ws, size_t ws_bytes,
                          void *stream);
+/* fpcc_allpairs_device without a workspace argument and without allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The workspace is a fixed device array of
+ * FPCC_NOALLOC_SCRATCH_BYTES inside the library, one per device (held from the
+ * first call on a device until the process ends); the call makes no hipMalloc or
+ * hipFree.  Every call on one device shares that array: calls that overlap on a
+ * device (other streams, other threads) race and give wrong results.  The job's
+ * constant tables (FFT plans and twiddles, Gram tile lists, filter banks) must
+ * exist on the device: one fpcc_allpairs_device call of a job with the same
+ * kind, stations, window length, lags, stack and filter parameters builds them.
+ * Returns 0; -1 / -3 as fpcc_allpairs_device; -5 if the tables are not built
+ * yet; -6 if the job's workspace (fpcc_workspace_size) exceeds the scratch. */
+#define FPCC_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int fpcc_allpairs_device_noalloc(const fpcc_job *job, const float *d_x, float *d_y, void *stream);
 /* Host buffers: copies x in, runs, copies y back, through pinned staging moved
  * on up to 8 host threads (no more than the CPUs the process may run on; pageable
  * copies if pinned memory is refused).  Each host-entry call (this one and the
