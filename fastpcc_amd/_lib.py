@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfastpcc.so")
 
 FPCC_PCC2, FPCC_PCC1, FPCC_PCCV, FPCC_GNCC, FPCC_CC1B, FPCC_WPCC = range(6)
-STACK_NONE, STACK_PLAIN, STACK_ZSUM, STACK_GRAM, STACK_GRAM16 = range(5)
+STACK_NONE, STACK_PLAIN, STACK_ZSUM, STACK_GRAM, STACK_GRAM16, STACK_PWS = range(6)
 
 
 class FpccJob(ctypes.Structure):
@@ -31,6 +31,7 @@ class FpccJob(ctypes.Structure):
         ("V", ctypes.c_uint),
         ("wtype", ctypes.c_int),
         ("op1", ctypes.c_double),
+        ("pws_power", ctypes.c_double),
     ]
 
 

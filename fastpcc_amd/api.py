@@ -206,11 +206,11 @@ def cc1b_set(x1, x2, lag1, lag2):
 KIND = {"pcc2": _lib.FPCC_PCC2, "pcc1": _lib.FPCC_PCC1, "pccv": _lib.FPCC_PCCV, "gncc": _lib.FPCC_GNCC,
         "cc1b": _lib.FPCC_CC1B, "wpcc": _lib.FPCC_WPCC}
 STACK = {"none": _lib.STACK_NONE, "plain": _lib.STACK_PLAIN, "zsum": _lib.STACK_ZSUM, "gram": _lib.STACK_GRAM,
-         "gram16": _lib.STACK_GRAM16}
+         "gram16": _lib.STACK_GRAM16, "pws": _lib.STACK_PWS}
 
 
 def make_job(kind, n_stations, n_windows, win_len, lag1, lag2, stack="plain", station_stride=None, v=2.0,
-             pmin=0.0, pmax=0.0, V=8, wtype=-1, op1=0.0):
+             pmin=0.0, pmax=0.0, V=8, wtype=-1, op1=0.0, pws_power=2.0):
     job = FpccJob()
     job.kind = KIND.get(kind, kind)
     job.stack = STACK.get(stack, stack)
@@ -220,6 +220,7 @@ def make_job(kind, n_stations, n_windows, win_len, lag1, lag2, stack="plain", st
     job.station_stride = station_stride if station_stride is not None else n_windows * win_len
     job.lag1, job.lag2 = lag1, lag2
     job.v, job.pmin, job.pmax, job.V, job.wtype, job.op1 = v, pmin, pmax, V, wtype, op1
+    job.pws_power = pws_power
     return job
 
 

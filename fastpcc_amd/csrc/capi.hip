@@ -476,11 +476,11 @@ bool needs_large(int kind, int N, unsigned Nz)
 bool stack_supported(int kind, int stack)
 {
     switch (kind) {
-    case FPCC_PCC2: return stack >= FPCC_STACK_NONE && stack <= FPCC_STACK_GRAM16;
-    case FPCC_GNCC: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN;
+    case FPCC_PCC2: return stack >= FPCC_STACK_NONE && stack <= FPCC_STACK_PWS;
+    case FPCC_GNCC: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN || stack == FPCC_STACK_PWS;
     case FPCC_CC1B:
         return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN || stack == FPCC_STACK_GRAM ||
-               stack == FPCC_STACK_GRAM16;
+               stack == FPCC_STACK_GRAM16 || stack == FPCC_STACK_PWS;
     case FPCC_PCC1:
     case FPCC_PCCV: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN;
     case FPCC_WPCC: return stack == FPCC_STACK_NONE || stack == FPCC_STACK_PLAIN || stack == FPCC_STACK_GRAM;
@@ -514,6 +514,11 @@ int resolve(const fpcc_job *job, Dims &d)
     if (d.large && d.Nz < 64) d.Nz = 64;
     if (d.large && !large_supported(job->kind, d.N, (int)d.Nz)) {
         fprintf(stderr, "fastpcc: unsupported window length N=%d (Nz=%u)\n", d.N, d.Nz);
+        return -1;
+    }
+    if (job->stack == FPCC_STACK_PWS && (d.large || !pws_supported((int)d.Nz) || !(job->pws_power >= 0))) {
+        fprintf(stderr, "fastpcc: the phase-weighted stack needs a zero-pad length of at most 8192 (Nz=%u) and a "
+                        "power >= 0\n", d.Nz);
         return -1;
     }
     if (job->kind == FPCC_WPCC) {
@@ -674,11 +679,17 @@ int pair_stage(const fpcc_job *job, const Dims &d, char *ws, float *y, hipStream
             const float scale = 0.5f * (1 / ((float)d.Nz * (float)d.N)) / (float)d.W;
             return launch_pair_c2r(G, gl, (int)d.P, d.Nz, d.N, d.W, d.lag0, d.L, scale, 0, y, st);
         }
+        if (job->stack == FPCC_STACK_PWS)  // the reference's 1/((float)Nz*(float)N) of each window
+            return launch_pair_pws(0, ws + l.a, nullptr, d.Nz, d.S, d.W, (int)d.Nz, d.lag0, d.L,
+                                   1 / ((float)d.Nz * (float)d.N), (float)job->pws_power, 0, y, st);
         return launch_pair_pcc2((const cf32 *)(ws + l.a), d.S, d.W, d.N, d.Nz, d.lag0, d.L, job->stack, 0,
                                 (int)d.P, y, st);
     case FPCC_GNCC:
-        return launch_pair_gn(ws + l.a, ws + l.a, (const double *)(ws + l.b), true, d.S, d.W, d.N, d.Nz, d.lag0,
-                              d.L, 0, mode, y, st);
+        r = launch_pair_gn(ws + l.a, ws + l.a, (const double *)(ws + l.b), true, d.S, d.W, d.N, d.Nz, d.lag0, d.L, 0,
+                           mode, y, st);
+        if (r || job->stack != FPCC_STACK_PWS) return r;
+        return launch_pair_pws(2, ws + l.a, ws + l.a, real_ld(d.Nz), d.S, d.W, (int)d.Nz, d.lag0, d.L, 0.f,
+                               (float)job->pws_power, 1, y, st);
     case FPCC_CC1B:
         if (is_gram(job->stack)) {
             int ntiles = 0;
@@ -703,8 +714,11 @@ int pair_stage(const fpcc_job *job, const Dims &d, char *ws, float *y, hipStream
             return launch_pair_c2r(G, gl, (int)d.P, d.Nz, d.N, d.W, d.lag0, d.L, scale, 1, y, st,
                                    (const float *)(ws + l.f), d.S);
         }
-        return launch_pair_gn(ws + l.a, ws + l.b, nullptr, false, d.S, d.W, d.N, d.Nz, d.lag0, d.L, 1, mode, y,
-                              st, (const float *)(ws + l.f));
+        r = launch_pair_gn(ws + l.a, ws + l.b, nullptr, false, d.S, d.W, d.N, d.Nz, d.lag0, d.L, 1, mode, y, st,
+                           (const float *)(ws + l.f));
+        if (r || job->stack != FPCC_STACK_PWS) return r;
+        return launch_pair_pws(1, ws + l.a, ws + l.b, real_ld(d.Nz), d.S, d.W, (int)d.Nz, d.lag0, d.L, 0.f,
+                               (float)job->pws_power, 1, y, st);
     case FPCC_PCC1:
     case FPCC_PCCV:
         return launch_pcc_time((const cf32 *)(ws + l.a), d.S, d.W, d.N, d.lag0, d.L,

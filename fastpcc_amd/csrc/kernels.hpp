@@ -24,6 +24,15 @@ int launch_pcc_time(const cf32 *d_phi, int S, int W, int N, int lag_first, int L
 int launch_pair_pcc2(const cf32 *d_F, int S, int W, int N, int Nz, int lag0, int L, int mode, int pair_base,
                      int npairs, float *d_y, hipStream_t st);
 
+// Phase-weighted stack over windows (pws.hip) for Nz <= 8192 (pws_supported).
+// src 0: PCC2 phasor spectra F[s*W+w][ld]; 1: FP32 half spectra (1-bit: Xa the
+// x >= 0 signs, Xb the x > 0 signs); 2: FP64 half spectra (GNCC, Xa = Xb).
+// multiply 0: y[p][l] = mean over windows of Re(a_w) * cscale, times the
+// coherence; 1: y[p][l] (a linear stack written before) times the coherence.
+bool pws_supported(int Nz);
+int launch_pair_pws(int src, const void *d_Xa, const void *d_Xb, long long ld, int S, int W, int Nz, int lag0, int L,
+                    float cscale, float power, int multiply, float *d_y, hipStream_t st);
+
 // Stack over windows as a per-frequency Gram (gram.hip).
 long long gram_ld(int nk);                       // row length of G for nk bins
 // Layout of G: bin k of pair p at (k >> sh) * gs + p * pl + (k & (2^sh - 1)).

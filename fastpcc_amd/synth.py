@@ -44,3 +44,26 @@ def pair_index(i, j, S):
 def pair_list(S):
     """(i, j) for p = 0..P-1 in the library's pair order."""
     return [(i, j) for i in range(S) for j in range(i + 1, S)]
+
+
+def morlet_bank(scales, Nz, w0=0.0):
+    """A caller-built Morlet filter bank for cuda_tspcc2_set_freq, from any list of
+    scales (in samples): per scale s the support Ls = min(2 ceil(5 s) + 1, Nz) and
+    origin center = Ls // 2 (src/FWTa/wavelet_def_v7.c:312-322), samples
+    w[u] = (sqrt(pi) s)^(-1/2) exp(i w0 t) exp(-t^2/2), t = (u - center)/s (the
+    Morlet of :38-52; w0 = pi sqrt(2/ln 2) when 0).  Returns dict(scale, wc, Ls,
+    center, K0) with K0 = sum 1/s, the normalisation tspcc2_set uses (C = 1/(K0 N))."""
+    if w0 == 0.0:
+        w0 = np.pi * np.sqrt(2 / np.log(2))
+    scales = np.asarray(scales, dtype=np.float64)
+    wc, Ls, center = [], [], []
+    for s in scales:
+        n = min(2 * int(np.ceil(5 * s)) + 1, Nz)
+        c = n // 2
+        t = (np.arange(n) - c) / s
+        k = 1 / np.sqrt(np.sqrt(np.pi) * s)
+        wc.append(k * np.exp(1j * w0 * t) * np.exp(-0.5 * t * t))
+        Ls.append(n)
+        center.append(c)
+    return dict(scale=scales, wc=wc, Ls=np.array(Ls, dtype=np.uint32), center=np.array(center, dtype=np.int32),
+                K0=float(np.sum(1 / scales)))

@@ -115,6 +115,16 @@ int cuda_tspcc2_set_freq(float **y0, float **x01, float **x02, unsigned int N, u
  * GRAM16;
  * PCC1 and PCC-v NONE, PLAIN (time-domain sums, N <= 8192); WPCC NONE, PLAIN
  * (= GRAM: the sum over scales and windows is one per-frequency Gram).
+ *
+ * FPCC_STACK_PWS (PCC2, GNCC and 1-bit, zero-pad length Nz <= 8192): the
+ * phase-weighted stack (Schimmel & Paulssen 1997) of the per-window correlations
+ * c_w, the stacking the reference leaves to the external ts-PWS program
+ * (src/PCC_fullpair_1b.c:568-569):
+ *     y[p][l] = (1/W sum_w c_w(l)) * |1/W sum_w u_w(l)|^pws_power,
+ * u_w(l) = a_w(l)/|a_w(l)| (0 where a_w(l) = 0), a_w(l) the analytic signal of
+ * window w's whole circular correlation (all Nz lags, before the lag window is
+ * cut; as AnalyticSignal computes it, src/FFTapps.c:45-80) at lag l.  pws_power 0
+ * gives the linear stack; 2 is the usual choice.
  * ------------------------------------------------------------------------- */
 enum { FPCC_PCC2 = 0, FPCC_PCC1 = 1, FPCC_PCCV = 2, FPCC_GNCC = 3, FPCC_CC1B = 4, FPCC_WPCC = 5 };
 enum {
@@ -122,7 +132,8 @@ enum {
     FPCC_STACK_PLAIN = 1, /* mean of per-window correlations (one IFFT per pair-window) */
     FPCC_STACK_ZSUM = 2,  /* sum of folded cross spectra over windows, one IFFT per pair */
     FPCC_STACK_GRAM = 3,  /* per-frequency Hermitian rank-W Gram (FP32 MFMA), one IFFT per pair */
-    FPCC_STACK_GRAM16 = 4 /* the same Gram on FP16 MFMA with split hi+lo operands            */
+    FPCC_STACK_GRAM16 = 4, /* the same Gram on FP16 MFMA with split hi+lo operands           */
+    FPCC_STACK_PWS = 5     /* phase-weighted stack of the per-window correlations (below)    */
 };
 
 typedef struct {
@@ -138,6 +149,7 @@ typedef struct {
     unsigned int V;                  /* WPCC voices per octave                         */
     int wtype;                       /* WPCC wavelet: -1 Morlet, -2 complete Morlet, -3 MexHat */
     double op1;                      /* WPCC Morlet w0 (0 = pi*sqrt(2/ln 2))           */
+    double pws_power;                /* FPCC_STACK_PWS: power of the phase coherence (>= 0) */
 } fpcc_job;
 
 /* Number of output floats for a job (0 if the job is invalid). */

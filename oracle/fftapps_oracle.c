@@ -714,21 +714,16 @@ static int orc64_cc1b(double **y, float **x1, float **x2, unsigned N, unsigned T
 /* WPCC (src/FFTapps.c:1076-1255, CPU branch), FP64 throughout               */
 /* ------------------------------------------------------------------------- */
 
-static int tspcc2_core(float **yf, double **yd, float **x1, float **x2, unsigned N, unsigned Tr,
-                       int Lag1, int Lag2, double pmin, double pmax, unsigned V, int type, double op1)
+/* The correlation of tspcc2_set (:1116-1252) with a given filter bank: S
+ * wavelets wc[s] of Ls[s] samples with origin center[s], scales scale[s], the
+ * record placed at Ls0 = Ls[S-1] in Nz zeros, output C * Re IFFT(Y) over L lags
+ * from lag.  cuda_tspcc2_set_freq (src/ccs_cuda.cu:898-1036) computes the same
+ * steps with a caller-built bank and C; the phasor here is the CPU AmpNorm. */
+static void wpcc_with_bank(float **yf, double **yd, float **x1, float **x2, unsigned N, unsigned Tr,
+                           unsigned Nz, unsigned S, const double *scale, double complex *const *wc,
+                           const unsigned *Ls_, const int *center, double C, int lag, int L)
 {
-    int L = abs(Lag2 - Lag1 + 1); /* as written at :1083 */
-    unsigned ua1 = abs(Lag1), ua2 = abs(Lag2);
-    if (ua1 >= N || ua2 >= N) return -3;
-    for (unsigned tr = 0; tr < Tr; tr++) {
-        if (yf) memset(yf[tr], 0, L * sizeof(float));
-        else memset(yd[tr], 0, L * sizeof(double));
-    }
-    orc_wfamily *wf = orc_wpcc_family(N, Lag1, Lag2, pmin, pmax, V, type, op1);
-    if (!wf) return -3;
-    const unsigned Nz = wf->Nz, S = wf->S, Ls0 = wf->Ls0;
-    const int lag = (Lag2 >= Lag1) ? Lag1 : Lag2;
-
+    const unsigned Ls0 = Ls_[S - 1];
     /* FFTs of the wavelet family (:1137-1149): w[c..Ls) at 0, w[0..c) at Nz-c */
     offt_plan_d *fZ = offt_make_d(Nz, 0), *bZ = offt_make_d(Nz, 1);
     size_t wl = maxz(offt_scratch_len_d(fZ), offt_scratch_len_d(bZ));
@@ -737,15 +732,14 @@ static int tspcc2_core(float **yf, double **yd, float **x1, float **x2, unsigned
         double complex *ws = malloc(sizeof(double complex) * wl);
         for (unsigned s = 0; s < S; s++) {
             double complex *f = fw + (size_t)s * Nz;
-            const double complex *pc = wf->wc[s];
-            int c = wf->center[s], Ls = (int)wf->Ls[s];
+            const double complex *pc = wc[s];
+            int c = center[s], Ls = (int)Ls_[s];
             memcpy(f, pc + c, (Ls - c) * sizeof(double complex));
             memcpy(f + Nz - c, pc, c * sizeof(double complex));
             offt_execute_ws_d(fZ, f, f, ws);
         }
         free(ws);
     }
-    const double C = 1. / (wf->K0 * (double)N);
     const double da2 = 1. / (double)Nz;
 #pragma omp parallel
     {
@@ -771,7 +765,7 @@ static int tspcc2_core(float **yf, double **yd, float **x1, float **x2, unsigned
                 offt_execute_ws_d(bZ, x2w, x2w, ws);
                 orc_ampnorm(x2w, Nz);
                 offt_execute_ws_d(fZ, x2w, x2w, ws);
-                double da3 = 1. / (wf->scale[s] * (double)Nz);
+                double da3 = 1. / (scale[s] * (double)Nz);
                 for (unsigned n = 0; n < Nz; n++) ywt[n] += da3 * conj(x1w[n]) * x2w[n];
             }
             offt_execute_ws_d(bZ, ywt, ywt, ws);
@@ -789,7 +783,49 @@ static int tspcc2_core(float **yf, double **yd, float **x1, float **x2, unsigned
     }
     free(fw);
     offt_destroy_d(fZ); offt_destroy_d(bZ);
+}
+
+static int tspcc2_core(float **yf, double **yd, float **x1, float **x2, unsigned N, unsigned Tr,
+                       int Lag1, int Lag2, double pmin, double pmax, unsigned V, int type, double op1)
+{
+    int L = abs(Lag2 - Lag1 + 1); /* as written at :1083 */
+    unsigned ua1 = abs(Lag1), ua2 = abs(Lag2);
+    if (ua1 >= N || ua2 >= N) return -3;
+    for (unsigned tr = 0; tr < Tr; tr++) {
+        if (yf) memset(yf[tr], 0, L * sizeof(float));
+        else memset(yd[tr], 0, L * sizeof(double));
+    }
+    orc_wfamily *wf = orc_wpcc_family(N, Lag1, Lag2, pmin, pmax, V, type, op1);
+    if (!wf) return -3;
+    const int lag = (Lag2 >= Lag1) ? Lag1 : Lag2;
+    wpcc_with_bank(yf, yd, x1, x2, N, Tr, wf->Nz, wf->S, wf->scale, wf->wc, wf->Ls, wf->center,
+                   1. / (wf->K0 * (double)N), lag, L);
     orc_wfamily_free(wf);
+    return 0;
+}
+
+int orc_wpcc_bank_f64(double *y, const float *x1, const float *x2, unsigned N, unsigned Tr, unsigned Nz,
+                      int Lag1, int Lag2, unsigned S, const double *scale, const double *wflat,
+                      const unsigned *Ls, const int *center, double C)
+{
+    if (!S || !Tr) return -1;
+    const int L = Lag2 >= Lag1 ? Lag2 - Lag1 + 1 : Lag1 - Lag2 + 1;
+    const int lag = Lag2 >= Lag1 ? Lag1 : Lag2;
+    double complex **wc = malloc(sizeof(double complex *) * S);
+    size_t off = 0;
+    for (unsigned s = 0; s < S; s++) {
+        wc[s] = (double complex *)(wflat + 2 * off);
+        off += Ls[s];
+    }
+    double **py = malloc(sizeof(double *) * Tr);
+    float **p1 = malloc(sizeof(float *) * Tr), **p2 = malloc(sizeof(float *) * Tr);
+    for (unsigned t = 0; t < Tr; ++t) {
+        py[t] = y + (size_t)t * L;
+        p1[t] = (float *)x1 + (size_t)t * N;
+        p2[t] = (float *)x2 + (size_t)t * N;
+    }
+    wpcc_with_bank(NULL, py, p1, p2, N, Tr, Nz, S, scale, wc, Ls, center, C, lag, L);
+    free(wc); free(py); free(p1); free(p2);
     return 0;
 }
 
@@ -797,6 +833,102 @@ int orc_tspcc2_set(float **y, float **x1, float **x2, unsigned N, unsigned Tr, i
                    double pmin, double pmax, unsigned V, int type, double op1)
 {
     return tspcc2_core(y, NULL, x1, x2, N, Tr, Lag1, Lag2, pmin, pmax, V, type, op1);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Phase-weighted stack of one pair's per-window correlations, all FP64       */
+/* ------------------------------------------------------------------------- */
+
+/* The reference stops at the per-window correlations and leaves their stacking
+ * to the external ts-PWS program (src/PCC_fullpair_1b.c:568-569, README.md:10-11).
+ * The phase-weighted stack (Schimmel & Paulssen 1997, GJI 130:497-505) of W
+ * windows' correlations c_w(l) is
+ *     PWS(l) = (1/W sum_w c_w(l)) * |1/W sum_w u_w(l)|^power,
+ * u_w(l) = a_w(l)/|a_w(l)| (0 where a_w(l) = 0), a_w the analytic signal of the
+ * window's correlation.  Here a_w is AnalyticSignal (src/FFTapps.c:45-80) of the
+ * whole circular correlation r_w[0..Nz) (the real sequence the lag window is cut
+ * from), read at the window's lags (lag0 + l) mod Nz: the Hilbert transform sees
+ * the correlation beyond the lag window, so the window's ends carry no truncation
+ * artefacts.  c_w is the kind's per-window output (all-FP64 build); r_w is, per
+ * kind, Re IFFT(conj F1 F2) of the PCC2 phasor spectra, IFFT(conj S1 S2) of the
+ * 1-bit sign spectra, IFFT(conj X1 X2) of GNCC's records (unnormalised: every
+ * kind's normalisation is a positive factor, which leaves u unchanged). */
+static void circ_corr_d(int kind, double *r, const float *x1, const float *x2, unsigned N, unsigned Nz)
+{
+    offt_plan_d *fN = offt_make_d(N, 0), *bN = offt_make_d(N, 1);
+    offt_plan_d *fZ = offt_make_d(Nz, 0), *bZ = offt_make_d(Nz, 1);
+    size_t wl = maxz(maxz(offt_scratch_len_d(fN), offt_scratch_len_d(bN)),
+                     maxz(offt_scratch_len_d(fZ), offt_scratch_len_d(bZ)));
+    double complex *a = calloc(Nz, sizeof(double complex)), *b = calloc(Nz, sizeof(double complex));
+    double complex *ws = malloc(sizeof(double complex) * wl);
+    double *xd = malloc(sizeof(double) * N);
+    const float *xs[2] = {x1, x2};
+    double complex *ab[2] = {a, b};
+    for (int j = 0; j < 2; j++) {
+        double complex *v = ab[j];
+        if (kind == ORC_PCC2) {
+            for (unsigned n = 0; n < N; ++n) xd[n] = xs[j][n];
+            analytic_d_plan(v, xd, N, fN, bN, ws);
+            orc_ampnormf_d(v, N);
+        } else if (kind == ORC_CC1B) {
+            /* :1037, :1042 */
+            for (unsigned n = 0; n < N; n++) v[n] = j == 0 ? (xs[j][n] >= 0 ? 1 : -1) : (xs[j][n] > 0 ? 1 : -1);
+        } else {
+            for (unsigned n = 0; n < N; n++) v[n] = xs[j][n];
+        }
+        for (unsigned n = N; n < Nz; n++) v[n] = 0;
+        offt_execute_ws_d(fZ, v, v, ws);
+    }
+    for (unsigned n = 0; n < Nz; n++) a[n] = conj(a[n]) * b[n];
+    offt_execute_ws_d(bZ, a, a, ws);
+    for (unsigned n = 0; n < Nz; n++) r[n] = creal(a[n]);
+    free(a); free(b); free(ws); free(xd);
+    offt_destroy_d(fN); offt_destroy_d(bN); offt_destroy_d(fZ); offt_destroy_d(bZ);
+}
+
+int orc_pws_f64(int kind, double *y, const float *x, unsigned S, unsigned W, unsigned N, int Lag1, int Lag2,
+                unsigned pair_begin, unsigned pair_end, double power)
+{
+    if (kind != ORC_PCC2 && kind != ORC_CC1B && kind != ORC_GNCC) return -1;
+    int L, lag;
+    lag_setup(Lag1, Lag2, &L, &lag);
+    unsigned ua1 = abs(Lag1), ua2 = abs(Lag2);
+    if (ua1 >= N || ua2 >= N) return -3;
+    const unsigned Nz = nz_rule(N, ua1 > ua2 ? ua1 : ua2), Ntot = W * N, P = S * (S - 1) / 2;
+    if (pair_end > P) pair_end = P;
+    double *c = malloc(sizeof(double) * (size_t)W * L);  /* c_w(l) of every window */
+    double *r = malloc(sizeof(double) * Nz);
+    double complex *z = malloc(sizeof(double complex) * Nz);
+    double *sc = malloc(sizeof(double) * L);
+    double complex *su = malloc(sizeof(double complex) * L);
+    unsigned p = 0;
+    int rc = 0;
+    for (unsigned i = 0; i < S && p < pair_end; ++i)
+        for (unsigned j = i + 1; j < S && p < pair_end; ++j, ++p) {
+            if (p < pair_begin) continue;
+            const float *x1 = x + (size_t)i * Ntot, *x2 = x + (size_t)j * Ntot;
+            rc = orc_run_f64(kind, c, x1, x2, N, W, Lag1, Lag2, 2.0, 0, 0, 8, -1, 0);
+            if (rc) goto done;
+            for (int l = 0; l < L; ++l) { sc[l] = 0; su[l] = 0; }
+            for (unsigned w = 0; w < W; ++w) {
+                circ_corr_d(kind, r, x1 + (size_t)w * N, x2 + (size_t)w * N, N, Nz);
+                orc_analytic_d(z, r, Nz);
+                for (int l = 0; l < L; ++l) {
+                    const double complex v = z[((long)lag + l + Nz) % Nz];
+                    const double m = cabs(v);
+                    sc[l] += c[(size_t)w * L + l];
+                    if (m > 0) su[l] += v / m;
+                }
+            }
+            double *dst = y + (size_t)(p - pair_begin) * L;
+            for (int l = 0; l < L; ++l) {
+                const double coh = power == 0 ? 1.0 : pow(cabs(su[l] / W), power);
+                dst[l] = sc[l] / W * coh;
+            }
+        }
+done:
+    free(c); free(r); free(z); free(sc); free(su);
+    return rc;
 }
 
 /* ------------------------------------------------------------------------- */

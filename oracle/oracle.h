@@ -61,6 +61,14 @@ int orc_allpairs_f64(int kind, double *y, const float *x, unsigned S, unsigned W
                      int Lag1, int Lag2, int stacked, unsigned pair_begin, unsigned pair_end,
                      double v, double pmin, double pmax, unsigned V, int type, double op1);
 
+/* Phase-weighted stack over W windows (Schimmel & Paulssen 1997) of the all-FP64
+ * per-window correlations of pairs [pair_begin, pair_end): y [pairs][L].  The
+ * instantaneous phase is that of AnalyticSignal (:45-80) of each window's whole
+ * circular correlation (Nz lags), read at the lag window.  kind: ORC_PCC2,
+ * ORC_CC1B or ORC_GNCC.  power 0 gives the linear stack. */
+int orc_pws_f64(int kind, double *y, const float *x, unsigned S, unsigned W, unsigned N, int Lag1, int Lag2,
+                unsigned pair_begin, unsigned pair_end, double power);
+
 /* ---- wavelet filter bank (src/FWTa/wavelet_def_v7.c:204-339) ---- */
 typedef struct {
     unsigned S;           /* number of scales = J*V             */
@@ -79,6 +87,13 @@ typedef struct {
 orc_wfamily *orc_wpcc_family(unsigned N, int Lag1, int Lag2, double pmin, double pmax, unsigned V,
                              int type, double op1);
 void orc_wfamily_free(orc_wfamily *w);
+/* The WPCC correlation with a caller-built filter bank (the semantics of
+ * cuda_tspcc2_set_freq, src/ccs_cuda.cu:898-1036, with the CPU AmpNorm phasor of
+ * src/FFTapps.c:1116-1252), all FP64: x1/x2 [Tr][N], y [Tr][|Lag2-Lag1|+1] from
+ * min(Lag1, Lag2); wflat holds the S wavelets concatenated, interleaved re,im. */
+int orc_wpcc_bank_f64(double *y, const float *x1, const float *x2, unsigned N, unsigned Tr, unsigned Nz,
+                      int Lag1, int Lag2, unsigned S, const double *scale, const double *wflat,
+                      const unsigned *Ls, const int *center, double C);
 double orc_erfi(double z);
 /* Flat export for ctypes: fills scale/Ls/center (length S) and the wavelet samples
  * concatenated (interleaved re,im) into wbuf (length 2*sum Ls); returns S or <0. */

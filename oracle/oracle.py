@@ -49,6 +49,9 @@ def lib():
         L.orc_wpcc_family_export.argtypes = [u, i, i, d, d, u, i, d, ctypes.POINTER(ctypes.c_uint),
                                              dp, dp, ctypes.POINTER(ctypes.c_uint),
                                              ctypes.POINTER(ctypes.c_int), dp, u]
+        L.orc_wpcc_bank_f64.argtypes = [dp, fp, fp, u, u, u, i, i, u, dp, dp, ctypes.POINTER(ctypes.c_uint),
+                                         ctypes.POINTER(ctypes.c_int), d]
+        L.orc_pws_f64.argtypes = [i, dp, fp, u, u, u, i, i, u, u, d]
         L.orc_qabsmedian.argtypes = [fp, i]
         L.orc_qabsmedian.restype = ctypes.c_float
         L.orc_qmedian.argtypes = [fp, i]
@@ -118,6 +121,22 @@ def allpairs(kind, x, W, N, lag1, lag2, stacked=True, f64=False, pair_begin=0, p
     return y
 
 
+def pws(kind, x, W, N, lag1, lag2, power, pair_begin=0, pair_end=None):
+    """FP64 phase-weighted stack over W windows of pairs [pair_begin, pair_end): [pairs][L]."""
+    kind = KINDS.get(kind, kind)
+    x = _f32(x)
+    S = x.shape[0]
+    P = S * (S - 1) // 2
+    if pair_end is None or pair_end > P:
+        pair_end = P
+    y = np.zeros((pair_end - pair_begin, out_len(kind, lag1, lag2)))
+    r = lib().orc_pws_f64(kind, _ptr(y, ctypes.c_double), _ptr(x, ctypes.c_float), S, W, N, lag1, lag2,
+                          pair_begin, pair_end, power)
+    if r != 0:
+        raise ValueError(f"oracle returned {r}")
+    return y
+
+
 def max_threads():
     return lib().orc_max_threads()
 
@@ -176,6 +195,27 @@ def wpcc_family(N, lag1, lag2, pmin, pmax, V, wtype=-1, op1=0.0):
         off += int(Ls[s])
     return dict(S=S, J=meta[1], V=meta[2], Nz=meta[3], Ls0=meta[4], s0=fl[0], K0=fl[1], op1=fl[2],
                 scale=scale, Ls=Ls, center=center, wc=waves)
+
+
+def wpcc_bank(x1, x2, Nz, lag1, lag2, scale, wc, Ls, center, C):
+    """FP64 WPCC of one pair over Tr windows with a caller-built filter bank (the
+    semantics of cuda_tspcc2_set_freq with the CPU phasor): y [Tr][|lag2-lag1|+1]."""
+    x1 = _f32(np.atleast_2d(x1))
+    x2 = _f32(np.atleast_2d(x2))
+    Tr, N = x1.shape
+    S = len(scale)
+    L = abs(lag2 - lag1) + 1
+    y = np.zeros((Tr, L))
+    scale = np.ascontiguousarray(scale, dtype=np.float64)
+    Ls = np.ascontiguousarray(Ls, dtype=np.uint32)
+    center = np.ascontiguousarray(center, dtype=np.int32)
+    wflat = np.ascontiguousarray(np.concatenate([np.asarray(w, dtype=np.complex128) for w in wc])).view(np.float64)
+    r = lib().orc_wpcc_bank_f64(_ptr(y, ctypes.c_double), _ptr(x1, ctypes.c_float), _ptr(x2, ctypes.c_float), N, Tr,
+                                Nz, lag1, lag2, S, _ptr(scale, ctypes.c_double), _ptr(wflat, ctypes.c_double),
+                                _ptr(Ls, ctypes.c_uint), _ptr(center, ctypes.c_int), float(C))
+    if r != 0:
+        raise ValueError(f"oracle returned {r}")
+    return y
 
 
 # ---- per-station preprocessing (src/PCC_fullpair_1b.c:881-925, 1065-1251) ----

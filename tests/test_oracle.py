@@ -385,3 +385,109 @@ def test_one_sided_lag_windows_are_slices(orc, kind, f64):
             ref = full[:, l1 + N - 1:l1 + N - 1 + y.shape[1]]
         assert y.shape == ref.shape
         np.testing.assert_allclose(y, ref, rtol=0, atol=1e-9 if f64 else 1e-6)
+
+
+def test_morlet_bank_matches_the_reference_family(orc):
+    """The caller-bank builder used for the literal 30-scale configuration-4 run
+    reproduces the family tspcc2_set builds (Morlet, 2-200 samples, 8 voices)."""
+    from fastpcc_amd import synth
+    f = orc.wpcc_family(86400, -2000, 2000, 2.0, 200.0, 8, -1, 0.0)
+    b = synth.morlet_bank(f["scale"], f["Nz"])
+    assert np.array_equal(b["Ls"], f["Ls"]) and np.array_equal(b["center"], f["center"])
+    assert abs(b["K0"] - f["K0"]) <= 1e-12 * f["K0"]
+    for w, wr in zip(b["wc"], f["wc"]):
+        assert np.abs(w - wr).max() <= 1e-13
+
+
+def test_wpcc_bank_oracle_equals_tspcc2(orc):
+    """The oracle's WPCC with a given bank, fed the family tspcc2_set builds, is
+    tspcc2_set's FP64 result (the same steps), for Morlet and Mexican hat."""
+    rng = np.random.default_rng(9)
+    x1 = rng.standard_normal((2, 600)).astype(np.float32)
+    x2 = rng.standard_normal((2, 600)).astype(np.float32)
+    for wt in (-1, -3):
+        f = orc.wpcc_family(600, -80, 60, 4.0, 40.0, 3, wt, 0.0)
+        y = orc.wpcc_bank(x1, x2, f["Nz"], -80, 60, f["scale"], f["wc"], f["Ls"], f["center"], 1.0 / (f["K0"] * 600))
+        yr = orc.run("wpcc", x1, x2, -80, 60, f64=True, pmin=4.0, pmax=40.0, V=3, wtype=wt)
+        assert np.array_equal(y, yr)
+
+
+# ---- phase-weighted stack (downstream of the reference: src/PCC_fullpair_1b.c:568-569) ----
+
+def _nz(N, M):
+    nz = 1
+    while nz < N + M:
+        nz *= 2
+    return nz
+
+
+def _numpy_pws(kind, x, W, N, lag, power):
+    """Independent PWS: scipy's analytic signal (phasors and the Hilbert transform of
+    each window's circular correlation), numpy FFTs, the mean over windows."""
+    S = x.shape[0]
+    Nz = _nz(N, lag)
+    lags = np.arange(-lag, lag + 1)
+    out = []
+    for i in range(S):
+        for j in range(i + 1, S):
+            sc, su = 0.0, 0.0
+            for w in range(W):
+                a = x[i, w * N:(w + 1) * N].astype(np.float64)
+                b = x[j, w * N:(w + 1) * N].astype(np.float64)
+                if kind == "pcc2":
+                    za, zb = ss.hilbert(a), ss.hilbert(b)
+                    za = za / (np.abs(za) + 1e-6 * np.abs(za).max())
+                    zb = zb / (np.abs(zb) + 1e-6 * np.abs(zb).max())
+                    r = np.fft.ifft(np.conj(np.fft.fft(za, Nz)) * np.fft.fft(zb, Nz)).real * Nz
+                    c = r[lags % Nz] / (Nz * N)
+                else:  # 1-bit: x1 >= 0 and x2 > 0 give +1
+                    sa, sb = np.where(a >= 0, 1.0, -1.0), np.where(b > 0, 1.0, -1.0)
+                    r = np.fft.ifft(np.conj(np.fft.fft(sa, Nz)) * np.fft.fft(sb, Nz)).real * Nz
+                    c = r[lags % Nz] / Nz / (N - np.abs(lags))
+                z = ss.hilbert(r)[lags % Nz]
+                sc = sc + c
+                su = su + z / np.abs(z)
+            out.append(sc / W * np.abs(su / W) ** power)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("kind", ["pcc2", "cc1b"])
+def test_pws_matches_independent_numpy(orc, kind):
+    x, _ = synth.make_stations(4, 3 * 200, 30, seed=77)
+    y = orc.pws(kind, x, 3, 200, -40, 40, 2.0)
+    yr = _numpy_pws(kind, x, 3, 200, 40, 2.0)
+    assert np.abs(y - yr).max() <= 1e-12
+
+
+@pytest.mark.parametrize("kind", ["pcc2", "cc1b", "gncc"])
+def test_pws_power_zero_is_the_linear_stack(orc, kind):
+    x, _ = synth.make_stations(4, 4 * 300, 50, seed=5)
+    y = orc.pws(kind, x, 4, 300, -60, 60, 0.0)
+    yl = orc.allpairs(kind, x, 4, 300, -60, 60, stacked=True, f64=True)
+    assert np.abs(y - yl).max() <= 1e-14
+
+
+@pytest.mark.parametrize("kind", ["pcc2", "cc1b", "gncc"])
+def test_pws_identical_windows_give_the_linear_stack(orc, kind):
+    """Equal windows have equal phases: coherence 1, PWS = linear stack = the window's correlation."""
+    x1, _ = synth.make_stations(3, 256, 40, seed=8)
+    x = np.tile(x1, (1, 5))
+    y = orc.pws(kind, x, 5, 256, -50, 50, 2.5)
+    yl = orc.allpairs(kind, x, 5, 256, -50, 50, stacked=True, f64=True)
+    y1 = orc.allpairs(kind, x1, 1, 256, -50, 50, stacked=True, f64=True)
+    assert np.abs(y - yl).max() <= 1e-12 and np.abs(yl - y1).max() <= 1e-12
+
+
+@pytest.mark.parametrize("kind", ["pcc2", "cc1b", "gncc"])
+def test_pws_sign_flipped_window(orc, kind):
+    """Windows (w, -w, w) at the second station: the correlations and phases of the
+    middle window are negated, so the mean is c/3 and the coherence |u/3|, and
+    PWS = c / 3^(1 + power); a flipped pair of windows alone cancels to 0."""
+    a, _ = synth.make_stations(2, 300, 40, seed=11)
+    x = np.stack([np.tile(a[0], 3), np.concatenate([a[1], -a[1], a[1]])])
+    c = orc.allpairs(kind, a, 1, 300, -70, 70, stacked=True, f64=True)
+    for power in (1.0, 2.0, 3.5):
+        y = orc.pws(kind, x, 3, 300, -70, 70, power)
+        assert np.abs(y - c / 3 ** (1 + power)).max() <= 1e-12 * max(1.0, np.abs(c).max())
+    y2 = orc.pws(kind, x[:, :600], 2, 300, -70, 70, 2.0)
+    assert np.abs(y2).max() <= 1e-15
