@@ -54,11 +54,35 @@ def kernel_flops(name, cfg, Nz):
     return P * (Nz // 2 + 1) * 2 * W * 8
 
 
-# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by
-# the same counters' readings for a 1 GiB copy (profiles/
-# pcc2_allpairs_256st_gram16_hbm_traffic.md), keyed by (config, stack, kernel).
-MEASURED_TRAFFIC = {("c3", "gram16", "gram_split16"): 1635.7e6, ("c3", "gram16", "pair_c2r"): 1340.5e6,
-                    ("c3", "gram16", "station_pcc"): 495.8e6, ("c3", "gram16", "spectra_to_u"): 907.2e6}
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by
+# the same counters' readings for a 1 GiB copy), read from the committed summary
+# of the counter runs for the headline workload; the summary names the commit it
+# was measured at, which the JSON line repeats.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pcc2_allpairs_256st_gram16_hbm_traffic.md")
+TRAFFIC_KERNELS = {"k_station_pcc_3600": "station_pcc", "k_spectra_to_u": "spectra_to_u",
+                   "k_gram16_dma": "gram_split16", "k_pair_c2r_4096": "pair_c2r", "k_station_u_3600": "station_u"}
+
+
+def measured_traffic():
+    """{bench kernel name: traffic bytes per launch}, and the commit line of the summary."""
+    out, commit = {}, None
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return out, None
+    for ln in lines:
+        if ln.lower().startswith("measured at commit"):
+            commit = ln.split()[-1].strip("`.")
+        cells = [c.strip() for c in ln.split("|")]
+        if len(cells) < 8 or not cells[1].startswith("fpcc::"):
+            continue
+        sym = cells[1][len("fpcc::"):].split("<")[0]
+        if sym in TRAFFIC_KERNELS:
+            try:
+                out[TRAFFIC_KERNELS[sym]] = float(cells[6]) * 1e6
+            except ValueError:
+                pass
+    return out, commit
 
 
 def kernel_bytes(name, cfg, Nz):
@@ -103,30 +127,135 @@ def cpu_baseline(x, cfg, seconds):
                       f"oracle pcc2_set restatement, OpenMP over windows of one pair per call"}
 
 
-def serve_run(n_requests, rate):
-    """Poisson stream through the request server after a short warm-up (plans and
-    filter banks of every zero-pad length are built on first use); then the
-    network-day stream (station subsets of one shared window) with the resident
-    spectra cache on and off."""
-    from fastpcc_amd import serving, Server
+def cpu_rate(kind, x, W, N, lag, budget_s, v=2.0):
+    """The oracle's reference-precision restatement of the reference's *_set call for
+    `kind` on this host's cores: one station pair per call with OpenMP over its
+    windows (as the reference's one-pair-per-run driver), pairs in order until all
+    are done or budget_s has passed."""
+    import oracle
+    S = x.shape[0]
+    P = S * (S - 1) // 2
+    done, t0, chunk = 0, time.perf_counter(), 1
+    while done < P:
+        n = min(chunk, P - done)
+        oracle.allpairs(kind, x, W, N, -lag, lag, stacked=True, pair_begin=done, pair_end=done + n, v=v)
+        done += n
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+        chunk = max(1, min(256, int(done / el * 0.25)))
+    el = time.perf_counter() - t0
+    return {"value": done * W / el, "unit": "pair-correlations/s", "cores": oracle.max_threads(), "kind": "port",
+            "sample": (f"all {P} pairs" if done == P else f"first {done} of {P} pairs") +
+                      f" x {W} windows ({el:.1f} s)"}
+
+
+def _check_served(reqs, ys, idx, net):
+    """Served outputs of requests idx against the same job run alone (fpcc_allpairs,
+    bit for bit) and against the oracle on the first and last pair (reference
+    precision; WPCC against its FP64 restatement, as the reference computes it)."""
+    import numpy as np
+    import oracle
+    from fastpcc_amd import serving, api
+    out = {"requests": len(idx), "alone_bit_exact": True, "max_abs_err_vs_oracle": 0.0, "checked": []}
+    for k in idx:
+        rq = reqs[k]
+        job = serving.job_for(rq, net.shape[1])
+        x = np.ascontiguousarray(net[rq["s0"]:rq["s0"] + rq["S"], rq["t0"]:rq["t0"] + rq["N"]])
+        kw = serving.WPCC_PARAMS if rq["kind"] == "wpcc" else {}
+        alone = api.allpairs(x, 1, rq["N"], -rq["lag"], rq["lag"], kind=rq["kind"], stack="none", **kw).ravel()
+        y = np.asarray(ys[k])
+        same = bool(np.array_equal(alone, y))
+        out["alone_bit_exact"] &= same
+        P = rq["S"] * (rq["S"] - 1) // 2
+        L = 2 * rq["lag"] + 1
+        err = 0.0
+        for pb in (0, P - 1):
+            yr = oracle.allpairs(rq["kind"], x, 1, rq["N"], -rq["lag"], rq["lag"], stacked=True,
+                                 f64=rq["kind"] == "wpcc", pair_begin=pb, pair_end=pb + 1, **kw)
+            err = max(err, float(np.abs(y.reshape(P, L)[pb] - yr[0]).max()))
+        out["max_abs_err_vs_oracle"] = max(out["max_abs_err_vs_oracle"], err)
+        out["checked"].append({"kind": rq["kind"], "stations": rq["S"], "N": rq["N"], "lags": rq["lag"],
+                               "alone_bit_exact": same, "max_abs_err_vs_oracle": err})
+        del job
+    return out
+
+
+def serve_run(n_requests, rate, sweep_rates, sweep_n, x3=None, y3_ref=None, check=True):
+    """The serving workload through the request server: after a warm-up (plans and
+    filter banks of every zero-pad length are built on first use), the Poisson
+    stream at `rate` with a sample of its outputs checked; an offered-load sweep
+    (the same mix at increasing arrival rates, up to and past saturation); the
+    256-station headline job submitted through the server; and the network-day
+    stream (station subsets of one shared window) with the resident spectra cache
+    on and off."""
+    import numpy as np
+    from fastpcc_amd import serving, Server, make_job, pinned_empty
     net = serving.make_network()
     kw = dict(device_bytes=96 << 30, pinned_bytes=4 << 30, max_inflight=16, n_streams=4)
     warm = serving.make_requests(48, rate, seed=99)
     reqs = serving.make_requests(n_requests, rate, seed=7)
+    # two requests of each kind (WPCC: windows up to 30000 samples, for the FP64 oracle's time)
+    idx = []
+    for kind in serving.KINDS:
+        cand = [k for k, r in enumerate(reqs) if r["kind"] == kind and (kind != "wpcc" or r["N"] <= 30000)]
+        idx += cand[:2]
     pool = serving.OutputPool(24, max(serving.out_len(r) for r in reqs + warm))
     with Server(**kw) as srv:
         serving.run(warm, net, server=srv, realtime=False, pool=pool, keep=False)
     with Server(**kw) as srv:
-        stats, _, _ = serving.run(reqs, net, server=srv, realtime=True, pool=pool, keep=False)
+        stats, _, ys = serving.run(reqs, net, server=srv, realtime=True, pool=pool, keep=False,
+                                   keep_idx=idx if check else ())
     out = serving.summary(stats, reqs, rate)
     out["workload"] = ("Poisson stream, stations U{4..128}, one window of N log-uniform in [1800, 86400], "
                        "lags +-U{200..2000} (< N), PCC2/GNCC/1-bit/WPCC 50/20/10/20 (WPCC: Morlet, periods "
                        "2-200 samples, 8 voices), inputs from a pinned 128 x 86400 network, outputs into a "
                        "pool of 24 pinned buffers (pageable when all are busy); no station keys (every "
                        "request starts at its own time, so no station window repeats)")
+    if check:
+        out["served_output_check"] = _check_served(reqs, ys, idx, net)
+    del ys
+    sweep = []
+    for i, r in enumerate(sweep_rates):
+        rq = serving.make_requests(sweep_n, r, seed=300 + i)
+        with Server(**kw) as srv:
+            st, _, _ = serving.run(rq, net, server=srv, realtime=True, pool=pool, keep=False)
+        e = serving.summary(st, rq, r)
+        sweep.append({k: e[k] for k in ("arrival_rate_per_s", "requests", "completed", "completed_per_s",
+                                        "pair_correlations_per_s", "p50_ms", "p99_ms", "max_ms",
+                                        "inflight_peak", "wall_s")})
+    out["load_sweep"] = {"requests_per_rate": sweep_n, "rates": sweep,
+                         "note": "the same mix at increasing Poisson arrival rates; past saturation the "
+                                 "completion rate levels off and latency grows with the queue"}
     del pool
+    if x3 is not None:
+        # the headline job (256 stations x 24 windows of 3600, split-FP16 stack) through
+        # the server from pinned host memory: four requests in a row, overlapping the
+        # input and output copies of one with the kernels of another
+        S3 = x3.shape[0]
+        P3 = S3 * (S3 - 1) // 2
+        job = make_job("pcc2", S3, 24, 3600, -1000, 1000, stack="gram16")
+        xp = pinned_empty(x3.shape)
+        xp[:] = x3
+        ys3 = [pinned_empty(P3 * 2001) for _ in range(4)]
+        with Server(**kw) as srv:
+            srv.wait(srv.submit(job, xp, ys3[0]))  # plans
+            t0 = time.perf_counter()
+            ids = [srv.submit(job, xp, y) for y in ys3]
+            lat = []
+            for rid in ids:
+                st = srv.wait(rid)
+                lat.append((st["t_done"] - st["t_submit"]) * 1e3)
+            wall = time.perf_counter() - t0
+        c3 = {"requests": 4, "wall_ms": round(wall * 1e3, 2), "latency_ms": [round(v, 2) for v in lat],
+              "pair_correlations_per_s": 4 * P3 * 24 / wall,
+              "note": "PCIe-inclusive: 88.5 MB in and 261 MB out per request, pinned host buffers"}
+        if y3_ref is not None:
+            c3["equal_to_device_path"] = bool(all(np.array_equal(np.asarray(y), y3_ref) for y in ys3))
+        out["headline_job_served"] = c3
+        del ys3, xp
     day_rate = 200.0
-    day = serving.make_network_day_requests(max(100, n_requests // 4), day_rate)
+    day = serving.make_network_day_requests(max(100, n_requests // 12), day_rate)
     out["network_day"] = {"workload": "Poisson 200/s, stations: random subsets of 8-64 of the 128, the same "
                                       "21600-sample window (lags +-1000), PCC2/WPCC 50/50"}
     for name, cb in (("cache", 16 << 30), ("no_cache", 0)):
@@ -150,9 +279,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=1000,
+    ap.add_argument("--serve-requests", type=int, default=3000,
                     help="requests of the serving run (0: skip); 3000 is the full stream")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="mean arrivals per second")
+    ap.add_argument("--sweep-rates", default="100,200,400,800,1600,3200",
+                    help="arrival rates of the offered-load sweep ('' for none)")
+    ap.add_argument("--sweep-requests", type=int, default=400, help="requests per rate of the sweep")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay of the headline step")
     ap.add_argument("--no-host", action="store_true", help="skip the PCIe-inclusive host-buffer timing")
     ap.add_argument("--no-pre", action="store_true", help="skip the preprocessing timing")
@@ -210,14 +342,23 @@ def main():
         check = {"pairs": npair, "max_abs_err_vs_oracle": float(np.abs(y[:npair] - yr).max()),
                  "max_abs_err_vs_fp64": float(np.abs(y[:npair] - y64).max())}
 
-    lib.fpcc_profile_reset()
-    lib.fpcc_profile_enable(1)
+    # the headline: K steps with no per-kernel timing inside the timed loop
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         runner.run(x_dev, stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    # the same K steps again with a HIP event pair around every launch (on the
+    # stream it runs on), for the per-kernel durations and the roofline
+    lib.fpcc_profile_reset()
+    lib.fpcc_profile_enable(1)
+    torch.cuda.synchronize()
+    tp0 = time.perf_counter()
+    for _ in range(args.steps):
+        runner.run(x_dev, stream)
+    torch.cuda.synchronize()
+    tp1 = time.perf_counter()
     lib.fpcc_profile_enable(0)
     names = ctypes.create_string_buffer(4096)
     tot = (ctypes.c_double * 16)()
@@ -228,7 +369,9 @@ def main():
         kernels[nm] = {"avg_ms": tot[k] / max(cnt[k], 1), "launches": int(cnt[k])}
 
     ms_per_step = (t1 - t0) * 1e3 / args.steps
+    ms_per_step_profiled = (tp1 - tp0) * 1e3 / args.steps
     value = P * W / (ms_per_step / 1e3)
+    traffic, traffic_commit = measured_traffic() if (args.config, args.stack) == ("c3", "gram16") else ({}, None)
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"]) if kernels else None
     roof = None
     for k, v in kernels.items():
@@ -238,6 +381,9 @@ def main():
         f = kernel_flops(k, cfg, Nz)
         if f:
             v["algorithmic_TFLOPs"] = round(f / (v["avg_ms"] * 1e-3) / 1e12, 2)
+    for k, v in kernels.items():
+        if k in traffic:
+            v["traffic_MB"] = round(traffic[k] / 1e6, 1)
     if dom:
         f = kernel_flops(dom, cfg, Nz)
         b = kernel_bytes(dom, cfg, Nz)
@@ -251,14 +397,17 @@ def main():
             ach = f / (kernels[dom]["avg_ms"] * 1e-3) / 1e12
             peak = MFMA_PEAK_TFLOPS[dom]
             roof = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
-                    "frac": round(ach / peak, 4), "traffic": MEASURED_TRAFFIC.get((args.config, args.stack, dom)),
+                    "frac": round(ach / peak, 4), "traffic": traffic.get(dom),
                     "algorithmic_flops": f}
         else:
             b = kernel_bytes(dom, cfg, Nz)
             ach = b / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
             roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get((args.config, args.stack, dom)),
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic.get(dom),
                     "algorithmic_bytes": b}
+        if roof["traffic"] is not None:
+            roof["traffic_source"] = (os.path.relpath(TRAFFIC_FILE, ROOT) +
+                                      (f" (measured at commit {traffic_commit})" if traffic_commit else ""))
 
     variants = {}
     if not args.no_graph:
@@ -378,8 +527,10 @@ def main():
         x2d = torch.from_numpy(x2).to(dev)
         P2 = S2 * (S2 - 1) // 2
         kinds = {}
-        for kind, kst, kv, nst in (("pcc2", "gram16", 2.0, 10), ("gncc", "plain", 2.0, 10), ("cc1b", "gram16", 2.0, 10),
-                                   ("pcc1", "plain", 2.0, 2), ("pccv", "plain", 1.5, 1)):
+        for name, kind, kst, kv, nst in (("pcc2", "pcc2", "gram16", 2.0, 10), ("pcc2_plain", "pcc2", "plain", 2.0, 5),
+                                         ("pcc2_pws", "pcc2", "pws", 2.0, 5), ("gncc", "gncc", "plain", 2.0, 10),
+                                         ("cc1b", "cc1b", "gram16", 2.0, 10), ("cc1b_pws", "cc1b", "pws", 2.0, 5),
+                                         ("pcc1", "pcc1", "plain", 2.0, 2), ("pccv", "pccv", "plain", 1.5, 1)):
             kj = fastpcc_amd.make_job(kind, S2, W2c, N2c, -l2, l2, stack=kst, v=kv)
             kr = fastpcc_amd.DeviceRunner(kj, dev)
             kr.run(x2d, stream)
@@ -395,14 +546,40 @@ def main():
             if not args.no_check:
                 import oracle
                 yk = kr.y.view(P2, 2 * l2 + 1).cpu().numpy()[:2]
-                y64 = oracle.allpairs(kind, x2, W2c, N2c, -l2, l2, stacked=True, pair_end=2, f64=True, v=kv)
+                if kst == "pws":
+                    y64 = oracle.pws(kind, x2, W2c, N2c, -l2, l2, 2.0, pair_end=2)
+                else:
+                    y64 = oracle.allpairs(kind, x2, W2c, N2c, -l2, l2, stacked=True, pair_end=2, f64=True, v=kv)
                 ent["max_abs_err_vs_fp64"] = float(np.abs(yk - y64).max())
-            kinds[kind] = ent
+            if not args.no_cpu and name in ("pcc2", "gncc", "cc1b", "pcc1", "pccv"):
+                # the reference's CPU path for this kind on the same network (all pairs
+                # within the budget, else the first pairs)
+                ent["cpu_baseline"] = cpu_rate(kind, x2, W2c, N2c, l2, 3.0 if kind in ("pcc1", "pccv") else 12.0, kv)
+            kinds[name] = ent
             del kr
         variants["kinds_c2"] = {"unit": "pair-correlations/s", "kinds": kinds,
                                 "workload": "64 stations x 86400 samples, 24 windows of 3600, nfft 8192, lags "
-                                            "+-1000, all 2016 pairs, mean stack"}
+                                            "+-1000, all 2016 pairs, mean stack (pws: phase-weighted stack, "
+                                            "power 2)"}
         del x2d
+        # configuration 1: 8 stations x 3600 samples, all 28 pairs, lags +-500, PCC2
+        x1c, _ = synth.make_stations(8, 3600, 200, seed=1)
+        x1d = torch.from_numpy(x1c).to(dev)
+        j1 = fastpcc_amd.make_job("pcc2", 8, 1, 3600, -500, 500, stack="plain")
+        r1 = fastpcc_amd.DeviceRunner(j1, dev)
+        r1.run(x1d, stream)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        for _ in range(50):
+            r1.run(x1d, stream)
+        torch.cuda.synchronize()
+        c1ms = (time.perf_counter() - ta) * 1e3 / 50
+        c1 = {"ms_per_step": round(c1ms, 4), "value": 28 / (c1ms / 1e3), "unit": "pair-correlations/s",
+              "steps": 50, "workload": "PCC2, 8 stations x 3600 samples, one window, all 28 pairs, lags +-500"}
+        if not args.no_cpu:
+            c1["cpu_baseline"] = cpu_rate("pcc2", x1c, 1, 3600, 500, 10.0)
+        variants["c1"] = c1
+        del r1, x1d
     if not args.no_paper:
         # the reference's published benchmark (SRL 2019, doc/Ventosa_S_2019_j-srl_tow_pldvPCC.pdf p.4): 649
         # day-long windows of one station pair, N = 21600 (dt = 4 s), lags +-12000 s = +-3000 samples, per-window
@@ -478,7 +655,10 @@ def main():
 
     serving = None
     if args.serve_requests > 0:
-        serving = serve_run(args.serve_requests, args.serve_rate)
+        rates = [float(r) for r in args.sweep_rates.split(",") if r]
+        y_ref = runner.y.view(P, L).cpu().numpy().ravel() if args.config == "c3" and args.stack == "gram16" else None
+        serving = serve_run(args.serve_requests, args.serve_rate, rates, args.sweep_requests,
+                            x3=x if y_ref is not None else None, y3_ref=y_ref, check=not args.no_check)
 
     out = {
         "metric": "PCC2 pair-correlations/s, 1 GPU: FP32 vs split-FP16 MFMA stack; p99 latency",
@@ -488,6 +668,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
+        "ms_per_step_profiled": ms_per_step_profiled,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

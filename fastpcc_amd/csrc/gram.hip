@@ -687,6 +687,133 @@ k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__res
     }
 }
 
+// The per pair-window PCC2 pair stage for Nz = 8192 on the same transform.
+// Persistent workgroups walk the pairs; for each window w of pair (i, j) thread t
+// forms, for its bins k = t + NT q (k < M/2) and their mirrors M - k, the folded
+// cross spectrum Gf(k) = Y(k) + conj Y(Nz-k), Y = conj(F_i,w) F_j,w, from the bins
+// k, Nz-k, M-k and M+k of both stations' spectra, folds it into the half-length
+// spectrum in registers as it writes LDS, and runs the radix-8^4 inverse; the
+// next window's bins load while the current one transforms.  MODE 0 writes every
+// window's lags, MODE 1 the mean of the windows' lags (accumulated in registers),
+// MODE 2 sums the folded spectra over the windows and transforms once a pair.
+// One complex bin through a buffer resource at a byte offset (0 beyond the row).
+// The 64-bit result is reinterpreted whole: taking its two 32-bit lanes apart
+// compiled to one dword load read twice.
+__device__ __forceinline__ cf32 ld_bin(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(cf32, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
+template <int NT, int LPT, int MODE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
+k_pair_plain_4096(const cf32 *__restrict__ F, int S, int W, int P, const cf32 *__restrict__ twM,
+                  const cf32 *__restrict__ twc, int lag0, int L, float scale, float *__restrict__ y)
+{
+    constexpr int M = 4096, Nz = 8192;
+    constexpr int H = M / 2 / NT;  // bins k < M/2 a thread
+    static_assert(NT == 512, "one radix-8 butterfly a thread per pass");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cf32 *lds = reinterpret_cast<cf32 *>(smem_raw), *alt = lds + M;
+    const int t = threadIdx.x;
+    const cf32 w0 = twc[t];  // twc[k] = exp(2 pi i k / Nz); twc[t + NT q] = w0 twc[NT q]
+    cf32 sa[MODE == 2 ? H : 1], sc[MODE == 2 ? H : 1], sph = cf32{0.f, 0.f};
+    float acc[LPT];
+    for (int p = blockIdx.x; p < P; p += gridDim.x) {
+        int i = 0, j = 0;
+        pair_index(p, S, i, j);
+        // uniform across the workgroup: scalar registers hold the row addresses
+        i = __builtin_amdgcn_readfirstlane(i);
+        j = __builtin_amdgcn_readfirstlane(j);
+#pragma unroll
+        for (int q = 0; q < LPT; ++q) acc[q] = 0.f;
+        for (int w = 0; w < W; ++w) {
+            // the two rows as buffer resources (scalar registers) read at 32-bit lane
+            // offsets: plain pointers with lane indices made the compiler keep one
+            // 64-bit address per load across the loop and spill
+            const __amdgpu_buffer_rsrc_t ri =
+                __builtin_amdgcn_make_buffer_rsrc((void *)(F + ((long long)i * W + w) * Nz), 0, Nz * 8, 0x00020000);
+            const __amdgpu_buffer_rsrc_t rj =
+                __builtin_amdgcn_make_buffer_rsrc((void *)(F + ((long long)j * W + w) * Nz), 0, Nz * 8, 0x00020000);
+            // Gf(k) = Y(k) + conj Y(Nz - k) at k (a) and M - k (c), Y = conj(F_i) F_j,
+            // folded into LDS as soon as each bin group is in (MODE 2 sums the windows)
+            auto fold_store = [&](int h, cf32 av, cf32 cv) {
+                if (h == 0 && t == 0) {  // DC and Nyquist of a real signal's spectrum are real
+                    av.y = 0.f;
+                    cv.y = 0.f;
+                }
+                const cf32 wk = h == 0 ? w0 : cmul(w0, twc[NT * h]);
+                lds[t + NT * h] = c2r_fold(av, cv, wk);
+                if (h > 0 || t > 0) lds[M - t - NT * h] = c2r_fold(cv, av, cf32{-wk.x, wk.y});
+            };
+#pragma unroll
+            for (int h = 0; h < H; ++h) {
+                const unsigned k = (unsigned)t + NT * h;
+                const unsigned kn = (Nz - k) & (Nz - 1), km = M - k, kp = M + k;
+                const cf32 y0 = cmulc(ld_bin(ri, 8 * k), ld_bin(rj, 8 * k));
+                const cf32 y1 = cmulc(ld_bin(ri, 8 * kn), ld_bin(rj, 8 * kn));
+                const cf32 y2 = cmulc(ld_bin(ri, 8 * km), ld_bin(rj, 8 * km));
+                const cf32 y3 = cmulc(ld_bin(ri, 8 * kp), ld_bin(rj, 8 * kp));
+                const cf32 av = cf32{y0.x + y1.x, y0.y - y1.y}, cv = cf32{y2.x + y3.x, y2.y - y3.y};
+                if (MODE == 2) {
+                    sa[h] = w == 0 ? av : cf32{sa[h].x + av.x, sa[h].y + av.y};
+                    sc[h] = w == 0 ? cv : cf32{sc[h].x + cv.x, sc[h].y + cv.y};
+                    if (w + 1 == W) fold_store(h, sa[h], sc[h]);
+                } else {
+                    fold_store(h, av, cv);
+                }
+                // two of the four bin groups' loads in flight at a time
+                if (h & 1) __builtin_amdgcn_sched_barrier(0);
+            }
+            cf32 ph = cf32{0.f, 0.f};
+            if (t == 0) {
+                const cf32 u0 = cmulc(ld_bin(ri, 8 * (M / 2)), ld_bin(rj, 8 * (M / 2)));
+                const cf32 u1 = cmulc(ld_bin(ri, 8 * (Nz - M / 2)), ld_bin(rj, 8 * (Nz - M / 2)));
+                ph = cf32{u0.x + u1.x, u0.y - u1.y};
+            }
+            if (MODE == 2) {
+                sph = w == 0 ? ph : cf32{sph.x + ph.x, sph.y + ph.y};
+                ph = sph;
+                if (w + 1 < W) continue;
+            }
+            if (t == 0) lds[M / 2] = cf32{2.f * ph.x, -2.f * ph.y};
+            // an opaque copy of the twiddle table's address each window keeps the
+            // compiler from holding every pass's twiddles in registers across the loop
+            const cf32 *tw = twM;
+            asm volatile("" : "+s"(tw));
+            __syncthreads();
+            ct_pass_oop<float, M, NT, +1, 8, 1, 0, 2>(lds, alt, tw);
+            ct_pass_oop<float, M, NT, +1, 8, 8, 2, 2>(alt, lds, tw + 7);
+            ct_pass_oop<float, M, NT, +1, 8, 64, 2, 0>(lds, alt, tw + 7 + 56);
+            ct_pass_oop<float, M, NT, +1, 8, 512, 0, 0>(alt, lds, tw + 7 + 56 + 448);
+            float *dst = y + (MODE == 0 ? ((long long)p * W + w) * L : (long long)p * L);
+#pragma unroll
+            for (int q = 0; q < LPT; ++q) {
+                const int l = t + NT * q;
+                if (l < L) {
+                    const int n = (lag0 + l) & (Nz - 1);
+                    const cf32 v = lds[n >> 1];
+                    const float val = scale * ((n & 1) ? v.y : v.x);
+                    if (MODE == 1) acc[q] += val;
+                    else dst[l] = val;
+                }
+            }
+            if (MODE == 1 && w + 1 == W) {
+#pragma unroll
+                for (int q = 0; q < LPT; ++q) {
+                    const int l = t + NT * q;
+                    if (l < L) dst[l] = acc[q] / (float)W;
+                }
+            }
+            // the next fold writes the buffer the last pass read (every lane is past
+            // that pass's barrier); these outputs are read before the next
+            // transform's first barrier, ahead of its first pass writing them over
+            cf32 *const tmp = lds;
+            lds = alt;
+            alt = tmp;
+        }
+    }
+}
+
 // ---- host side ----
 static void allow_lds(const void *fn, size_t bytes)
 {
@@ -849,6 +976,68 @@ static int launch_c2r_4096(const cf32 *d_G, const GLayout &gl, int P, const cf32
     hipLaunchKernelGGL(fn, dim3(nwg), dim3(kT), lds, st, d_G, gl, P, twr, twc, lag0, L, scale, N, d_y, ed);
     FPCC_CHECK(hipGetLastError());
     return 0;
+}
+
+template <int LPT, int MODE>
+static int launch_plain_4096_t(const cf32 *d_F, int S, int W, int P, const cf32 *twr, const cf32 *twc, int lag0,
+                               int L, float scale, float *d_y, hipStream_t st, const char *name)
+{
+    constexpr int kT = kC2rThreads;
+    const auto fn = k_pair_plain_4096<kT, LPT, MODE>;
+    const size_t lds = sizeof(cf32) * 2 * 4096;
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        return -1;
+    static std::atomic<int> per_cu_cached{0};  // the same on every call and device of this build
+    int per_cu = per_cu_cached.load(std::memory_order_relaxed);
+    if (!per_cu) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, kT, lds) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        per_cu_cached.store(per_cu, std::memory_order_relaxed);
+    }
+    const int nwg = P < per_cu * ncu ? P : per_cu * ncu;
+    ProfScope prof(name, st);
+    hipLaunchKernelGGL(fn, dim3(nwg), dim3(kT), lds, st, d_F, S, W, P, twr, twc, lag0, L, scale, d_y);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <int MODE>
+static int launch_plain_4096_m(int lpt, const cf32 *d_F, int S, int W, int P, const cf32 *twr, const cf32 *twc,
+                               int lag0, int L, float scale, float *d_y, hipStream_t st, const char *name)
+{
+    switch (lpt) {
+    case 1: return launch_plain_4096_t<1, MODE>(d_F, S, W, P, twr, twc, lag0, L, scale, d_y, st, name);
+    case 2: return launch_plain_4096_t<2, MODE>(d_F, S, W, P, twr, twc, lag0, L, scale, d_y, st, name);
+    case 4: return launch_plain_4096_t<4, MODE>(d_F, S, W, P, twr, twc, lag0, L, scale, d_y, st, name);
+    case 8: return launch_plain_4096_t<8, MODE>(d_F, S, W, P, twr, twc, lag0, L, scale, d_y, st, name);
+    case 16: return launch_plain_4096_t<16, MODE>(d_F, S, W, P, twr, twc, lag0, L, scale, d_y, st, name);
+    }
+    return -1;
+}
+
+int launch_pair_pcc2_4096(const cf32 *d_F, int S, int W, int N, int lag0, int L, int mode, float *d_y,
+                          hipStream_t st)
+{
+    constexpr int Nz = 8192;
+    const long long P = (long long)S * (S - 1) / 2;
+    if (P < 1 || P > 0x7fffffff || W < 1 || L < 1 || L > Nz || mode < 0 || mode > 2) return -1;
+    const void *twc = get_c2r_twiddles(Nz, false);
+    const cf32 *twr = (const cf32 *)get_order_twiddles(4096, {8, 8, 8, 8});
+    if (!twc || !twr) return -1;
+    int lpt = 1;
+    while (lpt * kC2rThreads < L) lpt *= 2;
+    // the folded spectra are twice the half spectrum of the real correlation: 1/2
+    // times the reference's 1/((float)Nz*(float)N) (mode 2 also takes the mean's 1/W)
+    float scale = 0.5f * (1 / ((float)Nz * (float)N));
+    if (mode == 2) scale /= (float)W;
+    const cf32 *tc = (const cf32 *)twc;
+    switch (mode) {
+    case 0: return launch_plain_4096_m<0>(lpt, d_F, S, W, (int)P, twr, tc, lag0, L, scale, d_y, st, "pair_pcc2_none");
+    case 1: return launch_plain_4096_m<1>(lpt, d_F, S, W, (int)P, twr, tc, lag0, L, scale, d_y, st, "pair_pcc2_plain");
+    default: return launch_plain_4096_m<2>(lpt, d_F, S, W, (int)P, twr, tc, lag0, L, scale, d_y, st, "pair_pcc2_zsum");
+    }
 }
 
 int launch_pair_c2r(const cf32 *d_G, GLayout gl, int P, int Nz, int N, int W, int lag0, int L, float scale,

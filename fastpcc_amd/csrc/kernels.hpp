@@ -33,6 +33,11 @@ bool pws_supported(int Nz);
 int launch_pair_pws(int src, const void *d_Xa, const void *d_Xb, long long ld, int S, int W, int Nz, int lag0, int L,
                     float cscale, float power, int multiply, float *d_y, hipStream_t st);
 
+// The same for Nz = 8192 and all pairs (gram.hip: persistent workgroups, the fold
+// in registers, the radix-8^4 inverse of the Gram path's pair stage).
+int launch_pair_pcc2_4096(const cf32 *d_F, int S, int W, int N, int lag0, int L, int mode, float *d_y,
+                          hipStream_t st);
+
 // Stack over windows as a per-frequency Gram (gram.hip).
 long long gram_ld(int nk);                       // row length of G for nk bins
 // Layout of G: bin k of pair p at (k >> sh) * gs + p * pl + (k & (2^sh - 1)).

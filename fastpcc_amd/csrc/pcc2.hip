@@ -407,6 +407,8 @@ int launch_station_phasor(const float *d_x, long long station_stride, int S, int
 int launch_pair_pcc2(const cf32 *d_F, int S, int W, int N, int Nz, int lag0, int L, int mode, int pair_base,
                      int npairs, float *d_y, hipStream_t st)
 {
+    if (Nz == 8192 && pair_base == 0 && npairs == S * (S - 1) / 2)
+        return launch_pair_pcc2_4096(d_F, S, W, N, lag0, L, mode, d_y, st);
     const int M = Nz / 2;
     const DevPlan &pM = get_plan(M, false);
     const void *twc = get_c2r_twiddles(Nz, false);

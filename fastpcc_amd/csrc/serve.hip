@@ -41,6 +41,8 @@ namespace {
 using clock_t_ = std::chrono::steady_clock;
 
 constexpr int kMaxRegions = 4;
+// p50/p99 are taken over the latencies of the last kLatencyKeep completed requests
+constexpr size_t kLatencyKeep = (size_t)1 << 20;
 constexpr int kCopyStations = 128;  // stations per cache copy launch (offsets go by value)
 
 // A HIP event shared by the request that fills cache entries and the requests
@@ -127,7 +129,11 @@ struct Server {
     // statistics
     unsigned long long n_submit = 0, n_done = 0, n_fail = 0, n_cancel = 0, n_reject = 0;
     unsigned long long n_hit = 0, n_miss = 0, n_batched = 0, n_batch_launch = 0;
+    // latencies of the last kLatencyKeep completed requests (a ring: memory stays
+    // bounded in a long-running loop); sum and max cover every completed request
     std::vector<double> lat_ms;
+    size_t lat_next = 0;
+    double lat_sum = 0, lat_max = 0;
     double first_start = -1, last_done = 0;
     unsigned long long pair_corr_done = 0;
 
@@ -687,7 +693,12 @@ void completer(Server *s)
         } else {
             r.state = FPCC_REQ_DONE;
             s->n_done++;
-            s->lat_ms.push_back((r.t_done - r.t_submit) * 1e3);
+            const double ms = (r.t_done - r.t_submit) * 1e3;
+            if (s->lat_ms.size() < kLatencyKeep) s->lat_ms.push_back(ms);
+            else s->lat_ms[s->lat_next] = ms;
+            s->lat_next = (s->lat_next + 1) % kLatencyKeep;
+            s->lat_sum += ms;
+            s->lat_max = std::max(s->lat_max, ms);
             s->pair_corr_done += r.pair_corr;
             s->last_done = r.t_done;
         }
@@ -708,10 +719,10 @@ void fill_status(const Request &r, fpcc_request_status *st)
 
 bool final_state(int s) { return s == FPCC_REQ_DONE || s == FPCC_REQ_FAILED || s == FPCC_REQ_CANCELLED; }
 
-double percentile(std::vector<double> v, double q)
+// v sorted
+double percentile(const std::vector<double> &v, double q)
 {
     if (v.empty()) return 0;
-    std::sort(v.begin(), v.end());
     const double pos = q * (double)(v.size() - 1);
     const size_t i = (size_t)pos;
     const double f = pos - (double)i;
@@ -966,7 +977,9 @@ extern "C" int fpcc_server_stats(fpcc_server *h, fpcc_serving_stats *out)
 {
     if (!h || !out) return -1;
     Server *s = &h->impl;
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::vector<double> lat;  // copied under the lock, sorted outside it
+    std::unique_lock<std::mutex> lk(s->mu);
+    lat = s->lat_ms;
     memset(out, 0, sizeof(*out));
     out->submitted = s->n_submit;
     out->completed = s->n_done;
@@ -981,17 +994,17 @@ extern "C" int fpcc_server_stats(fpcc_server *h, fpcc_serving_stats *out)
     out->queued = s->queue.size();
     out->inflight = s->inflight;
     out->inflight_peak = s->inflight_peak;
-    out->p50_ms = fpcc::percentile(s->lat_ms, 0.50);
-    out->p99_ms = fpcc::percentile(s->lat_ms, 0.99);
-    out->max_ms = s->lat_ms.empty() ? 0 : *std::max_element(s->lat_ms.begin(), s->lat_ms.end());
-    double sum = 0;
-    for (double v : s->lat_ms) sum += v;
-    out->mean_ms = s->lat_ms.empty() ? 0 : sum / s->lat_ms.size();
+    out->max_ms = s->lat_max;
+    out->mean_ms = s->n_done ? s->lat_sum / (double)s->n_done : 0;
     out->pair_correlations = (double)s->pair_corr_done;
     out->busy_s = s->first_start < 0 ? 0 : s->last_done - s->first_start;
     out->device_bytes = s->darena.size();
     out->device_used = s->darena.used();
     out->device_peak = s->dev_peak;
+    lk.unlock();
+    std::sort(lat.begin(), lat.end());
+    out->p50_ms = fpcc::percentile(lat, 0.50);
+    out->p99_ms = fpcc::percentile(lat, 0.99);
     return 0;
 }
 

@@ -125,12 +125,14 @@ def out_len(req):
     return P * (2 * req["lag"] + 1)
 
 
-def run(reqs, net, server=None, realtime=True, keys=False, pool=None, keep=True, **server_kw):
+def run(reqs, net, server=None, realtime=True, keys=False, pool=None, keep=True, keep_idx=(), **server_kw):
     """Submits every request at its arrival time (realtime) or all at once, waits
     for all, and returns (server stats, per-request records, outputs).  keys:
     name each request's stations for the resident-spectra cache; pool: an
     OutputPool for outputs (a pageable array when it is empty); keep: return the
-    outputs (pool buffers are recycled when False)."""
+    outputs (pool buffers are recycled when False; the requests listed in
+    keep_idx keep theirs either way)."""
+    keep_idx = set(keep_idx)
     own = server is None
     srv = server or Server(**server_kw)
     recs = [None] * len(reqs)
@@ -146,11 +148,13 @@ def run(reqs, net, server=None, realtime=True, keys=False, pool=None, keep=True,
             st = srv.wait(ids[k])
             recs[k] = st
             if pooled[k] is not None:
-                if keep:
+                if keep or k in keep_idx:
                     ys[k] = ys[k].copy()
                 else:
                     ys[k] = None
                 pool.put(pooled[k])
+            elif not keep and k not in keep_idx:
+                ys[k] = None
         done.set()
 
     th = threading.Thread(target=waiter, daemon=True)
@@ -184,12 +188,15 @@ def run(reqs, net, server=None, realtime=True, keys=False, pool=None, keep=True,
 def summary(stats, reqs, rate):
     """JSON-ready summary of a serving run."""
     lat = stats
+    busy = max(lat.get("busy_s", 0.0), 1e-9)
     kinds = {k: sum(1 for r in reqs if r["kind"] == k) for k in sorted({r["kind"] for r in reqs})}
     return {
         "requests": len(reqs), "arrival_rate_per_s": rate, "completed": int(lat["completed"]),
         "failed": int(lat["failed"]), "p50_ms": round(lat["p50_ms"], 3), "p99_ms": round(lat["p99_ms"], 3),
         "mean_ms": round(lat["mean_ms"], 3), "max_ms": round(lat["max_ms"], 3),
         "pair_correlations_per_s": round(lat["pair_correlations"] / max(lat["wall_s"], 1e-9), 1),
+        "pair_correlations_per_busy_s": round(lat["pair_correlations"] / busy, 1),
+        "completed_per_s": round(lat["completed"] / max(lat["wall_s"], 1e-9), 2),
         "pair_correlations": int(lat["pair_correlations"]), "wall_s": round(lat["wall_s"], 3),
         "inflight_peak": int(lat["inflight_peak"]), "device_peak_GB": round(lat["device_peak"] / 1e9, 3),
         "cache_hits": int(lat["cache_hits"]), "cache_misses": int(lat["cache_misses"]),

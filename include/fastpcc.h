@@ -332,7 +332,8 @@ typedef struct {
     size_t cache_used;                           /* bytes of resident station rows              */
     unsigned long long batched_requests, batch_launches; /* requests whose station stages shared a launch; groups */
     unsigned long long queued, inflight, inflight_peak;
-    double p50_ms, p99_ms, mean_ms, max_ms; /* latency of completed requests          */
+    double p50_ms, p99_ms, mean_ms, max_ms; /* latency of completed requests (p50, p99: of the
+                                               last 2^20 of them)                          */
     double pair_correlations;              /* sum of pairs x windows delivered        */
     double busy_s;                         /* first admission to last completion      */
     size_t device_bytes, device_used, device_peak;
