@@ -63,10 +63,13 @@ k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz,
 {
     __shared__ cf32 td[64][33];
     __shared__ cf32 tm[64][33];
+    // R rows a tile: whole stations when W <= 64, so every (bin, station) row of U
+    // is written by one tile (whole 128-byte lines)
+    const int R = W <= 64 ? 64 / W * W : 64;
     const int k0 = blockIdx.x * 32;
-    const int sw0 = blockIdx.y * 64;
+    const int sw0 = blockIdx.y * R;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-    for (int r = ty; r < 64; r += 8) {
+    for (int r = ty; r < R; r += 8) {
         const int sw = sw0 + r;
         const int k = k0 + tx;
         cf32 d = cf32{0.f, 0.f}, m = cf32{0.f, 0.f};
@@ -90,7 +93,7 @@ k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz,
         for (int idx = threadIdx.x; idx < 32 * 16; idx += 256) {
             const int kk = idx >> 4, g = idx & 15;
             const int k = k0 + kk, sw = sw0 + 4 * g;
-            if (k >= nk || sw >= SW) continue;
+            if (k >= nk || sw >= SW || 4 * g >= R) continue;
             const int s = sw / W, w = sw - s * W;
             uint4 *row = reinterpret_cast<uint4 *>(U + ((long long)k * S + s) * W2);
             uint2 e0 = u_element(td[4 * g][kk], sigma), e1 = u_element(td[4 * g + 1][kk], sigma);
@@ -112,7 +115,7 @@ k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz,
     for (int idx = threadIdx.x; idx < 32 * 64; idx += 256) {
         const int kk = idx >> 6, r = idx & 63;
         const int k = k0 + kk, sw = sw0 + r;
-        if (k >= nk || sw >= SW) continue;
+        if (k >= nk || sw >= SW || r >= R) continue;
         const int s = sw / W, w = sw - s * W;
         uint2 *row = U + ((long long)k * S + s) * W2;
         if (grouped) {
@@ -185,10 +188,13 @@ k_gram(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, int W2
     unsigned *lds = reinterpret_cast<unsigned *>(smem_raw);
     // XCD-aware order: blocks b, b+8, ... run on one XCD; each XCD takes pairs of
     // adjacent frequency chunks (one 128-byte line of every G row) and all
-    // station blocks of a chunk back to back, sharing the chunk's U rows in L2.
+    // station blocks of the pair back to back, sharing its U rows in L2; the two
+    // chunks of a line go to consecutive workgroups, so both halves of each G line
+    // are written while it is in L2 and leave as one line.
     const int b = blockIdx.x, xcd = b & 7, slot = b >> 3;
-    const int cp = slot / nblk, blk = slot - cp * nblk;
-    const int chunk = ((cp >> 1) * 8 + xcd) * 2 + (cp & 1);
+    const int half = slot & 1, rest = slot >> 1;
+    const int line = rest / nblk, blk = rest - line * nblk;
+    const int chunk = (line * 8 + xcd) * 2 + half;
     if (chunk >= nchunks) return;
     const int bi = blocks[blk].x, bj = blocks[blk].y;
     const bool same = bi == bj && Ua == Ub;
@@ -382,9 +388,12 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
              GLayout gl)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    // the XCD-aware order of k_gram: the two chunks of a G line on consecutive
+    // workgroups of one XCD
     const int b = blockIdx.x, xcd = b & 7, slot = b >> 3;
-    const int cp = slot / nblk, blk = slot - cp * nblk;
-    const int chunk = ((cp >> 1) * 8 + xcd) * 2 + (cp & 1);
+    const int half = slot & 1, rest = slot >> 1;
+    const int line = rest / nblk, blk = rest - line * nblk;
+    const int chunk = (line * 8 + xcd) * 2 + half;
     if (chunk >= nchunks) return;
     const int bi = blocks[blk].x, bj = blocks[blk].y;
     const bool same = bi == bj && Ua == Ub;
@@ -882,7 +891,8 @@ int launch_spectra_to_u(const cf32 *d_F, long long ldF, int S, int W, int Nz, in
                         hipStream_t st, float sigma)
 {
     const int SW = S * W;
-    dim3 grid((nk + 31) / 32, (SW + 63) / 64);
+    const int R = W <= 64 ? 64 / W * W : 64;  // rows a tile (whole stations), as the kernel
+    dim3 grid((nk + 31) / 32, (SW + R - 1) / R);
     ProfScope prof("spectra_to_u", st);
     const int W2 = mirror ? 2 * W : W;
     hipLaunchKernelGGL(k_spectra_to_u, grid, dim3(256), 0, st, d_F, ldF, SW, W, Nz, nk, mirror, sigma,
