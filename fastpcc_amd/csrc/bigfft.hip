@@ -115,9 +115,10 @@ __device__ __forceinline__ cf32 tw_cr(const cf32 *__restrict__ twN, int c, int r
     return cmul(twN[kTwLo + m / kTwLo], twN[m % kTwLo]);
 }
 
-// Job-table launch (BATCH): batch index b runs over the station-windows of several
-// jobs; job j (jb.b0[j] <= b < jb.b0[j+1]) supplies x, its stride, W, N and the
-// destination rows (COL_FWD_INPUT only).
+// Job-table launch (BATCH): batch index b runs over the rows of several jobs; job j
+// (jb.b0[j] <= b < jb.b0[j+1]) supplies x, its stride, W, N and the destination
+// rows (COL_FWD_INPUT), or its rows and maxima (COL_INV_MAX, COL_FWD_AMPNORM), or
+// its rows (row passes; ROW_INV_MUL also its input spectra).
 __device__ __forceinline__ int batch_job(const RealBatch &jb, int b)
 {
     int j = 0;
@@ -221,11 +222,16 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
     if (BATCH) {
         const int j = batch_job(jb, b);
         b -= jb.b0[j];
-        a.x = jb.x[j];
-        a.x_stride = jb.stride[j];
-        a.W = jb.W[j];
-        a.N = jb.N[j];
-        a.dst = (cf32 *)jb.X[j];
+        if (a.op == COL_FWD_INPUT) {
+            a.x = jb.x[j];
+            a.x_stride = jb.stride[j];
+            a.W = jb.W[j];
+            a.N = jb.N[j];
+            a.dst = (cf32 *)jb.X[j];
+        } else {  // COL_INV_MAX, COL_FWD_AMPNORM: job j's rows and maxima
+            a.src = a.dst = (cf32 *)jb.X[j];
+            a.maxbuf = jb.mx[j];
+        }
     }
     const int c0 = blockIdx.x * kBigCols;
     const long long base = (long long)b * a.bstride;
@@ -383,6 +389,7 @@ k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, 
         const int j = batch_job(jb, b);
         b -= jb.b0[j];
         a.src = a.dst = (cf32 *)jb.X[j];
+        if (a.op == ROW_INV_MUL) a.X = (const cf32 *)jb.Xin[j];
     }
     const int r0 = blockIdx.x * kBigRows;
     const long long base = (long long)b * a.bstride;
@@ -640,7 +647,8 @@ int run_cols_batch(const BigPlan &bp, const ColArgs &a, const RealBatch &jb, hip
 {
     const int nbatch = jb.b0[jb.n];
     if (nbatch <= 0) return 0;
-    if (!bp.ok || a.op != COL_FWD_INPUT || nbatch > 65535) return -1;
+    if (!bp.ok || nbatch > 65535) return -1;
+    if (a.op != COL_FWD_INPUT && a.op != COL_INV_MAX && a.op != COL_FWD_AMPNORM) return -1;
     return launch_cols<true>(bp, a, jb, nbatch, st, name);
 }
 
@@ -692,7 +700,8 @@ int run_rows_batch(const BigPlan &bp, const RowArgs &a, const RealBatch &jb, hip
 {
     const int nbatch = jb.b0[jb.n];
     if (nbatch <= 0) return 0;
-    if (!bp.ok || a.op != ROW_FWD_PLAIN || nbatch > 65535) return -1;
+    if (!bp.ok || nbatch > 65535) return -1;
+    if (a.op != ROW_FWD_PLAIN && a.op != ROW_INV_MUL && a.op != ROW_FWD_SCALE) return -1;
     return launch_rows<true>(bp, a, jb, nbatch, st, name);
 }
 

@@ -901,6 +901,23 @@ namespace {
 // stride), grouped by kind, path and Nz.
 bool batch_key(const fpcc_job *job, const Dims &d, long long &key)
 {
+    if (job->kind == FPCC_WPCC) {
+        // jobs whose families share the filter bank (type, Nz, scales, voices, s0, w0):
+        // a hash of those; groups are checked with wpcc_same_bank as well
+        if (!d.fam || d.fam->type == 0) return false;
+        const WFamily &f = *d.fam;
+        unsigned long long h = 1469598103934665603ull;
+        auto mix = [&](unsigned long long v) { h = (h ^ v) * 1099511628211ull; };
+        mix((unsigned long long)(long long)f.type);
+        mix(f.Nz);
+        mix(f.S);
+        mix(f.V);
+        mix(f.Ls0);
+        mix(__builtin_bit_cast(unsigned long long, f.s0));
+        mix(__builtin_bit_cast(unsigned long long, f.op1));
+        key = (long long)((h << 8) | FPCC_WPCC);
+        return true;
+    }
     if (job->kind == FPCC_PCC2) {
         // the in-LDS station stage of any window length, except the specialised one
         if (d.large || station_pcc_specialised(d.N, (int)d.Nz)) return false;
@@ -930,6 +947,20 @@ int station_stage_group(const std::vector<const fpcc_job *> &jobs, const std::ve
         jb.b0[g + 1] = jb.b0[g] + D[g].S * D[g].W;
     }
     int r = 0;
+    if (kind == FPCC_WPCC) {
+        std::vector<long long> stride(n);
+        std::vector<int> S(n), W(n), N(n);
+        std::vector<char *> wsp(n);
+        for (int g = 0; g < n; ++g) {
+            stride[g] = (long long)jobs[g]->station_stride;
+            S[g] = D[g].S;
+            W[g] = D[g].W;
+            N[g] = D[g].N;
+            wsp[g] = ws[g] + layout(jobs[g], D[g]).a;
+        }
+        return wpcc_station_stage_batch(n, x.data(), stride.data(), S.data(), W.data(), N.data(), *D[0].fam,
+                                        wsp.data(), st);
+    }
     if (kind == FPCC_PCC2) {
         PccBatch pb{};
         pb.n = n;
@@ -1020,12 +1051,18 @@ extern "C" int fpcc_station_stage_batch_device(const fpcc_job *jobs, int njobs, 
         long long ki;
         std::vector<int> grp{i};
         if (batch_key(&jobs[i], D[i], ki)) {
-            long long sw = (long long)D[i].S * D[i].W;
+            // rows of one launch: station-windows (two-pass transforms) or, for WPCC,
+            // station-window scales; a launch takes at most 65535
+            const bool wpcc = jobs[i].kind == FPCC_WPCC;
+            const long long per = wpcc ? (long long)D[i].fam->S : 1;
+            long long sw = (long long)D[i].S * D[i].W * per;
             for (int j = i + 1; j < njobs && (int)grp.size() < kMaxBatchJobs; ++j) {
                 long long kj;
                 if (done[j] || !batch_key(&jobs[j], D[j], kj) || kj != ki) continue;
-                if (D[i].large && sw + (long long)D[j].S * D[j].W > 65535) continue;
-                sw += (long long)D[j].S * D[j].W;
+                if (wpcc && !wpcc_same_bank(*D[i].fam, *D[j].fam)) continue;
+                const long long add = (long long)D[j].S * D[j].W * per;
+                if ((D[i].large || wpcc) && sw + add > 65535) continue;
+                sw += add;
                 grp.push_back(j);
             }
         }

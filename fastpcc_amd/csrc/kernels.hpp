@@ -95,6 +95,11 @@ struct RealBatch {
     void *X[kMaxBatchJobs];
     double *E[kMaxBatchJobs];
     long long lde[kMaxBatchJobs];
+    // per-(station-window, scale) passes of WPCC: X holds job j's per-scale rows,
+    // Xin its input spectra (ROW_INV_MUL), mx its per-row maxima (COL_INV_MAX,
+    // COL_FWD_AMPNORM)
+    const void *Xin[kMaxBatchJobs];
+    float *mx[kMaxBatchJobs];
 };
 
 // Job table of a PCC2 station-stage launch over several jobs of one Nz <= 8192
@@ -152,6 +157,14 @@ size_t wpcc_workspace(int S, int W, int N, int Nz, int nscales);
 int wpcc_station_stage(const float *d_x, long long station_stride, int S, int W, int N, const WFamily &f,
                        char *ws, hipStream_t st);
 int wpcc_station_regions(int S, int W, int Nz, int nscales, size_t *off, size_t *per);
+// Station stages of n WPCC jobs with one filter bank (wpcc_batchable) in shared
+// launches: job j has S[j] stations of W[j] windows of N[j] samples at x[j]
+// (rows stride[j] apart) and its own workspace ws[j]; each job's rows equal
+// wpcc_station_stage's.
+int wpcc_station_stage_batch(int n, const float *const *x, const long long *stride, const int *S, const int *W,
+                             const int *N, const WFamily &f, char *const *ws, hipStream_t st);
+// Whether two families share the filter bank (type, Nz, scales, voices, s0, w0).
+bool wpcc_same_bank(const WFamily &a, const WFamily &b);
 // per_window: y[p][w][L] (one Gram per window), else mean over windows y[p][L].
 int wpcc_pair_stage(int S, int W, int N, const WFamily &f, int lag0, int L, bool per_window, char *ws, float *d_y,
                     hipStream_t st);

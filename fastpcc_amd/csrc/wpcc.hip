@@ -213,6 +213,75 @@ int wpcc_station_stage(const float *d_x, long long station_stride, int S, int W,
     return run_rows(bp, ra, (int)nb, st, "wpcc_fwd_rows");
 }
 
+bool wpcc_same_bank(const WFamily &a, const WFamily &b)
+{
+    // library families: the filter bank is fixed by these (as filter_bank's cache key)
+    return a.type != 0 && a.type == b.type && a.Nz == b.Nz && a.S == b.S && a.V == b.V && a.s0 == b.s0 &&
+           a.op1 == b.op1 && a.Ls0 == b.Ls0;
+}
+
+int wpcc_station_stage_batch(int n, const float *const *x, const long long *stride, const int *S, const int *W,
+                             const int *N, const WFamily &f, char *const *ws, hipStream_t st)
+{
+    if (n < 1 || n > kMaxBatchJobs) return -1;
+    const BigPlan &bp = big_plan((int)f.Nz);
+    if (!bp.ok) return -1;
+    FilterBank fb;
+    int r = filter_bank(f, st, fb);
+    if (r) return r;
+    const int nsc = (int)f.S;
+    const long long Nz = f.Nz;
+    // job tables: station-windows (input passes), station-window scales (the rest)
+    RealBatch jw{}, js{};
+    jw.n = js.n = n;
+    jw.b0[0] = js.b0[0] = 0;
+    for (int g = 0; g < n; ++g) {
+        const WpccLayout l = wpcc_layout(S[g], W[g], (int)Nz, nsc);
+        const int SW = S[g] * W[g];
+        jw.x[g] = x[g];
+        jw.stride[g] = stride[g];
+        jw.W[g] = W[g];
+        jw.N[g] = N[g];
+        jw.X[g] = ws[g] + l.X;
+        jw.b0[g + 1] = jw.b0[g] + SW;
+        js.X[g] = ws[g] + l.B;
+        js.Xin[g] = ws[g] + l.X;
+        js.mx[g] = (float *)(ws[g] + l.mx);
+        js.b0[g + 1] = js.b0[g] + SW * nsc;
+        if ((r = fill_zero(js.mx[g], (long long)SW * nsc, st))) return r;
+    }
+    if (js.b0[n] > 65535) return -1;
+    ColArgs ca{};
+    ca.op = COL_FWD_INPUT;  // X = FFT(in), in[Ls0 + n] = x[n] / Nz
+    ca.bstride = Nz;
+    ca.off = (int)f.Ls0;
+    ca.in_scale = (float)(1.0 / (double)Nz);
+    if ((r = run_cols_batch(bp, ca, jw, st, "wpcc_input_cols_batch"))) return r;
+    RowArgs ra{};
+    ra.op = ROW_FWD_PLAIN;
+    ra.bstride = Nz;
+    ra.S = 1;
+    if ((r = run_rows_batch(bp, ra, jw, st, "wpcc_input_rows_batch"))) return r;
+    ra = RowArgs{};
+    ra.op = ROW_INV_MUL;  // IFFT(X W_s), then max|z|^2, AmpNorm, FFT, * sqrt(1/(scale Nz))
+    ra.bstride = Nz;
+    ra.fw = fb.fw;
+    ra.S = nsc;
+    if ((r = run_rows_batch(bp, ra, js, st, "wpcc_scale_inv_rows_batch"))) return r;
+    ca = ColArgs{};
+    ca.op = COL_INV_MAX;
+    ca.bstride = Nz;
+    if ((r = run_cols_batch(bp, ca, js, st, "wpcc_scale_inv_cols_batch"))) return r;
+    ca.op = COL_FWD_AMPNORM;
+    if ((r = run_cols_batch(bp, ca, js, st, "wpcc_ampnorm_cols_batch"))) return r;
+    ra = RowArgs{};
+    ra.op = ROW_FWD_SCALE;
+    ra.bstride = Nz;
+    ra.S = nsc;
+    ra.wscale = fb.wscale;
+    return run_rows_batch(bp, ra, js, st, "wpcc_fwd_rows_batch");
+}
+
 int wpcc_pair_stage(int S, int W, int N, const WFamily &f, int lag0, int L, bool per_window, char *ws, float *d_y,
                     hipStream_t st)
 {

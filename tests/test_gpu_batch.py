@@ -98,3 +98,60 @@ def test_server_groups_uneven_pcc2(fp):
         assert st["batched_requests"] >= 2 and st["batch_launches"] >= 1
     for (job, x, y), ref in zip(items, refs):
         assert np.array_equal(y, ref)
+
+
+# WPCC jobs: uneven station counts, window counts and lengths; the first three share
+# a filter bank (Morlet, periods 4-40 samples, 3 voices, Nz 4096), the last one's
+# window needs Nz 8192 (its own bank: it runs alone)
+WPCC_KW = dict(pmin=4.0, pmax=40.0, V=3, wtype=-1)
+WPCC_GROUP = [(5, 1, 3000, 400, "none"), (7, 2, 2500, 300, "plain"), (3, 1, 3200, 150, "none"),
+              (4, 1, 3500, 200, "none")]
+
+
+def test_station_stage_batch_wpcc(fp, hip):
+    L = fp.api._lib.lib()
+    n = len(WPCC_GROUP)
+    jobs = (fp.api._lib.FpccJob * n)()
+    dx, dws, wsb, refs = [], [], [], []
+    for k, (S, W, N, lag, stack) in enumerate(WPCC_GROUP):
+        x, _ = synth.make_stations(S, W * N + 7 * k, 100, seed=90 + k)
+        job = fp.make_job("wpcc", S, W, N, -lag, lag, stack=stack, station_stride=x.shape[1], **WPCC_KW)
+        jobs[k] = job
+        d = hip.alloc(x.nbytes)
+        hip.h2d(d, x)
+        dx.append(d)
+        wsb.append(L.fpcc_workspace_size(ctypes.byref(job)))
+        dws.append(hip.alloc(wsb[-1]))
+        refs.append(fp.allpairs(x, W, N, -lag, lag, kind="wpcc", stack=stack, **WPCC_KW))
+    print("wpcc group: station stages of", [s[:3] for s in WPCC_GROUP], flush=True)
+    r = L.fpcc_station_stage_batch_device(jobs, n, (ctypes.c_void_p * n)(*dx), (ctypes.c_void_p * n)(*dws),
+                                          (ctypes.c_size_t * n)(*wsb), None)
+    assert r == 0
+    for k in range(n):
+        y = np.empty(refs[k].shape, np.float32)
+        dy = hip.alloc(y.nbytes)
+        assert L.fpcc_pair_stage_device(ctypes.byref(jobs[k]), dws[k], wsb[k], dy, None) == 0
+        hip.d2h(y, dy)
+        print(f"  {WPCC_GROUP[k]}: max|batch - alone| = {np.abs(y - refs[k]).max():.3e}", flush=True)
+        assert np.array_equal(y, refs[k])
+
+
+def test_server_groups_wpcc(fp):
+    """A burst of WPCC requests with one filter bank runs as a group; every result
+    equals the standalone run."""
+    items, refs = [], []
+    for k, (S, W, N, lag, stack) in enumerate(WPCC_GROUP[:3]):
+        x, _ = synth.make_stations(S, W * N, 100, seed=110 + k)
+        job = fp.make_job("wpcc", S, W, N, -lag, lag, stack=stack, station_stride=W * N, **WPCC_KW)
+        y = np.empty(int(np.prod(fp.output_shape(job))), np.float32)
+        items.append((job, x, y))
+        refs.append(fp.allpairs(x, W, N, -lag, lag, kind="wpcc", stack=stack, **WPCC_KW).ravel())
+    with fp.Server(device_bytes=2 << 30, pinned_bytes=128 << 20, max_inflight=16) as srv:
+        ids = srv.submit_many(items)
+        for rid in ids:
+            assert srv.wait(rid)["state"] == 2
+        st = srv.stats()
+        print(f"wpcc: batched {st['batched_requests']} in {st['batch_launches']} groups", flush=True)
+        assert st["batched_requests"] >= 2 and st["batch_launches"] >= 1 and st["device_used"] == 0
+    for (job, x, y), ref in zip(items, refs):
+        assert np.array_equal(y, ref)
