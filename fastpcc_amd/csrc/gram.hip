@@ -370,11 +370,15 @@ k_gram(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, int W2
 // and epilogue as k_gram<1>, but each bin's rows are copied by the memory unit
 // straight into LDS (buffer loads with the LDS flag, 16 bytes a lane, bounds-
 // checked by the buffer resource: out-of-range units read as zero) while the
-// MFMAs of the previous bin run; no registers are spent on staging.  An LDS row
-// holds the K/2 units of a station plus one or two pad units (an odd number of
-// 16-byte units), so the 16 rows of an operand read fall on distinct banks.  The
-// operand reads are inline ds_read_b128 that wait for themselves, which keeps the
-// compiler from waiting for the next bin's copies before every LDS read.
+// MFMAs of an earlier bin run; no registers are spent on staging.  With NB
+// staging buffers, bins f+1 .. f+NB-1 are in flight while bin f is multiplied:
+// each wave waits (vmcnt) only for its own copies of bin f+1, then a bare
+// s_barrier publishes them (__syncthreads would add a wait for every copy in
+// flight).  An LDS row holds the K/2 units of a station plus one or two pad
+// units (an odd number of 16-byte units), so the 16 rows of an operand read fall
+// on distinct banks.  The operand reads are inline ds_read_b128 that wait for
+// themselves, which keeps the compiler from waiting for the later bins' copies
+// before every LDS read.
 __host__ __device__ constexpr int dma_row_units(int K) { return K / 2 + ((K / 2) % 2 ? 2 : 1); }
 
 __device__ __forceinline__ unsigned lds_offset(const void *p)
@@ -382,6 +386,24 @@ __device__ __forceinline__ unsigned lds_offset(const void *p)
     return (unsigned)(size_t)((__attribute__((address_space(3))) const void *)p);
 }
 
+// wait until at most n of this wave's vector-memory operations are in flight (n uniform)
+__device__ __forceinline__ void wait_vm(int n)
+{
+    switch (n) {
+    case 0: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+    case 1: asm volatile("s_waitcnt vmcnt(1)" ::: "memory"); break;
+    case 2: asm volatile("s_waitcnt vmcnt(2)" ::: "memory"); break;
+    case 3: asm volatile("s_waitcnt vmcnt(3)" ::: "memory"); break;
+    case 4: asm volatile("s_waitcnt vmcnt(4)" ::: "memory"); break;
+    case 5: asm volatile("s_waitcnt vmcnt(5)" ::: "memory"); break;
+    case 6: asm volatile("s_waitcnt vmcnt(6)" ::: "memory"); break;
+    case 7: asm volatile("s_waitcnt vmcnt(7)" ::: "memory"); break;
+    case 8: asm volatile("s_waitcnt vmcnt(8)" ::: "memory"); break;
+    default: asm volatile("s_waitcnt vmcnt(9)" ::: "memory"); break;
+    }
+}
+
+template <int NB>
 __global__ void __launch_bounds__(kGThreads)
 k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, int K, int nk,
              const int2 *__restrict__ blocks, int nblk, int nchunks, float oscale, cf32 *__restrict__ G,
@@ -449,16 +471,24 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
 #pragma unroll
         for (int a = 0; a < 4; ++a) acc[f][a] = f4v{0.f, 0.f, 0.f, 0.f};
 
+    // copies this wave issues per bin
+    int nw = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) nw += j * 8 + wave < nchunk64;
+    // NB buffers: bins f + 1 .. f + NB - 1 in flight while bin f is multiplied
     dma(0, 0);
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
+#pragma unroll
+    for (int f = 1; f < NB - 1; ++f)
+        if (f < kGF) dma(f, f);
+    wait_vm(NB > 2 && kGF > 1 ? nw * (NB - 2) : 0);
+    __builtin_amdgcn_s_barrier();  // this wave's copies waited for above
     const int arow = 16 * ti + li;
     const int brow = (same ? 0 : kGB) + 32 * cj + li;
     const unsigned base = lds_offset(smem_raw);
 #pragma unroll
     for (int f = 0; f < kGF; ++f) {
-        const int buf = f & 1;
-        if (f + 1 < kGF) dma(f + 1, buf ^ 1);
+        const int buf = f % NB;
+        if (f + NB - 1 < kGF) dma(f + NB - 1, (f + NB - 1) % NB);
         if (active) {
             f4v r0 = acc[f][0], i0 = acc[f][1], r1 = acc[f][2], i1 = acc[f][3];
             unsigned pa = base + buf * bufb + arow * su * 16 + 32 * q;
@@ -503,8 +533,9 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
             acc[f][2] = r1;
             acc[f][3] = i1;
         }
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // bin f + 1 has landed
-        __syncthreads();
+        // bin f + 1 has landed (the later bins' copies may still be in flight)
+        wait_vm(nw * max(0, min(NB - 2, kGF - 2 - f)));
+        __builtin_amdgcn_s_barrier();  // this wave's copies waited for above
     }
 
     // epilogue in quarters (columns 16h..16h+15, held by the waves with cj = h/2,
@@ -948,10 +979,21 @@ int launch_gram_split16(const void *d_Ua, const void *d_Ub, int S, int W2, int m
     if (slots * 8 > 0x7fffffffLL) return -1;
     static_assert(2 * 2 * kGB * dma_row_units(kGK) * 16 <= kGramLds, "DMA staging fits the Gram's LDS");
     ProfScope prof("gram_split16", st);
-    allow_lds((const void *)k_gram16_dma, kGramLds);
-    hipLaunchKernelGGL(k_gram16_dma, dim3((unsigned)(slots * 8)), dim3(kGThreads), kGramLds, st,
-                       (const uint2 *)d_Ua, (const uint2 *)d_Ub, S, K, nk, d_tiles, ntiles, nchunks,
-                       1.0f / (sigma * sigma), d_G, gl);
+    // three staging buffers when they fit the 160 KB of LDS, else two
+    const size_t bufb = (size_t)2 * kGB * dma_row_units(K) * 16;
+    const size_t epi = (size_t)4 * kGB * (kGB / 4) * kGOut4;
+    const bool three = 3 * bufb <= 160 * 1024;
+    const size_t lds = std::max(epi, (three ? 3 : 2) * bufb);
+    const void *fn = three ? (const void *)k_gram16_dma<3> : (const void *)k_gram16_dma<2>;
+    allow_lds(fn, lds);
+    if (three)
+        hipLaunchKernelGGL(k_gram16_dma<3>, dim3((unsigned)(slots * 8)), dim3(kGThreads), lds, st,
+                           (const uint2 *)d_Ua, (const uint2 *)d_Ub, S, K, nk, d_tiles, ntiles, nchunks,
+                           1.0f / (sigma * sigma), d_G, gl);
+    else
+        hipLaunchKernelGGL(k_gram16_dma<2>, dim3((unsigned)(slots * 8)), dim3(kGThreads), lds, st,
+                           (const uint2 *)d_Ua, (const uint2 *)d_Ub, S, K, nk, d_tiles, ntiles, nchunks,
+                           1.0f / (sigma * sigma), d_G, gl);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }

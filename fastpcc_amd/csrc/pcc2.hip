@@ -113,7 +113,7 @@ k_station_pcc_batch(PccBatch jb, int Nz, FftPlan planZ, const cf32 *__restrict__
                     nullptr, smem_raw);
 }
 
-constexpr int kSpN = 3600, kSpNz = 8192, kSpM = 4096, kSpT = 256;
+constexpr int kSpN = 3600, kSpNz = 8192, kSpM = 4096, kSpT = 512;
 
 // The station kernel's 3600-point transforms: plain layout in and out, the stage
 // between the radix-16 and radix-9 passes XOR-swizzled (3% faster than one pad
@@ -128,7 +128,9 @@ __device__ __forceinline__ void fft_3600(cf32 *lds, const cf32 *__restrict__ tw)
 
 // Station stage specialised for N = 3600, Nz = 8192 (one day at 1 Hz cut in
 // hourly windows): compile-time FFTs in one padded LDS buffer of 4096 values
-// (35 KB; registers then allow three workgroups a CU).  The window occupies
+// (35 KB), 512 threads (the 4096-point transforms in radix-8 passes, a butterfly
+// a thread; 116 registers allow two workgroups, 16 waves, a CU: 6% faster than
+// 256 threads with radix-16 passes at 147 registers and 12 waves).  The window occupies
 // samples < 3600 < 4096 of the zero-padded 8192, so the 8192-point spectrum
 // splits into two 4096-point transforms:
 //   F[2m] = FFT_4096(z)[m],  F[2m+1] = FFT_4096(z[n] e^{-2 pi i n/8192})[m];
@@ -163,7 +165,7 @@ k_station_pcc_3600(const float *__restrict__ x, long long station_stride, int W,
     }
     m = block_max(m, red);
     const float eps = (float)(1e-6 * sqrt((double)m));
-    constexpr int kZ = (kSpN + kSpT - 1) / kSpT;  // 15 samples a thread (n = t + 256 i)
+    constexpr int kZ = (kSpN + kSpT - 1) / kSpT;  // samples a thread (n = t + kSpT i)
     cf32 *row = F + (long long)sw * kSpNz;
     cf32 z[kZ];
 #pragma unroll
@@ -181,7 +183,7 @@ k_station_pcc_3600(const float *__restrict__ x, long long station_stride, int W,
 #pragma unroll
     for (int i = 0; i < kSpM / kSpT; ++i) lds[ct_pad(t + kSpT * i)] = i < kZ ? z[i] : cf32{0.0f, 0.0f};
     __syncthreads();
-    ct_fft<float, kSpM, kSpT, -1, 16, 16, 16>(lds, twM);
+    ct_fft<float, kSpM, kSpT, -1, 8, 8, 8, 8>(lds, twM);
     cf32 ev[kSpM / kSpT];
 #pragma unroll
     for (int i = 0; i < kSpM / kSpT; ++i) ev[i] = lds[ct_pad(t + kSpT * i)];
@@ -193,7 +195,7 @@ k_station_pcc_3600(const float *__restrict__ x, long long station_stride, int W,
         lds[ct_pad(n)] = i < kZ && n < kSpN ? cmul(z[i], twZ[n]) : cf32{0.0f, 0.0f};
     }
     __syncthreads();
-    ct_fft<float, kSpM, kSpT, -1, 16, 16, 16>(lds, twM);
+    ct_fft<float, kSpM, kSpT, -1, 8, 8, 8, 8>(lds, twM);
     float4 *out = reinterpret_cast<float4 *>(row);
 #pragma unroll
     for (int i = 0; i < kSpM / kSpT; ++i) {
@@ -334,7 +336,7 @@ int launch_station_pcc(const float *d_x, long long station_stride, int S, int W,
     }
     const float da1 = (float)(1 / (double)N);
     if (station_pcc_specialised(N, Nz)) {
-        const void *twN = get_order_twiddles(kSpN, {16, 9, 25}), *twM = get_order_twiddles(kSpM, {16, 16, 16});
+        const void *twN = get_order_twiddles(kSpN, {16, 9, 25}), *twM = get_order_twiddles(kSpM, {8, 8, 8, 8});
         const void *twZ = get_full_twiddles(kSpNz, false);
         if (!twN || !twM || !twZ) return -1;
         size_t lds = sizeof(cf32) * ct_lds_elems<kSpM>() + sizeof(float) * 8;
