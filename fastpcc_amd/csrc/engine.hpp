@@ -43,7 +43,7 @@ constexpr int kTwLo = 512;
 const void *get_split_twiddles(int n);
 // Per-pass twiddles of an FP32 plan with the given radix order (the compile-time
 // FFT kernels fix their order; cached per device), nullptr if the order is invalid.
-const void *get_order_twiddles(int n, const std::vector<int> &radices);
+const void *get_order_twiddles(int n, const std::vector<int> &radices, bool f64 = false);
 // Radix list for n (host); returns false if unsupported.
 bool factor_plan(int n, FftPlan &p);
 // Plan with the given pass radices (false unless valid for n).

@@ -239,7 +239,7 @@ const void *get_c2r_twiddles(int nz, bool f64)
     return d;
 }
 
-const void *get_order_twiddles(int n, const std::vector<int> &radices)
+const void *get_order_twiddles(int n, const std::vector<int> &radices, bool f64)
 {
     static std::map<std::pair<int, std::vector<int>>, const void *> cache;
     FftPlan p{};
@@ -249,10 +249,11 @@ const void *get_order_twiddles(int n, const std::vector<int> &radices)
     std::lock_guard<std::mutex> lk(g_mu);
     std::vector<int> key = radices;
     key.push_back(dev);
+    key.push_back(f64 ? 1 : 0);
     auto it = cache.find({n, key});
     if (it != cache.end()) return it->second;
     if (!table_builds_allowed()) return nullptr;
-    const void *d = upload_table<float>(plan_twiddles(p));
+    const void *d = f64 ? upload_table<double>(plan_twiddles(p)) : upload_table<float>(plan_twiddles(p));
     cache.emplace(std::make_pair(n, key), d);
     return d;
 }

@@ -49,44 +49,34 @@ constexpr int kW64Rows = 1024;       // row length of the four-step split
 constexpr int kW64ColsPerWg = 16;
 constexpr int kW64RowsPerWg = 4;
 
-__device__ __forceinline__ double block_sum_d(double v, double *red)
+// Wave64 reductions (cross-lane shuffles), then one exchange through LDS between
+// the waves: two barriers a reduction instead of one per tree level.  red holds
+// one slot a wave; the leading barrier keeps the previous reduction's readers
+// ahead of this one's writers.
+template <typename T, typename Op>
+__device__ __forceinline__ T block_reduce(T v, T *red, Op op)
 {
-    red[threadIdx.x] = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    const double r = red[0];
+    if (lane == 0) red[wid] = v;
     __syncthreads();
+    T r = red[0];
+    for (int i = 1; i < nw; ++i) r = op(r, red[i]);
     return r;
 }
-
-__device__ __forceinline__ int block_sum_i(int v, int *red)
-{
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    const int r = red[0];
-    __syncthreads();
-    return r;
-}
-
-__device__ __forceinline__ unsigned block_max_u(unsigned v, unsigned *red)
-{
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
-    }
-    const unsigned r = red[0];
-    __syncthreads();
-    return r;
-}
+struct OpAdd {
+    template <typename T>
+    __device__ T operator()(T a, T b) const { return a + b; }
+};
+struct OpMax {
+    __device__ unsigned operator()(unsigned a, unsigned b) const { return a > b ? a : b; }
+    __device__ double operator()(double a, double b) const { return fmax(a, b); }
+};
+struct OpMin {
+    __device__ double operator()(double a, double b) const { return fmin(a, b); }
+};
 
 // Bit pattern of |v|: for non-negative floats the unsigned order is the value order.
 __device__ __forceinline__ unsigned abs_bits(float v) { return __float_as_uint(v) & 0x7fffffffu; }
@@ -96,11 +86,11 @@ k_pre_window(float *__restrict__ x, long long stride, int W, int N, const float 
              unsigned char *__restrict__ nzflag, float *__restrict__ rms, float *__restrict__ thr_out)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ double redd[kPreThreads];
+    __shared__ double redd[kPreThreads / 64];
+    __shared__ int redi[kPreThreads / 64];
+    __shared__ unsigned redu[kPreThreads / 64];
     __shared__ unsigned hist[256];
     __shared__ unsigned sel[2];
-    int *redi = reinterpret_cast<int *>(redd);
-    unsigned *redu = reinterpret_cast<unsigned *>(redd);
     float *cache = reinterpret_cast<float *>(smem_raw);
     const bool cached = N <= kPreCacheN;
     const int sw = blockIdx.x;
@@ -116,8 +106,8 @@ k_pre_window(float *__restrict__ x, long long stride, int W, int N, const float 
         nz |= (v != 0.0f);
         part += (double)(v * v);
     }
-    const double sumsq = block_sum_d(part, redd);
-    nz = block_sum_i(nz, redi) > 0;
+    const double sumsq = block_reduce(part, redd, OpAdd());
+    nz = block_reduce(nz, redi, OpAdd()) > 0;
     if (threadIdx.x == 0) {
         nzflag[sw] = (unsigned char)nz;
         rms[sw] = (float)sqrt(sumsq / (double)N);
@@ -136,11 +126,37 @@ k_pre_window(float *__restrict__ x, long long stride, int W, int N, const float 
                 if ((u & mask) == prefix) atomicAdd(&hist[(u >> shift) & 255u], 1u);
             }
             __syncthreads();
-            if (threadIdx.x == 0) {
-                int below = 0, d = 0;
-                while (d < 255 && below + (int)hist[d] <= k) below += (int)hist[d++];
-                sel[0] = (unsigned)d;
-                sel[1] = (unsigned)(k - below);
+            if (threadIdx.x < 64) {
+                // the digit: the first bin whose cumulative count passes k, found by a
+                // wave-wide scan of four bins a lane
+                const int l = threadIdx.x;
+                const unsigned h0 = hist[4 * l], h1 = hist[4 * l + 1], h2 = hist[4 * l + 2], h3 = hist[4 * l + 3];
+                const unsigned tot = h0 + h1 + h2 + h3;
+                unsigned inc = tot;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    const unsigned o = __shfl_up(inc, off, 64);
+                    if (l >= off) inc += o;
+                }
+                const unsigned exc = inc - tot, ku = (unsigned)k;
+                if (exc <= ku && ku < inc) {
+                    unsigned below = exc;
+                    int d = 4 * l;
+                    if (below + h0 <= ku) {
+                        below += h0;
+                        ++d;
+                        if (below + h1 <= ku) {
+                            below += h1;
+                            ++d;
+                            if (below + h2 <= ku) {
+                                below += h2;
+                                ++d;
+                            }
+                        }
+                    }
+                    sel[0] = (unsigned)d;
+                    sel[1] = ku - below;
+                }
             }
             __syncthreads();
             prefix |= sel[0] << shift;
@@ -162,8 +178,8 @@ k_pre_window(float *__restrict__ x, long long stride, int W, int N, const float 
                     mless = max(mless, u);
                 }
             }
-            less = block_sum_i(less, redi);
-            mless = block_max_u(mless, redu);
+            less = block_reduce(less, redi, OpAdd());
+            mless = block_reduce(mless, redu, OpMax());
             const float vk1 = less <= N / 2 - 1 ? vk : __uint_as_float(mless);
             med = (vk + vk1) / 2;
         }
@@ -188,7 +204,7 @@ k_pre_valid(const unsigned char *__restrict__ nzflag, const float *__restrict__ 
             unsigned char *__restrict__ valid)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ int redi[kPreThreads];
+    __shared__ int redi[kPreThreads / 64];
     __shared__ float midv[2];
     float *v = reinterpret_cast<float *>(smem_raw);
     unsigned char *f = reinterpret_cast<unsigned char *>(v + W);
@@ -199,7 +215,7 @@ k_pre_valid(const unsigned char *__restrict__ nzflag, const float *__restrict__ 
         f[w] = nzflag[(long long)s * W + w];
         cnt += f[w];
     }
-    const int n = block_sum_i(cnt, redi);
+    const int n = block_reduce(cnt, redi, OpAdd());
     float thr = 0.0f;
     const bool rule = nstd > 0.0f && n > 0;
     if (rule) {
@@ -286,21 +302,116 @@ k_white_amp(const float *__restrict__ x, long long stride, int W, int N, int M, 
     }
 }
 
-// AveWhite's filter for station s from its kept windows' amplitudes.
-__global__ void __launch_bounds__(kPreThreads)
+// Windows of 2049..4096 samples (Nz = 4096, M = 2048; the hourly windows at
+// 1 Hz): the same two kernels with compile-time FP64 transforms in place in one
+// padded LDS buffer (35 KB instead of two 32 KB buffers: four workgroups a CU
+// instead of two, and no runtime radix dispatch), radix order 4 x 8 x 8 x 8 with
+// its own FP64 twiddle table.  Bin k sits at lds[ct_pad(k)].
+constexpr int kW2M = 2048, kW2Threads = 256;
+
+__device__ __forceinline__ void white_fft_2048(const float *__restrict__ px, int N, cf64 *lds,
+                                               const cf64 *__restrict__ twM)
+{
+    for (int m = threadIdx.x; m < kW2M; m += kW2Threads) {
+        const int n0 = 2 * m, n1 = 2 * m + 1;
+        lds[ct_pad(m)] = cf64{n0 < N ? (double)px[n0] : 0.0, n1 < N ? (double)px[n1] : 0.0};
+    }
+    __syncthreads();
+    ct_fft<double, kW2M, kW2Threads, -1, 4, 8, 8, 8>(lds, twM);
+}
+
+__device__ __forceinline__ cf64 half_bin_ct(const cf64 *Z, int k, const cf64 *__restrict__ twc)
+{
+    constexpr int M = kW2M;
+    const cf64 zk = Z[ct_pad(k == M ? 0 : k)];
+    const cf64 zm = Z[ct_pad(k == 0 ? 0 : M - k)];
+    const cf64 zc = cf64{zm.x, -zm.y};
+    const cf64 ev = cf64{(zk.x + zc.x) * 0.5, (zk.y + zc.y) * 0.5};
+    const cf64 d = cf64{zk.x - zc.x, zk.y - zc.y};
+    const cf64 od = cf64{d.y * 0.5, -d.x * 0.5};
+    const cf64 wk = (k == M) ? cf64{-1.0, 0.0} : cf64{twc[k].x, -twc[k].y};
+    return ev + cmul(wk, od);
+}
+
+__global__ void __launch_bounds__(kW2Threads)
+k_white_amp_2048(const float *__restrict__ x, long long stride, int W, int N, const cf64 *__restrict__ twM,
+                 const cf64 *__restrict__ twc, const unsigned char *__restrict__ valid, double *__restrict__ A)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int sw = blockIdx.x;
+    if (!valid[sw]) return;
+    const int s = sw / W, w = sw - s * W;
+    cf64 *lds = reinterpret_cast<cf64 *>(smem_raw);
+    white_fft_2048(x + (long long)s * stride + (long long)w * N, N, lds, twM);
+    double *a = A + (long long)sw * (kW2M + 1);
+    for (int k = threadIdx.x; k <= kW2M; k += kW2Threads) {
+        const cf64 X = half_bin_ct(lds, k, twc);
+        a[k] = hypot(X.x, X.y);
+    }
+}
+
+__global__ void __launch_bounds__(kW2Threads)
+k_white_apply_2048(float *__restrict__ x, long long stride, int W, int N, const cf64 *__restrict__ twM,
+                   const cf64 *__restrict__ twc, const unsigned char *__restrict__ valid,
+                   const double *__restrict__ HS)
+{
+    constexpr int M = kW2M;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int sw = blockIdx.x;
+    if (!valid[sw]) return;
+    const int s = sw / W, w = sw - s * W;
+    float *px = x + (long long)s * stride + (long long)w * N;
+    cf64 *lds = reinterpret_cast<cf64 *>(smem_raw);
+    white_fft_2048(px, N, lds, twM);
+    const double *hs = HS + (long long)s * (M + 1);
+    const double da1 = 1. / (double)(2 * M);
+    // filtered bins, then the c2r packing F[k] = P + i e^{2 pi i k/Nz} Q, in place:
+    // the thread of pair {k, M-k} reads only Z[k] and Z[M-k] and writes F[k], F[M-k]
+    auto pack = [&](cf64 a, cf64 c, int k) {
+        const cf64 P = cf64{a.x + c.x, a.y - c.y};
+        const cf64 Q = cf64{a.x - c.x, a.y + c.y};
+        const cf64 wq = cmul(twc[k], Q);
+        return cf64{P.x - wq.y, P.y + wq.x};
+    };
+    for (int k = threadIdx.x; k <= M / 2; k += kW2Threads) {
+        cf64 a = half_bin_ct(lds, k, twc), c = half_bin_ct(lds, M - k, twc);
+        const double ga = da1 * hs[k], gc = da1 * hs[M - k];
+        a = cf64{a.x * ga, a.y * ga};
+        c = cf64{c.x * gc, c.y * gc};
+        if (k == 0) {  // c2r ignores the imaginary parts of DC and Nyquist
+            a.y = 0.0;
+            c.y = 0.0;
+        }
+        const cf64 fk = pack(a, c, k);
+        if (k > 0 && k < M / 2) lds[ct_pad(M - k)] = pack(c, a, M - k);
+        lds[ct_pad(k)] = fk;
+    }
+    __syncthreads();
+    ct_fft<double, M, kW2Threads, +1, 4, 8, 8, 8>(lds, twM);
+    for (int n = threadIdx.x; n < N; n += kW2Threads) {
+        const cf64 v = lds[ct_pad(n >> 1)];
+        px[n] = (float)((n & 1) ? v.y : v.x);
+    }
+}
+
+// AveWhite's filter for station s from its kept windows' amplitudes (one
+// workgroup a station: 1024 threads keep enough of the window average's loads in
+// flight).
+constexpr int kWhiteFilterThreads = 1024;
+__global__ void __launch_bounds__(kWhiteFilterThreads)
 k_white_filter(const double *__restrict__ A, const unsigned char *__restrict__ valid, int W, int M, int Nz,
                int N, int n1, int n2, double *__restrict__ Hs, double *__restrict__ HS)
 {
-    __shared__ double redd[kPreThreads];
+    __shared__ double redd[kWhiteFilterThreads / 64];
     __shared__ double win[11];
-    __shared__ int redi[kPreThreads];
+    __shared__ int redi[kWhiteFilterThreads / 64];
     const int s = blockIdx.x;
     const int Nh = M + 1;
     double *H = Hs + (long long)s * Nh;  // this station's scratch row
     const unsigned char *vs = valid + (long long)s * W;
     int cnt = 0;
     for (int w = threadIdx.x; w < W; w += blockDim.x) cnt += vs[w];
-    const int Tr = block_sum_i(cnt, redi);
+    const int Tr = block_reduce(cnt, redi, OpAdd());
     if (Tr == 0) return;
     const double rTr = 1. / (double)Tr;
     for (int k = threadIdx.x; k < Nh; k += blockDim.x) {
@@ -330,22 +441,8 @@ k_white_filter(const double *__restrict__ A, const unsigned char *__restrict__ v
         mn = fmin(mn, H[k]);
         mx = fmax(mx, H[k]);
     }
-    redd[threadIdx.x] = mn;
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) redd[threadIdx.x] = fmin(redd[threadIdx.x], redd[threadIdx.x + st]);
-        __syncthreads();
-    }
-    mn = redd[0];
-    __syncthreads();
-    redd[threadIdx.x] = mx;
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) redd[threadIdx.x] = fmax(redd[threadIdx.x], redd[threadIdx.x + st]);
-        __syncthreads();
-    }
-    mx = redd[0];
-    __syncthreads();
+    mn = block_reduce(mn, redd, OpMin());
+    mx = block_reduce(mx, redd, OpMax());
     if (mx > 1000 * mn) mx = 1000 * mn;
     const double scale = 1 / (mn * (double)Nz);
     for (int k = threadIdx.x; k < Nh; k += blockDim.x) {
@@ -653,7 +750,7 @@ int white_filter(const PreDims &d, const double *A, const unsigned char *valid, 
                  hipStream_t st)
 {
     ProfScope prof("white_filter", st);
-    hipLaunchKernelGGL(k_white_filter, dim3(d.S), dim3(kPreThreads), 0, st, A, valid, d.W, d.M, d.Nz, d.N, d.n1, d.n2,
+    hipLaunchKernelGGL(k_white_filter, dim3(d.S), dim3(kWhiteFilterThreads), 0, st, A, valid, d.W, d.M, d.Nz, d.N, d.n1, d.n2,
                        Hs, HS);
     FPCC_CHECK(hipGetLastError());
     return 0;
@@ -696,7 +793,26 @@ int preprocess_device(const fpcc_job *job, const fpcc_preproc *pp, float *d_x, c
         hipLaunchKernelGGL(k_pre_zero, dim3((unsigned)SW), dim3(256), 0, st, d_x, d.stride, d.W, d.N, d_valid);
         FPCC_CHECK(hipGetLastError());
     }
-    if (d.white && !d.lng) {
+    if (d.white && !d.lng && d.M == kW2M) {
+        const void *twc = get_c2r_twiddles(d.Nz, true), *twW = get_order_twiddles(kW2M, {4, 8, 8, 8}, true);
+        if (!twc || !twW) return -1;
+        const size_t lds = sizeof(cf64) * ct_lds_elems<kW2M>();
+        {
+            ProfScope prof("white_amp", st);
+            allow_lds((const void *)k_white_amp_2048, lds);
+            hipLaunchKernelGGL(k_white_amp_2048, dim3((unsigned)SW), dim3(kW2Threads), lds, st, (const float *)d_x,
+                               d.stride, d.W, d.N, (const cf64 *)twW, (const cf64 *)twc,
+                               (const unsigned char *)d_valid, A);
+            FPCC_CHECK(hipGetLastError());
+        }
+        if ((r = white_filter(d, A, d_valid, Hs, HS, st))) return r;
+        ProfScope prof("white_apply", st);
+        allow_lds((const void *)k_white_apply_2048, lds);
+        hipLaunchKernelGGL(k_white_apply_2048, dim3((unsigned)SW), dim3(kW2Threads), lds, st, d_x, d.stride, d.W,
+                           d.N, (const cf64 *)twW, (const cf64 *)twc, (const unsigned char *)d_valid,
+                           (const double *)HS);
+        FPCC_CHECK(hipGetLastError());
+    } else if (d.white && !d.lng) {
         const DevPlan &pM = get_plan(d.M, true);
         const void *twc = get_c2r_twiddles(d.Nz, true);
         if (!pM.ok || !twc) return -1;

@@ -46,7 +46,9 @@ def test_common_windows_and_no_clip(fp, orc):
     assert np.array_equal(y.view(np.int32), yr.view(np.int32))
 
 
-@pytest.mark.parametrize("N,band", [(3600, (0.02, 0.2)), (1000, (0.05, 0.45)), (8192, (0.01, 0.3)), (64, (0.1, 0.4))])
+# 2049..4096 samples take the compile-time 2048-point FP64 kernels; the others the runtime plans
+@pytest.mark.parametrize("N,band", [(3600, (0.02, 0.2)), (2049, (0.02, 0.2)), (4096, (0.05, 0.3)), (1000, (0.05, 0.45)),
+                                    (8192, (0.01, 0.3)), (64, (0.1, 0.4))])
 def test_whitening_matches_oracle(fp, orc, N, band):
     S, W = 3, 5
     print(f"whitening N={N} band={band}", flush=True)
