@@ -31,6 +31,13 @@ constexpr int kPwsThreads = 512;
 
 enum { SRC_FULL = 0, SRC_HALF32 = 1, SRC_HALF64 = 2 };
 
+// One complex bin through a buffer resource at a byte offset; the 64-bit result is
+// reinterpreted whole (taking its two 32-bit lanes apart compiles to one dword load).
+__device__ __forceinline__ cf32 ld_bin32(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(cf32, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
 // conj(a) b
 template <typename T>
 __device__ __forceinline__ cf32 conj_mul(const C2<T> &a, const C2<T> &b)
@@ -38,7 +45,10 @@ __device__ __forceinline__ cf32 conj_mul(const C2<T> &a, const C2<T> &b)
     return cf32{(float)(a.x * b.x + a.y * b.y), (float)(a.x * b.y - a.y * b.x)};
 }
 
-template <int SRC, int LPT>
+// CT4096: Nz = 8192 (M = 4096) with the compile-time radix-8 x 8 x 8 x 8 inverse of the
+// pair IFFT (out of place between the two buffers, the stage layouts plain or
+// XOR-swizzled; tw the twiddles of that radix order), else the runtime plan.
+template <int SRC, int LPT, bool CT4096>
 __global__ void __launch_bounds__(kPwsThreads)
 k_pair_pws(const void *__restrict__ Xa, const void *__restrict__ Xb, long long ld, int S, int W, int Nz,
            FftPlan plan, const cf32 *__restrict__ tw, const cf32 *__restrict__ twc, int lag0, int L, float cscale,
@@ -63,22 +73,26 @@ k_pair_pws(const void *__restrict__ Xa, const void *__restrict__ Xb, long long l
         cf32 A[KPT];
         float AM;
         if (SRC == SRC_FULL) {
-            const cf32 *Fi = (const cf32 *)Xa + ((long long)i * W + w) * ld;
-            const cf32 *Fj = (const cf32 *)Xa + ((long long)j * W + w) * ld;
+            // the two rows as buffer resources (scalar registers) read at 32-bit lane
+            // offsets: one 64-bit address a load would hold most of the registers
+            const __amdgpu_buffer_rsrc_t ri = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)((const cf32 *)Xa + ((long long)i * W + w) * ld), 0, Nz * 8, 0x00020000);
+            const __amdgpu_buffer_rsrc_t rj = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)((const cf32 *)Xa + ((long long)j * W + w) * ld), 0, Nz * 8, 0x00020000);
 #pragma unroll
             for (int q = 0; q < KPT; ++q) {
                 const int k = t + NT * q;
                 if (k < M) {
-                    const cf32 Y = conj_mul(Fi[k], Fj[k]);
+                    const cf32 Y = conj_mul(ld_bin32(ri, 8u * k), ld_bin32(rj, 8u * k));
                     if (k == 0) {
                         A[q] = cf32{Y.x, 0.f};
                     } else {
-                        const cf32 Yc = conj_mul(Fi[Nz - k], Fj[Nz - k]);
+                        const cf32 Yc = conj_mul(ld_bin32(ri, 8u * (Nz - k)), ld_bin32(rj, 8u * (Nz - k)));
                         A[q] = cf32{Y.x + Yc.x, Y.y - Yc.y};
                     }
                 }
             }
-            AM = conj_mul(Fi[M], Fj[M]).x;
+            AM = conj_mul(ld_bin32(ri, 8u * M), ld_bin32(rj, 8u * M)).x;
         } else {
             typedef typename std::conditional<SRC == SRC_HALF64, cf64, cf32>::type CT;
             const CT *Hi = (const CT *)Xa + ((long long)i * W + w) * ld;
@@ -104,7 +118,17 @@ k_pair_pws(const void *__restrict__ Xa, const void *__restrict__ Xb, long long l
                 if (k < M) buf0[k] = half ? cmul(A[q], twc[k]) : A[q];
             }
             __syncthreads();
-            const cf32 *res = fft_lds<float, +1>(buf0, buf1, plan, tw);
+            const cf32 *res;
+            if constexpr (CT4096) {
+                constexpr int MC = 4096;
+                ct_pass_oop<float, MC, NT, +1, 8, 1, 0, 2>(buf0, buf1, tw);
+                ct_pass_oop<float, MC, NT, +1, 8, 8, 2, 2>(buf1, buf0, tw + 7);
+                ct_pass_oop<float, MC, NT, +1, 8, 64, 2, 0>(buf0, buf1, tw + 7 + 56);
+                ct_pass_oop<float, MC, NT, +1, 8, 512, 0, 0>(buf1, buf0, tw + 7 + 56 + 448);
+                res = buf0;
+            } else {
+                res = fft_lds<float, +1>(buf0, buf1, plan, tw);
+            }
             const float edge = half ? -AM : AM;
 #pragma unroll
             for (int q = 0; q < LPT; ++q) {
@@ -145,13 +169,13 @@ k_pair_pws(const void *__restrict__ Xa, const void *__restrict__ Xb, long long l
 }
 
 template <int SRC>
-int launch_lpt(int lpt, dim3 grid, size_t lds, hipStream_t st, const void *Xa, const void *Xb, long long ld, int S,
-               int W, int Nz, const FftPlan &plan, const cf32 *tw, const cf32 *twc, int lag0, int L, float cscale,
-               float power, int multiply, float *y)
+int launch_lpt(int lpt, bool ct, dim3 grid, size_t lds, hipStream_t st, const void *Xa, const void *Xb, long long ld,
+               int S, int W, int Nz, const FftPlan &plan, const cf32 *tw, const cf32 *twc, int lag0, int L,
+               float cscale, float power, int multiply, float *y)
 {
 #define FPCC_PWS_CASE(N)                                                                                          \
     case N: {                                                                                                     \
-        const auto fn = k_pair_pws<SRC, N>;                                                                      \
+        const auto fn = ct ? k_pair_pws<SRC, N, true> : k_pair_pws<SRC, N, false>;                               \
         if (lds > 65536) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, \
                                                    (int)lds);                                                     \
         hipLaunchKernelGGL(fn, grid, dim3(kPwsThreads), lds, st, Xa, Xb, ld, S, W, Nz, plan, tw, twc, lag0, L,  \
@@ -190,16 +214,20 @@ int launch_pair_pws(int src, const void *d_Xa, const void *d_Xb, long long ld, i
     const size_t lds = sizeof(cf32) * 2 * (size_t)M;
     ProfScope prof("pair_pws", st);
     const dim3 grid((unsigned)P);
-    const cf32 *tw = (const cf32 *)pM.tw, *tc = (const cf32 *)twc;
+    // Nz 8192: the compile-time radix-8^4 inverse and its own twiddle order
+    const bool ct = M == 4096;
+    const cf32 *tw = ct ? (const cf32 *)get_order_twiddles(M, {8, 8, 8, 8}) : (const cf32 *)pM.tw;
+    if (!tw) return -1;
+    const cf32 *tc = (const cf32 *)twc;
     switch (src) {
     case SRC_FULL:
-        return launch_lpt<SRC_FULL>(lpt, grid, lds, st, d_Xa, d_Xb, ld, S, W, Nz, pM.plan, tw, tc, lag0, L, cscale,
+        return launch_lpt<SRC_FULL>(lpt, ct, grid, lds, st, d_Xa, d_Xb, ld, S, W, Nz, pM.plan, tw, tc, lag0, L, cscale,
                                     power, multiply, d_y);
     case SRC_HALF32:
-        return launch_lpt<SRC_HALF32>(lpt, grid, lds, st, d_Xa, d_Xb, ld, S, W, Nz, pM.plan, tw, tc, lag0, L,
+        return launch_lpt<SRC_HALF32>(lpt, ct, grid, lds, st, d_Xa, d_Xb, ld, S, W, Nz, pM.plan, tw, tc, lag0, L,
                                       cscale, power, multiply, d_y);
     case SRC_HALF64:
-        return launch_lpt<SRC_HALF64>(lpt, grid, lds, st, d_Xa, d_Xb, ld, S, W, Nz, pM.plan, tw, tc, lag0, L,
+        return launch_lpt<SRC_HALF64>(lpt, ct, grid, lds, st, d_Xa, d_Xb, ld, S, W, Nz, pM.plan, tw, tc, lag0, L,
                                       cscale, power, multiply, d_y);
     }
     return -1;
