@@ -246,7 +246,65 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
         const int s = b / a.W, w = b - s * a.W;
         xr = a.x + (long long)s * a.x_stride + (long long)w * a.N;
     }
-    for (int idx = threadIdx.x; idx < kBigCols * N1; idx += blockDim.x) {
+    // the value a point of the tile starts with, from its loaded source (u: the
+    // complex row element, xv: the input sample, in: inside the input)
+    auto start = [&](long long g, cf32 u, float xv, bool in) -> cf32 {
+        if (op == COL_FWD_INPUT || op == COL_FWD_CHIRP) {
+            if (!in) return cf32{0.f, 0.f};
+            float sv;
+            if (a.sign_rule == 1) sv = xv >= 0.f ? 1.f : -1.f;
+            else if (a.sign_rule == 2) sv = xv > 0.f ? 1.f : -1.f;
+            else sv = xv * a.in_scale;
+            if (op == COL_FWD_CHIRP) {
+                const cf32 ch = a.chirp[g - a.off];
+                return cf32{sv * ch.x, sv * ch.y};
+            }
+            return cf32{sv, 0.f};
+        }
+        if (op == COL_FWD_AMPNORM) {
+            const float rr = 1.0f / sqrtf(u.x * u.x + u.y * u.y + eps);
+            return __builtin_isnormal(rr) ? cf32{u.x * rr, u.y * rr} : cf32{0.f, 0.f};
+        }
+        if (op == COL_FWD_AMPNORMF) {
+            if (!in) return cf32{0.f, 0.f};
+            const float rr = 1.0f / (sqrtf(u.x * u.x + u.y * u.y) + eps);
+            return __builtin_isnormal(rr) ? cf32{u.x * rr, u.y * rr} : cf32{0.f, 0.f};
+        }
+        return u;
+    };
+    if constexpr (CT) {
+        // all of a thread's loads first (16 points of 256 columns-rows), then the work:
+        // a loop that loads, computes and stores one point at a time waits for every load
+        constexpr int PER = kBigCols * kCtN1 / 256;
+        const bool xin = op == COL_FWD_INPUT || op == COL_FWD_CHIRP;
+        cf32 u[PER];
+        float xv[PER];
+        bool in[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int idx = threadIdx.x + 256 * i, t = idx % kBigCols, r = idx / kBigCols, c = c0 + t;
+            const long long g = (long long)r * N2 + c;
+            u[i] = cf32{0.f, 0.f};
+            xv[i] = 0.f;
+            in[i] = false;
+            if (c < N2) {
+                if (xin) {
+                    const long long m = g - a.off;
+                    in[i] = m >= 0 && m < a.N;
+                    if (in[i]) xv[i] = xr[m];
+                } else {
+                    in[i] = op != COL_FWD_AMPNORMF || g < a.nvalid;
+                    if (in[i]) u[i] = a.src[sbase + g];
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int idx = threadIdx.x + 256 * i, t = idx % kBigCols, r = idx / kBigCols, c = c0 + t;
+            buf0[cix(t, r)] = c < N2 ? start((long long)r * N2 + c, u[i], xv[i], in[i]) : cf32{0.f, 0.f};
+        }
+    }
+    for (int idx = threadIdx.x; !CT && idx < kBigCols * N1; idx += blockDim.x) {
         const int t = idx % kBigCols, r = idx / kBigCols;
         const int c = c0 + t;
         cf32 v = cf32{0.f, 0.f};
@@ -314,7 +372,26 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
         p = a.pair0 + q;
         if (a.norm == 1) pair_index(p, a.S, i, j);
     }
-    for (int idx = threadIdx.x; idx < kBigCols * N1; idx += blockDim.x) {
+    bool done = false;
+    if constexpr (CT) {
+        if (fwd_out) {
+            // the step twiddles of all of a thread's points first, then the stores
+            constexpr int PER = kBigCols * kCtN1 / 256;
+            cf32 w[PER];
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const int idx = threadIdx.x + 256 * i, t = idx % kBigCols, r = idx / kBigCols, c = c0 + t;
+                w[i] = c < N2 ? tw_cr(twN, c, r) : cf32{0.f, 0.f};
+            }
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const int idx = threadIdx.x + 256 * i, t = idx % kBigCols, r = idx / kBigCols, c = c0 + t;
+                if (c < N2) a.dst[base + (long long)r * N2 + c] = cmul(res[cix(t, r)], w[i]);  // W_n^{n2 k1}
+            }
+            done = true;
+        }
+    }
+    for (int idx = threadIdx.x; !done && idx < kBigCols * N1; idx += blockDim.x) {
         const int t = idx % kBigCols, r = idx / kBigCols;
         const int c = c0 + t;
         if (c >= N2) continue;
@@ -409,7 +486,36 @@ k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, 
         cw0 = a.wsum ? 0 : a.wsel + (b - q * a.Wb);
         cw1 = a.wsum ? a.Wfull : cw0 + 1;
     }
-    for (int idx = threadIdx.x; idx < kBigRows * N2; idx += blockDim.x) {
+    bool loaded = false;
+    if constexpr (CTN2 != 0) {
+        if (op != ROW_INV_CROSS) {
+            // all of a thread's loads first, then the LDS writes (a loop that loads and
+            // stores one point at a time waits for every load)
+            constexpr int PER = kBigRows * CTN2 / 256;
+            cf32 u[PER], f[PER];
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const int idx = threadIdx.x + 256 * i, rr = idx / CTN2, c = idx - rr * CTN2, r = r0 + rr;
+                const long long g = (long long)r * CTN2 + c;
+                u[i] = f[i] = cf32{0.f, 0.f};
+                if (r < N1) {
+                    if (op == ROW_INV_MUL) {
+                        u[i] = Xs[g];
+                        f[i] = Fs[g];
+                    } else {
+                        u[i] = a.src[base + g];
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const int idx = threadIdx.x + 256 * i, rr = idx / CTN2, c = idx - rr * CTN2;
+                buf0[rix(rr, c)] = op == ROW_INV_MUL ? cmul(u[i], f[i]) : u[i];
+            }
+            loaded = true;
+        }
+    }
+    for (int idx = threadIdx.x; !loaded && idx < kBigRows * N2; idx += blockDim.x) {
         const int rr = idx / N2, c = idx - rr * N2;
         const int r = r0 + rr;
         cf32 v = cf32{0.f, 0.f};
@@ -464,7 +570,25 @@ k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, 
         }
     }
     const bool inv_out = inv || op == ROW_MASK_INV || op == ROW_MUL_INV;
-    for (int idx = threadIdx.x; idx < kBigRows * N2; idx += blockDim.x) {
+    bool stored = false;
+    if constexpr (CTN2 != 0) {
+        if (inv_out) {
+            constexpr int PER = kBigRows * CTN2 / 256;
+            cf32 w[PER];
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const int idx = threadIdx.x + 256 * i, rr = idx / CTN2, c = idx - rr * CTN2, r = r0 + rr;
+                w[i] = r < N1 ? tw_cr(twN, c, r) : cf32{0.f, 0.f};
+            }
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const int idx = threadIdx.x + 256 * i, rr = idx / CTN2, c = idx - rr * CTN2, r = r0 + rr;
+                if (r < N1) a.dst[base + (long long)r * CTN2 + c] = cmulc(w[i], res[rix(rr, c)]);  // W_n^{-n2 k1}
+            }
+            stored = true;
+        }
+    }
+    for (int idx = threadIdx.x; !stored && idx < kBigRows * N2; idx += blockDim.x) {
         const int rr = idx / N2, c = idx - rr * N2;
         const int r = r0 + rr;
         if (r >= N1) continue;
