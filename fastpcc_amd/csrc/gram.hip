@@ -69,19 +69,28 @@ k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz,
     const int k0 = blockIdx.x * 32;
     const int sw0 = blockIdx.y * R;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-    for (int r = ty; r < R; r += 8) {
-        const int sw = sw0 + r;
+    // all of a thread's loads first (R <= 64: at most 8 rows a thread), then the LDS
+    // writes: a loop that loads and stores one row at a time waits for every load
+    {
         const int k = k0 + tx;
-        cf32 d = cf32{0.f, 0.f}, m = cf32{0.f, 0.f};
-        if (sw < SW && k < nk) {
-            d = F[(long long)sw * ldF + k];
-            if (mirror) {
-                cf32 mm = F[(long long)sw * ldF + ((Nz - k) & (Nz - 1))];
-                m = cf32{mm.x, -mm.y};
+        cf32 d[8], m[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int r = ty + 8 * i, sw = sw0 + r;
+            d[i] = m[i] = cf32{0.f, 0.f};
+            if (r < R && sw < SW && k < nk) {
+                d[i] = F[(long long)sw * ldF + k];
+                if (mirror) m[i] = F[(long long)sw * ldF + ((Nz - k) & (Nz - 1))];
             }
         }
-        td[r][tx] = d;
-        tm[r][tx] = m;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int r = ty + 8 * i;
+            if (r < R) {
+                td[r][tx] = d[i];
+                tm[r][tx] = cf32{m[i].x, -m[i].y};
+            }
+        }
     }
     __syncthreads();
     // write: for each k of the tile, 64 station-windows (contiguous w runs per station)
