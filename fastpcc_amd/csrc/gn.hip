@@ -124,7 +124,11 @@ k_station_real_batch(RealBatch jb, int Nz, int sign_rule, FftPlan planM, const C
 }
 
 // mode 0: y[p][w][L]; 1: mean over windows.  norm 0: energies (GNCC); 1: overlap count.
-template <typename T>
+// CT4096: Nz = 8192 (M = 4096) with a compile-time radix-8^4 inverse in place in
+// one padded LDS buffer (bin k at ct_pad(k); twM the twiddles of that radix
+// order): 69 KB instead of two 64 KB buffers in FP64, two workgroups a CU instead
+// of one, and the fold's loads unrolled; else the runtime plan with two buffers.
+template <typename T, bool CT4096>
 __global__ void __launch_bounds__(512)
 k_pair_gn(const C2<T> *__restrict__ Xa, const C2<T> *__restrict__ Xb, long long ldx,
           const double *__restrict__ E, long long lde, int S, int W, int N, int Nz, FftPlan planM,
@@ -132,10 +136,11 @@ k_pair_gn(const C2<T> *__restrict__ Xa, const C2<T> *__restrict__ Xb, long long 
           float *__restrict__ y, const float *__restrict__ edge, int K)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int M = Nz >> 1;
+    const int M = CT4096 ? 4096 : Nz >> 1;
     C2<T> *buf0 = reinterpret_cast<C2<T> *>(smem_raw);
     C2<T> *buf1 = buf0 + M;
-    float *acc = reinterpret_cast<float *>(buf1 + M);
+    float *acc = reinterpret_cast<float *>(CT4096 ? buf0 + ct_lds_elems<4096>() : buf1 + M);
+    auto zi = [&](int k) { return CT4096 ? ct_pad(k) : k; };
     int p, w0, w1;
     if (mode == 0) {
         p = blockIdx.x / W;
@@ -154,7 +159,7 @@ k_pair_gn(const C2<T> *__restrict__ Xa, const C2<T> *__restrict__ Xb, long long 
     for (int w = w0; w < w1; ++w) {
         const C2<T> *X1 = Xa + (long long)(i * W + w) * ldx;
         const C2<T> *X2 = Xb + (long long)(j * W + w) * ldx;
-        for (int k = threadIdx.x; k < M; k += T_) {
+        auto fold = [&](int k) {
             C2<T> a = cmulc(X1[k], X2[k]);
             C2<T> c = cmulc(X1[M - k], X2[M - k]);
             if (k == 0) {  // c2r ignores the imaginary parts of DC and Nyquist
@@ -164,16 +169,27 @@ k_pair_gn(const C2<T> *__restrict__ Xa, const C2<T> *__restrict__ Xb, long long 
             C2<T> P = C2<T>{a.x + c.x, a.y - c.y};
             C2<T> Q = C2<T>{a.x - c.x, a.y + c.y};
             C2<T> wq = cmul(twc[k], Q);
-            buf0[k] = C2<T>{P.x - wq.y, P.y + wq.x};
+            buf0[zi(k)] = C2<T>{P.x - wq.y, P.y + wq.x};
+        };
+        C2<T> *z;
+        if constexpr (CT4096) {
+            // two bins' loads at a time keep FP64 within two workgroups a CU
+#pragma unroll 2
+            for (int q = 0; q < 4096 / 512; ++q) fold(threadIdx.x + 512 * q);
+            __syncthreads();
+            ct_fft<T, 4096, 512, +1, 8, 8, 8, 8>(buf0, twM);
+            z = buf0;
+        } else {
+            for (int k = threadIdx.x; k < M; k += T_) fold(k);
+            __syncthreads();
+            z = fft_lds<T, +1>(buf0, buf1, planM, twM);
         }
-        __syncthreads();
-        C2<T> *z = fft_lds<T, +1>(buf0, buf1, planM, twM);
         const double *P1 = E ? E + (long long)(i * W + w) * lde : nullptr;
         const double *P2 = E ? E + (long long)(j * W + w) * lde : nullptr;
         for (int l = threadIdx.x; l < L; l += T_) {
             const int lag = lag0 + l;
             const int n = lag & (Nz - 1);
-            C2<T> v = z[n >> 1];
+            C2<T> v = z[zi(n >> 1)];
             double c = (double)(((n & 1) ? v.y : v.x) * inv_nz);
             const int m = N - abs(lag);
             if (edge && m > 0 && m < K) c = edge_corr(edge, K, W, i, j, w, w + 1, lag, m, norm != 0);
@@ -277,22 +293,28 @@ int launch_pair_gn(const void *d_Xa, const void *d_Xb, const double *d_E, bool f
     const int blocks = mode == 0 ? P * W : P;
     if (!blocks) return 0;
     const size_t elem = f64 ? sizeof(cf64) : sizeof(cf32);
-    size_t lds = elem * 2 * (size_t)M + sizeof(float) * (size_t)L;
+    // Nz 8192: the compile-time in-place inverse and its own twiddle order
+    const bool ct = M == 4096;
+    const void *tw = ct ? get_order_twiddles(M, {8, 8, 8, 8}, f64) : pM.tw;
+    if (!tw) return -1;
+    size_t lds = elem * (ct ? (size_t)ct_lds_elems<4096>() : 2 * (size_t)M) + sizeof(float) * (size_t)L;
     if (lds > 160 * 1024) {
         fprintf(stderr, "fastpcc: lag window too long for the single-workgroup pair stage\n");
         return -1;
     }
     ProfScope prof(f64 ? "pair_gn_f64" : "pair_gn_f32", st);
     if (f64) {
-        allow_lds((const void *)k_pair_gn<double>, lds);
-        hipLaunchKernelGGL(k_pair_gn<double>, dim3(blocks), dim3(512), lds, st, (const cf64 *)d_Xa,
-                           (const cf64 *)d_Xb, real_ld(Nz), d_E, energy_ld(N), S, W, N, Nz, pM.plan,
-                           (const cf64 *)pM.tw, (const cf64 *)twc, lag0, L, norm, mode, d_y, d_edge, K);
+        const auto fn = ct ? k_pair_gn<double, true> : k_pair_gn<double, false>;
+        allow_lds((const void *)fn, lds);
+        hipLaunchKernelGGL(fn, dim3(blocks), dim3(512), lds, st, (const cf64 *)d_Xa, (const cf64 *)d_Xb,
+                           real_ld(Nz), d_E, energy_ld(N), S, W, N, Nz, pM.plan, (const cf64 *)tw,
+                           (const cf64 *)twc, lag0, L, norm, mode, d_y, d_edge, K);
     } else {
-        allow_lds((const void *)k_pair_gn<float>, lds);
-        hipLaunchKernelGGL(k_pair_gn<float>, dim3(blocks), dim3(512), lds, st, (const cf32 *)d_Xa,
-                           (const cf32 *)d_Xb, real_ld(Nz), d_E, energy_ld(N), S, W, N, Nz, pM.plan,
-                           (const cf32 *)pM.tw, (const cf32 *)twc, lag0, L, norm, mode, d_y, d_edge, K);
+        const auto fn = ct ? k_pair_gn<float, true> : k_pair_gn<float, false>;
+        allow_lds((const void *)fn, lds);
+        hipLaunchKernelGGL(fn, dim3(blocks), dim3(512), lds, st, (const cf32 *)d_Xa, (const cf32 *)d_Xb,
+                           real_ld(Nz), d_E, energy_ld(N), S, W, N, Nz, pM.plan, (const cf32 *)tw,
+                           (const cf32 *)twc, lag0, L, norm, mode, d_y, d_edge, K);
     }
     FPCC_CHECK(hipGetLastError());
     return 0;
