@@ -354,12 +354,20 @@ __device__ __forceinline__ void ct_pass(C2<T> *lds, const C2<T> *__restrict__ tw
     constexpr int NB = N / R;
     constexpr int BPT = (NB + NT - 1) / NT;
     C2<T> v[BPT][R];
+    // the twiddles are loaded with the LDS reads, ahead of the barrier (which the
+    // compiler does not move global loads across), so their latency overlaps it
+    C2<T> wt[BPT][R > 1 && NS > 1 ? R - 1 : 1];
 #pragma unroll
     for (int i = 0; i < BPT; ++i) {
         const int b = threadIdx.x + i * NT;
         if (NB % NT == 0 || b < NB) {
 #pragma unroll
             for (int r = 0; r < R; ++r) v[i][r] = lds[ct_lay<LIN>(b + r * NB)];
+            if (NS > 1) {
+                const int k = b % NS;
+#pragma unroll
+                for (int r = 1; r < R; ++r) wt[i][r - 1] = tw[(r - 1) * NS + k];
+            }
         }
     }
     __syncthreads();
@@ -371,7 +379,7 @@ __device__ __forceinline__ void ct_pass(C2<T> *lds, const C2<T> *__restrict__ tw
             if (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    C2<T> w = tw[(r - 1) * NS + k];
+                    C2<T> w = wt[i][r - 1];
                     if (DIR > 0) w.y = -w.y;
                     v[i][r] = cmul(v[i][r], w);
                 }
