@@ -46,11 +46,19 @@ int AnalyticSignal(double _Complex *y, double *x, unsigned int N);
 int pcc2_set(float **const y, float **const x1, float **const x2, const unsigned int N,
              const unsigned int Tr, const int Lag1, const int Lag2);
 /* Geometrically normalised correlation, replaces ccgn_set, src/FFTapps.c:883-978.
- * FP64 FFTs and overlap energies like the reference's fftw path. */
+ * FP64 FFTs and overlap energies like the reference's fftw path.  Above a zero-pad
+ * length of 8192 the transforms are FP32, and the lags whose overlap is under N/64
+ * samples are summed exactly in FP64 from the windows' edge samples instead (the
+ * FP32 transform's error divided by a short overlap's small energy would dominate
+ * there). */
 int ccgn_set(float **const y, float **const x1, float **const x2, const unsigned int N,
              const unsigned int Tr, const int Lag1, const int Lag2);
 /* 1-bit GNCC, replaces cc1b_set, src/FFTapps.c:980-1073.  Signs x1 >= 0 -> +1 and
- * x2 > 0 -> +1 (the reference's asymmetric rule, :1037 and :1042); FP32 FFTs. */
+ * x2 > 0 -> +1 (the reference's asymmetric rule, :1037 and :1042); FP32 FFTs.
+ * The lags whose overlap m is under max(64, N/64) samples (at most N) are exact
+ * integer sums of sign products from the windows' edge samples, divided by m: there
+ * the result is the exact value, which the reference's FP32 transform misses by up
+ * to about 1.5e-5 near |lag| = N-1 (the transform's absolute error over a small m). */
 int cc1b_set(float **const y, float **const x1, float **const x2, const unsigned int N,
              const unsigned int Tr, const int Lag1, const int Lag2);
 
