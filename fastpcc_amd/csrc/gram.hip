@@ -755,8 +755,9 @@ __device__ __forceinline__ cf32 ld_bin(__amdgpu_buffer_rsrc_t r, unsigned off)
 
 template <int NT, int LPT, int MODE>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
-k_pair_plain_4096(const cf32 *__restrict__ F, int S, int W, int P, const cf32 *__restrict__ twM,
-                  const cf32 *__restrict__ twc, int lag0, int L, float scale, float *__restrict__ y)
+k_pair_plain_4096(const cf32 *__restrict__ F, int S, int W, int P, const int *__restrict__ order,
+                  const cf32 *__restrict__ twM, const cf32 *__restrict__ twc, int lag0, int L, float scale,
+                  float *__restrict__ y)
 {
     constexpr int M = 4096, Nz = 8192;
     constexpr int H = M / 2 / NT;  // bins k < M/2 a thread
@@ -767,7 +768,10 @@ k_pair_plain_4096(const cf32 *__restrict__ F, int S, int W, int P, const cf32 *_
     const cf32 w0 = twc[t];  // twc[k] = exp(2 pi i k / Nz); twc[t + NT q] = w0 twc[NT q]
     cf32 sa[MODE == 2 ? H : 1], sc[MODE == 2 ? H : 1], sph = cf32{0.f, 0.f};
     float acc[LPT];
-    for (int p = blockIdx.x; p < P; p += gridDim.x) {
+    for (int u = blockIdx.x; u < P; u += gridDim.x) {
+        // pairs in tiles of stations (order[u]): the workgroups running together read
+        // the spectra of a few tiles' stations, which stay in the caches
+        const int p = __builtin_amdgcn_readfirstlane(order[u]);
         int i = 0, j = 0;
         pair_index(p, S, i, j);
         // uniform across the workgroup: scalar registers hold the row addresses
@@ -920,6 +924,33 @@ static const int2 *block_list(int S, int B, int &ntiles)
 
 const int2 *gram_tiles(int S, int &ntiles) { return block_list(S, kGB, ntiles); }
 
+// The pairs in tiles of T x T stations (tiles row-major over the upper triangle of
+// blocks, pairs row-major in a tile) as a device table cached per device and
+// station count: order[u] is the u-th pair's row-major index p.
+static const int *pair_tile_order(int S, int T)
+{
+    static std::map<std::tuple<int, int, int>, const int *> cache;
+    static std::mutex mu;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find({dev, S, T});
+    if (it != cache.end()) return it->second;
+    if (!table_builds_allowed()) return nullptr;
+    std::vector<int> o;
+    o.reserve((size_t)S * (S - 1) / 2);
+    for (int bi = 0; bi * T < S; ++bi)
+        for (int bj = bi; bj * T < S; ++bj)
+            for (int i = bi * T; i < std::min(S, (bi + 1) * T); ++i)
+                for (int j = std::max(i + 1, bj * T); j < std::min(S, (bj + 1) * T); ++j)
+                    o.push_back((int)((long long)i * S - (long long)i * (i + 1) / 2 + (j - i - 1)));
+    void *d = nullptr;
+    if (o.empty() || hipMalloc(&d, sizeof(int) * o.size()) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, o.data(), sizeof(int) * o.size(), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    cache[{dev, S, T}] = (const int *)d;
+    return (const int *)d;
+}
+
 float gram16_sigma(int bound)
 {
     int e = 0;
@@ -1043,6 +1074,10 @@ template <int LPT, int MODE>
 static int launch_plain_4096_t(const cf32 *d_F, int S, int W, int P, const cf32 *twr, const cf32 *twc, int lag0,
                                int L, float scale, float *d_y, hipStream_t st, const char *name)
 {
+    // tiles of 16 stations: two tiles' spectra (32 stations x W windows x 64 KB, 49 MB
+    // at W 24) cover the pairs the resident workgroups run at once
+    const int *order = pair_tile_order(S, 16);
+    if (!order) return -1;
     constexpr int kT = kC2rThreads;
     const auto fn = k_pair_plain_4096<kT, LPT, MODE>;
     const size_t lds = sizeof(cf32) * 2 * 4096;
@@ -1059,7 +1094,7 @@ static int launch_plain_4096_t(const cf32 *d_F, int S, int W, int P, const cf32 
     }
     const int nwg = P < per_cu * ncu ? P : per_cu * ncu;
     ProfScope prof(name, st);
-    hipLaunchKernelGGL(fn, dim3(nwg), dim3(kT), lds, st, d_F, S, W, P, twr, twc, lag0, L, scale, d_y);
+    hipLaunchKernelGGL(fn, dim3(nwg), dim3(kT), lds, st, d_F, S, W, P, order, twr, twc, lag0, L, scale, d_y);
     FPCC_CHECK(hipGetLastError());
     return 0;
 }
