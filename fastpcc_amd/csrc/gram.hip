@@ -412,7 +412,7 @@ __device__ __forceinline__ void wait_vm(int n)
     }
 }
 
-template <int NB>
+template <int NB, bool F32>
 __global__ void __launch_bounds__(kGThreads)
 k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, int K, int nk,
              const int2 *__restrict__ blocks, int nblk, int nchunks, float oscale, cf32 *__restrict__ G,
@@ -500,10 +500,38 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
         if (f + NB - 1 < kGF) dma(f + NB - 1, (f + NB - 1) % NB);
         if (active) {
             f4v r0 = acc[f][0], i0 = acc[f][1], r1 = acc[f][2], i1 = acc[f][3];
-            unsigned pa = base + buf * bufb + arow * su * 16 + 32 * q;
-            unsigned pb0 = base + buf * bufb + brow * su * 16 + 32 * q;
+            unsigned pa = base + buf * bufb + arow * su * 16 + (F32 ? 16 : 32) * q;
+            unsigned pb0 = base + buf * bufb + brow * su * 16 + (F32 ? 16 : 32) * q;
             unsigned pb1 = pb0 + 16 * su * 16;
-            for (int stp = 0; stp < K / 16; ++stp) {
+            // FP32 rows (k_gram<0>'s operands): 8 complex K values (64 bytes) a step,
+            // lane quarter q holding values 2q and 2q + 1, on 16x16x4 FP32 MFMAs
+            for (int mm = 0; F32 && mm < K / 8; ++mm) {
+                f4v a, b0, b1;
+                asm volatile(
+                    "ds_read_b128 %0, %3\n\t"
+                    "ds_read_b128 %1, %4\n\t"
+                    "ds_read_b128 %2, %5\n\t"
+                    "s_waitcnt lgkmcnt(0)"
+                    : "=&v"(a), "=&v"(b0), "=&v"(b1)
+                    : "v"(pa), "v"(pb0), "v"(pb1)
+                    : "memory");
+                pa += 64;
+                pb0 += 64;
+                pb1 += 64;
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const float ar = a[2 * e], ai = a[2 * e + 1];
+                    r0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b0[2 * e], r0, 0, 0, 0);
+                    i0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b0[2 * e + 1], i0, 0, 0, 0);
+                    r1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b1[2 * e], r1, 0, 0, 0);
+                    i1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b1[2 * e + 1], i1, 0, 0, 0);
+                    r0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, b0[2 * e + 1], r0, 0, 0, 0);
+                    i0 = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, b0[2 * e], i0, 0, 0, 0);
+                    r1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, b1[2 * e + 1], r1, 0, 0, 0);
+                    i1 = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, b1[2 * e], i1, 0, 0, 0);
+                }
+            }
+            for (int stp = 0; !F32 && stp < K / 16; ++stp) {
                 u4v ah, al, bh0, bl0, bh1, bl1;
                 asm volatile(
                     "ds_read_b128 %0, %6\n\t"
@@ -995,13 +1023,55 @@ static int launch_gram(bool split, const void *d_Ua, const void *d_Ub, int S, in
     return 0;
 }
 
+static int launch_gram_dma(bool f32, const void *d_Ua, const void *d_Ub, int S, int K, int nk, const int2 *d_tiles,
+                           int ntiles, float oscale, cf32 *d_G, GLayout gl, hipStream_t st);
+
 int launch_gram_f32(const cf32 *d_Ua, const cf32 *d_Ub, int S, int W2, int m0, int K, int nk, const int2 *d_tiles,
                     int ntiles, cf32 *d_G, GLayout gl, hipStream_t st)
 {
+    // whole rows of a multiple of 8 values (up to 70): staged by the memory unit straight into LDS
+    if (m0 == 0 && K == W2 && K % 8 == 0 && dma_row_units(K) <= 36 && (long long)S * K * 8 * kGF < 0x80000000LL)
+        return launch_gram_dma(true, d_Ua, d_Ub, S, K, nk, d_tiles, ntiles, 1.0f, d_G, gl, st);
     return launch_gram(false, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles, ntiles, 1.0f, d_G, gl, st);
 }
 
 bool gram16_grouped(int W2) { return W2 % 16 == 0 && W2 <= kGK; }
+
+// The DMA-staged Gram on rows of K 8-byte elements (f32: FP32 complex; else the
+// grouped split words), K a multiple of 8 (16 for the split words) and at most 70:
+// three staging buffers when they fit the 160 KB of LDS, else two.  -1 when the
+// shape does not qualify.
+static int launch_gram_dma(bool f32, const void *d_Ua, const void *d_Ub, int S, int K, int nk, const int2 *d_tiles,
+                           int ntiles, float oscale, cf32 *d_G, GLayout gl, hipStream_t st)
+{
+    if (S < 2 || nk < 1 || K < 8 || K % (f32 ? 8 : 16) || dma_row_units(K) > 36) return -1;
+    const int nchunks = (nk + kGF - 1) / kGF;
+    if (!layout_ok(gl, S, nchunks)) return -1;
+    if ((long long)S * K * 8 * kGF >= 0x80000000LL) return -1;  // chunk offsets stay below the pad offset
+    const long long slots = (long long)(nchunks + 15) / 16 * 2 * ntiles;
+    if (slots * 8 > 0x7fffffffLL) return -1;
+    const size_t bufb = (size_t)2 * kGB * dma_row_units(K) * 16;
+    const size_t epi = (size_t)4 * kGB * (kGB / 4) * kGOut4;
+    const bool three = 3 * bufb <= 160 * 1024;
+    const size_t lds = std::max(epi, (three ? 3 : 2) * bufb);
+    const void *fn = f32 ? (three ? (const void *)k_gram16_dma<3, true> : (const void *)k_gram16_dma<2, true>)
+                         : (three ? (const void *)k_gram16_dma<3, false> : (const void *)k_gram16_dma<2, false>);
+    allow_lds(fn, lds);
+    ProfScope prof(f32 ? "gram_f32" : "gram_split16", st);
+#define FPCC_GRAM_DMA(NB, F)                                                                                       \
+    hipLaunchKernelGGL((k_gram16_dma<NB, F>), dim3((unsigned)(slots * 8)), dim3(kGThreads), lds, st,             \
+                       (const uint2 *)d_Ua, (const uint2 *)d_Ub, S, K, nk, d_tiles, ntiles, nchunks, oscale, d_G, gl)
+    if (f32) {
+        if (three) FPCC_GRAM_DMA(3, true);
+        else FPCC_GRAM_DMA(2, true);
+    } else {
+        if (three) FPCC_GRAM_DMA(3, false);
+        else FPCC_GRAM_DMA(2, false);
+    }
+#undef FPCC_GRAM_DMA
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
 
 int launch_gram_split16(const void *d_Ua, const void *d_Ub, int S, int W2, int m0, int K, int nk,
                         const int2 *d_tiles, int ntiles, float sigma, cf32 *d_G, GLayout gl, hipStream_t st)
@@ -1011,31 +1081,9 @@ int launch_gram_split16(const void *d_Ua, const void *d_Ub, int S, int W2, int m
         return launch_gram(true, d_Ua, d_Ub, S, W2, m0, K, nk, d_tiles, ntiles, 1.0f / (sigma * sigma), d_G, gl,
                            st);
     // grouped rows (written so by launch_spectra_to_u): the whole row, DMA-staged
-    if (S < 2 || nk < 1 || m0 != 0 || K != W2) return -1;
-    const int nchunks = (nk + kGF - 1) / kGF;
-    if (!layout_ok(gl, S, nchunks)) return -1;
-    if ((long long)S * K * 8 * kGF >= 0x80000000LL) return -1;  // chunk offsets stay below the pad offset
-    const long long slots = (long long)(nchunks + 15) / 16 * 2 * ntiles;
-    if (slots * 8 > 0x7fffffffLL) return -1;
+    if (m0 != 0 || K != W2) return -1;
     static_assert(2 * 2 * kGB * dma_row_units(kGK) * 16 <= kGramLds, "DMA staging fits the Gram's LDS");
-    ProfScope prof("gram_split16", st);
-    // three staging buffers when they fit the 160 KB of LDS, else two
-    const size_t bufb = (size_t)2 * kGB * dma_row_units(K) * 16;
-    const size_t epi = (size_t)4 * kGB * (kGB / 4) * kGOut4;
-    const bool three = 3 * bufb <= 160 * 1024;
-    const size_t lds = std::max(epi, (three ? 3 : 2) * bufb);
-    const void *fn = three ? (const void *)k_gram16_dma<3> : (const void *)k_gram16_dma<2>;
-    allow_lds(fn, lds);
-    if (three)
-        hipLaunchKernelGGL(k_gram16_dma<3>, dim3((unsigned)(slots * 8)), dim3(kGThreads), lds, st,
-                           (const uint2 *)d_Ua, (const uint2 *)d_Ub, S, K, nk, d_tiles, ntiles, nchunks,
-                           1.0f / (sigma * sigma), d_G, gl);
-    else
-        hipLaunchKernelGGL(k_gram16_dma<2>, dim3((unsigned)(slots * 8)), dim3(kGThreads), lds, st,
-                           (const uint2 *)d_Ua, (const uint2 *)d_Ub, S, K, nk, d_tiles, ntiles, nchunks,
-                           1.0f / (sigma * sigma), d_G, gl);
-    FPCC_CHECK(hipGetLastError());
-    return 0;
+    return launch_gram_dma(false, d_Ua, d_Ub, S, K, nk, d_tiles, ntiles, 1.0f / (sigma * sigma), d_G, gl, st);
 }
 
 // 512 threads, radix 8 x 8 x 8 x 8: eight values a thread per pass and the next
