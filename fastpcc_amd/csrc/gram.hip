@@ -412,6 +412,15 @@ __device__ __forceinline__ void wait_vm(int n)
     }
 }
 
+// workgroup barrier over LDS only: this wave's LDS operations complete, then the
+// barrier; its global stores stay in flight (__syncthreads would wait for them)
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 template <int NB, bool F32>
 __global__ void __launch_bounds__(kGThreads)
 k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, int K, int nk,
@@ -582,26 +591,38 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
     for (int h = 0; h < 4; ++h) {
         const int tt = h & 1;
         if (cj == (h >> 1) && active) {
+            const unsigned ob = lds_offset(smem_raw) + ((16 * (16 * ti + 4 * q) + li) * kGOut4) * 4;
 #pragma unroll
-            for (int f = 0; f < kGF; ++f)
+            for (int f = 0; f < kGF; f += 2)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int il = 16 * ti + 4 * q + r, jl = li;
-                    const f4v re = tt ? acc[f][2] : acc[f][0], im = tt ? acc[f][3] : acc[f][1];
-                    *reinterpret_cast<float2 *>(out + (il * 16 + jl) * kGOut4 + 2 * f) =
-                        float2{re[r] * oscale, im[r] * oscale};
+                    const f4v re0 = tt ? acc[f][2] : acc[f][0], im0 = tt ? acc[f][3] : acc[f][1];
+                    const f4v re1 = tt ? acc[f + 1][2] : acc[f + 1][0], im1 = tt ? acc[f + 1][3] : acc[f + 1][1];
+                    const f4v v{re0[r] * oscale, im0[r] * oscale, re1[r] * oscale, im1[r] * oscale};
+                    // staged with inline stores: no copy to LDS is in flight here, and
+                    // the global stores of the previous quarter need not complete first
+                    asm volatile("ds_write_b128 %0, %1" ::"v"(ob + (r * 16 * kGOut4 + 2 * f) * 4), "v"(v) : "memory");
                 }
         }
-        __syncthreads();
-        for (int e = t; e < kGB * 16 * 4; e += kGThreads) {
-            const int pl = e >> 2, qq = e & 3;
-            const int i = bi * kGB + (pl >> 4), j = bj * kGB + 16 * h + (pl & 15);
-            if (i >= j || j >= S) continue;
-            const long long p = (long long)i * S - (long long)i * (i + 1) / 2 + (j - i - 1);
-            *reinterpret_cast<f4v *>(G + gaddr(gl, p, k0 + 2 * qq)) =
-                *reinterpret_cast<const f4v *>(out + pl * kGOut4 + 4 * qq);
+        lds_barrier();
+        // each thread's LDS reads issued together, then its stores
+        constexpr int kE = kGB * 16 * 4 / kGThreads;
+        f4v ev[kE];
+#pragma unroll
+        for (int u = 0; u < kE; ++u) {
+            const int e = t + u * kGThreads, pl = e >> 2, qq = e & 3;
+            ev[u] = *reinterpret_cast<const f4v *>(out + pl * kGOut4 + 4 * qq);
         }
-        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kE; ++u) {
+            const int e = t + u * kGThreads, pl = e >> 2, qq = e & 3;
+            const int i = bi * kGB + (pl >> 4), j = bj * kGB + 16 * h + (pl & 15);
+            if (i < j && j < S) {
+                const long long p = (long long)i * S - (long long)i * (i + 1) / 2 + (j - i - 1);
+                *reinterpret_cast<f4v *>(G + gaddr(gl, p, k0 + 2 * qq)) = ev[u];
+            }
+        }
+        if (h < 3) lds_barrier();
     }
 }
 
