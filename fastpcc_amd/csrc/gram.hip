@@ -693,7 +693,38 @@ __device__ __forceinline__ cf32 c2r_fold(cf32 a, cf32 c, cf32 w)
     return cf32{P.x - wq.y, P.y + wq.x};
 }
 
-template <int NT, bool COUNT>
+// The last radix-8 pass of the 4096-point inverse (stride 512, plain layouts) for
+// lags within +-1024 of zero: only the outputs b and b + 3584 (real lags 2b, 2b + 1
+// and 7168 + 2b, 7169 + 2b) are read, so thread b forms only outputs 0 and 7 of its
+// butterfly, with the operations of the full radix-8 DFT that lead to them.
+__device__ __forceinline__ void c2r_last_pass_ends(const cf32 *in, cf32 *out, const cf32 *__restrict__ tw)
+{
+    constexpr int NS = 512;
+    const int b = threadIdx.x;
+    cf32 v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = in[b + r * NS];
+#pragma unroll
+    for (int r = 1; r < 8; ++r) {
+        cf32 w = tw[(r - 1) * NS + b];
+        w.y = -w.y;
+        v[r] = cmul(v[r], w);
+    }
+    // even and odd radix-4 halves: outputs 0 and 3 of each
+    const cf32 es02 = v[0] + v[4], ed02 = v[0] - v[4];
+    const cf32 es13 = v[2] + v[6], ed13 = rot<+1>(v[2] - v[6]);
+    const cf32 os02 = v[1] + v[5], od02 = v[1] - v[5];
+    const cf32 os13 = v[3] + v[7], od13 = rot<+1>(v[3] - v[7]);
+    const cf32 e0 = es02 + es13, e3 = ed02 - ed13;
+    const cf32 o0 = os02 + os13, o3 = od02 - od13;
+    constexpr float h = 0.70710678118654752440f;
+    const cf32 o3w = cf32{(-o3.x - o3.y) * h, (-o3.y + o3.x) * h};  // * W8^3
+    out[b] = e0 + o0;
+    out[b + 7 * NS] = e3 - o3w;
+    __syncthreads();
+}
+
+template <int NT, bool COUNT, bool ENDS>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
 k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__restrict__ twM,
                 const cf32 *__restrict__ twc, int lag0, int L, float scale, int N, float *__restrict__ y,
@@ -751,7 +782,8 @@ k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__res
         ct_pass_oop<float, M, NT, +1, 8, 1, 0, 2>(lds, alt, twM);
         ct_pass_oop<float, M, NT, +1, 8, 8, 2, 2>(alt, lds, twM + 7);
         ct_pass_oop<float, M, NT, +1, 8, 64, 2, 0>(lds, alt, twM + 7 + 56);
-        ct_pass_oop<float, M, NT, +1, 8, 512, 0, 0>(alt, lds, twM + 7 + 56 + 448);
+        if (ENDS) c2r_last_pass_ends(alt, lds, twM + 7 + 56 + 448);
+        else ct_pass_oop<float, M, NT, +1, 8, 512, 0, 0>(alt, lds, twM + 7 + 56 + 448);
         float *dst = y + p * L;
         // lags whose overlap is under ed.K samples come from the edge sums below
         for (int l = t; l < L; l += NT) {
@@ -1117,7 +1149,9 @@ static int launch_c2r_4096(const cf32 *d_G, const GLayout &gl, int P, const cf32
                            int L, float scale, int N, float *d_y, const C2rEdge &ed, hipStream_t st)
 {
     constexpr int kT = kC2rThreads;
-    const auto fn = k_pair_c2r_4096<kT, COUNT>;
+    // lags within +-1024: the last pass forms only the outputs they read
+    const bool ends = lag0 >= -1024 && lag0 + L - 1 <= 1023;
+    const auto fn = ends ? k_pair_c2r_4096<kT, COUNT, true> : k_pair_c2r_4096<kT, COUNT, false>;
     const size_t lds = sizeof(cf32) * 2 * 4096;
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -1125,12 +1159,12 @@ static int launch_c2r_4096(const cf32 *d_G, const GLayout &gl, int P, const cf32
         return -1;
     // one persistent workgroup per resident slot (the query is the same on every
     // call and device of this build; concurrent first calls store the same value)
-    static std::atomic<int> per_cu_cached{0};
-    int per_cu = per_cu_cached.load(std::memory_order_relaxed);
+    static std::atomic<int> per_cu_cached[2] = {{0}, {0}};
+    int per_cu = per_cu_cached[ends].load(std::memory_order_relaxed);
     if (!per_cu) {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, kT, lds) != hipSuccess || per_cu < 1)
             per_cu = 1;
-        per_cu_cached.store(per_cu, std::memory_order_relaxed);
+        per_cu_cached[ends].store(per_cu, std::memory_order_relaxed);
     }
     const int nwg = P < per_cu * ncu ? P : per_cu * ncu;
     ProfScope prof("pair_c2r", st);
