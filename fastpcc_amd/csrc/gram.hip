@@ -585,13 +585,17 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
     }
 
     // epilogue in quarters (columns 16h..16h+15, held by the waves with cj = h/2,
-    // tt = h%2): pair (il, jl) at dwords (il*16 + jl)*kGOut4 in LDS, then 16 bytes
-    // (two bins) a lane, four lanes a pair: 64-byte segments of G rows
+    // tt = h%2): pair pl = il*16 + jl as 64 bytes at 64 pl in LDS, its 16-byte units
+    // (two bins each) permuted by u ^ ((pl >> 1 ^ pl >> 2) & 3), which puts the
+    // 16 lanes of a write (16 pairs, one unit) and of a read (4 pairs, 4 units) on
+    // distinct banks; then 16 bytes a lane, four lanes a pair: 64-byte segments of G rows
     float *out = reinterpret_cast<float *>(smem_raw);
+    const int swl = ((li >> 1) ^ (li >> 2)) & 3;          // writer lane: pl & 15 = li
+    const int swr = ((t >> 3) ^ (t >> 4)) & 3;            // reader lane: pl & 15 = (t >> 2) & 15
     for (int h = 0; h < 4; ++h) {
         const int tt = h & 1;
         if (cj == (h >> 1) && active) {
-            const unsigned ob = lds_offset(smem_raw) + ((16 * (16 * ti + 4 * q) + li) * kGOut4) * 4;
+            const unsigned ob = lds_offset(smem_raw) + (16 * (16 * ti + 4 * q) + li) * 64;
 #pragma unroll
             for (int f = 0; f < kGF; f += 2)
 #pragma unroll
@@ -601,7 +605,8 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
                     const f4v v{re0[r] * oscale, im0[r] * oscale, re1[r] * oscale, im1[r] * oscale};
                     // staged with inline stores: no copy to LDS is in flight here, and
                     // the global stores of the previous quarter need not complete first
-                    asm volatile("ds_write_b128 %0, %1" ::"v"(ob + (r * 16 * kGOut4 + 2 * f) * 4), "v"(v) : "memory");
+                    asm volatile("ds_write_b128 %0, %1" ::"v"(ob + r * 16 * 64 + (((f >> 1) ^ swl) << 4)), "v"(v)
+                                 : "memory");
                 }
         }
         lds_barrier();
@@ -611,7 +616,7 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
 #pragma unroll
         for (int u = 0; u < kE; ++u) {
             const int e = t + u * kGThreads, pl = e >> 2, qq = e & 3;
-            ev[u] = *reinterpret_cast<const f4v *>(out + pl * kGOut4 + 4 * qq);
+            ev[u] = *reinterpret_cast<const f4v *>(out + pl * 16 + 4 * (qq ^ swr));
         }
 #pragma unroll
         for (int u = 0; u < kE; ++u) {
@@ -1104,7 +1109,7 @@ static int launch_gram_dma(bool f32, const void *d_Ua, const void *d_Ub, int S, 
     const long long slots = (long long)(nchunks + 15) / 16 * 2 * ntiles;
     if (slots * 8 > 0x7fffffffLL) return -1;
     const size_t bufb = (size_t)2 * kGB * dma_row_units(K) * 16;
-    const size_t epi = (size_t)4 * kGB * (kGB / 4) * kGOut4;
+    const size_t epi = (size_t)kGB * 16 * 64;  // an epilogue quarter: 64 x 16 pairs of 64 bytes
     const bool three = 3 * bufb <= 160 * 1024;
     const size_t lds = std::max(epi, (three ? 3 : 2) * bufb);
     const void *fn = f32 ? (three ? (const void *)k_gram16_dma<3, true> : (const void *)k_gram16_dma<2, true>)
