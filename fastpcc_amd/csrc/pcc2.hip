@@ -115,22 +115,23 @@ k_station_pcc_batch(PccBatch jb, int Nz, FftPlan planZ, const cf32 *__restrict__
 
 constexpr int kSpN = 3600, kSpNz = 8192, kSpM = 4096, kSpT = 512;
 
-// The station kernel's 3600-point transforms: plain layout in and out, the stage
-// between the radix-16 and radix-9 passes XOR-swizzled (3% faster than one pad
-// slot per 16; the 4096-point transforms keep ct_pad, which measured faster there).
+// The station kernel's 3600-point transforms, out of place a -> b -> a -> b (one
+// barrier a pass): plain layout in and out, the stage between the radix-16 and
+// radix-9 passes XOR-swizzled (3% faster than one pad slot per 16).
 template <int DIR>
-__device__ __forceinline__ void fft_3600(cf32 *lds, const cf32 *__restrict__ tw)
+__device__ __forceinline__ void fft_3600(cf32 *a, cf32 *b, const cf32 *__restrict__ tw)
 {
-    ct_pass<float, kSpN, kSpT, DIR, 16, 1, 0, 3>(lds, tw);
-    ct_pass<float, kSpN, kSpT, DIR, 9, 16, 3, 0>(lds, tw + 15);
-    ct_pass<float, kSpN, kSpT, DIR, 25, 144, 0, 0>(lds, tw + 15 + 8 * 16);
+    ct_pass_oop<float, kSpN, kSpT, DIR, 16, 1, 0, 3>(a, b, tw);
+    ct_pass_oop<float, kSpN, kSpT, DIR, 9, 16, 3, 0>(b, a, tw + 15);
+    ct_pass_oop<float, kSpN, kSpT, DIR, 25, 144, 0, 0>(a, b, tw + 15 + 8 * 16);
 }
 
 // Station stage specialised for N = 3600, Nz = 8192 (one day at 1 Hz cut in
-// hourly windows): compile-time FFTs in one padded LDS buffer of 4096 values
-// (35 KB), 512 threads (the 4096-point transforms in radix-8 passes, a butterfly
-// a thread; 116 registers allow two workgroups, 16 waves, a CU: 6% faster than
-// 256 threads with radix-16 passes at 147 registers and 12 waves).  The window occupies
+// hourly windows): compile-time FFTs out of place between two LDS buffers of 4096
+// values (64 KB), 512 threads (the 4096-point transforms in radix-8 passes, a
+// butterfly a thread; 110 registers and 64 KB allow two workgroups, 16 waves, a CU:
+// 6% faster than 256 threads with radix-16 passes at 147 registers and 12 waves;
+// out of place, 2.5% faster than in place in one padded buffer).  The window occupies
 // samples < 3600 < 4096 of the zero-padded 8192, so the 8192-point spectrum
 // splits into two 4096-point transforms:
 //   F[2m] = FFT_4096(z)[m],  F[2m+1] = FFT_4096(z[n] e^{-2 pi i n/8192})[m];
@@ -142,22 +143,23 @@ k_station_pcc_3600(const float *__restrict__ x, long long station_stride, int W,
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cf32 *lds = reinterpret_cast<cf32 *>(smem_raw);
-    float *red = reinterpret_cast<float *>(lds + ct_lds_elems<kSpM>());
+    float *red = reinterpret_cast<float *>(lds + 2 * kSpM);
     const int sw = blockIdx.x, t = threadIdx.x;
     const int s = sw / W, w = sw - s * W;
     const float *in = x + (long long)s * station_stride + (long long)w * kSpN;
+    cf32 *alt = lds + kSpM;
     for (int n = t; n < kSpN; n += kSpT) lds[n] = cf32{in[n], 0.0f};
     __syncthreads();
-    fft_3600<-1>(lds, twN);
+    fft_3600<-1>(lds, alt, twN);  // spectrum in alt
     constexpr int half = (kSpN + 1) >> 1;
     const float two_da1 = 2 * da1;
     for (int k = t; k < kSpN; k += kSpT) {
         const float g = k == 0 ? da1 : k < half ? two_da1 : k == (kSpN >> 1) ? da1 : 0.0f;
-        cf32 v = lds[k];
-        lds[k] = cf32{v.x * g, v.y * g};
+        cf32 v = alt[k];
+        alt[k] = cf32{v.x * g, v.y * g};
     }
     __syncthreads();
-    fft_3600<+1>(lds, twN);
+    fft_3600<+1>(alt, lds, twN);  // analytic signal in lds
     float m = 0.0f;
     for (int n = t; n < kSpN; n += kSpT) {
         cf32 v = lds[n];
@@ -179,28 +181,36 @@ k_station_pcc_3600(const float *__restrict__ x, long long station_stride, int W,
         }
     }
     __syncthreads();
-    // even bins
+    // the 4096-point transforms out of place between lds and alt, one barrier a
+    // pass, the stages between passes 1-2 and 2-3 XOR-swizzled (every wave's LDS
+    // accesses free of bank conflicts); even bins: lds -> alt -> lds -> alt -> lds
 #pragma unroll
-    for (int i = 0; i < kSpM / kSpT; ++i) lds[ct_pad(t + kSpT * i)] = i < kZ ? z[i] : cf32{0.0f, 0.0f};
+    for (int i = 0; i < kSpM / kSpT; ++i) lds[t + kSpT * i] = i < kZ ? z[i] : cf32{0.0f, 0.0f};
     __syncthreads();
-    ct_fft<float, kSpM, kSpT, -1, 8, 8, 8, 8>(lds, twM);
+    ct_pass_oop<float, kSpM, kSpT, -1, 8, 1, 0, 2>(lds, alt, twM);
+    ct_pass_oop<float, kSpM, kSpT, -1, 8, 8, 2, 2>(alt, lds, twM + 7);
+    ct_pass_oop<float, kSpM, kSpT, -1, 8, 64, 2, 0>(lds, alt, twM + 7 + 56);
+    ct_pass_oop<float, kSpM, kSpT, -1, 8, 512, 0, 0>(alt, lds, twM + 7 + 56 + 448);
     cf32 ev[kSpM / kSpT];
 #pragma unroll
-    for (int i = 0; i < kSpM / kSpT; ++i) ev[i] = lds[ct_pad(t + kSpT * i)];
-    __syncthreads();
-    // odd bins
+    for (int i = 0; i < kSpM / kSpT; ++i) ev[i] = lds[t + kSpT * i];
+    // odd bins: alt -> lds -> alt -> lds -> alt (alt was last read by the pass above,
+    // lds is read above before the barrier that follows)
 #pragma unroll
     for (int i = 0; i < kSpM / kSpT; ++i) {
         const int n = t + kSpT * i;
-        lds[ct_pad(n)] = i < kZ && n < kSpN ? cmul(z[i], twZ[n]) : cf32{0.0f, 0.0f};
+        alt[n] = i < kZ && n < kSpN ? cmul(z[i], twZ[n]) : cf32{0.0f, 0.0f};
     }
     __syncthreads();
-    ct_fft<float, kSpM, kSpT, -1, 8, 8, 8, 8>(lds, twM);
+    ct_pass_oop<float, kSpM, kSpT, -1, 8, 1, 0, 2>(alt, lds, twM);
+    ct_pass_oop<float, kSpM, kSpT, -1, 8, 8, 2, 2>(lds, alt, twM + 7);
+    ct_pass_oop<float, kSpM, kSpT, -1, 8, 64, 2, 0>(alt, lds, twM + 7 + 56);
+    ct_pass_oop<float, kSpM, kSpT, -1, 8, 512, 0, 0>(lds, alt, twM + 7 + 56 + 448);
     float4 *out = reinterpret_cast<float4 *>(row);
 #pragma unroll
     for (int i = 0; i < kSpM / kSpT; ++i) {
         const int k = t + kSpT * i;
-        const cf32 o = lds[ct_pad(k)];
+        const cf32 o = alt[k];
         out[k] = float4{ev[i].x, ev[i].y, o.x, o.y};
     }
 }
@@ -339,7 +349,7 @@ int launch_station_pcc(const float *d_x, long long station_stride, int S, int W,
         const void *twN = get_order_twiddles(kSpN, {16, 9, 25}), *twM = get_order_twiddles(kSpM, {8, 8, 8, 8});
         const void *twZ = get_full_twiddles(kSpNz, false);
         if (!twN || !twM || !twZ) return -1;
-        size_t lds = sizeof(cf32) * ct_lds_elems<kSpM>() + sizeof(float) * 8;
+        size_t lds = sizeof(cf32) * 2 * kSpM + sizeof(float) * 8;
         allow_lds((const void *)k_station_pcc_3600, lds);
         ProfScope prof("station_pcc", st);
         hipLaunchKernelGGL(k_station_pcc_3600, dim3(S * W), dim3(kSpT), lds, st, d_x, station_stride, W,
