@@ -440,6 +440,39 @@ __device__ __forceinline__ void ct_fft(C2<T> *lds, const C2<T> *__restrict__ tw)
     ct_passes<T, N, NT, DIR, 1, 0, Rs...>(lds, tw);
 }
 
+// The last radix-8 pass of the 4096-point inverse (stride 512, plain layouts) for
+// lags within +-1024 of zero: only the outputs b and b + 3584 (real lags 2b, 2b + 1
+// and 7168 + 2b, 7169 + 2b) are read, so thread b of a 512-thread workgroup forms
+// only outputs 0 and 7 of its butterfly, with the operations of the full radix-8
+// DFT that lead to them.
+__device__ __forceinline__ void ct_last_pass_ends_4096(const C2<float> *in, C2<float> *out,
+                                                       const C2<float> *__restrict__ tw)
+{
+    constexpr int NS = 512;
+    const int b = threadIdx.x;
+    C2<float> v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = in[b + r * NS];
+#pragma unroll
+    for (int r = 1; r < 8; ++r) {
+        C2<float> w = tw[(r - 1) * NS + b];
+        w.y = -w.y;
+        v[r] = cmul(v[r], w);
+    }
+    // even and odd radix-4 halves: outputs 0 and 3 of each
+    const C2<float> es02 = v[0] + v[4], ed02 = v[0] - v[4];
+    const C2<float> es13 = v[2] + v[6], ed13 = rot<+1>(v[2] - v[6]);
+    const C2<float> os02 = v[1] + v[5], od02 = v[1] - v[5];
+    const C2<float> os13 = v[3] + v[7], od13 = rot<+1>(v[3] - v[7]);
+    const C2<float> e0 = es02 + es13, e3 = ed02 - ed13;
+    const C2<float> o0 = os02 + os13, o3 = od02 - od13;
+    constexpr float h = 0.70710678118654752440f;
+    const C2<float> o3w = C2<float>{(-o3.x - o3.y) * h, (-o3.y + o3.x) * h};  // * W8^3
+    out[b] = e0 + o0;
+    out[b + 7 * NS] = e3 - o3w;
+    __syncthreads();
+}
+
 // ---- workgroup reductions (wave64) ----
 template <typename T>
 __device__ __forceinline__ T wave_max(T v)

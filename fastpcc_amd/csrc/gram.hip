@@ -698,37 +698,6 @@ __device__ __forceinline__ cf32 c2r_fold(cf32 a, cf32 c, cf32 w)
     return cf32{P.x - wq.y, P.y + wq.x};
 }
 
-// The last radix-8 pass of the 4096-point inverse (stride 512, plain layouts) for
-// lags within +-1024 of zero: only the outputs b and b + 3584 (real lags 2b, 2b + 1
-// and 7168 + 2b, 7169 + 2b) are read, so thread b forms only outputs 0 and 7 of its
-// butterfly, with the operations of the full radix-8 DFT that lead to them.
-__device__ __forceinline__ void c2r_last_pass_ends(const cf32 *in, cf32 *out, const cf32 *__restrict__ tw)
-{
-    constexpr int NS = 512;
-    const int b = threadIdx.x;
-    cf32 v[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = in[b + r * NS];
-#pragma unroll
-    for (int r = 1; r < 8; ++r) {
-        cf32 w = tw[(r - 1) * NS + b];
-        w.y = -w.y;
-        v[r] = cmul(v[r], w);
-    }
-    // even and odd radix-4 halves: outputs 0 and 3 of each
-    const cf32 es02 = v[0] + v[4], ed02 = v[0] - v[4];
-    const cf32 es13 = v[2] + v[6], ed13 = rot<+1>(v[2] - v[6]);
-    const cf32 os02 = v[1] + v[5], od02 = v[1] - v[5];
-    const cf32 os13 = v[3] + v[7], od13 = rot<+1>(v[3] - v[7]);
-    const cf32 e0 = es02 + es13, e3 = ed02 - ed13;
-    const cf32 o0 = os02 + os13, o3 = od02 - od13;
-    constexpr float h = 0.70710678118654752440f;
-    const cf32 o3w = cf32{(-o3.x - o3.y) * h, (-o3.y + o3.x) * h};  // * W8^3
-    out[b] = e0 + o0;
-    out[b + 7 * NS] = e3 - o3w;
-    __syncthreads();
-}
-
 template <int NT, bool COUNT, bool ENDS>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
 k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__restrict__ twM,
@@ -787,7 +756,7 @@ k_pair_c2r_4096(const cf32 *__restrict__ G, GLayout gl, int P, const cf32 *__res
         ct_pass_oop<float, M, NT, +1, 8, 1, 0, 2>(lds, alt, twM);
         ct_pass_oop<float, M, NT, +1, 8, 8, 2, 2>(alt, lds, twM + 7);
         ct_pass_oop<float, M, NT, +1, 8, 64, 2, 0>(lds, alt, twM + 7 + 56);
-        if (ENDS) c2r_last_pass_ends(alt, lds, twM + 7 + 56 + 448);
+        if (ENDS) ct_last_pass_ends_4096(alt, lds, twM + 7 + 56 + 448);
         else ct_pass_oop<float, M, NT, +1, 8, 512, 0, 0>(alt, lds, twM + 7 + 56 + 448);
         float *dst = y + p * L;
         // lags whose overlap is under ed.K samples come from the edge sums below
@@ -839,7 +808,7 @@ __device__ __forceinline__ cf32 ld_bin(__amdgpu_buffer_rsrc_t r, unsigned off)
     return __builtin_bit_cast(cf32, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
 
-template <int NT, int LPT, int MODE>
+template <int NT, int LPT, int MODE, bool ENDS>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
 k_pair_plain_4096(const cf32 *__restrict__ F, int S, int W, int P, const int *__restrict__ order,
                   const cf32 *__restrict__ twM, const cf32 *__restrict__ twc, int lag0, int L, float scale,
@@ -923,7 +892,8 @@ k_pair_plain_4096(const cf32 *__restrict__ F, int S, int W, int P, const int *__
             ct_pass_oop<float, M, NT, +1, 8, 1, 0, 2>(lds, alt, tw);
             ct_pass_oop<float, M, NT, +1, 8, 8, 2, 2>(alt, lds, tw + 7);
             ct_pass_oop<float, M, NT, +1, 8, 64, 2, 0>(lds, alt, tw + 7 + 56);
-            ct_pass_oop<float, M, NT, +1, 8, 512, 0, 0>(alt, lds, tw + 7 + 56 + 448);
+            if (ENDS) ct_last_pass_ends_4096(alt, lds, tw + 7 + 56 + 448);
+            else ct_pass_oop<float, M, NT, +1, 8, 512, 0, 0>(alt, lds, tw + 7 + 56 + 448);
             float *dst = y + (MODE == 0 ? ((long long)p * W + w) * L : (long long)p * L);
 #pragma unroll
             for (int q = 0; q < LPT; ++q) {
@@ -1178,7 +1148,7 @@ static int launch_c2r_4096(const cf32 *d_G, const GLayout &gl, int P, const cf32
     return 0;
 }
 
-template <int LPT, int MODE>
+template <int LPT, int MODE, bool ENDS>
 static int launch_plain_4096_t(const cf32 *d_F, int S, int W, int P, const cf32 *twr, const cf32 *twc, int lag0,
                                int L, float scale, float *d_y, hipStream_t st, const char *name)
 {
@@ -1187,7 +1157,7 @@ static int launch_plain_4096_t(const cf32 *d_F, int S, int W, int P, const cf32 
     const int *order = pair_tile_order(S, 16);
     if (!order) return -1;
     constexpr int kT = kC2rThreads;
-    const auto fn = k_pair_plain_4096<kT, LPT, MODE>;
+    const auto fn = k_pair_plain_4096<kT, LPT, MODE, ENDS>;
     const size_t lds = sizeof(cf32) * 2 * 4096;
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -1211,13 +1181,17 @@ template <int MODE>
 static int launch_plain_4096_m(int lpt, const cf32 *d_F, int S, int W, int P, const cf32 *twr, const cf32 *twc,
                                int lag0, int L, float scale, float *d_y, hipStream_t st, const char *name)
 {
+    // lags within +-1024: the last pass forms only the outputs they read
+    const bool ends = lag0 >= -1024 && lag0 + L - 1 <= 1023;
+#define FPCC_PLAIN(LPT, E) launch_plain_4096_t<LPT, MODE, E>(d_F, S, W, P, twr, twc, lag0, L, scale, d_y, st, name)
     switch (lpt) {
-    case 1: return launch_plain_4096_t<1, MODE>(d_F, S, W, P, twr, twc, lag0, L, scale, d_y, st, name);
-    case 2: return launch_plain_4096_t<2, MODE>(d_F, S, W, P, twr, twc, lag0, L, scale, d_y, st, name);
-    case 4: return launch_plain_4096_t<4, MODE>(d_F, S, W, P, twr, twc, lag0, L, scale, d_y, st, name);
-    case 8: return launch_plain_4096_t<8, MODE>(d_F, S, W, P, twr, twc, lag0, L, scale, d_y, st, name);
-    case 16: return launch_plain_4096_t<16, MODE>(d_F, S, W, P, twr, twc, lag0, L, scale, d_y, st, name);
+    case 1: return ends ? FPCC_PLAIN(1, true) : FPCC_PLAIN(1, false);
+    case 2: return ends ? FPCC_PLAIN(2, true) : FPCC_PLAIN(2, false);
+    case 4: return ends ? FPCC_PLAIN(4, true) : FPCC_PLAIN(4, false);
+    case 8: return FPCC_PLAIN(8, false);
+    case 16: return FPCC_PLAIN(16, false);
     }
+#undef FPCC_PLAIN
     return -1;
 }
 

@@ -48,7 +48,7 @@ __device__ __forceinline__ cf32 conj_mul(const C2<T> &a, const C2<T> &b)
 // CT4096: Nz = 8192 (M = 4096) with the compile-time radix-8 x 8 x 8 x 8 inverse of the
 // pair IFFT (out of place between the two buffers, the stage layouts plain or
 // XOR-swizzled; tw the twiddles of that radix order), else the runtime plan.
-template <int SRC, int LPT, bool CT4096>
+template <int SRC, int LPT, bool CT4096, bool ENDS = false>
 __global__ void __launch_bounds__(kPwsThreads)
 k_pair_pws(const void *__restrict__ Xa, const void *__restrict__ Xb, long long ld, int S, int W, int Nz,
            FftPlan plan, const cf32 *__restrict__ tw, const cf32 *__restrict__ twc, int lag0, int L, float cscale,
@@ -124,7 +124,9 @@ k_pair_pws(const void *__restrict__ Xa, const void *__restrict__ Xb, long long l
                 ct_pass_oop<float, MC, NT, +1, 8, 1, 0, 2>(buf0, buf1, tw);
                 ct_pass_oop<float, MC, NT, +1, 8, 8, 2, 2>(buf1, buf0, tw + 7);
                 ct_pass_oop<float, MC, NT, +1, 8, 64, 2, 0>(buf0, buf1, tw + 7 + 56);
-                ct_pass_oop<float, MC, NT, +1, 8, 512, 0, 0>(buf1, buf0, tw + 7 + 56 + 448);
+                // lags within +-1024: only the outputs they read
+                if (ENDS) ct_last_pass_ends_4096(buf1, buf0, tw + 7 + 56 + 448);
+                else ct_pass_oop<float, MC, NT, +1, 8, 512, 0, 0>(buf1, buf0, tw + 7 + 56 + 448);
                 res = buf0;
             } else {
                 res = fft_lds<float, +1>(buf0, buf1, plan, tw);
@@ -173,9 +175,11 @@ int launch_lpt(int lpt, bool ct, dim3 grid, size_t lds, hipStream_t st, const vo
                int S, int W, int Nz, const FftPlan &plan, const cf32 *tw, const cf32 *twc, int lag0, int L,
                float cscale, float power, int multiply, float *y)
 {
+    const bool ends = ct && lag0 >= -1024 && lag0 + L - 1 <= 1023;
 #define FPCC_PWS_CASE(N)                                                                                          \
     case N: {                                                                                                     \
-        const auto fn = ct ? k_pair_pws<SRC, N, true> : k_pair_pws<SRC, N, false>;                               \
+        const auto fn = ends ? k_pair_pws<SRC, N, true, (N <= 4)>                                                 \
+                             : ct ? k_pair_pws<SRC, N, true> : k_pair_pws<SRC, N, false>;                        \
         if (lds > 65536) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, \
                                                    (int)lds);                                                     \
         hipLaunchKernelGGL(fn, grid, dim3(kPwsThreads), lds, st, Xa, Xb, ld, S, W, Nz, plan, tw, twc, lag0, L,  \
