@@ -470,12 +470,19 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
         voff0[j] = pu < upr ? (unsigned)(((long long)st * K + 2 * pu) * 8) : 0x80000000u;
     }
 
+    // copies holding rows of real stations: a side's rows 0 .. n-1 fill its first
+    // ceil(n su / 64) copies, so a block that ends past the last station (networks of
+    // under 64 stations) copies only those (its other rows feed only pairs that are
+    // never stored)
+    const int na = (min(kGB, S - bi * kGB) * su + 63) / 64, nb = (min(kGB, S - bj * kGB) * su + 63) / 64;
+    auto used = [&](int c64) { return c64 < nchunk64 && (c64 < su ? c64 < na : c64 - su < nb); };
+
     auto dma = [&](int f, int buf) {
         const unsigned boff = (unsigned)(f * binb);
 #pragma unroll
         for (int j = 0; j < 9; ++j) {
             const int c64 = j * 8 + wave;  // scalar
-            if (c64 >= nchunk64) break;
+            if (!used(c64)) continue;
             __builtin_amdgcn_raw_ptr_buffer_load_lds(c64 >= su ? rb : ra,
                                                      (__attribute__((address_space(3))) void *)(lds + buf * bufb +
                                                                                                 c64 * 1024),
@@ -492,7 +499,7 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
     // copies this wave issues per bin
     int nw = 0;
 #pragma unroll
-    for (int j = 0; j < 9; ++j) nw += j * 8 + wave < nchunk64;
+    for (int j = 0; j < 9; ++j) nw += used(j * 8 + wave);
     // NB buffers: bins f + 1 .. f + NB - 1 in flight while bin f is multiplied
     dma(0, 0);
 #pragma unroll
@@ -628,6 +635,148 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
             }
         }
         if (h < 3) lds_barrier();
+    }
+}
+
+// Gram of a network of at most 32 stations (one diagonal block, where the 64 x 64
+// kernel keeps two of its eight waves busy): a workgroup a chunk of 8 bins, every
+// bin's rows copied into LDS at once (each bin's region a whole number of 1 KB
+// pieces), wave f multiplying bin f's 32 x 32 block (its three upper 16 x 16
+// tiles) with the operand reads and MFMA sequences of k_gram16_dma, then the
+// results staged through LDS (64 bytes a pair, 16-byte units permuted by the pair
+// index) into 64-byte runs of G.
+template <bool F32>
+__global__ void __launch_bounds__(kGThreads)
+k_gram_small(const uint2 *__restrict__ U, int S, int K, int nk, int nchunks, float oscale, cf32 *__restrict__ G,
+             GLayout gl)
+{
+    static_assert(kGThreads / 64 == kGF, "a wave a bin");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int b = blockIdx.x, xcd = b & 7, slot = b >> 3;
+    const int chunk = ((slot >> 1) * 8 + xcd) * 2 + (slot & 1);  // a G line's two chunks on one XCD
+    if (chunk >= nchunks) return;
+    const int k0 = chunk * kGF;
+    const int t = threadIdx.x, lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);  // the bin this wave multiplies
+    const int li = lane & 15, q = lane >> 4;
+    const int su = dma_row_units(K), upr = K / 2;
+    const int binlds = (su + 1) / 2 * 1024;  // 32 rows of su 16-byte units, rounded up to 1 KB pieces
+    const int npb = (S * su + 63) / 64;       // pieces holding the rows of real stations
+    const long long binb = (long long)S * K * 8;
+    const unsigned nrec = (unsigned)((long long)min(kGF, nk - k0) * binb);
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(U + (long long)k0 * S * K), 0, nrec, 0x00020000);
+    for (int g = wave; g < kGF * npb; g += kGThreads / 64) {  // uniform
+        const int f = g / npb, c = g - f * npb;
+        const int p = c * 64 + lane, row = p / su, pu = p - row * su;
+        const unsigned off = pu < upr ? (unsigned)(f * binb + ((long long)row * K + 2 * pu) * 8) : 0x80000000u;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            r, (__attribute__((address_space(3))) void *)(smem_raw + f * binlds + c * 1024), 16, off, 0, 0, 0);
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+
+    // tiles (0,0), (0,1), (1,1): rows 16 ti .., columns 16 tj ..
+    f4v acc[3][2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) acc[a][0] = acc[a][1] = f4v{0.f, 0.f, 0.f, 0.f};
+    const unsigned base = lds_offset(smem_raw) + wave * binlds;
+    unsigned p0 = base + li * su * 16 + (F32 ? 16 : 32) * q;
+    unsigned p1 = p0 + 16 * su * 16;
+    for (int mm = 0; F32 && mm < K / 8; ++mm) {
+        f4v x0, x1;
+        asm volatile(
+            "ds_read_b128 %0, %2\n\t"
+            "ds_read_b128 %1, %3\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&v"(x0), "=&v"(x1)
+            : "v"(p0), "v"(p1)
+            : "memory");
+        p0 += 64;
+        p1 += 64;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const f4v &A = a == 2 ? x1 : x0, &B = a == 0 ? x0 : x1;
+                const float ar = A[2 * e], ai = A[2 * e + 1];
+                f4v re = acc[a][0], im = acc[a][1];
+                re = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, B[2 * e], re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, B[2 * e + 1], im, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, B[2 * e + 1], re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, B[2 * e], im, 0, 0, 0);
+                acc[a][0] = re;
+                acc[a][1] = im;
+            }
+        }
+    }
+    for (int stp = 0; !F32 && stp < K / 16; ++stp) {
+        u4v h0, l0, h1, l1;
+        asm volatile(
+            "ds_read_b128 %0, %4\n\t"
+            "ds_read_b128 %1, %4 offset:16\n\t"
+            "ds_read_b128 %2, %5\n\t"
+            "ds_read_b128 %3, %5 offset:16\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&v"(h0), "=&v"(l0), "=&v"(h1), "=&v"(l1)
+            : "v"(p0), "v"(p1)
+            : "memory");
+        p0 += 128;
+        p1 += 128;
+        const h8v H0 = __builtin_bit_cast(h8v, h0), L0 = __builtin_bit_cast(h8v, l0);
+        const h8v H1 = __builtin_bit_cast(h8v, h1), L1 = __builtin_bit_cast(h8v, l1);
+        const h8v RH0 = __builtin_bit_cast(h8v, rot_cplx16(h0)), RL0 = __builtin_bit_cast(h8v, rot_cplx16(l0));
+        const h8v RH1 = __builtin_bit_cast(h8v, rot_cplx16(h1)), RL1 = __builtin_bit_cast(h8v, rot_cplx16(l1));
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const h8v &Ah = a == 2 ? H1 : H0, &Al = a == 2 ? L1 : L0;
+            const h8v &Rh = a == 2 ? RH1 : RH0, &Rl = a == 2 ? RL1 : RL0;
+            const h8v &Bh = a == 0 ? H0 : H1, &Bl = a == 0 ? L0 : L1;
+            f4v re = acc[a][0], im = acc[a][1];
+            re = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah, Bh, re, 0, 0, 0);
+            im = __builtin_amdgcn_mfma_f32_16x16x32_f16(Rh, Bh, im, 0, 0, 0);
+            re = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah, Bl, re, 0, 0, 0);
+            im = __builtin_amdgcn_mfma_f32_16x16x32_f16(Rh, Bl, im, 0, 0, 0);
+            re = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al, Bh, re, 0, 0, 0);
+            im = __builtin_amdgcn_mfma_f32_16x16x32_f16(Rl, Bh, im, 0, 0, 0);
+            acc[a][0] = re;
+            acc[a][1] = im;
+        }
+    }
+    __builtin_amdgcn_s_barrier();  // every wave's operand reads waited for above
+
+    // slot ps = 32 i + j: bins at 64 ps, unit u = f >> 1 at ((u ^ sw(ps)) << 4)
+    const unsigned eb = lds_offset(smem_raw);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int ti = a == 2 ? 1 : 0, tj = a == 0 ? 0 : 1;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            const int i = 16 * ti + 4 * q + rr, j = 16 * tj + li, ps = 32 * i + j;
+            const int sw = ((ps >> 1) ^ (ps >> 2)) & 3;
+            const float2 v{acc[a][0][rr] * oscale, acc[a][1][rr] * oscale};
+            asm volatile("ds_write_b64 %0, %1" ::"v"(eb + ps * 64 + ((((wave >> 1) ^ sw) << 4) | ((wave & 1) << 3))),
+                         "v"(v)
+                         : "memory");
+        }
+    }
+    lds_barrier();
+    constexpr int kE = 32 * 32 * 4 / kGThreads;
+    f4v ev[kE];
+#pragma unroll
+    for (int u = 0; u < kE; ++u) {
+        const int e = t + u * kGThreads, ps = e >> 2, qq = e & 3;
+        const int sw = ((ps >> 1) ^ (ps >> 2)) & 3;
+        ev[u] = *reinterpret_cast<const f4v *>(smem_raw + ps * 64 + ((qq ^ sw) << 4));
+    }
+#pragma unroll
+    for (int u = 0; u < kE; ++u) {
+        const int e = t + u * kGThreads, ps = e >> 2, qq = e & 3;
+        const int i = ps >> 5, j = ps & 31;
+        if (i < j && j < S) {
+            const long long p = (long long)i * S - (long long)i * (i + 1) / 2 + (j - i - 1);
+            *reinterpret_cast<f4v *>(G + gaddr(gl, p, k0 + 2 * qq)) = ev[u];
+        }
     }
 }
 
@@ -1076,6 +1225,22 @@ static int launch_gram_dma(bool f32, const void *d_Ua, const void *d_Ub, int S, 
     const int nchunks = (nk + kGF - 1) / kGF;
     if (!layout_ok(gl, S, nchunks)) return -1;
     if ((long long)S * K * 8 * kGF >= 0x80000000LL) return -1;  // chunk offsets stay below the pad offset
+    if (S <= 32 && d_Ua == d_Ub) {
+        // one diagonal block: every wave on a bin of its own
+        const long long slots = (long long)(nchunks + 15) / 16 * 2;
+        const size_t lds = std::max((size_t)32 * 32 * 64, (size_t)kGF * ((dma_row_units(K) + 1) / 2) * 1024);
+        const void *fn = f32 ? (const void *)k_gram_small<true> : (const void *)k_gram_small<false>;
+        allow_lds(fn, lds);
+        ProfScope prof(f32 ? "gram_f32" : "gram_split16", st);
+        if (f32)
+            hipLaunchKernelGGL(k_gram_small<true>, dim3((unsigned)(slots * 8)), dim3(kGThreads), lds, st,
+                               (const uint2 *)d_Ua, S, K, nk, nchunks, oscale, d_G, gl);
+        else
+            hipLaunchKernelGGL(k_gram_small<false>, dim3((unsigned)(slots * 8)), dim3(kGThreads), lds, st,
+                               (const uint2 *)d_Ua, S, K, nk, nchunks, oscale, d_G, gl);
+        FPCC_CHECK(hipGetLastError());
+        return 0;
+    }
     const long long slots = (long long)(nchunks + 15) / 16 * 2 * ntiles;
     if (slots * 8 > 0x7fffffffLL) return -1;
     const size_t bufb = (size_t)2 * kGB * dma_row_units(K) * 16;

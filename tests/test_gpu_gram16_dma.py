@@ -58,3 +58,19 @@ def test_dma_gram16_headline_shape(fp, orc):
     e64 = _check_ranges(orc, "pcc2", x, W, N, lag, y16, ranges)
     print(f"headline shape: max|gram16-gram32| over {P} pairs={e32:.3e}, max|gram16-fp64| (48 pairs)={e64:.3e}")
     assert e32 <= TOL16 and e64 <= TOL16
+
+
+@pytest.mark.parametrize("stack", ["gram", "gram16"])
+@pytest.mark.parametrize("S,W", [(2, 8), (5, 24), (17, 8), (32, 24), (33, 24)])
+def test_small_network_gram_matches_fp64(fp, orc, stack, S, W):
+    """Networks of at most 32 stations take the Gram kernel that gives each of its
+    eight waves a bin of its own (33 stations, the 64 x 64 block kernel): every
+    pair against the FP64 oracle, FP32 and split-FP16 operands."""
+    N, lag = 1000, 200
+    print(f"{stack} S={S} W={W} N={N} lags +-{lag}", flush=True)
+    x, _ = synth.make_stations(S, W * N, lag // 2, seed=S * 7 + W)
+    y = fp.allpairs(x, W, N, -lag, lag, kind="pcc2", stack=stack)
+    y64 = orc.allpairs("pcc2", x, W, N, -lag, lag, stacked=True, f64=True)
+    err = float(np.abs(y - y64).max())
+    print(f"  max|gpu-fp64| over all {S * (S - 1) // 2} pairs = {err:.3e}")
+    assert err <= (TOL16 if stack == "gram16" else 1e-5)
