@@ -228,6 +228,12 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
             a.W = jb.W[j];
             a.N = jb.N[j];
             a.dst = (cf32 *)jb.X[j];
+        } else if (a.op == COL_FWD_AMPNORMF) {  // job j's analytic rows (Xin, N long) into its rows X
+            a.src = (const cf32 *)jb.Xin[j];
+            a.src_bstride = jb.N[j];
+            a.nvalid = jb.N[j];
+            a.maxbuf = jb.mx[j];
+            a.dst = (cf32 *)jb.X[j];
         } else {  // COL_INV_MAX, COL_FWD_AMPNORM: job j's rows and maxima
             a.src = a.dst = (cf32 *)jb.X[j];
             a.maxbuf = jb.mx[j];
@@ -772,7 +778,8 @@ int run_cols_batch(const BigPlan &bp, const ColArgs &a, const RealBatch &jb, hip
     const int nbatch = jb.b0[jb.n];
     if (nbatch <= 0) return 0;
     if (!bp.ok || nbatch > 65535) return -1;
-    if (a.op != COL_FWD_INPUT && a.op != COL_INV_MAX && a.op != COL_FWD_AMPNORM) return -1;
+    if (a.op != COL_FWD_INPUT && a.op != COL_INV_MAX && a.op != COL_FWD_AMPNORM && a.op != COL_FWD_AMPNORMF)
+        return -1;
     return launch_cols<true>(bp, a, jb, nbatch, st, name);
 }
 

@@ -919,8 +919,9 @@ bool batch_key(const fpcc_job *job, const Dims &d, long long &key)
         return true;
     }
     if (job->kind == FPCC_PCC2) {
-        // the in-LDS station stage of any window length, except the specialised one
-        if (d.large || station_pcc_specialised(d.N, (int)d.Nz)) return false;
+        // the in-LDS station stage of any window length, except the specialised one;
+        // long windows of one Nz share their zero-padded transforms
+        if (!d.large && station_pcc_specialised(d.N, (int)d.Nz)) return false;
     } else if (job->kind != FPCC_GNCC && job->kind != FPCC_CC1B) {
         return false;
     }
@@ -959,6 +960,22 @@ int station_stage_group(const std::vector<const fpcc_job *> &jobs, const std::ve
             wsp[g] = ws[g] + layout(jobs[g], D[g]).a;
         }
         return wpcc_station_stage_batch(n, x.data(), stride.data(), S.data(), W.data(), N.data(), *D[0].fam,
+                                        wsp.data(), st);
+    }
+    if (kind == FPCC_PCC2 && large) {
+        if (jb.b0[n] > 65535) return -1;
+        std::vector<long long> stride(n);
+        std::vector<int> S(n), W(n), N(n), stack(n);
+        std::vector<char *> wsp(n);
+        for (int g = 0; g < n; ++g) {
+            stride[g] = jb.stride[g];
+            S[g] = D[g].S;
+            W[g] = D[g].W;
+            N[g] = D[g].N;
+            stack[g] = jobs[g]->stack;  // the layout large_station_stage uses
+            wsp[g] = ws[g] + layout(jobs[g], D[g]).a;
+        }
+        return large_pcc2_station_batch(n, x.data(), stride.data(), S.data(), W.data(), N.data(), stack.data(), Nz,
                                         wsp.data(), st);
     }
     if (kind == FPCC_PCC2) {

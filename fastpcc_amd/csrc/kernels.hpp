@@ -140,6 +140,12 @@ bool large_supported(int kind, int N, int Nz);
 size_t large_workspace(int kind, int stack, int S, int W, int N, int Nz);
 int large_station_stage(int kind, const float *x, long long stride, int S, int W, int N, int Nz, int stack,
                         char *ws, hipStream_t st);
+// The long-window PCC2 station stage of n jobs of one Nz (window lengths, station
+// and window counts and strides may differ): each job's analytic signal on its own
+// plan, then AmpNormf and the zero-padded forward transform of all their
+// station-windows as shared launches.
+int large_pcc2_station_batch(int n, const float *const *x, const long long *stride, const int *S, const int *W,
+                             const int *N, const int *stack, int Nz, char *const *ws, hipStream_t st);
 int large_pair_stage(int kind, int stack, int S, int W, int N, int Nz, int lag0, int L, double v, char *ws, float *y,
                      hipStream_t st);
 // Workspace regions holding the station stage's per-station output (what the pair

@@ -329,6 +329,39 @@ int large_station_stage(int kind, const float *x, long long stride, int S, int W
     return -1;
 }
 
+int large_pcc2_station_batch(int n, const float *const *x, const long long *stride, const int *S, const int *W,
+                             const int *N, const int *stack, int Nz, char *const *ws, hipStream_t st)
+{
+    if (n < 1 || n > kMaxBatchJobs) return -1;
+    RealBatch jb{};
+    jb.n = n;
+    jb.b0[0] = 0;
+    int r;
+    for (int g = 0; g < n; ++g) {
+        // each job's analytic signal on the plan of its own window length
+        const LargeLayout l = large_layout(FPCC_PCC2, stack[g], S[g], W[g], N[g], Nz);
+        cf32 *Z = (cf32 *)(ws[g] + l.Z);
+        float *mx = (float *)(ws[g] + l.mx);
+        if ((r = analytic_big(x[g], stride[g], S[g], W[g], N[g], Z, mx, (cf32 *)(ws[g] + l.B), st))) return r;
+        jb.Xin[g] = Z;
+        jb.N[g] = N[g];
+        jb.mx[g] = mx;
+        jb.X[g] = ws[g] + l.F;
+        jb.b0[g + 1] = jb.b0[g] + S[g] * W[g];
+    }
+    if (jb.b0[n] > 65535) return -1;
+    // AmpNormf and the zero-padded forward transform of every job's station-windows
+    const BigPlan &bz = big_plan(Nz);
+    ColArgs ca;
+    ca.op = COL_FWD_AMPNORMF;
+    ca.bstride = Nz;
+    if ((r = run_cols_batch(bz, ca, jb, st, "pcc2_pad_cols_batch"))) return r;
+    RowArgs ra;
+    ra.op = ROW_FWD_PLAIN;
+    ra.bstride = Nz;
+    return run_rows_batch(bz, ra, jb, st, "pcc2_pad_rows_batch");
+}
+
 int large_pair_stage(int kind, int stack, int S, int W, int N, int Nz, int lag0, int L, double v, char *ws, float *y,
                      hipStream_t st)
 {

@@ -210,9 +210,11 @@ int fpcc_pair_stage_device(const fpcc_job *job, void *d_ws, size_t ws_bytes, flo
  * (d_x[j], d_ws[j], ws_bytes[j]).  Jobs of uneven station count, window count,
  * window length and stride share kernel launches when their transforms agree
  * (GNCC and 1-bit GNCC of equal Nz; PCC2 of equal Nz <= 8192, each job with its
- * own window-length FFT; WPCC whose families share the filter bank: wavelet type,
- * Nz, scales, voices, first scale and w0; up to 16 jobs per launch); the others
- * run one by one.  The results equal fpcc_station_stage_device per job. */
+ * own window-length FFT; PCC2 of equal Nz > 8192, whose AmpNormf and zero-padded
+ * transforms share launches after each job's analytic signal on its own plan;
+ * WPCC whose families share the filter bank: wavelet type, Nz, scales, voices,
+ * first scale and w0; up to 16 jobs per launch); the others run one by one.  The
+ * results equal fpcc_station_stage_device per job. */
 int fpcc_station_stage_batch_device(const fpcc_job *jobs, int njobs, const float *const *d_x, void *const *d_ws,
                                     const size_t *ws_bytes, void *stream);
 
@@ -299,8 +301,8 @@ int fpcc_graph_destroy(fpcc_graph *graph);
  *
  * Uneven jobs in one launch: when requests whose station stages can share
  * kernels (fpcc_station_stage_batch_device: GNCC or 1-bit of one zero-pad length,
- * PCC2 of one zero-pad length <= 8192, WPCC of one filter bank; without station
- * keys) are next in line together, they are admitted as a group and their
+ * PCC2 of one zero-pad length, WPCC of one filter bank; without station keys) are
+ * next in line together, they are admitted as a group and their
  * station stages run as one set of launches.
  *
  * Return codes: 0 ok; -1 invalid argument or GPU failure; -3 |lag| >= N;

@@ -1,8 +1,8 @@
 """Uneven jobs in one launch: fpcc_station_stage_batch_device runs the station
-stages of GNCC, 1-bit and PCC2 jobs with different station counts, window
-counts, window lengths and strides (one Nz per group) as shared launches; each
-job's pair stage then gives exactly its standalone result.  Device memory comes from the
-HIP runtime the library itself links."""
+stages of GNCC, 1-bit and PCC2 jobs (long windows too) with different station
+counts, window counts, window lengths and strides (one Nz per group) as shared
+launches; each job's pair stage then gives exactly its standalone result.
+Device memory comes from the HIP runtime the library itself links."""
 import ctypes
 
 import numpy as np
@@ -23,6 +23,10 @@ GROUPS = [
      ("pcc2", 4, 3, 2800, 1000, "gram16")],
     # PCC2, Nz 8192: the specialised 3600-sample job runs alone, the others share a launch
     [("pcc2", 4, 2, 3600, 1000, "none"), ("pcc2", 6, 1, 6000, 2000, "plain"), ("pcc2", 3, 2, 5000, 1500, "zsum")],
+    # long-window PCC2, Nz 32768: analytic signals on four plans (10007 samples: Bluestein), the zero-padded
+    # transforms shared
+    [("pcc2", 4, 1, 9000, 2000, "none"), ("pcc2", 3, 2, 12000, 3000, "plain"), ("pcc2", 5, 1, 10007, 1000, "none"),
+     ("pcc2", 2, 2, 16000, 500, "zsum")],
 ]
 
 
