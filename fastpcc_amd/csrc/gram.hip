@@ -12,7 +12,8 @@
 //   Re G = sum_m ur_i ur_j + ui_i ui_j,   Im G = sum_m ur_i ui_j - ui_i ur_j.
 // Stages:
 //   k_spectra_to_u   F[s*W+w][k] -> U[k][s][m], k <= Nz/2  (LDS-tiled transpose)
-//   k_gram, k_gram16_dma  U -> G[p][k] (pair-major rows of ld >= Nz/2+1 bins)
+//   k_gram, k_gram16_dma, k_gram_small  U -> G[p][k] (pair-major rows of ld >= Nz/2+1
+//                    bins, or runs of 128 bins)
 //   k_pair_c2r       G[p] -> half-length complex IFFT -> lag window -> y[p][L]
 #include <algorithm>
 #include <atomic>
@@ -153,7 +154,7 @@ k_spectra_to_u(const cf32 *__restrict__ F, long long ldF, int SW, int W, int Nz,
 // LDS (double-buffered, register prefetch of the next step); wave (ti, cj)
 // owns the 16 x 32 sub-block rows 16ti.., columns 32cj.. and keeps its
 // accumulators for all kGF frequencies in registers, so each pair's kGF bins
-// leave as one 64-byte segment (staged through LDS in two halves).
+// leave as one 64-byte segment (staged through LDS in quarters of the block).
 // SPLIT = 0: FP32 MFMA 16x16x4, A[l%16][l/16], B[l/16][l%16], D[4(l/16)+r][l%16];
 //   K index (m, component), lane quarter q holds m = 8mm + 2q + e (e = 0, 1).
 // SPLIT = 1: U holds sigma-scaled FP16 hi and lo parts; FP16 MFMA 16x16x32 with
