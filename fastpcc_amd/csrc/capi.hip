@@ -896,6 +896,80 @@ extern "C" int fpcc_pair_stage_device(const fpcc_job *job, void *d_ws, size_t ws
 
 namespace {
 
+bool pair_batch_mode(const fpcc_job *job, const Dims &d, int &mode)
+{
+    if (job->kind != FPCC_PCC2 || d.large || d.Nz != 8192) return false;
+    switch (job->stack) {
+    case FPCC_STACK_NONE: mode = 0; return true;
+    case FPCC_STACK_PLAIN: mode = 1; return true;
+    case FPCC_STACK_ZSUM: mode = 2; return true;
+    default: return false;
+    }
+}
+
+}  // namespace
+
+namespace fpcc {
+bool job_pair_batch_mode(const fpcc_job *job, int *mode)
+{
+    Dims d;
+    int m = 0;
+    if (resolve(job, d) || !pair_batch_mode(job, d, m)) return false;
+    if (mode) *mode = m;
+    return true;
+}
+}  // namespace fpcc
+
+extern "C" int fpcc_pair_stage_batch_device(const fpcc_job *jobs, int njobs, void *const *d_ws,
+                                            const size_t *ws_bytes, float *const *d_y, void *stream)
+{
+    if (njobs < 0 || (njobs > 0 && (!jobs || !d_ws || !ws_bytes || !d_y))) return -1;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<Dims> D(njobs);
+    std::vector<int> mode(njobs, -1);
+    for (int j = 0; j < njobs; ++j) {
+        int r = resolve(&jobs[j], D[j]);
+        if (r) return r;
+        if (!d_ws[j] || !d_y[j] || ws_bytes[j] < layout(&jobs[j], D[j]).total) return -1;
+        int m;
+        if (pair_batch_mode(&jobs[j], D[j], m)) mode[j] = m;
+    }
+    std::vector<char> done(njobs, 0);
+    for (int i = 0; i < njobs; ++i) {
+        if (done[i]) continue;
+        std::vector<int> grp{i};
+        if (mode[i] >= 0)
+            for (int j = i + 1; j < njobs && (int)grp.size() < kMaxBatchJobs; ++j)
+                if (!done[j] && mode[j] == mode[i]) grp.push_back(j);
+        for (int g : grp) done[g] = 1;
+        int r;
+        if (grp.size() == 1) {
+            r = pair_stage(&jobs[i], D[i], (char *)d_ws[i], d_y[i], st);
+        } else {
+            const int n = (int)grp.size();
+            std::vector<const cf32 *> F(n);
+            std::vector<int> S(n), W(n), N(n), lag0(n), L(n);
+            std::vector<float *> y(n);
+            for (int k = 0; k < n; ++k) {
+                const int g = grp[k];
+                F[k] = (const cf32 *)((char *)d_ws[g] + layout(&jobs[g], D[g]).a);
+                S[k] = D[g].S;
+                W[k] = D[g].W;
+                N[k] = D[g].N;
+                lag0[k] = D[g].lag0;
+                L[k] = D[g].L;
+                y[k] = d_y[g];
+            }
+            r = launch_pair_pcc2_4096_batch(n, F.data(), S.data(), W.data(), N.data(), lag0.data(), L.data(),
+                                            mode[i], y.data(), st);
+        }
+        if (r) return r;
+    }
+    return 0;
+}
+
+namespace {
+
 // Jobs whose station stages can share launches: GNCC and 1-bit (real-input
 // transforms of Nz points; any station count, window count, window length and
 // stride), grouped by kind, path and Nz.

@@ -958,11 +958,11 @@ __device__ __forceinline__ cf32 ld_bin(__amdgpu_buffer_rsrc_t r, unsigned off)
     return __builtin_bit_cast(cf32, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
 
-template <int NT, int LPT, int MODE, bool ENDS>
+template <int NT, int LPT, int MODE, bool ENDS, bool BATCH>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
 k_pair_plain_4096(const cf32 *__restrict__ F, int S, int W, int P, const int *__restrict__ order,
                   const cf32 *__restrict__ twM, const cf32 *__restrict__ twc, int lag0, int L, float scale,
-                  float *__restrict__ y)
+                  float *__restrict__ y, PlainBatch jb)
 {
     constexpr int M = 4096, Nz = 8192;
     constexpr int H = M / 2 / NT;  // bins k < M/2 a thread
@@ -974,9 +974,23 @@ k_pair_plain_4096(const cf32 *__restrict__ F, int S, int W, int P, const int *__
     cf32 sa[MODE == 2 ? H : 1], sc[MODE == 2 ? H : 1], sph = cf32{0.f, 0.f};
     float acc[LPT];
     for (int u = blockIdx.x; u < P; u += gridDim.x) {
+        int uj = u;
+        if (BATCH) {  // job j's pairs are u0[j] <= u < u0[j+1]: its spectra, order, lags and output
+            int j = 0;
+            while (j + 1 < jb.n && u >= jb.u0[j + 1]) ++j;
+            uj = u - jb.u0[j];
+            F = jb.F[j];
+            S = jb.S[j];
+            W = jb.W[j];
+            order = jb.order[j];
+            lag0 = jb.lag0[j];
+            L = jb.L[j];
+            scale = jb.scale[j];
+            y = jb.y[j];
+        }
         // pairs in tiles of stations (order[u]): the workgroups running together read
         // the spectra of a few tiles' stations, which stay in the caches
-        const int p = __builtin_amdgcn_readfirstlane(order[u]);
+        const int p = __builtin_amdgcn_readfirstlane(order[uj]);
         int i = 0, j = 0;
         pair_index(p, S, i, j);
         // uniform across the workgroup: scalar registers hold the row addresses
@@ -1323,7 +1337,7 @@ static int launch_plain_4096_t(const cf32 *d_F, int S, int W, int P, const cf32 
     const int *order = pair_tile_order(S, 16);
     if (!order) return -1;
     constexpr int kT = kC2rThreads;
-    const auto fn = k_pair_plain_4096<kT, LPT, MODE, ENDS>;
+    const auto fn = k_pair_plain_4096<kT, LPT, MODE, ENDS, false>;
     const size_t lds = sizeof(cf32) * 2 * 4096;
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -1338,7 +1352,8 @@ static int launch_plain_4096_t(const cf32 *d_F, int S, int W, int P, const cf32 
     }
     const int nwg = P < per_cu * ncu ? P : per_cu * ncu;
     ProfScope prof(name, st);
-    hipLaunchKernelGGL(fn, dim3(nwg), dim3(kT), lds, st, d_F, S, W, P, order, twr, twc, lag0, L, scale, d_y);
+    hipLaunchKernelGGL(fn, dim3(nwg), dim3(kT), lds, st, d_F, S, W, P, order, twr, twc, lag0, L, scale, d_y,
+                       PlainBatch{});
     FPCC_CHECK(hipGetLastError());
     return 0;
 }
@@ -1381,6 +1396,97 @@ int launch_pair_pcc2_4096(const cf32 *d_F, int S, int W, int N, int lag0, int L,
     case 0: return launch_plain_4096_m<0>(lpt, d_F, S, W, (int)P, twr, tc, lag0, L, scale, d_y, st, "pair_pcc2_none");
     case 1: return launch_plain_4096_m<1>(lpt, d_F, S, W, (int)P, twr, tc, lag0, L, scale, d_y, st, "pair_pcc2_plain");
     default: return launch_plain_4096_m<2>(lpt, d_F, S, W, (int)P, twr, tc, lag0, L, scale, d_y, st, "pair_pcc2_zsum");
+    }
+}
+
+// The per-window PCC2 pair stages of several jobs at Nz 8192 (one stack mode) in
+// one launch of the persistent pair kernel: a job table of spectra, station and
+// window counts, lag windows and outputs, the pairs of all jobs walked as one
+// sequence (each job's in its own tile order).
+template <int LPT, int MODE, bool ENDS>
+static int launch_plain_4096_batch_t(const PlainBatch &jb, int P, const cf32 *twr, const cf32 *twc, hipStream_t st)
+{
+    constexpr int kT = kC2rThreads;
+    const auto fn = k_pair_plain_4096<kT, LPT, MODE, ENDS, true>;
+    const size_t lds = sizeof(cf32) * 2 * 4096;
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        return -1;
+    static std::atomic<int> per_cu_cached{0};
+    int per_cu = per_cu_cached.load(std::memory_order_relaxed);
+    if (!per_cu) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, kT, lds) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        per_cu_cached.store(per_cu, std::memory_order_relaxed);
+    }
+    const int nwg = P < per_cu * ncu ? P : per_cu * ncu;
+    ProfScope prof(MODE == 0 ? "pair_pcc2_none_batch" : MODE == 1 ? "pair_pcc2_plain_batch" : "pair_pcc2_zsum_batch",
+                   st);
+    hipLaunchKernelGGL(fn, dim3(nwg), dim3(kT), lds, st, (const cf32 *)nullptr, 0, 0, P, (const int *)nullptr, twr,
+                       twc, 0, 0, 0.f, (float *)nullptr, jb);
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <int MODE>
+static int launch_plain_4096_batch_m(int lpt, bool ends, const PlainBatch &jb, int P, const cf32 *twr,
+                                     const cf32 *twc, hipStream_t st)
+{
+#define FPCC_PLAINB(LPT, E) launch_plain_4096_batch_t<LPT, MODE, E>(jb, P, twr, twc, st)
+    switch (lpt) {
+    case 1: return ends ? FPCC_PLAINB(1, true) : FPCC_PLAINB(1, false);
+    case 2: return ends ? FPCC_PLAINB(2, true) : FPCC_PLAINB(2, false);
+    case 4: return ends ? FPCC_PLAINB(4, true) : FPCC_PLAINB(4, false);
+    case 8: return FPCC_PLAINB(8, false);
+    case 16: return FPCC_PLAINB(16, false);
+    }
+#undef FPCC_PLAINB
+    return -1;
+}
+
+int launch_pair_pcc2_4096_batch(int n, const cf32 *const *d_F, const int *S, const int *W, const int *N,
+                                const int *lag0, const int *L, int mode, float *const *d_y, hipStream_t st)
+{
+    constexpr int Nz = 8192;
+    if (n < 1 || n > kMaxBatchJobs || mode < 0 || mode > 2) return -1;
+    const void *twc = get_c2r_twiddles(Nz, false);
+    const cf32 *twr = (const cf32 *)get_order_twiddles(4096, {8, 8, 8, 8});
+    if (!twc || !twr) return -1;
+    PlainBatch jb{};
+    jb.n = n;
+    long long P = 0;
+    int Lmax = 1;
+    bool ends = true;
+    for (int j = 0; j < n; ++j) {
+        const long long Pj = (long long)S[j] * (S[j] - 1) / 2;
+        if (Pj < 1 || W[j] < 1 || L[j] < 1 || L[j] > Nz) return -1;
+        jb.order[j] = pair_tile_order(S[j], 16);
+        if (!jb.order[j]) return -1;
+        jb.u0[j] = (int)P;
+        P += Pj;
+        jb.F[j] = d_F[j];
+        jb.S[j] = S[j];
+        jb.W[j] = W[j];
+        jb.lag0[j] = lag0[j];
+        jb.L[j] = L[j];
+        // as launch_pair_pcc2_4096: 1/2 times 1/((float)Nz*(float)N), mode 2 also 1/W
+        float sc = 0.5f * (1 / ((float)Nz * (float)N[j]));
+        if (mode == 2) sc /= (float)W[j];
+        jb.scale[j] = sc;
+        jb.y[j] = d_y[j];
+        Lmax = std::max(Lmax, L[j]);
+        ends = ends && lag0[j] >= -1024 && lag0[j] + L[j] - 1 <= 1023;
+    }
+    if (P > 0x7fffffff) return -1;
+    jb.u0[n] = (int)P;
+    int lpt = 1;
+    while (lpt * kC2rThreads < Lmax) lpt *= 2;
+    const cf32 *tc = (const cf32 *)twc;
+    switch (mode) {
+    case 0: return launch_plain_4096_batch_m<0>(lpt, ends, jb, (int)P, twr, tc, st);
+    case 1: return launch_plain_4096_batch_m<1>(lpt, ends, jb, (int)P, twr, tc, st);
+    default: return launch_plain_4096_batch_m<2>(lpt, ends, jb, (int)P, twr, tc, st);
     }
 }
 

@@ -116,6 +116,23 @@ struct PccBatch {
     float da1[kMaxBatchJobs];
     cf32 *F[kMaxBatchJobs];
 };
+// Job table of a per-window PCC2 pair-stage launch over several jobs at Nz 8192
+// (gram.hip): job j's pairs are u0[j] <= u < u0[j+1] of the launch's sequence.
+struct PlainBatch {
+    int n;
+    int u0[kMaxBatchJobs + 1];
+    const cf32 *F[kMaxBatchJobs];
+    const int *order[kMaxBatchJobs];
+    int S[kMaxBatchJobs], W[kMaxBatchJobs], lag0[kMaxBatchJobs], L[kMaxBatchJobs];
+    float scale[kMaxBatchJobs];
+    float *y[kMaxBatchJobs];
+};
+// The pair stages of n PCC2 jobs at Nz 8192 with one stack mode (0 none, 1 plain,
+// 2 zsum): spectra d_F[j] of S[j] stations x W[j] windows, window length N[j],
+// lags lag0[j] .. lag0[j] + L[j] - 1, output d_y[j]; the same results as
+// launch_pair_pcc2 per job.
+int launch_pair_pcc2_4096_batch(int n, const cf32 *const *d_F, const int *S, const int *W, const int *N,
+                                const int *lag0, const int *L, int mode, float *const *d_y, hipStream_t st);
 // True when (N, Nz) runs the compile-time specialised station kernel (those
 // jobs are not batched: the batch kernel is the generic one).
 bool station_pcc_specialised(int N, int Nz);
@@ -183,5 +200,9 @@ int job_station_regions(const fpcc_job *job, size_t *off, size_t *per);
 // Whether the job's station stage can share launches with other jobs of the same
 // key (fpcc_station_stage_batch_device).
 bool job_batch_key(const fpcc_job *job, long long *key);
+// Whether the job's pair stage can share a launch with other jobs of the same mode
+// (fpcc_pair_stage_batch_device: PCC2 windows in LDS at Nz 8192, stacks none,
+// plain and zsum); mode 0, 1 or 2.
+bool job_pair_batch_mode(const fpcc_job *job, int *mode);
 
 }  // namespace fpcc

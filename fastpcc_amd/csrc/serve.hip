@@ -340,7 +340,8 @@ int issue(Server *s, Request &r)
 
 // A group of uncached requests whose station stages share launches: inputs on
 // their own streams, the grouped station stage on the first one's stream (joined
-// by events both ways), then each pair stage and output copy on its own stream.
+// by events both ways), the pair stages that share launches on that stream too,
+// then every other pair stage and each output copy on its request's own stream.
 int issue_group(Server *s, std::vector<Request *> &g)
 {
     (void)hipSetDevice(s->device);
@@ -378,14 +379,37 @@ int issue_group(Server *s, std::vector<Request *> &g)
         wsb.push_back(r->ws_bytes);
     }
     int rc = fpcc_station_stage_batch_device(jobs.data(), (int)jobs.size(), xs.data(), wss.data(), wsb.data(), bst);
-    hipEvent_t done;
+    // pair stages that can share launches (fpcc_pair_stage_batch_device) run together
+    // on the group's stream, the others each on its own
+    std::vector<char> shared(g.size(), 0);
+    std::vector<fpcc_job> pj;
+    std::vector<void *> pw;
+    std::vector<size_t> pb;
+    std::vector<float *> py;
+    for (size_t k = 0; k < g.size(); ++k) {
+        int m;
+        if (fpcc::job_pair_batch_mode(&g[k]->dev_job, &m)) {
+            shared[k] = 1;
+            pj.push_back(g[k]->dev_job);
+            pw.push_back(dbase + g[k]->off_ws);
+            pb.push_back(g[k]->ws_bytes);
+            py.push_back((float *)(dbase + g[k]->off_y));
+        }
+    }
+    if (pj.size() < 2) std::fill(shared.begin(), shared.end(), 0);
+    hipEvent_t done, pdone = nullptr;  // station stages done; shared pair stages done
     if (!rc && (!event(done) || hipEventRecord(done, bst) != hipSuccess)) rc = -1;
-    for (Request *r : g) {
+    if (!rc && pj.size() >= 2) {
+        rc = fpcc_pair_stage_batch_device(pj.data(), (int)pj.size(), pw.data(), pb.data(), py.data(), bst);
+        if (!rc && (!event(pdone) || hipEventRecord(pdone, bst) != hipSuccess)) rc = -1;
+    }
+    for (size_t k = 0; k < g.size(); ++k) {
+        Request *r = g[k];
         if (rc) break;
         hipStream_t st = s->streams[r->stream];
-        if (r != g[0] && hipStreamWaitEvent(st, done, 0) != hipSuccess) rc = -1;
-        if (!rc) rc = fpcc_pair_stage_device(&r->dev_job, dbase + r->off_ws, r->ws_bytes, (float *)(dbase + r->off_y),
-                                             st);
+        if (r != g[0] && hipStreamWaitEvent(st, shared[k] ? pdone : done, 0) != hipSuccess) rc = -1;
+        if (!rc && !shared[k])
+            rc = fpcc_pair_stage_device(&r->dev_job, dbase + r->off_ws, r->ws_bytes, (float *)(dbase + r->off_y), st);
         if (!rc) rc = copy_out(s, *r);
     }
     cleanup();

@@ -217,6 +217,14 @@ int fpcc_pair_stage_device(const fpcc_job *job, void *d_ws, size_t ws_bytes, flo
  * results equal fpcc_station_stage_device per job. */
 int fpcc_station_stage_batch_device(const fpcc_job *jobs, int njobs, const float *const *d_x, void *const *d_ws,
                                     const size_t *ws_bytes, void *stream);
+/* Pair stages of njobs jobs on one stream, after their station stages, each from
+ * its own workspace into its own output (d_ws[j], ws_bytes[j], d_y[j]).  PCC2 jobs
+ * whose windows fit one workgroup at Nz 8192 (2048 < N <= 4096) with stacks NONE,
+ * PLAIN or ZSUM share one launch of the persistent pair kernel per stack (up to
+ * 16 jobs of uneven station count, window count, window length and lags); the
+ * others run one by one.  The results equal fpcc_pair_stage_device per job. */
+int fpcc_pair_stage_batch_device(const fpcc_job *jobs, int njobs, void *const *d_ws, const size_t *ws_bytes,
+                                 float *const *d_y, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Per-station preprocessing (the steps src/PCC_fullpair_1b.c runs on each
@@ -303,7 +311,8 @@ int fpcc_graph_destroy(fpcc_graph *graph);
  * kernels (fpcc_station_stage_batch_device: GNCC or 1-bit of one zero-pad length,
  * PCC2 of one zero-pad length, WPCC of one filter bank; without station keys) are
  * next in line together, they are admitted as a group and their
- * station stages run as one set of launches.
+ * station stages run as one set of launches, their pair stages too where
+ * fpcc_pair_stage_batch_device shares them.
  *
  * Return codes: 0 ok; -1 invalid argument or GPU failure; -3 |lag| >= N;
  * -4 the request's footprint exceeds the server's arenas (never admissible);

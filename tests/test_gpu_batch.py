@@ -1,8 +1,9 @@
 """Uneven jobs in one launch: fpcc_station_stage_batch_device runs the station
 stages of GNCC, 1-bit and PCC2 jobs (long windows too) with different station
 counts, window counts, window lengths and strides (one Nz per group) as shared
-launches; each job's pair stage then gives exactly its standalone result.
-Device memory comes from the HIP runtime the library itself links."""
+launches; each job's pair stage then gives exactly its standalone result, also
+when fpcc_pair_stage_batch_device runs the pair stages of PCC2 jobs at Nz 8192 in
+shared launches.  Device memory comes from the HIP runtime the library itself links."""
 import ctypes
 
 import numpy as np
@@ -59,6 +60,41 @@ def test_station_stage_batch_exact(fp, hip, group):
             hip.d2h(y, dy)
             print(f"{specs[k]}: max|batch - alone| = {np.abs(y - refs[k]).max():.3e}")
             assert np.array_equal(y, refs[k])
+
+
+# PCC2 at Nz 8192 (2048 < N <= 4096): uneven stations, windows, window lengths and lags;
+# stacks none, plain and zsum share a launch per stack, gram and the others run alone
+PAIR_SPECS = [("pcc2", 5, 2, 3000, 800, "none"), ("pcc2", 9, 1, 2500, 900, "none"), ("pcc2", 4, 3, 3600, 1000, "plain"),
+              ("pcc2", 6, 2, 4000, 2000, "plain"), ("pcc2", 3, 2, 2100, 300, "zsum"), ("pcc2", 7, 1, 3300, 3000, "none"),
+              ("pcc2", 4, 2, 3600, 700, "gram16"), ("pcc2", 2, 4, 2900, 1000, "zsum")]
+
+
+def test_pair_stage_batch_exact(fp, hip):
+    """fpcc_pair_stage_batch_device: every job's output equals its standalone run,
+    bit for bit, whether its pair stage shared a launch or not."""
+    L = fp.api._lib.lib()
+    n = len(PAIR_SPECS)
+    jobs = (fp.api._lib.FpccJob * n)()
+    dws, wsb, dys, refs = [], [], [], []
+    for k, (kind, S, W, N, lag, stack) in enumerate(PAIR_SPECS):
+        x, _ = synth.make_stations(S, W * N + 7 * k, 200, seed=110 + k)
+        job = fp.make_job(kind, S, W, N, -lag, lag, stack=stack, station_stride=x.shape[1])
+        jobs[k] = job
+        dx = hip.alloc(x.nbytes)
+        hip.h2d(dx, x)
+        wsb.append(L.fpcc_workspace_size(ctypes.byref(job)))
+        dws.append(hip.alloc(wsb[-1]))
+        assert L.fpcc_station_stage_device(ctypes.byref(jobs[k]), dx, dws[-1], wsb[-1], None) == 0
+        refs.append(fp.allpairs(x, W, N, -lag, lag, kind=kind, stack=stack))
+        dys.append(hip.alloc(refs[-1].nbytes))
+    r = L.fpcc_pair_stage_batch_device(jobs, n, (ctypes.c_void_p * n)(*dws), (ctypes.c_size_t * n)(*wsb),
+                                       (ctypes.c_void_p * n)(*dys), None)
+    assert r == 0
+    for k in range(n):
+        y = np.empty(refs[k].shape, np.float32)
+        hip.d2h(y, dys[k])
+        print(f"{PAIR_SPECS[k]}: max|batch - alone| = {np.abs(y - refs[k]).max():.3e}")
+        assert np.array_equal(y, refs[k])
 
 
 def test_server_groups_uneven_jobs(fp):
