@@ -896,15 +896,33 @@ extern "C" int fpcc_pair_stage_device(const fpcc_job *job, void *d_ws, size_t ws
 
 namespace {
 
+// Pair stages that can share a launch: PCC2 windows in LDS at Nz 8192 with stacks
+// none, plain and zsum (mode 0, 1, 2 of the persistent pair kernel); GNCC and 1-bit
+// windows in LDS with stacks none and plain (mode 0, 1 of their pair kernel).  Jobs
+// of one key (kind, Nz, mode) share.
 bool pair_batch_mode(const fpcc_job *job, const Dims &d, int &mode)
 {
-    if (job->kind != FPCC_PCC2 || d.large || d.Nz != 8192) return false;
-    switch (job->stack) {
-    case FPCC_STACK_NONE: mode = 0; return true;
-    case FPCC_STACK_PLAIN: mode = 1; return true;
-    case FPCC_STACK_ZSUM: mode = 2; return true;
-    default: return false;
+    if (d.large) return false;
+    if (job->kind == FPCC_PCC2) {
+        if (d.Nz != 8192) return false;
+        switch (job->stack) {
+        case FPCC_STACK_NONE: mode = 0; return true;
+        case FPCC_STACK_PLAIN: mode = 1; return true;
+        case FPCC_STACK_ZSUM: mode = 2; return true;
+        default: return false;
+        }
     }
+    if (job->kind == FPCC_GNCC || job->kind == FPCC_CC1B) {
+        if (job->stack != FPCC_STACK_NONE && job->stack != FPCC_STACK_PLAIN) return false;
+        mode = job->stack == FPCC_STACK_NONE ? 0 : 1;
+        return true;
+    }
+    return false;
+}
+
+long long pair_batch_key(const fpcc_job *job, const Dims &d, int mode)
+{
+    return ((long long)d.Nz << 16) | ((long long)job->kind << 4) | mode;
 }
 
 }  // namespace
@@ -927,24 +945,48 @@ extern "C" int fpcc_pair_stage_batch_device(const fpcc_job *jobs, int njobs, voi
     hipStream_t st = (hipStream_t)stream;
     std::vector<Dims> D(njobs);
     std::vector<int> mode(njobs, -1);
+    std::vector<long long> key(njobs, -1);
     for (int j = 0; j < njobs; ++j) {
         int r = resolve(&jobs[j], D[j]);
         if (r) return r;
         if (!d_ws[j] || !d_y[j] || ws_bytes[j] < layout(&jobs[j], D[j]).total) return -1;
         int m;
-        if (pair_batch_mode(&jobs[j], D[j], m)) mode[j] = m;
+        if (pair_batch_mode(&jobs[j], D[j], m)) {
+            mode[j] = m;
+            key[j] = pair_batch_key(&jobs[j], D[j], m);
+        }
     }
     std::vector<char> done(njobs, 0);
     for (int i = 0; i < njobs; ++i) {
         if (done[i]) continue;
         std::vector<int> grp{i};
-        if (mode[i] >= 0)
+        if (key[i] >= 0)
             for (int j = i + 1; j < njobs && (int)grp.size() < kMaxBatchJobs; ++j)
-                if (!done[j] && mode[j] == mode[i]) grp.push_back(j);
+                if (!done[j] && key[j] == key[i]) grp.push_back(j);
         for (int g : grp) done[g] = 1;
         int r;
         if (grp.size() == 1) {
             r = pair_stage(&jobs[i], D[i], (char *)d_ws[i], d_y[i], st);
+        } else if (jobs[i].kind != FPCC_PCC2) {
+            const bool gn = jobs[i].kind == FPCC_GNCC;
+            GnBatch jb{};
+            jb.n = (int)grp.size();
+            for (int k = 0; k < jb.n; ++k) {
+                const int g = grp[k];
+                const Layout l = layout(&jobs[g], D[g]);
+                char *ws = (char *)d_ws[g];
+                jb.Xa[k] = ws + l.a;
+                jb.Xb[k] = gn ? ws + l.a : ws + l.b;
+                jb.E[k] = gn ? (const double *)(ws + l.b) : nullptr;
+                jb.edge[k] = gn ? nullptr : (const float *)(ws + l.f);
+                jb.S[k] = D[g].S;
+                jb.W[k] = D[g].W;
+                jb.N[k] = D[g].N;
+                jb.lag0[k] = D[g].lag0;
+                jb.L[k] = D[g].L;
+                jb.y[k] = d_y[g];
+            }
+            r = launch_pair_gn_batch(jb, gn, (int)D[i].Nz, gn ? 0 : 1, mode[i], st);
         } else {
             const int n = (int)grp.size();
             std::vector<const cf32 *> F(n);

@@ -128,14 +128,34 @@ k_station_real_batch(RealBatch jb, int Nz, int sign_rule, FftPlan planM, const C
 // one padded LDS buffer (bin k at ct_pad(k); twM the twiddles of that radix
 // order): 69 KB instead of two 64 KB buffers in FP64, two workgroups a CU instead
 // of one, and the fold's loads unrolled; else the runtime plan with two buffers.
-template <typename T, bool CT4096>
+// BATCH: workgroups b0[jj] <= blockIdx.x < b0[jj+1] run job jj of the table (its
+// spectra, energies, counts, lag window, output and window edges).
+template <typename T, bool CT4096, bool BATCH>
 __global__ void __launch_bounds__(512)
 k_pair_gn(const C2<T> *__restrict__ Xa, const C2<T> *__restrict__ Xb, long long ldx,
           const double *__restrict__ E, long long lde, int S, int W, int N, int Nz, FftPlan planM,
           const C2<T> *__restrict__ twM, const C2<T> *__restrict__ twc, int lag0, int L, int norm, int mode,
-          float *__restrict__ y, const float *__restrict__ edge, int K)
+          float *__restrict__ y, const float *__restrict__ edge, int K, GnBatch jb)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    int bid = blockIdx.x;
+    if (BATCH) {
+        int jj = 0;
+        while (jj + 1 < jb.n && bid >= jb.b0[jj + 1]) ++jj;
+        bid -= jb.b0[jj];
+        Xa = (const C2<T> *)jb.Xa[jj];
+        Xb = (const C2<T> *)jb.Xb[jj];
+        E = jb.E[jj];
+        lde = jb.lde[jj];
+        S = jb.S[jj];
+        W = jb.W[jj];
+        N = jb.N[jj];
+        lag0 = jb.lag0[jj];
+        L = jb.L[jj];
+        y = jb.y[jj];
+        edge = jb.edge[jj];
+        K = jb.K[jj];
+    }
     const int M = CT4096 ? 4096 : Nz >> 1;
     C2<T> *buf0 = reinterpret_cast<C2<T> *>(smem_raw);
     C2<T> *buf1 = buf0 + M;
@@ -143,11 +163,11 @@ k_pair_gn(const C2<T> *__restrict__ Xa, const C2<T> *__restrict__ Xb, long long 
     auto zi = [&](int k) { return CT4096 ? ct_pad(k) : k; };
     int p, w0, w1;
     if (mode == 0) {
-        p = blockIdx.x / W;
-        w0 = blockIdx.x % W;
+        p = bid / W;
+        w0 = bid % W;
         w1 = w0 + 1;
     } else {
-        p = blockIdx.x;
+        p = bid;
         w0 = 0;
         w1 = W;
     }
@@ -304,17 +324,63 @@ int launch_pair_gn(const void *d_Xa, const void *d_Xb, const double *d_E, bool f
     }
     ProfScope prof(f64 ? "pair_gn_f64" : "pair_gn_f32", st);
     if (f64) {
-        const auto fn = ct ? k_pair_gn<double, true> : k_pair_gn<double, false>;
+        const auto fn = ct ? k_pair_gn<double, true, false> : k_pair_gn<double, false, false>;
         allow_lds((const void *)fn, lds);
         hipLaunchKernelGGL(fn, dim3(blocks), dim3(512), lds, st, (const cf64 *)d_Xa, (const cf64 *)d_Xb,
                            real_ld(Nz), d_E, energy_ld(N), S, W, N, Nz, pM.plan, (const cf64 *)tw,
-                           (const cf64 *)twc, lag0, L, norm, mode, d_y, d_edge, K);
+                           (const cf64 *)twc, lag0, L, norm, mode, d_y, d_edge, K, GnBatch{});
     } else {
-        const auto fn = ct ? k_pair_gn<float, true> : k_pair_gn<float, false>;
+        const auto fn = ct ? k_pair_gn<float, true, false> : k_pair_gn<float, false, false>;
         allow_lds((const void *)fn, lds);
         hipLaunchKernelGGL(fn, dim3(blocks), dim3(512), lds, st, (const cf32 *)d_Xa, (const cf32 *)d_Xb,
                            real_ld(Nz), d_E, energy_ld(N), S, W, N, Nz, pM.plan, (const cf32 *)tw,
-                           (const cf32 *)twc, lag0, L, norm, mode, d_y, d_edge, K);
+                           (const cf32 *)twc, lag0, L, norm, mode, d_y, d_edge, K, GnBatch{});
+    }
+    FPCC_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_pair_gn_batch(GnBatch jb, bool f64, int Nz, int norm, int mode, hipStream_t st)
+{
+    if (jb.n < 1 || jb.n > kMaxBatchJobs || mode < 0 || mode > 1) return -1;
+    const int M = Nz / 2;
+    const DevPlan &pM = get_plan(M, f64);
+    const void *twc = get_c2r_twiddles(Nz, f64);
+    if (!pM.ok || !twc) return -1;
+    int Lmax = 1;
+    long long blocks = 0;
+    for (int j = 0; j < jb.n; ++j) {
+        const long long P = (long long)jb.S[j] * (jb.S[j] - 1) / 2;
+        jb.b0[j] = (int)blocks;
+        blocks += mode == 0 ? P * jb.W[j] : P;
+        Lmax = std::max(Lmax, jb.L[j]);
+        jb.lde[j] = energy_ld(jb.N[j]);
+        jb.K[j] = jb.edge[j] ? edge_len(jb.N[j]) : 0;
+    }
+    if (blocks > 0x7fffffff) return -1;
+    jb.b0[jb.n] = (int)blocks;
+    if (!blocks) return 0;
+    const size_t elem = f64 ? sizeof(cf64) : sizeof(cf32);
+    const bool ct = M == 4096;
+    const void *tw = ct ? get_order_twiddles(M, {8, 8, 8, 8}, f64) : pM.tw;
+    if (!tw) return -1;
+    size_t lds = elem * (ct ? (size_t)ct_lds_elems<4096>() : 2 * (size_t)M) + sizeof(float) * (size_t)Lmax;
+    if (lds > 160 * 1024) return -1;
+    ProfScope prof(f64 ? "pair_gn_f64_batch" : "pair_gn_f32_batch", st);
+    if (f64) {
+        const auto fn = ct ? k_pair_gn<double, true, true> : k_pair_gn<double, false, true>;
+        allow_lds((const void *)fn, lds);
+        hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(512), lds, st, (const cf64 *)nullptr,
+                           (const cf64 *)nullptr, real_ld(Nz), (const double *)nullptr, 0LL, 0, 0, 0, Nz, pM.plan,
+                           (const cf64 *)tw, (const cf64 *)twc, 0, 0, norm, mode, (float *)nullptr,
+                           (const float *)nullptr, 0, jb);
+    } else {
+        const auto fn = ct ? k_pair_gn<float, true, true> : k_pair_gn<float, false, true>;
+        allow_lds((const void *)fn, lds);
+        hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(512), lds, st, (const cf32 *)nullptr,
+                           (const cf32 *)nullptr, real_ld(Nz), (const double *)nullptr, 0LL, 0, 0, 0, Nz, pM.plan,
+                           (const cf32 *)tw, (const cf32 *)twc, 0, 0, norm, mode, (float *)nullptr,
+                           (const float *)nullptr, 0, jb);
     }
     FPCC_CHECK(hipGetLastError());
     return 0;

@@ -151,6 +151,24 @@ int launch_station_real_batch(const RealBatch &jb, int Nz, int sign_rule, bool f
 // mode 0: y[p][w][L]; 1: mean over windows.
 int launch_pair_gn(const void *d_Xa, const void *d_Xb, const double *d_E, bool f64, int S, int W, int N, int Nz,
                    int lag0, int L, int norm, int mode, float *d_y, hipStream_t st, const float *d_edge = nullptr);
+// Job table of a GNCC / 1-bit pair-stage launch over several jobs of one Nz, norm
+// and mode: per job the spectra (Xa, Xb), energies, counts, lag window, output and
+// window edges; b0, lde and K are filled by launch_pair_gn_batch.
+struct GnBatch {
+    int n;
+    int b0[kMaxBatchJobs + 1];
+    const void *Xa[kMaxBatchJobs];
+    const void *Xb[kMaxBatchJobs];
+    const double *E[kMaxBatchJobs];
+    long long lde[kMaxBatchJobs];
+    int S[kMaxBatchJobs], W[kMaxBatchJobs], N[kMaxBatchJobs], lag0[kMaxBatchJobs], L[kMaxBatchJobs],
+        K[kMaxBatchJobs];
+    float *y[kMaxBatchJobs];
+    const float *edge[kMaxBatchJobs];
+};
+// The pair stages of the jobs of jb in one launch, the same results as
+// launch_pair_gn per job.
+int launch_pair_gn_batch(GnBatch jb, bool f64, int Nz, int norm, int mode, hipStream_t st);
 
 // Windows longer than one workgroup's LDS (large.hip).
 bool large_supported(int kind, int N, int Nz);

@@ -218,11 +218,14 @@ int fpcc_pair_stage_device(const fpcc_job *job, void *d_ws, size_t ws_bytes, flo
 int fpcc_station_stage_batch_device(const fpcc_job *jobs, int njobs, const float *const *d_x, void *const *d_ws,
                                     const size_t *ws_bytes, void *stream);
 /* Pair stages of njobs jobs on one stream, after their station stages, each from
- * its own workspace into its own output (d_ws[j], ws_bytes[j], d_y[j]).  PCC2 jobs
- * whose windows fit one workgroup at Nz 8192 (2048 < N <= 4096) with stacks NONE,
- * PLAIN or ZSUM share one launch of the persistent pair kernel per stack (up to
- * 16 jobs of uneven station count, window count, window length and lags); the
- * others run one by one.  The results equal fpcc_pair_stage_device per job. */
+ * its own workspace into its own output (d_ws[j], ws_bytes[j], d_y[j]).  Jobs of
+ * uneven station count, window count, window length and lags share launches (up
+ * to 16 a launch): PCC2 jobs whose windows fit one workgroup at Nz 8192
+ * (2048 < N <= 4096) with stacks NONE, PLAIN or ZSUM, one launch of the
+ * persistent pair kernel per stack; GNCC and 1-bit GNCC jobs whose windows fit one
+ * workgroup (Nz <= 8192) with stacks NONE or PLAIN, one launch per kind, Nz and
+ * stack.  The others run one by one.  The results equal fpcc_pair_stage_device
+ * per job. */
 int fpcc_pair_stage_batch_device(const fpcc_job *jobs, int njobs, void *const *d_ws, const size_t *ws_bytes,
                                  float *const *d_y, void *stream);
 

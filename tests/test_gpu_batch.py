@@ -2,8 +2,8 @@
 stages of GNCC, 1-bit and PCC2 jobs (long windows too) with different station
 counts, window counts, window lengths and strides (one Nz per group) as shared
 launches; each job's pair stage then gives exactly its standalone result, also
-when fpcc_pair_stage_batch_device runs the pair stages of PCC2 jobs at Nz 8192 in
-shared launches.  Device memory comes from the HIP runtime the library itself links."""
+when fpcc_pair_stage_batch_device runs the pair stages of PCC2 jobs at Nz 8192 and
+of GNCC and 1-bit jobs in shared launches.  Device memory comes from the HIP runtime the library itself links."""
 import ctypes
 
 import numpy as np
@@ -69,14 +69,23 @@ PAIR_SPECS = [("pcc2", 5, 2, 3000, 800, "none"), ("pcc2", 9, 1, 2500, 900, "none
               ("pcc2", 4, 2, 3600, 700, "gram16"), ("pcc2", 2, 4, 2900, 1000, "zsum")]
 
 
-def test_pair_stage_batch_exact(fp, hip):
+# GNCC and 1-bit: Nz 8192 (compile-time inverse) and 4096 (runtime plan), stacks none
+# and plain, uneven lags (1-bit up to the window edges, from the exact edge sums)
+GN_PAIR_SPECS = [("gncc", 5, 2, 3000, 800, "none"), ("gncc", 4, 1, 2500, 2400, "none"), ("gncc", 6, 2, 3900, 300, "plain"),
+                 ("gncc", 3, 3, 2200, 1000, "plain"), ("gncc", 4, 2, 1500, 700, "none"), ("gncc", 5, 1, 1200, 1100, "none"),
+                 ("cc1b", 4, 2, 3000, 2999, "none"), ("cc1b", 6, 1, 2600, 500, "none"), ("cc1b", 3, 2, 3500, 900, "plain"),
+                 ("cc1b", 5, 2, 2300, 1500, "plain"), ("gncc", 4, 1, 3000, 600, "pws")]
+
+
+@pytest.mark.parametrize("specs", [PAIR_SPECS, GN_PAIR_SPECS], ids=["pcc2", "gncc_1bit"])
+def test_pair_stage_batch_exact(fp, hip, specs):
     """fpcc_pair_stage_batch_device: every job's output equals its standalone run,
     bit for bit, whether its pair stage shared a launch or not."""
     L = fp.api._lib.lib()
-    n = len(PAIR_SPECS)
+    n = len(specs)
     jobs = (fp.api._lib.FpccJob * n)()
     dws, wsb, dys, refs = [], [], [], []
-    for k, (kind, S, W, N, lag, stack) in enumerate(PAIR_SPECS):
+    for k, (kind, S, W, N, lag, stack) in enumerate(specs):
         x, _ = synth.make_stations(S, W * N + 7 * k, 200, seed=110 + k)
         job = fp.make_job(kind, S, W, N, -lag, lag, stack=stack, station_stride=x.shape[1])
         jobs[k] = job
@@ -93,7 +102,7 @@ def test_pair_stage_batch_exact(fp, hip):
     for k in range(n):
         y = np.empty(refs[k].shape, np.float32)
         hip.d2h(y, dys[k])
-        print(f"{PAIR_SPECS[k]}: max|batch - alone| = {np.abs(y - refs[k]).max():.3e}")
+        print(f"{specs[k]}: max|batch - alone| = {np.abs(y - refs[k]).max():.3e}")
         assert np.array_equal(y, refs[k])
 
 
