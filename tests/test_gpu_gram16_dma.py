@@ -61,7 +61,7 @@ def test_dma_gram16_headline_shape(fp, orc):
 
 
 @pytest.mark.parametrize("stack", ["gram", "gram16"])
-@pytest.mark.parametrize("S,W", [(2, 8), (5, 24), (17, 8), (32, 24), (33, 24)])
+@pytest.mark.parametrize("S,W", [(2, 8), (5, 24), (17, 8), (20, 16), (31, 32), (32, 24), (33, 24)])
 def test_small_network_gram_matches_fp64(fp, orc, stack, S, W):
     """Networks of at most 32 stations take the Gram kernel that gives each of its
     eight waves a bin of its own (33 stations, the 64 x 64 block kernel): every
