@@ -425,6 +425,46 @@ __device__ __forceinline__ void ct_pass_oop(const C2<T> *in, C2<T> *out, const C
     __syncthreads();
 }
 
+// The first pass of ct_pass_oop (stride 1, no twiddles), one butterfly a thread,
+// with thread b's inputs (elements b + r N/R, r < R) already in its registers v:
+// the transform starts without staging its input in LDS.  Outputs to `out` in
+// layout LOUT, then a barrier.
+template <typename T, int N, int NT, int DIR, int R, int LOUT>
+__device__ __forceinline__ void ct_first_pass_regs(C2<T> *v, C2<T> *out)
+{
+    static_assert(N / R <= NT, "one butterfly a thread");
+    const int b = threadIdx.x;
+    if (N / R == NT || b < N / R) {
+        Dft<R, DIR>::run(v);
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[ct_lay<LOUT>(b * R + r)] = v[r];
+    }
+    __syncthreads();
+}
+
+// The last pass of ct_pass_oop (stride N/R), at most one butterfly a thread, with
+// its outputs kept in registers: thread b < N/R holds elements b + r N/R (r < R),
+// the positions of its inputs (threads past N/R leave v alone).  No barrier: the
+// caller orders any later write of `in` after every thread's reads.
+template <typename T, int N, int NT, int DIR, int R, int LIN>
+__device__ __forceinline__ void ct_last_pass_regs(const C2<T> *in, C2<T> *v, const C2<T> *__restrict__ tw)
+{
+    constexpr int NS = N / R;
+    static_assert(NS <= NT, "one butterfly a thread");
+    const int b = threadIdx.x;
+    if (NS == NT || b < NS) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = in[ct_lay<LIN>(b + r * NS)];
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            C2<T> w = tw[(r - 1) * NS + b];
+            if (DIR > 0) w.y = -w.y;
+            v[r] = cmul(v[r], w);
+        }
+        Dft<R, DIR>::run(v);
+    }
+}
+
 template <typename T, int N, int NT, int DIR, int NS, int OFF, int R, int... Rest>
 __device__ __forceinline__ void ct_passes(C2<T> *lds, const C2<T> *__restrict__ tw)
 {

@@ -115,17 +115,6 @@ k_station_pcc_batch(PccBatch jb, int Nz, FftPlan planZ, const cf32 *__restrict__
 
 constexpr int kSpN = 3600, kSpNz = 8192, kSpM = 4096, kSpT = 512;
 
-// The station kernel's 3600-point transforms, out of place a -> b -> a -> b (one
-// barrier a pass): plain layout in and out, the stage between the radix-16 and
-// radix-9 passes XOR-swizzled (3% faster than one pad slot per 16).
-template <int DIR>
-__device__ __forceinline__ void fft_3600(cf32 *a, cf32 *b, const cf32 *__restrict__ tw)
-{
-    ct_pass_oop<float, kSpN, kSpT, DIR, 16, 1, 0, 3>(a, b, tw);
-    ct_pass_oop<float, kSpN, kSpT, DIR, 9, 16, 3, 0>(b, a, tw + 15);
-    ct_pass_oop<float, kSpN, kSpT, DIR, 25, 144, 0, 0>(a, b, tw + 15 + 8 * 16);
-}
-
 // Station stage specialised for N = 3600, Nz = 8192 (one day at 1 Hz cut in
 // hourly windows): compile-time FFTs out of place between two LDS buffers of 4096
 // values (64 KB), 512 threads (the 4096-point transforms in radix-8 passes, a
@@ -139,7 +128,8 @@ __device__ __forceinline__ void fft_3600(cf32 *a, cf32 *b, const cf32 *__restric
 // for the odd ones, and each lane writes (F[2m], F[2m+1]) as one 16-byte store.
 __global__ void __launch_bounds__(kSpT)
 k_station_pcc_3600(const float *__restrict__ x, long long station_stride, int W, const cf32 *__restrict__ twN,
-                   const cf32 *__restrict__ twM, const cf32 *__restrict__ twZ, float da1, cf32 *__restrict__ F)
+                   const cf32 *__restrict__ twI, const cf32 *__restrict__ twM, const cf32 *__restrict__ twZ, float da1,
+                   cf32 *__restrict__ F)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cf32 *lds = reinterpret_cast<cf32 *>(smem_raw);
@@ -148,26 +138,56 @@ k_station_pcc_3600(const float *__restrict__ x, long long station_stride, int W,
     const int s = sw / W, w = sw - s * W;
     const float *in = x + (long long)s * station_stride + (long long)w * kSpN;
     cf32 *alt = lds + kSpM;
-    for (int n = t; n < kSpN; n += kSpT) lds[n] = cf32{in[n], 0.0f};
-    __syncthreads();
-    fft_3600<-1>(lds, alt, twN);  // spectrum in alt
+    constexpr int kB = kSpN / 16, kB25 = kSpN / 25;  // butterflies of the radix-16 and radix-25 passes
+    cf32 v[25];
+    // forward transform, radices 16, 9, 25: the window straight from global memory
+    // into the first pass's registers (thread b < 225: samples b + 225 r), the last
+    // pass's outputs kept in registers (thread b < 144: bins b + 144 r)
+    if (t < kB) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = cf32{in[t + kB * r], 0.0f};
+    }
+    ct_first_pass_regs<float, kSpN, kSpT, -1, 16, 3>(v, alt);
+    ct_pass_oop<float, kSpN, kSpT, -1, 9, 16, 3, 0>(alt, lds, twN + 15);
+    ct_last_pass_regs<float, kSpN, kSpT, -1, 25, 0>(lds, v, twN + 15 + 8 * 16);
+    // the analytic-signal gain in registers: bins 1 .. 1799 doubled, 0 and 1800 kept,
+    // the rest zero
     constexpr int half = (kSpN + 1) >> 1;
     const float two_da1 = 2 * da1;
-    for (int k = t; k < kSpN; k += kSpT) {
-        const float g = k == 0 ? da1 : k < half ? two_da1 : k == (kSpN >> 1) ? da1 : 0.0f;
-        cf32 v = alt[k];
-        alt[k] = cf32{v.x * g, v.y * g};
+    if (t < kB25) {
+#pragma unroll
+        for (int r = 0; r < 25; ++r) {
+            const int k = t + kB25 * r;
+            const float g = (k == 0 || k == (kSpN >> 1)) ? da1 : k < half ? two_da1 : 0.0f;
+            v[r] = cf32{v[r].x * g, v[r].y * g};
+        }
     }
-    __syncthreads();
-    fft_3600<+1>(alt, lds, twN);  // analytic signal in lds
+    // inverse transform, radices 25, 9, 16 (twiddles twI): thread b < 144's first
+    // butterfly takes bins b + 144 r, its own registers, and writes alt (last read
+    // by the forward's second pass, before its barrier); the last pass leaves
+    // samples b + 225 r (b < 225) of the analytic signal in registers
+    ct_first_pass_regs<float, kSpN, kSpT, +1, 25, 0>(v, alt);
+    ct_pass_oop<float, kSpN, kSpT, +1, 9, 25, 0, 0>(alt, lds, twI + 24);
+    ct_last_pass_regs<float, kSpN, kSpT, +1, 16, 0>(lds, v, twI + 24 + 8 * 25);
+    // AmpNormf: max |z|^2 from registers; the samples go through alt (last read by
+    // the inverse's second pass) to the 4096-point transforms' distribution, and
+    // the one barrier of the block maximum orders those writes before the reads
+    // (red is written once in this kernel, so no barrier guards its earlier use)
     float m = 0.0f;
-    for (int n = t; n < kSpN; n += kSpT) {
-        cf32 v = lds[n];
-        m = fmaxf(m, v.x * v.x + v.y * v.y);
+    if (t < kB) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            m = fmaxf(m, v[r].x * v[r].x + v[r].y * v[r].y);
+            alt[t + kB * r] = v[r];
+        }
     }
-    m = block_max(m, red);
+    m = wave_max(m);
+    if ((t & 63) == 0) red[t >> 6] = m;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSpT / 64; ++i) m = fmaxf(m, red[i]);
     const float eps = (float)(1e-6 * sqrt((double)m));
-    constexpr int kZ = (kSpN + kSpT - 1) / kSpT;  // samples a thread (n = t + kSpT i)
+    constexpr int kZ = kSpM / kSpT;  // samples a thread (n = t + kSpT i), zero past 3600
     cf32 *row = F + (long long)sw * kSpNz;
     cf32 z[kZ];
 #pragma unroll
@@ -175,44 +195,43 @@ k_station_pcc_3600(const float *__restrict__ x, long long station_stride, int W,
         const int n = t + kSpT * i;
         z[i] = cf32{0.0f, 0.0f};
         if (n < kSpN) {
-            const cf32 v = lds[n];
-            const float r = 1.0f / (sqrtf(v.x * v.x + v.y * v.y) + eps);
-            if (__builtin_isnormal(r)) z[i] = cf32{v.x * r, v.y * r};
+            const cf32 a = alt[n];
+            const float q = 1.0f / (sqrtf(a.x * a.x + a.y * a.y) + eps);
+            if (__builtin_isnormal(q)) z[i] = cf32{a.x * q, a.y * q};
         }
     }
-    __syncthreads();
     // the 4096-point transforms out of place between lds and alt, one barrier a
     // pass, the stages between passes 1-2 and 2-3 XOR-swizzled (every wave's LDS
-    // accesses free of bank conflicts); even bins: lds -> alt -> lds -> alt -> lds
+    // accesses free of bank conflicts).  Thread t's radix-8 butterfly of the first
+    // pass takes elements t + 512 i, its own z[i], from registers, and that of the
+    // last pass gives elements t + 512 i, the bins it stores, in registers.  Even
+    // bins: registers -> lds -> alt -> lds -> registers (lds was last read by the
+    // inverse's last pass, before the barrier above; alt, holding the analytic
+    // signal, is written only after the first pass's barrier, when every thread
+    // has its z)
 #pragma unroll
-    for (int i = 0; i < kSpM / kSpT; ++i) lds[t + kSpT * i] = i < kZ ? z[i] : cf32{0.0f, 0.0f};
-    __syncthreads();
-    ct_pass_oop<float, kSpM, kSpT, -1, 8, 1, 0, 2>(lds, alt, twM);
-    ct_pass_oop<float, kSpM, kSpT, -1, 8, 8, 2, 2>(alt, lds, twM + 7);
-    ct_pass_oop<float, kSpM, kSpT, -1, 8, 64, 2, 0>(lds, alt, twM + 7 + 56);
-    ct_pass_oop<float, kSpM, kSpT, -1, 8, 512, 0, 0>(alt, lds, twM + 7 + 56 + 448);
-    cf32 ev[kSpM / kSpT];
-#pragma unroll
-    for (int i = 0; i < kSpM / kSpT; ++i) ev[i] = lds[t + kSpT * i];
-    // odd bins: alt -> lds -> alt -> lds -> alt (alt was last read by the pass above,
-    // lds is read above before the barrier that follows)
-#pragma unroll
-    for (int i = 0; i < kSpM / kSpT; ++i) {
-        const int n = t + kSpT * i;
-        alt[n] = i < kZ && n < kSpN ? cmul(z[i], twZ[n]) : cf32{0.0f, 0.0f};
-    }
-    __syncthreads();
-    ct_pass_oop<float, kSpM, kSpT, -1, 8, 1, 0, 2>(alt, lds, twM);
+    for (int i = 0; i < kZ; ++i) v[i] = z[i];
+    ct_first_pass_regs<float, kSpM, kSpT, -1, 8, 2>(v, lds);
     ct_pass_oop<float, kSpM, kSpT, -1, 8, 8, 2, 2>(lds, alt, twM + 7);
     ct_pass_oop<float, kSpM, kSpT, -1, 8, 64, 2, 0>(alt, lds, twM + 7 + 56);
-    ct_pass_oop<float, kSpM, kSpT, -1, 8, 512, 0, 0>(lds, alt, twM + 7 + 56 + 448);
+    cf32 ev[kSpM / kSpT];
+    ct_last_pass_regs<float, kSpM, kSpT, -1, 8, 0>(lds, ev, twM + 7 + 56 + 448);
+    // odd bins: registers -> alt -> lds -> alt -> registers (alt was last read by
+    // the even transform's third pass, before its barrier; lds is written only
+    // after the first pass's barrier, when every thread has read the last pass's
+    // inputs from it)
+#pragma unroll
+    for (int i = 0; i < kZ; ++i) {
+        const int n = t + kSpT * i;
+        v[i] = n < kSpN ? cmul(z[i], twZ[n]) : cf32{0.0f, 0.0f};
+    }
+    ct_first_pass_regs<float, kSpM, kSpT, -1, 8, 2>(v, alt);
+    ct_pass_oop<float, kSpM, kSpT, -1, 8, 8, 2, 2>(alt, lds, twM + 7);
+    ct_pass_oop<float, kSpM, kSpT, -1, 8, 64, 2, 0>(lds, alt, twM + 7 + 56);
+    ct_last_pass_regs<float, kSpM, kSpT, -1, 8, 0>(alt, v, twM + 7 + 56 + 448);
     float4 *out = reinterpret_cast<float4 *>(row);
 #pragma unroll
-    for (int i = 0; i < kSpM / kSpT; ++i) {
-        const int k = t + kSpT * i;
-        const cf32 o = alt[k];
-        out[k] = float4{ev[i].x, ev[i].y, o.x, o.y};
-    }
+    for (int i = 0; i < kSpM / kSpT; ++i) out[t + kSpT * i] = float4{ev[i].x, ev[i].y, v[i].x, v[i].y};
 }
 
 // AmpNormf (src/FFTapps.c:178-205) in place on rows of N complex values.
@@ -346,14 +365,15 @@ int launch_station_pcc(const float *d_x, long long station_stride, int S, int W,
     }
     const float da1 = (float)(1 / (double)N);
     if (station_pcc_specialised(N, Nz)) {
-        const void *twN = get_order_twiddles(kSpN, {16, 9, 25}), *twM = get_order_twiddles(kSpM, {8, 8, 8, 8});
+        const void *twN = get_order_twiddles(kSpN, {16, 9, 25}), *twI = get_order_twiddles(kSpN, {25, 9, 16});
+        const void *twM = get_order_twiddles(kSpM, {8, 8, 8, 8});
         const void *twZ = get_full_twiddles(kSpNz, false);
-        if (!twN || !twM || !twZ) return -1;
+        if (!twN || !twI || !twM || !twZ) return -1;
         size_t lds = sizeof(cf32) * 2 * kSpM + sizeof(float) * 8;
         allow_lds((const void *)k_station_pcc_3600, lds);
         ProfScope prof("station_pcc", st);
         hipLaunchKernelGGL(k_station_pcc_3600, dim3(S * W), dim3(kSpT), lds, st, d_x, station_stride, W,
-                           (const cf32 *)twN, (const cf32 *)twM, (const cf32 *)twZ, da1, d_F);
+                           (const cf32 *)twN, (const cf32 *)twI, (const cf32 *)twM, (const cf32 *)twZ, da1, d_F);
         FPCC_CHECK(hipGetLastError());
         return 0;
     }
