@@ -173,6 +173,33 @@ __device__ __forceinline__ void ctb_pass(cf32 *lds, const cf32 *__restrict__ tw)
     __syncthreads();
 }
 
+// Register-ended passes of a tile's in-place transforms (transform t's point p at
+// lds[t * LDT + ct_pad(p)]), one butterfly b of transform t a call.  First pass
+// (stride 1): the butterfly's inputs, points b + r NPT/R, are in v; its outputs go
+// to LDS (the caller puts a barrier after every thread's call).
+template <int NPT, int LDT, int DIR, int R>
+__device__ __forceinline__ void ctb_first_regs(cf32 *v, int t, int b, cf32 *lds)
+{
+    Dft<R, DIR>::run(v);
+#pragma unroll
+    for (int r = 0; r < R; ++r) lds[t * LDT + ct_pad(b * R + r)] = v[r];
+}
+// Last pass (stride NPT/R): the butterfly's outputs, points b + r NPT/R, stay in v.
+template <int NPT, int LDT, int DIR, int R>
+__device__ __forceinline__ void ctb_last_regs(const cf32 *lds, int t, int b, cf32 *v, const cf32 *__restrict__ tw)
+{
+    constexpr int NS = NPT / R;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = lds[t * LDT + ct_pad(b + r * NS)];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+        cf32 w = tw[(r - 1) * NS + b];
+        if (DIR > 0) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+    }
+    Dft<R, DIR>::run(v);
+}
+
 template <int NPT, int NTR, int LDT, int DIR, int NS, int OFF, int R, int... Rest>
 __device__ __forceinline__ void ctb_passes(cf32 *lds, const cf32 *__restrict__ tw)
 {
@@ -278,131 +305,21 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
         }
         return u;
     };
-    if constexpr (CT) {
-        // all of a thread's loads first (16 points of 256 columns-rows), then the work:
-        // a loop that loads, computes and stores one point at a time waits for every load
-        constexpr int PER = kBigCols * kCtN1 / 256;
-        const bool xin = op == COL_FWD_INPUT || op == COL_FWD_CHIRP;
-        cf32 u[PER];
-        float xv[PER];
-        bool in[PER];
-#pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int idx = threadIdx.x + 256 * i, t = idx % kBigCols, r = idx / kBigCols, c = c0 + t;
-            const long long g = (long long)r * N2 + c;
-            u[i] = cf32{0.f, 0.f};
-            xv[i] = 0.f;
-            in[i] = false;
-            if (c < N2) {
-                if (xin) {
-                    const long long m = g - a.off;
-                    in[i] = m >= 0 && m < a.N;
-                    if (in[i]) xv[i] = xr[m];
-                } else {
-                    in[i] = op != COL_FWD_AMPNORMF || g < a.nvalid;
-                    if (in[i]) u[i] = a.src[sbase + g];
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int idx = threadIdx.x + 256 * i, t = idx % kBigCols, r = idx / kBigCols, c = c0 + t;
-            buf0[cix(t, r)] = c < N2 ? start((long long)r * N2 + c, u[i], xv[i], in[i]) : cf32{0.f, 0.f};
-        }
-    }
-    for (int idx = threadIdx.x; !CT && idx < kBigCols * N1; idx += blockDim.x) {
-        const int t = idx % kBigCols, r = idx / kBigCols;
-        const int c = c0 + t;
-        cf32 v = cf32{0.f, 0.f};
-        if (c < N2) {
-            const long long g = (long long)r * N2 + c;
-            if (op == COL_FWD_INPUT || op == COL_FWD_CHIRP) {
-                const long long m = g - a.off;
-                if (m >= 0 && m < a.N) {
-                    const float xv = xr[m];
-                    float s;
-                    if (a.sign_rule == 1) s = xv >= 0.f ? 1.f : -1.f;
-                    else if (a.sign_rule == 2) s = xv > 0.f ? 1.f : -1.f;
-                    else s = xv * a.in_scale;
-                    v = cf32{s, 0.f};
-                    if (op == COL_FWD_CHIRP) v = cf32{s * a.chirp[m].x, s * a.chirp[m].y};
-                }
-            } else if (op == COL_FWD_AMPNORM) {
-                const cf32 u = a.src[sbase + g];
-                const float rr = 1.0f / sqrtf(u.x * u.x + u.y * u.y + eps);
-                if (__builtin_isnormal(rr)) v = cf32{u.x * rr, u.y * rr};
-            } else if (op == COL_FWD_AMPNORMF) {
-                if (g < a.nvalid) {
-                    const cf32 u = a.src[sbase + g];
-                    const float rr = 1.0f / (sqrtf(u.x * u.x + u.y * u.y) + eps);
-                    if (__builtin_isnormal(rr)) v = cf32{u.x * rr, u.y * rr};
-                }
-            } else {
-                v = a.src[sbase + g];
-            }
-        }
-        buf0[cix(t, r)] = v;
-    }
-    __syncthreads();
     const bool inv = op == COL_INV_MAX || op == COL_INV_LAGWIN || op == COL_CHIRP_MID || op == COL_INV_CHIRP_MAX;
-    cf32 *res;
-    if (CT) res = inv ? ct_cols_fft<+1>(buf0, tw1) : ct_cols_fft<-1>(buf0, tw1);
-    else
-        res = inv ? fft_lds_batch<float, +1>(buf0, buf1, p1, tw1, kBigCols, ld)
-                  : fft_lds_batch<float, -1>(buf0, buf1, p1, tw1, kBigCols, ld);
-    if (op == COL_CHIRP_MID) {
-        // second transform's chirped input: conj(mask X) w = mask conj(conv) (|w| = 1)
-        const int half = (a.N + 1) >> 1;
-        for (int idx = threadIdx.x; idx < kBigCols * N1; idx += blockDim.x) {
-            const int t = idx % kBigCols, r = idx / kBigCols;
-            const long long g = (long long)r * N2 + c0 + t;
-            float gm = 0.f;
-            if (c0 + t < N2 && g < a.N) {
-                if (g == 0) gm = a.da1;
-                else if (g < half) gm = 2 * a.da1;
-                else if (!(a.N & 1) && g == (a.N >> 1)) gm = a.da1;
-            }
-            const cf32 v = res[cix(t, r)];
-            res[cix(t, r)] = cf32{gm * v.x, -gm * v.y};
-        }
-        __syncthreads();
-        res = CT ? ct_cols_fft<-1>(res, tw1) : fft_lds_batch<float, -1>(res, res == buf0 ? buf1 : buf0, p1, tw1, kBigCols, ld);
-    }
     const bool fwd_out = !inv || op == COL_CHIRP_MID;
     float m = 0.f;
     // LAGWIN batch decode
-    int p = 0, w = 0, i = 0, j = 0;
+    int p = 0, w = 0, i = 0, j = 0;  // pair, window, stations (LAGWIN)
     if (op == COL_INV_LAGWIN) {
         const int q = b / a.Wb;
         w = a.wsel + (b - q * a.Wb);
         p = a.pair0 + q;
         if (a.norm == 1) pair_index(p, a.S, i, j);
     }
-    bool done = false;
-    if constexpr (CT) {
-        if (fwd_out) {
-            // the step twiddles of all of a thread's points first, then the stores
-            constexpr int PER = kBigCols * kCtN1 / 256;
-            cf32 w[PER];
-#pragma unroll
-            for (int i = 0; i < PER; ++i) {
-                const int idx = threadIdx.x + 256 * i, t = idx % kBigCols, r = idx / kBigCols, c = c0 + t;
-                w[i] = c < N2 ? tw_cr(twN, c, r) : cf32{0.f, 0.f};
-            }
-#pragma unroll
-            for (int i = 0; i < PER; ++i) {
-                const int idx = threadIdx.x + 256 * i, t = idx % kBigCols, r = idx / kBigCols, c = c0 + t;
-                if (c < N2) a.dst[base + (long long)r * N2 + c] = cmul(res[cix(t, r)], w[i]);  // W_n^{n2 k1}
-            }
-            done = true;
-        }
-    }
-    for (int idx = threadIdx.x; !done && idx < kBigCols * N1; idx += blockDim.x) {
-        const int t = idx % kBigCols, r = idx / kBigCols;
+    // one output point of the transformed tile: column t, row r, value v
+    auto emit = [&](int t, int r, cf32 v) {
         const int c = c0 + t;
-        if (c >= N2) continue;
         const long long g = (long long)r * N2 + c;
-        cf32 v = res[cix(t, r)];
         if (fwd_out) {
             a.dst[base + g] = cmul(v, tw_cr(twN, c, r));  // W_n^{n2 k1}
         } else if (op == COL_INV_CHIRP_MAX) {
@@ -446,6 +363,154 @@ k_big_cols(ColArgs a, int N1, int N2, FftPlan p1, const cf32 *__restrict__ tw1, 
                 if (a.accumulate) a.y[(long long)p * a.L + l] += (float)o;
                 else a.y[((long long)p * a.Wfull + w) * a.L + l] = (float)o;
             }
+        }
+    };
+    bool regs = false;  // transformed in registers and emitted
+    if constexpr (CT) {
+        // all of a thread's loads first (16 points of 256 columns-rows), then the work:
+        // a loop that loads, computes and stores one point at a time waits for every load
+        constexpr int PER = kBigCols * kCtN1 / 256;
+        const bool xin = op == COL_FWD_INPUT || op == COL_FWD_CHIRP;
+        cf32 u[PER];
+        float xv[PER];
+        bool in[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int idx = threadIdx.x + 256 * i, t = idx % kBigCols, r = idx / kBigCols, c = c0 + t;
+            const long long g = (long long)r * N2 + c;
+            u[i] = cf32{0.f, 0.f};
+            xv[i] = 0.f;
+            in[i] = false;
+            if (c < N2) {
+                if (xin) {
+                    const long long m = g - a.off;
+                    in[i] = m >= 0 && m < a.N;
+                    if (in[i]) xv[i] = xr[m];
+                } else {
+                    in[i] = op != COL_FWD_AMPNORMF || g < a.nvalid;
+                    if (in[i]) u[i] = a.src[sbase + g];
+                }
+            }
+        }
+        cf32 v[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int idx = threadIdx.x + 256 * i, t = idx % kBigCols, r = idx / kBigCols, c = c0 + t;
+            v[i] = c < N2 ? start((long long)r * N2 + c, u[i], xv[i], in[i]) : cf32{0.f, 0.f};
+        }
+        if (op != COL_CHIRP_MID) {
+            // thread (column t, q) holds rows q + 16 i: the inputs of the first radix-16
+            // butterfly q of column t; the second pass's butterfly q gives rows q + 16 i
+            // back, the points this thread stores.  One barrier between the passes.
+            const int t = threadIdx.x % kBigCols, q = threadIdx.x / kBigCols, c = c0 + t;
+            if (inv) ctb_first_regs<kCtN1, kCtLD, +1, 16>(v, t, q, buf0);
+            else ctb_first_regs<kCtN1, kCtLD, -1, 16>(v, t, q, buf0);
+            __syncthreads();
+            if (inv) ctb_last_regs<kCtN1, kCtLD, +1, 16>(buf0, t, q, v, tw1 + 15);
+            else ctb_last_regs<kCtN1, kCtLD, -1, 16>(buf0, t, q, v, tw1 + 15);
+            if (fwd_out) {
+                // the step twiddles of all of a thread's points first, then the stores
+                cf32 wt[PER];
+#pragma unroll
+                for (int i = 0; i < PER; ++i) wt[i] = c < N2 ? tw_cr(twN, c, q + 16 * i) : cf32{0.f, 0.f};
+#pragma unroll
+                for (int i = 0; i < PER; ++i)
+                    if (c < N2) a.dst[base + (long long)(q + 16 * i) * N2 + c] = cmul(v[i], wt[i]);  // W_n^{n2 k1}
+            } else {
+#pragma unroll
+                for (int i = 0; i < PER; ++i)
+                    if (c < N2) emit(t, q + 16 * i, v[i]);
+            }
+            regs = true;
+        } else {
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const int idx = threadIdx.x + 256 * i, t = idx % kBigCols, r = idx / kBigCols;
+                buf0[cix(t, r)] = v[i];
+            }
+        }
+    }
+    if (!regs) {
+        for (int idx = threadIdx.x; !CT && idx < kBigCols * N1; idx += blockDim.x) {
+            const int t = idx % kBigCols, r = idx / kBigCols;
+            const int c = c0 + t;
+            cf32 v = cf32{0.f, 0.f};
+            if (c < N2) {
+                const long long g = (long long)r * N2 + c;
+                if (op == COL_FWD_INPUT || op == COL_FWD_CHIRP) {
+                    const long long m = g - a.off;
+                    if (m >= 0 && m < a.N) {
+                        const float xv = xr[m];
+                        float s;
+                        if (a.sign_rule == 1) s = xv >= 0.f ? 1.f : -1.f;
+                        else if (a.sign_rule == 2) s = xv > 0.f ? 1.f : -1.f;
+                        else s = xv * a.in_scale;
+                        v = cf32{s, 0.f};
+                        if (op == COL_FWD_CHIRP) v = cf32{s * a.chirp[m].x, s * a.chirp[m].y};
+                    }
+                } else if (op == COL_FWD_AMPNORM) {
+                    const cf32 u = a.src[sbase + g];
+                    const float rr = 1.0f / sqrtf(u.x * u.x + u.y * u.y + eps);
+                    if (__builtin_isnormal(rr)) v = cf32{u.x * rr, u.y * rr};
+                } else if (op == COL_FWD_AMPNORMF) {
+                    if (g < a.nvalid) {
+                        const cf32 u = a.src[sbase + g];
+                        const float rr = 1.0f / (sqrtf(u.x * u.x + u.y * u.y) + eps);
+                        if (__builtin_isnormal(rr)) v = cf32{u.x * rr, u.y * rr};
+                    }
+                } else {
+                    v = a.src[sbase + g];
+                }
+            }
+            buf0[cix(t, r)] = v;
+        }
+        __syncthreads();
+        cf32 *res;
+        if (CT) res = inv ? ct_cols_fft<+1>(buf0, tw1) : ct_cols_fft<-1>(buf0, tw1);
+        else
+            res = inv ? fft_lds_batch<float, +1>(buf0, buf1, p1, tw1, kBigCols, ld)
+                      : fft_lds_batch<float, -1>(buf0, buf1, p1, tw1, kBigCols, ld);
+        if (op == COL_CHIRP_MID) {
+            // second transform's chirped input: conj(mask X) w = mask conj(conv) (|w| = 1)
+            const int half = (a.N + 1) >> 1;
+            for (int idx = threadIdx.x; idx < kBigCols * N1; idx += blockDim.x) {
+                const int t = idx % kBigCols, r = idx / kBigCols;
+                const long long g = (long long)r * N2 + c0 + t;
+                float gm = 0.f;
+                if (c0 + t < N2 && g < a.N) {
+                    if (g == 0) gm = a.da1;
+                    else if (g < half) gm = 2 * a.da1;
+                    else if (!(a.N & 1) && g == (a.N >> 1)) gm = a.da1;
+                }
+                const cf32 v = res[cix(t, r)];
+                res[cix(t, r)] = cf32{gm * v.x, -gm * v.y};
+            }
+            __syncthreads();
+            res = CT ? ct_cols_fft<-1>(res, tw1) : fft_lds_batch<float, -1>(res, res == buf0 ? buf1 : buf0, p1, tw1, kBigCols, ld);
+        }
+        bool done = false;
+        if constexpr (CT) {
+            if (fwd_out) {
+                // the step twiddles of all of a thread's points first, then the stores
+                constexpr int PER = kBigCols * kCtN1 / 256;
+                cf32 w[PER];
+    #pragma unroll
+                for (int i = 0; i < PER; ++i) {
+                    const int idx = threadIdx.x + 256 * i, t = idx % kBigCols, r = idx / kBigCols, c = c0 + t;
+                    w[i] = c < N2 ? tw_cr(twN, c, r) : cf32{0.f, 0.f};
+                }
+    #pragma unroll
+                for (int i = 0; i < PER; ++i) {
+                    const int idx = threadIdx.x + 256 * i, t = idx % kBigCols, r = idx / kBigCols, c = c0 + t;
+                    if (c < N2) a.dst[base + (long long)r * N2 + c] = cmul(res[cix(t, r)], w[i]);  // W_n^{n2 k1}
+                }
+                done = true;
+            }
+        }
+        for (int idx = threadIdx.x; !done && idx < kBigCols * N1; idx += blockDim.x) {
+            const int t = idx % kBigCols, r = idx / kBigCols;
+            if (c0 + t >= N2) continue;
+            emit(t, r, res[cix(t, r)]);
         }
     }
     if (op == COL_INV_MAX || op == COL_INV_CHIRP_MAX) {
@@ -494,6 +559,69 @@ k_big_rows(RowArgs a, int N1, int N2, FftPlan p2, const cf32 *__restrict__ tw2, 
     }
     bool loaded = false;
     if constexpr (CTN2 != 0) {
+        if (op == ROW_FWD_PLAIN || op == ROW_FWD_SCALE || op == ROW_INV_MUL || op == ROW_INV_PLAIN) {
+            // transformed in registers between a first and a last pass: thread (row t, q)
+            // loads points q + 64 k of its row (coalesced across q), the inputs of the
+            // first pass's butterfly q; the last pass's butterflies give points
+            // b + (CTN2/8) k back, which the thread stores
+            constexpr int LD = ct_row_ld(CTN2), R1 = CTN2 == 512 ? 8 : 16, NB1 = CTN2 / R1;
+            constexpr int NSL = CTN2 / 8, BPL = kBigRows * NSL / 256;  // last pass: radix 8, stride NSL
+            static_assert(kBigRows * NB1 == 256, "one first-pass butterfly a thread");
+            const bool inv = op == ROW_INV_MUL || op == ROW_INV_PLAIN;
+            const int t = threadIdx.x / NB1, q = threadIdx.x % NB1, r = r0 + t;
+            cf32 v[R1], f[R1];
+#pragma unroll
+            for (int k = 0; k < R1; ++k) {
+                const long long g = (long long)r * CTN2 + q + NB1 * k;
+                v[k] = f[k] = cf32{0.f, 0.f};
+                if (r < N1) {
+                    if (op == ROW_INV_MUL) {
+                        v[k] = Xs[g];
+                        f[k] = Fs[g];
+                    } else {
+                        v[k] = a.src[base + g];
+                    }
+                }
+            }
+            if (op == ROW_INV_MUL) {
+#pragma unroll
+                for (int k = 0; k < R1; ++k) v[k] = cmul(v[k], f[k]);
+            }
+            if (inv) ctb_first_regs<CTN2, LD, +1, R1>(v, t, q, buf0);
+            else ctb_first_regs<CTN2, LD, -1, R1>(v, t, q, buf0);
+            __syncthreads();
+            if (inv) ctb_pass<CTN2, kBigRows, LD, +1, 8, R1>(buf0, tw2 + (R1 - 1));
+            else ctb_pass<CTN2, kBigRows, LD, -1, 8, R1>(buf0, tw2 + (R1 - 1));
+            cf32 o[BPL][8];
+#pragma unroll
+            for (int i = 0; i < BPL; ++i) {
+                const int bb = threadIdx.x + 256 * i;
+                if (inv) ctb_last_regs<CTN2, LD, +1, 8>(buf0, bb / NSL, bb % NSL, o[i], tw2 + (R1 - 1) + 7 * R1);
+                else ctb_last_regs<CTN2, LD, -1, 8>(buf0, bb / NSL, bb % NSL, o[i], tw2 + (R1 - 1) + 7 * R1);
+            }
+            const float sc = op == ROW_FWD_SCALE ? a.wscale[b % a.S] : 1.0f;
+            // the step twiddles of all of a thread's points first, then the stores
+            cf32 w[BPL][8];
+#pragma unroll
+            for (int i = 0; i < BPL; ++i) {
+                const int bb = threadIdx.x + 256 * i, rl = r0 + bb / NSL, bl = bb % NSL;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) w[i][k] = inv && rl < N1 ? tw_cr(twN, bl + NSL * k, rl) : cf32{0.f, 0.f};
+            }
+#pragma unroll
+            for (int i = 0; i < BPL; ++i) {
+                const int bb = threadIdx.x + 256 * i, rl = r0 + bb / NSL, bl = bb % NSL;
+                if (rl >= N1) continue;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int c = bl + NSL * k;
+                    const cf32 x = o[i][k];
+                    a.dst[base + (long long)rl * CTN2 + c] =
+                        inv ? cmulc(w[i][k], x) : op == ROW_FWD_SCALE ? cf32{x.x * sc, x.y * sc} : x;  // W_n^{-n2 k1}
+                }
+            }
+            return;
+        }
         if (op != ROW_INV_CROSS) {
             // all of a thread's loads first, then the LDS writes (a loop that loads and
             // stores one point at a time waits for every load)
