@@ -472,18 +472,20 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
     }
 
     // copies holding rows of real stations: a side's rows 0 .. n-1 fill its first
-    // ceil(n su / 64) copies, so a block that ends past the last station (networks of
-    // under 64 stations) copies only those (its other rows feed only pairs that are
-    // never stored)
-    const int na = (min(kGB, S - bi * kGB) * su + 63) / 64, nb = (min(kGB, S - bj * kGB) * su + 63) / 64;
-    auto used = [&](int c64) { return c64 < nchunk64 && (c64 < su ? c64 < na : c64 - su < nb); };
+    // ceil(n su / 64) copies, and only the last block ends past the last station, as
+    // side b (bi < bj: side a is whole) or as the one side of a diagonal block, so
+    // those copies are the first `lim` of the workgroup's; a block that ends past the
+    // last station copies only those (its other rows feed only pairs that are never
+    // stored).  A prefix keeps the copy loops' early exit (a test of each copy
+    // against both sides' counts ran the headline Gram 6% slower).
+    const int lim = same ? (min(kGB, S - bi * kGB) * su + 63) / 64 : su + (min(kGB, S - bj * kGB) * su + 63) / 64;
 
     auto dma = [&](int f, int buf) {
         const unsigned boff = (unsigned)(f * binb);
 #pragma unroll
         for (int j = 0; j < 9; ++j) {
             const int c64 = j * 8 + wave;  // scalar
-            if (!used(c64)) continue;
+            if (c64 >= lim) break;
             __builtin_amdgcn_raw_ptr_buffer_load_lds(c64 >= su ? rb : ra,
                                                      (__attribute__((address_space(3))) void *)(lds + buf * bufb +
                                                                                                 c64 * 1024),
@@ -500,7 +502,7 @@ k_gram16_dma(const uint2 *__restrict__ Ua, const uint2 *__restrict__ Ub, int S, 
     // copies this wave issues per bin
     int nw = 0;
 #pragma unroll
-    for (int j = 0; j < 9; ++j) nw += used(j * 8 + wave);
+    for (int j = 0; j < 9; ++j) nw += j * 8 + wave < lim;
     // NB buffers: bins f + 1 .. f + NB - 1 in flight while bin f is multiplied
     dma(0, 0);
 #pragma unroll
