@@ -175,11 +175,37 @@ k_pair_gn(const C2<T> *__restrict__ Xa, const C2<T> *__restrict__ Xb, long long 
     pair_index(p, S, i, j);
     const int T_ = blockDim.x;
     const T inv_nz = T(1) / (T)Nz;
-    for (int l = threadIdx.x; l < L; l += T_) acc[l] = 0.0f;
+    // every lag within +-1024 (the 4096-point inverse): thread t's butterflies take
+    // the bins it folds, t + 512 q, from registers, and its last one gives the
+    // transform values t and t + 3584 (lags 2t, 2t + 1 and 2t - 1024, 2t - 1023),
+    // normalised and summed over the windows in registers
+    // (FP64 only: the FP32 kernel keeps four waves a SIMD only without it)
+    constexpr bool kRegs = CT4096 && sizeof(T) == 8;
+    const bool ends = kRegs && lag0 >= -1024 && lag0 + L - 1 <= 1023;
+    float ar[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (!ends)
+        for (int l = threadIdx.x; l < L; l += T_) acc[l] = 0.0f;
+    // the normalised correlation at lag (lag0 + l) of window w from its transform value
+    auto lagval = [&](int l, T vt, int w, const double *P1, const double *P2) -> float {
+        const int lag = lag0 + l;
+        double c = (double)(vt * inv_nz);
+        const int m = N - abs(lag);
+        if (edge && m > 0 && m < K) c = edge_corr(edge, K, W, i, j, w, w + 1, lag, m, norm != 0);
+        double den;
+        if (norm == 0) {
+            const int a1 = max(0, -lag), b1 = min(N, N - lag);
+            const int a2 = max(0, lag), b2 = min(N, N + lag);
+            double e1 = P1[b1] - P1[a1], e2 = P2[b2] - P2[a2];
+            den = sqrt(e1 * e2);
+        } else {
+            den = (double)(N - abs(lag));
+        }
+        return (float)(c / den);
+    };
     for (int w = w0; w < w1; ++w) {
         const C2<T> *X1 = Xa + (long long)(i * W + w) * ldx;
         const C2<T> *X2 = Xb + (long long)(j * W + w) * ldx;
-        auto fold = [&](int k) {
+        auto foldv = [&](int k) {
             C2<T> a = cmulc(X1[k], X2[k]);
             C2<T> c = cmulc(X1[M - k], X2[M - k]);
             if (k == 0) {  // c2r ignores the imaginary parts of DC and Nyquist
@@ -189,8 +215,35 @@ k_pair_gn(const C2<T> *__restrict__ Xa, const C2<T> *__restrict__ Xb, long long 
             C2<T> P = C2<T>{a.x + c.x, a.y - c.y};
             C2<T> Q = C2<T>{a.x - c.x, a.y + c.y};
             C2<T> wq = cmul(twc[k], Q);
-            buf0[zi(k)] = C2<T>{P.x - wq.y, P.y + wq.x};
+            return C2<T>{P.x - wq.y, P.y + wq.x};
         };
+        auto fold = [&](int k) { buf0[zi(k)] = foldv(k); };
+        const double *P1 = E ? E + (long long)(i * W + w) * lde : nullptr;
+        const double *P2 = E ? E + (long long)(j * W + w) * lde : nullptr;
+        if constexpr (kRegs) {
+            if (ends) {
+                C2<T> v[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    v[q] = foldv(threadIdx.x + 512 * q);
+                    // two bins' loads at a time keep FP64 within two workgroups a CU
+                    if (q & 1) __builtin_amdgcn_sched_barrier(0);
+                }
+                ct_first_pass_regs<T, 4096, 512, +1, 8, 1>(v, buf0);
+                ct_pass<T, 4096, 512, +1, 8, 8, 1, 1>(buf0, twM + 7);
+                ct_pass<T, 4096, 512, +1, 8, 64, 1, 1>(buf0, twM + 7 + 56);
+                ct_last_pass_regs<T, 4096, 512, +1, 8, 1>(buf0, v, twM + 7 + 56 + 448);
+                const int t2 = 2 * threadIdx.x;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int l = (e < 2 ? t2 + e : t2 - 1024 + (e - 2)) - lag0;
+                    const T vt = e == 0 ? v[0].x : e == 1 ? v[0].y : e == 2 ? v[7].x : v[7].y;
+                    if (l >= 0 && l < L) ar[e] += lagval(l, vt, w, P1, P2);
+                }
+                __syncthreads();  // every lane has read buf0 before the next window's first pass
+                continue;
+            }
+        }
         C2<T> *z;
         if constexpr (CT4096) {
             // two bins' loads at a time keep FP64 within two workgroups a CU
@@ -204,35 +257,24 @@ k_pair_gn(const C2<T> *__restrict__ Xa, const C2<T> *__restrict__ Xb, long long 
             __syncthreads();
             z = fft_lds<T, +1>(buf0, buf1, planM, twM);
         }
-        const double *P1 = E ? E + (long long)(i * W + w) * lde : nullptr;
-        const double *P2 = E ? E + (long long)(j * W + w) * lde : nullptr;
         for (int l = threadIdx.x; l < L; l += T_) {
-            const int lag = lag0 + l;
-            const int n = lag & (Nz - 1);
+            const int n = (lag0 + l) & (Nz - 1);
             C2<T> v = z[zi(n >> 1)];
-            double c = (double)(((n & 1) ? v.y : v.x) * inv_nz);
-            const int m = N - abs(lag);
-            if (edge && m > 0 && m < K) c = edge_corr(edge, K, W, i, j, w, w + 1, lag, m, norm != 0);
-            double den;
-            if (norm == 0) {
-                const int a1 = max(0, -lag), b1 = min(N, N - lag);
-                const int a2 = max(0, lag), b2 = min(N, N + lag);
-                double e1 = P1[b1] - P1[a1], e2 = P2[b2] - P2[a2];
-                den = sqrt(e1 * e2);
-            } else {
-                den = (double)(N - abs(lag));
-            }
-            acc[l] += (float)(c / den);
+            acc[l] += lagval(l, (n & 1) ? v.y : v.x, w, P1, P2);
         }
         __syncthreads();
     }
-    if (mode == 0) {
-        float *dst = y + ((long long)p * W + w0) * L;
-        for (int l = threadIdx.x; l < L; l += T_) dst[l] = acc[l];
+    const float sc = mode == 0 ? 1.0f : 1.0f / (float)W;
+    float *dst = mode == 0 ? y + ((long long)p * W + w0) * L : y + (long long)p * L;
+    if (ends) {
+        const int t2 = 2 * threadIdx.x;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int l = (e < 2 ? t2 + e : t2 - 1024 + (e - 2)) - lag0;
+            if (l >= 0 && l < L) dst[l] = mode == 0 ? ar[e] : ar[e] * sc;
+        }
     } else {
-        const float inv = 1.0f / (float)W;
-        float *dst = y + (long long)p * L;
-        for (int l = threadIdx.x; l < L; l += T_) dst[l] = acc[l] * inv;
+        for (int l = threadIdx.x; l < L; l += T_) dst[l] = mode == 0 ? acc[l] : acc[l] * sc;
     }
 }
 
