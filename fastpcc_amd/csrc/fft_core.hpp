@@ -465,6 +465,34 @@ __device__ __forceinline__ void ct_last_pass_regs(const C2<T> *in, C2<T> *v, con
     }
 }
 
+// ct_last_pass_ends_4096 (below) with its two outputs kept in registers: lo =
+// value b, hi = value b + 3584 of the transform, for thread b.  No barrier.
+__device__ __forceinline__ void ct_last_pass_ends_4096_regs(const C2<float> *in, const C2<float> *__restrict__ tw,
+                                                            C2<float> &lo, C2<float> &hi)
+{
+    constexpr int NS = 512;
+    const int b = threadIdx.x;
+    C2<float> v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = in[b + r * NS];
+#pragma unroll
+    for (int r = 1; r < 8; ++r) {
+        C2<float> w = tw[(r - 1) * NS + b];
+        w.y = -w.y;
+        v[r] = cmul(v[r], w);
+    }
+    const C2<float> es02 = v[0] + v[4], ed02 = v[0] - v[4];
+    const C2<float> es13 = v[2] + v[6], ed13 = rot<+1>(v[2] - v[6]);
+    const C2<float> os02 = v[1] + v[5], od02 = v[1] - v[5];
+    const C2<float> os13 = v[3] + v[7], od13 = rot<+1>(v[3] - v[7]);
+    const C2<float> e0 = es02 + es13, e3 = ed02 - ed13;
+    const C2<float> o0 = os02 + os13, o3 = od02 - od13;
+    constexpr float h = 0.70710678118654752440f;
+    const C2<float> o3w = C2<float>{(-o3.x - o3.y) * h, (-o3.y + o3.x) * h};  // * W8^3
+    lo = e0 + o0;
+    hi = e3 - o3w;
+}
+
 template <typename T, int N, int NT, int DIR, int NS, int OFF, int R, int... Rest>
 __device__ __forceinline__ void ct_passes(C2<T> *lds, const C2<T> *__restrict__ tw)
 {

@@ -107,6 +107,49 @@ k_pair_pws(const void *__restrict__ Xa, const void *__restrict__ Xb, long long l
             }
             AM = conj_mul(Hi[M], Hj[M]).x;
         }
+        if constexpr (CT4096 && ENDS && LPT == 4) {
+            // every lag within +-1024: thread t's first butterflies take its bins
+            // t + 512 q from registers (A, then A e^{2 pi i k/Nz}) and its last ones
+            // give transform values t and t + 3584 of each half, i.e. lags 2t, 2t + 1,
+            // 2t - 1024 and 2t - 1023, summed into registers.  Half 0 runs
+            // registers -> buf1 -> buf0 -> buf1 -> registers and half 1 registers ->
+            // buf0 -> buf1 -> buf0 -> registers: each first pass writes the buffer
+            // the previous transform's third pass read (before its barrier), and the
+            // second pass writes the one the previous last pass read only after the
+            // first pass's barrier.  Six barriers a window.
+            constexpr int MC = 4096;
+            static_assert(KPT * NT == MC, "a first-pass butterfly's bins a thread");
+            cf32 e[2][2];
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                cf32 v[KPT];
+#pragma unroll
+                for (int q = 0; q < KPT; ++q) v[q] = half ? cmul(A[q], twc[t + NT * q]) : A[q];
+                cf32 *b0 = half ? buf0 : buf1, *b1 = half ? buf1 : buf0;
+                ct_first_pass_regs<float, MC, NT, +1, 8, 2>(v, b0);
+                ct_pass_oop<float, MC, NT, +1, 8, 8, 2, 2>(b0, b1, tw + 7);
+                ct_pass_oop<float, MC, NT, +1, 8, 64, 2, 0>(b1, b0, tw + 7 + 56);
+                ct_last_pass_ends_4096_regs(b0, tw + 7 + 56 + 448, e[half][0], e[half][1]);
+            }
+            // lag 2t + half from value t, lag 2t - 1024 + half from value t + 3584
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int half = g & 1, hi = g >> 1;
+                const int l = 2 * t - (hi ? 1024 : 0) + half - lag0;
+                if (l >= 0 && l < L) {
+                    const cf32 v = e[half][hi];
+                    const cf32 zz = cf32{v.x + (half ? -AM : AM), v.y};
+                    sc[g] += zz.x * cscale;
+                    const float m2 = zz.x * zz.x + zz.y * zz.y;
+                    if (m2 > 0.f) {
+                        const float r = rsqrtf(m2);
+                        su[g].x += zz.x * r;
+                        su[g].y += zz.y * r;
+                    }
+                }
+            }
+            continue;
+        }
         cf32 z[LPT];
 #pragma unroll
         for (int q = 0; q < LPT; ++q) z[q] = cf32{0.f, 0.f};
@@ -160,8 +203,10 @@ k_pair_pws(const void *__restrict__ Xa, const void *__restrict__ Xb, long long l
     const float invW = 1.f / (float)W;
 #pragma unroll
     for (int q = 0; q < LPT; ++q) {
-        const int l = t + NT * q;
-        if (l >= L) continue;
+        // the lag of accumulator q: l = t + NT q, or with the register path above
+        // (lags within +-1024) 2t + (q & 1) - 1024 (q >> 1) - lag0
+        const int l = (CT4096 && ENDS && LPT == 4) ? 2 * t + (q & 1) - ((q >> 1) ? 1024 : 0) - lag0 : t + NT * q;
+        if (l < 0 || l >= L) continue;
         const float ux = su[q].x * invW, uy = su[q].y * invW;
         const float m2 = ux * ux + uy * uy;
         const float coh = power == 0.f ? 1.f : (m2 > 0.f ? powf(m2, 0.5f * power) : 0.f);
@@ -175,10 +220,11 @@ int launch_lpt(int lpt, bool ct, dim3 grid, size_t lds, hipStream_t st, const vo
                int S, int W, int Nz, const FftPlan &plan, const cf32 *tw, const cf32 *twc, int lag0, int L,
                float cscale, float power, int multiply, float *y)
 {
+    // lags within +-1024: the register-ended kernel, four lags a thread
     const bool ends = ct && lag0 >= -1024 && lag0 + L - 1 <= 1023;
 #define FPCC_PWS_CASE(N)                                                                                          \
     case N: {                                                                                                     \
-        const auto fn = ends ? k_pair_pws<SRC, N, true, (N <= 4)>                                                 \
+        const auto fn = ends ? k_pair_pws<SRC, 4, true, true>                                                     \
                              : ct ? k_pair_pws<SRC, N, true> : k_pair_pws<SRC, N, false>;                        \
         if (lds > 65536) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, \
                                                    (int)lds);                                                     \
