@@ -118,7 +118,7 @@ constexpr int kSpN = 3600, kSpNz = 8192, kSpM = 4096, kSpT = 512;
 // Station stage specialised for N = 3600, Nz = 8192 (one day at 1 Hz cut in
 // hourly windows): compile-time FFTs out of place between two LDS buffers of 4096
 // values (64 KB), 512 threads (the 4096-point transforms in radix-8 passes, a
-// butterfly a thread; 110 registers and 64 KB allow two workgroups, 16 waves, a CU:
+// butterfly a thread; 116 registers and 64 KB allow two workgroups, 16 waves, a CU:
 // 6% faster than 256 threads with radix-16 passes at 147 registers and 12 waves;
 // out of place, 2.5% faster than in place in one padded buffer).  The window occupies
 // samples < 3600 < 4096 of the zero-padded 8192, so the 8192-point spectrum
