@@ -89,10 +89,17 @@ k_pcc_time(const cf32 *__restrict__ phi, int S, int W, int N, int lag_first, int
                     for (int t = lo; t < hi; ++t) part += term(s2[t + tid], s1[t]);
                 }
             };
-            if (pcc1) {
+            // PCC1, and PCC-v at v = 1 (|.|^2 to the power 1/2), take square roots; PCC-v
+            // at v = 2 takes the squared magnitudes themselves (powf(x, 1) = x)
+            if (pcc1 || V == 0.5f) {
                 run([](cf32 a, cf32 b) {
                     const float sx = a.x + b.x, sy = a.y + b.y, dx = a.x - b.x, dy = a.y - b.y;
                     return hw_sqrt(sx * sx + sy * sy) - hw_sqrt(dx * dx + dy * dy);
+                });
+            } else if (V == 1.0f) {
+                run([](cf32 a, cf32 b) {
+                    const float sx = a.x + b.x, sy = a.y + b.y, dx = a.x - b.x, dy = a.y - b.y;
+                    return (sx * sx + sy * sy) - (dx * dx + dy * dy);
                 });
             } else if (V > 0.f) {
                 run([V](cf32 a, cf32 b) {
