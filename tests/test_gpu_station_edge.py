@@ -1,9 +1,11 @@
-"""The specialised station stage (hourly windows of 3600 samples, nfft 8192) on
-windows that reach its special cases, against the FP64 oracle: a window of zeros
+"""The station stages on windows that reach their special cases, against the FP64
+oracle: the specialised kernel (hourly windows of 3600 samples, nfft 8192), the
+runtime-plan kernel (2000 samples, nfft 4096) and the four-step path (5000 samples,
+nfft 16384), each with a window of zeros
 (AmpNormf's reciprocal is not normal, the phasors are zero), a constant window (only
 bin 0, kept at the analytic gain da1 instead of doubled) and an alternating window
-(only bin 1800 = N/2, also kept at da1), beside noise windows; per-window output
-and the split-FP16 mean stack."""
+(only bin N/2, also kept at da1), beside noise windows; per-window output and the
+split-FP16 mean stack."""
 import numpy as np
 import pytest
 
@@ -13,8 +15,8 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
-def special_network(seed):
-    S, W, N = 6, 4, 3600
+def special_network(seed, N=3600):
+    S, W = 6, 4
     x, _ = synth.make_stations(S, W * N, 300, seed=seed)
     x = x.astype(np.float32)
     x[0, 1 * N:2 * N] = 0.0                                   # a silent hour
@@ -25,9 +27,10 @@ def special_network(seed):
     return x, S, W, N
 
 
+@pytest.mark.parametrize("N", [3600, 2000, 5000])
 @pytest.mark.parametrize("stack", ["none", "gram16"])
-def test_station_special_windows(fp, orc, stack):
-    x, S, W, N = special_network(29)
+def test_station_special_windows(fp, orc, stack, N):
+    x, S, W, N = special_network(29, N)
     print(f"pcc2/{stack} S={S} W={W} N={N}: zero, constant and alternating windows", flush=True)
     y = fp.allpairs(x, W, N, -1000, 1000, kind="pcc2", stack=stack)
     y64 = orc.allpairs("pcc2", x, W, N, -1000, 1000, stacked=(stack != "none"), f64=True)
