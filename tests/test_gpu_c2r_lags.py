@@ -2,8 +2,9 @@
 inside [-1024, 1023] take the last pass that forms only the outputs those lags
 read (the transform's outputs 0..511 and 3584..4095), the others the full pass.
 The Gram stacks' inverse (PCC2 and 1-bit, FP32 and split-FP16), the per-window
-PCC2 path (every window, their mean, the summed spectra) and the phase-weighted
-stack, every pair against the FP64 oracle."""
+PCC2 path (every window, their mean, the summed spectra), the phase-weighted
+stack and GNCC's FP64 inverse (every window, their mean), every pair against the FP64
+oracle."""
 import numpy as np
 import pytest
 
@@ -57,3 +58,18 @@ def test_pws_lag_windows(fp, orc, kind, lag1, lag2):
     err = float(np.abs(y - yr).max())
     print(f"  max|gpu-fp64|={err:.3e}")
     assert y.shape == yr.shape and err <= TOL
+
+
+@pytest.mark.parametrize("stack", ["none", "plain"])
+@pytest.mark.parametrize("lag1,lag2", WINDOWS)
+def test_gncc_lag_windows(fp, orc, stack, lag1, lag2):
+    """GNCC's FP64 pair inverse: windows inside [-1024, 1023] take the path whose
+    butterflies start and end in registers, the others the LDS path."""
+    S, W, N = 4, 3, 3600
+    print(f"gncc/{stack} S={S} W={W} N={N} lags ({lag1},{lag2})", flush=True)
+    x, _ = synth.make_stations(S, W * N, 300, seed=23 + lag2 - lag1)
+    y = fp.allpairs(x, W, N, lag1, lag2, kind="gncc", stack=stack)
+    y64 = orc.allpairs("gncc", x, W, N, lag1, lag2, stacked=(stack != "none"), f64=True)
+    err = float(np.abs(y.reshape(y64.shape) - y64).max())
+    print(f"  max|gpu-fp64|={err:.3e}")
+    assert err <= TOL
