@@ -669,15 +669,15 @@ k_gram_small(const uint2 *__restrict__ U, int S, int K, int nk, int nchunks, flo
     const unsigned nrec = (unsigned)((long long)min(kGF, nk - k0) * binb);
     const __amdgpu_buffer_rsrc_t r =
         __builtin_amdgcn_make_buffer_rsrc((void *)(U + (long long)k0 * S * K), 0, nrec, 0x00020000);
-    for (int g = wave; g < kGF * npb; g += kGThreads / 64) {  // uniform
-        const int f = g / npb, c = g - f * npb;
+    // wave f copies bin f's rows and multiplies them as soon as its own copies have
+    // landed: no barrier before the products (no wave reads another's bin)
+    for (int c = 0; c < npb; ++c) {  // uniform
         const int p = c * 64 + lane, row = p / su, pu = p - row * su;
-        const unsigned off = pu < upr ? (unsigned)(f * binb + ((long long)row * K + 2 * pu) * 8) : 0x80000000u;
+        const unsigned off = pu < upr ? (unsigned)(wave * binb + ((long long)row * K + 2 * pu) * 8) : 0x80000000u;
         __builtin_amdgcn_raw_ptr_buffer_load_lds(
-            r, (__attribute__((address_space(3))) void *)(smem_raw + f * binlds + c * 1024), 16, off, 0, 0, 0);
+            r, (__attribute__((address_space(3))) void *)(smem_raw + wave * binlds + c * 1024), 16, off, 0, 0, 0);
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __builtin_amdgcn_s_barrier();
 
     // tiles (0,0), (0,1), (1,1): rows 16 ti .., columns 16 tj ..
     f4v acc[3][2];
